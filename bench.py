@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark of the isolated narrow-phase workload on one MI355X.
+
+Workload: 2^24 unique triangle pairs, vertices uniform in a 1 mm cube,
+separation log-uniform in [1e-7, 1e-2] m, 5% near-parallel edge pairs
+(angle < 1e-4 rad), 1% intersecting; generated in HBM with a fixed seed.
+A step is one pass of the narrow phase over all pairs.  The timed region
+brackets exactly --steps passes with device synchronisation and HIP events on
+the stream the kernels run on.  Inputs are resident in HBM before timing.
+
+Prints ONE JSON line.  The CPU baseline (the C restatement of the reference,
+OpenMP over pairs, on this host's cores) is timed on a bounded sample of the
+same pairs, and the GPU results of that sample are checked against it.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--pairs-log2", type=int, default=24)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="target wall time of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--methods", default="bf64,penalty64,hybrid64,mixed,learned")
+    return ap.parse_args()
+
+
+def time_method(batch, method, steps, warmup):
+    from delta_amd import _lib
+    L = _lib.lib()
+    for _ in range(warmup):
+        batch.run(method)
+    _lib.check(L.dh_synchronize(), "sync")
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    _lib.check(L.dh_event_create(C.byref(e0)), "ev")
+    _lib.check(L.dh_event_create(C.byref(e1)), "ev")
+    t0 = time.perf_counter()
+    _lib.check(L.dh_event_record(e0, None), "rec")
+    for _ in range(steps):
+        batch.run(method)
+    _lib.check(L.dh_event_record(e1, None), "rec")
+    _lib.check(L.dh_synchronize(), "sync")
+    wall = time.perf_counter() - t0
+    ms = C.c_float()
+    _lib.check(L.dh_event_elapsed_ms(C.byref(ms), e0, e1), "elapsed")
+    L.dh_event_destroy(e0)
+    L.dh_event_destroy(e1)
+    return ms.value / steps, wall
+
+
+def cpu_baseline(batch, target_s):
+    """Time the oracle (OpenMP over pairs) on a bounded prefix of the pairs and
+    check the GPU's bf64 results on that prefix bit for bit."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    path = os.path.join(ROOT, "oracle", "liboracle_omp.so")
+    if not os.path.exists(path):
+        return None, None
+    orc = C.CDLL(path)
+    orc.or_tri_pairs_omp.argtypes = [C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    soa_all = batch.soa.download()
+    batch.run("bf64")
+    P_g, Q_g, d_g, _ = batch.results()
+
+    def run(m):
+        soa = np.ascontiguousarray(soa_all[:, :m])
+        P = np.empty((3, m))
+        Q = np.empty((3, m))
+        d = np.empty(m)
+        t0 = time.perf_counter()
+        orc.or_tri_pairs_omp(0, soa.ctypes.data, m, 2e-4, P.ctypes.data, Q.ctypes.data,
+                             d.ctypes.data, None, threads)
+        return time.perf_counter() - t0, P, Q, d
+
+    m = min(batch.n, 1 << 16)
+    run(m)  # warm-up
+    dt, _, _, _ = run(m)
+    rate = m / dt
+    m = int(min(batch.n, max(m, rate * target_s)))
+    dt, P, Q, d = run(m)
+    mism = int(np.count_nonzero(d.view(np.uint64) != d_g[:m].view(np.uint64)) +
+               np.count_nonzero(P.view(np.uint64) != P_g[:, :m].view(np.uint64)) +
+               np.count_nonzero(Q.view(np.uint64) != Q_g[:, :m].view(np.uint64)))
+    return ({"value": m / dt, "unit": "triangle-pair checks/s", "cores": threads,
+             "kind": "port",
+             "sample": f"first {m} of the {batch.n} pairs, FP64 brute force (oracle bfSolver, "
+                       f"OpenMP over pairs, {threads} threads), {dt:.1f} s"},
+            {"pairs": m, "bitwise_mismatches": mism})
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a "
+                                   "design note only (bench.py --gpus 1)"}))
+        sys.exit(2)
+    from delta_amd import narrow
+    n = 1 << a.pairs_log2
+    print(f"[bench] generating {n} pairs in HBM", file=sys.stderr, flush=True)
+    batch = narrow.PairBatch(n, seed=3, with_f32=True)
+    results = {}
+    for method in a.methods.split(","):
+        try:
+            ms, wall = time_method(batch, method, a.steps, a.warmup)
+        except Exception as ex:  # method not built yet
+            print(f"[bench] {method}: {ex}", file=sys.stderr, flush=True)
+            continue
+        results[method] = {"ms_per_step": ms, "checks_per_s": n / (ms * 1e-3), "wall_s": wall}
+        print(f"[bench] {method}: {ms:.3f} ms/pass, {n / (ms * 1e-3):.3e} pairs/s",
+              file=sys.stderr, flush=True)
+    head = "learned" if "learned" in results else ("mixed" if "mixed" in results else "bf64")
+    r = results[head]
+    bf = results.get("bf64")
+    # roofline of the dominant kernel (one launch per pass)
+    if head == "bf64":
+        bytes_per = narrow.BYTES_F64
+    else:
+        bytes_per = narrow.BYTES_F32_BULK
+    achieved = bytes_per * n / (r["ms_per_step"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+            "kernel": f"dh_tri_pairs[{head}]", "bytes_per_pair": bytes_per}
+    extra = {}
+    if bf:
+        fl = narrow.FLOP_BF * n / (bf["ms_per_step"] * 1e-3) / 1e12
+        extra["bf64_fp64_tflops"] = fl
+        extra["bf64_fp64_frac"] = fl / PEAK_FP64_TFLOPS
+        extra["bf64_checks_per_s"] = bf["checks_per_s"]
+        extra["speedup_vs_bf64"] = r["checks_per_s"] / bf["checks_per_s"]
+    out = {
+        "metric": "triangle-pair checks/s, learned-hybrid vs plain FP64 BF, at 1e-6 dist err",
+        "value": r["checks_per_s"],
+        "unit": "triangle-pair checks/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "bf64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "isolated narrow-phase microbench: 2^24 random triangle pairs, "
+                               "1 mm cube, separation log-uniform 1e-7..1e-2 m, 5% near-parallel, "
+                               "1% intersecting",
+                   "pairs": n, "method": head},
+        "roofline": roof,
+        "methods": {k: {"ms_per_step": v["ms_per_step"], "checks_per_s": v["checks_per_s"]}
+                    for k, v in results.items()},
+    }
+    out.update(extra)
+    if not a.no_cpu:
+        cb, chk = cpu_baseline(batch, a.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["bf64_parity_sample"] = chk
+        if cb:
+            out["speedup_vs_cpu"] = r["checks_per_s"] / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

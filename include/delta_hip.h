@@ -1,0 +1,263 @@
+/*
+ * delta_hip.h — C ABI of the MI355X contact-detection and time-stepping engine.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns a status code (DH_OK == 0).  Pointers documented as "device" must be
+ * HBM allocations made with dh_malloc (or any hipMalloc'd memory of the same
+ * process); "host" pointers are ordinary process memory.  No entry point takes
+ * ownership of caller buffers.
+ *
+ * Geometry convention (same as the reference): a particle is a flat triangle
+ * soup stored as three arrays x[], y[], z[] of length 3*ntri, where
+ * x[3*t+k] is the x coordinate of vertex k of triangle t
+ * (delta/geometry/mesh/Mesh.cpp:248-302, delta/core/data/ParticleRecord.h:78-84).
+ * All geometry arithmetic of the exact paths is IEEE binary64 without fused
+ * multiply-add, in the reference's operation order.
+ */
+#ifndef DELTA_HIP_H
+#define DELTA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+  DH_OK = 0,
+  DH_ERR_ARG = 1,        /* null pointer, negative size, unknown method      */
+  DH_ERR_HIP = 2,        /* a HIP runtime call failed (see dh_last_error)    */
+  DH_ERR_CAPACITY = 3,   /* more contacts than `cap`; *n_out holds the need  */
+  DH_ERR_REF_UB = 4,     /* input on which the reference reads out of bounds */
+  DH_ERR_NO_DEVICE = 5,  /* no GPU visible                                   */
+  DH_ERR_UNSUPPORTED = 6 /* method not available for this entry point        */
+};
+
+/* ---------------------------------------------------------------- methods */
+/* Particle-pair detectors: one value per reference free function. */
+enum {
+  DH_BF = 0,                     /* detection::bf              bf.cpp:28-149      */
+  DH_PENALTY = 1,                /* detection::penalty         penalty.cpp:113-241 */
+  DH_HYBRID_BATCH = 2,           /* hybridWithPerBatchFallBack hybrid.cpp:338-487  */
+  DH_HYBRID_TRIANGLE = 3,        /* hybridWithPerTriangleFallBack hybrid.cpp:59-200 */
+  DH_PENALTY_STAT = 4,           /* penaltyStat                penalty.cpp:50-111  */
+  DH_HYBRID_BATCH_STAT = 5,      /* hybridBatchStat            hybrid.cpp:489-627  */
+  DH_HYBRID_TRIANGLE_STAT = 6,   /* hybridTriangleStat         hybrid.cpp:202-336  */
+  DH_GJK = 7                     /* detection::gjk             gjk.cpp:25-100      */
+};
+
+/* Unique triangle-pair kernels (the isolated narrow-phase workload). */
+enum {
+  DH_PAIR_BF64 = 0,        /* bfSolver, FP64, bit-exact                            */
+  DH_PAIR_PENALTY64 = 1,   /* penaltySolver (4 fixed Newton steps), FP64, bit-exact */
+  DH_PAIR_HYBRID64 = 2,    /* penalty, bfSolver where the pair's `failed` is set   */
+  DH_PAIR_MIXED = 3,       /* FP32 feature screen + FP64 exact finalisation        */
+  DH_PAIR_LEARNED = 4,     /* MLP warm start / feature guess + FP64 certificate    */
+  DH_PAIR_PENALTY_ADAPT64 = 5 /* penaltySolver(..., int& its), <=16 its, FP64       */
+};
+
+/* flags[] bits written by dh_tri_pairs */
+enum {
+  DH_FLAG_FAILED = 1,      /* penalty `failed` bit of the last Newton step         */
+  DH_FLAG_RECHECK = 2,     /* FP64 path taken (screen could not certify)           */
+  DH_FLAG_EDGE = 4,        /* edge-edge feature won (ptmin > ssmin)                */
+  DH_FLAG_CERTIFIED = 8    /* learned guess certified without the full BF          */
+};
+
+/* ------------------------------------------------------------ contact record */
+/* Mirror of delta::contact::contactpoint (contactpoint.h:42-103) plus the
+ * triangle indices and the batched-pair index, so that contact sets compare
+ * as sets of (master, slave, triA, triB). */
+typedef struct dh_contact {
+  double x[3];        /* (P+Q)/2                                           */
+  double normal[3];   /* (P-Q)/|P-Q|                                        */
+  double depth;       /* eps_total - |P-Q|                                  */
+  double P[3];        /* closest point on the triangle of particle A        */
+  double Q[3];        /* closest point on the triangle of particle B        */
+  double eps_total;   /* epsA + epsB                                        */
+  int32_t master;     /* id of particle A                                   */
+  int32_t slave;      /* id of particle B                                   */
+  int32_t penetration;/* always 1 for the 11-argument constructor           */
+  int32_t friction;   /* frictionA && frictionB                             */
+  int32_t triA;       /* triangle index within A                            */
+  int32_t triB;       /* triangle index within B                            */
+  int32_t pair;       /* index into the particle-pair list of the call      */
+  int32_t pad_;
+} dh_contact;
+
+/* Statistics the reference keeps in static globals (penalty.cpp:33-48,
+ * hybrid.cpp:27-57).  The caller owns and zeroes it; calls accumulate. */
+typedef struct dh_stats {
+  int64_t newton_histogram[17]; /* penaltyStat: _numberOfNewtonIterations     */
+  int64_t penalty_fails;        /* _numberOfPenaltyFails                      */
+  int64_t batch_fails;          /* _numberOfBatchFails                        */
+  int64_t batch_size;           /* _batchSize                                 */
+  double batch_error;           /* _batchError                                */
+  int64_t triangle_checks;      /* nTriA*nTriB per particle pair (Engine.cpp:451) */
+  int64_t pairs_evaluated;      /* triangle-pair evaluations actually run     */
+  int64_t fallback_rows;        /* rows recomputed with bfSolver              */
+} dh_stats;
+
+/* ------------------------------------------------------------ housekeeping */
+int dh_init(void);                          /* selects device 0, idempotent  */
+const char* dh_last_error(void);
+int dh_device_count(int* n);
+int dh_malloc(void** dev_ptr, size_t bytes);
+int dh_free(void* dev_ptr);
+int dh_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int dh_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int dh_memset(void* dev_ptr, int value, size_t bytes);
+int dh_synchronize(void);
+/* Timing on the library stream (or a caller stream when `stream` != NULL). */
+int dh_event_create(void** ev);
+int dh_event_destroy(void* ev);
+int dh_event_record(void* ev, void* stream);
+int dh_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ------------------------------------------------ particle-pair detection */
+/* Host-pointer form of delta::contact::detection::{bf, penalty, hybrid*,
+ * penaltyStat, hybrid*Stat} (bf.h:65-81, penalty.h:62-115, hybrid.h:57-170).
+ * nTriA/nTriB are triangle counts.  Contacts are written to `out` (host) in
+ * the reference's emission order (row iA outer, iB inner); `*n_out` is the
+ * number of contacts.  If more than `cap` would be written, DH_ERR_CAPACITY
+ * is returned and *n_out holds the required capacity.  `stats` may be NULL. */
+int dh_detect_pair(int method,
+                   const double* xA, const double* yA, const double* zA,
+                   int nTriA, double epsA, int frictionA, int idA,
+                   const double* xB, const double* yB, const double* zB,
+                   int nTriB, double epsB, int frictionB, int idB,
+                   dh_contact* out, int cap, int* n_out, dh_stats* stats);
+
+/* Device-resident batch over particle pairs.  Geometry of all particles is
+ * concatenated: triangles of particle p are [tri_offset[p], tri_offset[p+1]).
+ * pairs[2*k], pairs[2*k+1] are particle indices (A, B).  Contacts of pair k
+ * are written contiguously in reference order, pairs in ascending k, to
+ * out_dev; *n_out (host) receives the count.  pair_count_dev (may be NULL)
+ * receives the per-pair contact count (device, npairs ints). */
+typedef struct dh_geometry {
+  const double* x;          /* device, 3*ntri_total */
+  const double* y;
+  const double* z;
+  const int* tri_offset;    /* device, nparticles+1 */
+  const double* eps;        /* device, nparticles   */
+  const int* friction;      /* device, nparticles   */
+  const int* id;            /* device, nparticles   */
+  int nparticles;
+  int ntri_total;
+} dh_geometry;
+
+int dh_detect_pairs(int method, const dh_geometry* geom,
+                    const int* pairs_dev, int npairs,
+                    dh_contact* out_dev, int cap, int* n_out,
+                    int* pair_count_dev, dh_stats* stats, void* stream);
+
+/* -------------------------------------------------- unique triangle pairs */
+/* soa_dev: 18 device arrays of n doubles, packed as soa_dev[k*n + i] with
+ * k = 0..2 x of A's vertices 0..2, 3..5 y of A, 6..8 z of A, 9..17 the same
+ * for B (the bfSolver argument layout, bf.h:95-108).  Outputs (device):
+ * P[3*n] and Q[3*n] (component-major: P[c*n+i]), d[n] = |Q-P| as bf.cpp:117,
+ * flags[n] (may be NULL).  eps_total is the contact threshold epsA+epsB used by
+ * the penalty tolerance (eps_total/16, penalty.cpp:198) and by the mixed path's
+ * recheck band.  soa32_dev (may be NULL) is the same data rounded to binary32,
+ * read by DH_PAIR_MIXED / DH_PAIR_LEARNED for the screening pass. */
+int dh_tri_pairs(int method, const double* soa_dev, const float* soa32_dev,
+                 int64_t n, double eps_total,
+                 double* P_dev, double* Q_dev, double* d_dev, uint8_t* flags_dev,
+                 void* stream);
+
+/* Seeded synthetic unique-pair workload (no counterpart in the reference; the
+ * isolated narrow-phase benchmark input).  A: vertices uniform in [0,box)^3.
+ * Separation s log-uniform in [sep_min, sep_max].  frac_parallel of the pairs
+ * put an edge of B at angle < max_angle to an edge of A, at distance s, with
+ * those edges as the closest features; frac_intersect are pushed through A;
+ * the rest translate a random B along a random axis until the true distance
+ * is s (bisection with bisection_steps steps).  Writes soa_dev (18*n, the
+ * dh_tri_pairs layout), target_dev[n] = s and kind_dev[n] in {0 general,
+ * 1 near-parallel, 2 intersecting} (both may be NULL).  p may be NULL for
+ * the defaults (1e-3, 1e-7, 1e-2, 0.05, 0.01, 1e-4, 48). */
+typedef struct dh_gen_params {
+  double box, sep_min, sep_max, frac_parallel, frac_intersect, max_angle;
+  int bisection_steps;
+} dh_gen_params;
+int dh_gen_tri_pairs(uint64_t seed, int64_t n, const dh_gen_params* p, double* soa_dev,
+                     double* target_dev, uint8_t* kind_dev, void* stream);
+
+/* Element-wise binary64 -> binary32 rounding of n values (device). */
+int dh_convert_f32(const double* src_dev, float* dst_dev, int64_t n, void* stream);
+
+/* Number of pairs of the last dh_tri_pairs call per path (host). */
+typedef struct dh_pair_counters {
+  int64_t screened;     /* pairs that went through the FP32 screen          */
+  int64_t rechecked;    /* pairs finished by the full FP64 bfSolver         */
+  int64_t features_fp64;/* feature evaluations done in FP64 by finalisation */
+  int64_t certified;    /* pairs certified from the learned guess           */
+} dh_pair_counters;
+int dh_tri_pairs_counters(dh_pair_counters* out);
+
+/* ------------------------------------------------------- forces / dynamics */
+/* Rigid-body state, one record per particle (device, AoS of small arrays). */
+typedef struct dh_bodies {
+  double* pos;        /* 3n  spatial centre of mass      (_centreOfMass)      */
+  double* refpos;     /* 3n  referential centre of mass  (_refCentreOfMass)   */
+  double* linvel;     /* 3n  linear velocity                                  */
+  double* angvel;     /* 3n  spatial angular velocity    (_angularVelocity)   */
+  double* refangvel;  /* 3n  referential angular velocity                     */
+  double* rot;        /* 9n  orientation, column-major (NVMUL, algo.h:980)    */
+  double* inertia;    /* 9n  referential inertia                              */
+  double* inverse;    /* 9n  its inverse                                      */
+  double* mass;       /* n   (may be +inf for obstacles)                      */
+  int* obstacle;      /* n   1: never moved                                   */
+  int* material;      /* n                                                    */
+  int n;
+} dh_bodies;
+
+/* delta::contact::forces::getContactsForces (forces.cpp:291-396) for a batch
+ * of particle pairs whose contacts were produced by dh_detect_pairs.
+ * For pair k, contacts [pair_offset[k], pair_offset[k+1]) of contacts_dev are
+ * applied to A = pairs[2k] (normal as stored) and to B = pairs[2k+1] (normal
+ * negated, Collision.cpp:177-185).  Per-particle force/torque (device, 3n
+ * each) are overwritten with the sum over the particle's pairs, in ascending
+ * partner index, of the per-pair sums in contact order.  is_sphere selects
+ * springSphere.  Constants: spring k, damping ratio zeta, friction coefficient. */
+typedef struct dh_force_params {
+  double spring;      /* SPRING 1E4   (forces.cpp:11) */
+  double damper;      /* DAMPER 0.5   (forces.cpp:12) */
+  double friction;    /* FRICTION 0.5 (forces.cpp:14) */
+  int is_sphere;
+} dh_force_params;
+
+int dh_contacts_forces(const dh_bodies* bodies, const int* pairs_dev, int npairs,
+                       const dh_contact* contacts_dev, const int* pair_offset_dev,
+                       const dh_force_params* fp,
+                       double* force_dev, double* torque_dev, void* stream);
+
+/* Engine::deriveForces tail + Engine::updatePosition (Engine.cpp:511-565):
+ * v += dt F/m, updateAngular, v[gaxis] += dt*g, x += dt v, updateRotationMatrix,
+ * and every vertex replica x = c + R (X - C_ref).  Obstacles are skipped. */
+int dh_integrate(dh_bodies* bodies, const double* force_dev, const double* torque_dev,
+                 double dt, const double gravity[3],
+                 double* x_dev, double* y_dev, double* z_dev,
+                 const double* refx_dev, const double* refy_dev, const double* refz_dev,
+                 const int* tri_offset_dev, void* stream);
+
+/* Host-pointer single-particle forms of the dynamics free functions
+ * (dynamics.h:45-95), for drop-in callers. */
+int dh_update_angular(double refAngular[3], const double rotation[9],
+                      const double inertia[9], const double inverse[9],
+                      const double torque[3], double step);
+int dh_update_rotation_matrix(double angular[3], const double refAngular[3],
+                              double rotation[9], double step);
+
+/* ------------------------------------------------------------ broad phase */
+/* Uniform-grid candidate search over particle AABBs (inflated by eps).
+ * Writes candidate pairs (A<B) to pairs_dev (2*cap ints), sorted ascending;
+ * *npairs receives the count (DH_ERR_CAPACITY if more than cap). */
+int dh_broadphase(const dh_geometry* geom, int* pairs_dev, int cap, int* npairs,
+                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DELTA_HIP_H */

@@ -1,0 +1,993 @@
+/*
+ * oracle.c — CPU restatement of the reference path (test infrastructure only;
+ * see oracle.h for what may call it and how each function is pinned).
+ *
+ * Compile with -O2 -ffp-contract=off: the reference's gcc build (delta/Makefile
+ * :22-23,41) has no -march and hence no fused multiply-add, so every product
+ * below is rounded before it is added, in the order written.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_DOT(a, b) ((a)[0] * (b)[0] + (a)[1] * (b)[1] + (a)[2] * (b)[2])
+
+static void sub3(const double a[3], const double b[3], double r[3]) {
+  r[0] = a[0] - b[0];
+  r[1] = a[1] - b[1];
+  r[2] = a[2] - b[2];
+}
+
+/* ---------------------------------------------------------------------------
+ * Segment-segment part of bfSolver (bf.cpp:409-713).  The nine edge pairs are
+ * enumerated as (edge k of A) x (edge m of B), j = 3k+m.  Edge k of a triangle
+ * with vertices v0,v1,v2 runs from its origin v[(k+1)%3] towards v[k]: the
+ * reference's u = p1-p2 (origin p2), p2-p3 (origin p3), p3-p1 (origin p1),
+ * and likewise v = p4-p5 (origin p5), p5-p6, p6-p4.
+ * The clamped solve is the classic segment-segment closest-point routine with
+ * the reference's thresholds (D < 1e-30, |sN|,|tN| < 1e-30 -> 0).
+ * ------------------------------------------------------------------------- */
+static void segseg_params(double a, double b, double c, double d, double e,
+                          double* sc_out, double* tc_out) {
+  const double small_num = 1E-30;
+  double D = a * c - b * b;
+  double sD = D, tD = D, sN, tN;
+  if (D < small_num) {
+    sN = 0.0;
+    sD = 1.0;
+    tN = e;
+    tD = c;
+  } else {
+    sN = b * e - c * d;
+    tN = a * e - b * d;
+    if (sN < 0.0) {
+      sN = 0.0;
+      tN = e;
+      tD = c;
+    } else if (sN > sD) {
+      sN = sD;
+      tN = e + b;
+      tD = c;
+    }
+  }
+  if (tN < 0.0) {
+    tN = 0.0;
+    if (-d < 0.0) {
+      sN = 0.0;
+    } else if (-d > a) {
+      sN = sD;
+    } else {
+      sN = -d;
+      sD = a;
+    }
+  } else if (tN > tD) {
+    tN = tD;
+    if ((-d + b) < 0.0) {
+      sN = 0;
+    } else if ((-d + b) > a) {
+      sN = sD;
+    } else {
+      sN = (-d + b);
+      sD = a;
+    }
+  }
+  *sc_out = (fabs(sN) < small_num) ? 0.0 : sN / sD;
+  *tc_out = (fabs(tN) < small_num) ? 0.0 : tN / tD;
+}
+
+/* ---------------------------------------------------------------------------
+ * Point-triangle part (bf.cpp:875-1100): squared distance from a point to the
+ * triangle base + s*E0 + t*E1 with the seven-region case analysis, given
+ * a=E0.E0, b=E0.E1, c=E1.E1, d=E0.(base-pt), e=E1.(base-pt), f=|base-pt|^2.
+ * ------------------------------------------------------------------------- */
+static double pt_tri_sqr(double a, double b, double c, double d, double e, double f,
+                         double* s_out, double* t_out) {
+  double det = a * c - b * b;
+  double s = b * e - c * d;
+  double t = b * d - a * e;
+  double sqr = 0.0;
+#define QUAD_FORM (s * (a * s + b * t + 2 * d) + t * (b * s + c * t + 2 * e) + f)
+  if ((s + t) <= det) {
+    if (s < 0) {
+      if (t < 0) {
+        if (d < 0) {
+          t = 0;
+          if (-d >= a) {
+            s = 1;
+            sqr = a + 2 * d + f;
+          } else {
+            s = -d / a;
+            sqr = d * s + f;
+          }
+        } else {
+          s = 0;
+          if (e >= 0) {
+            t = 0;
+            sqr = f;
+          } else if (-e >= c) {
+            t = 1;
+            sqr = c + 2 * e + f;
+          } else {
+            t = -e / c;
+            sqr = e * t + f;
+          }
+        }
+      } else {
+        s = 0;
+        if (e >= 0) {
+          t = 0;
+          sqr = f;
+        } else if (-e >= c) {
+          t = 1;
+          sqr = c + 2 * e + f;
+        } else {
+          t = -e / c;
+          sqr = e * t + f;
+        }
+      }
+    } else if (t < 0) {
+      t = 0;
+      if (d >= 0) {
+        s = 0;
+        sqr = f;
+      } else if (-d >= a) {
+        s = 1;
+        sqr = a + 2 * d + f;
+      } else {
+        s = -d / a;
+        sqr = d * s + f;
+      }
+    } else {
+      double inv_det = 1 / det;
+      s = s * inv_det;
+      t = t * inv_det;
+      sqr = QUAD_FORM;
+    }
+  } else {
+    if (s < 0) {
+      double tmp0 = b + d;
+      double tmp1 = c + e;
+      if (tmp1 > tmp0) {
+        double numer = tmp1 - tmp0;
+        double denom = a - 2 * b + c;
+        if (numer >= denom) {
+          s = 1;
+          t = 0;
+          sqr = a + 2 * d + f;
+        } else {
+          s = numer / denom;
+          t = 1 - s;
+          sqr = QUAD_FORM;
+        }
+      } else {
+        s = 0;
+        if (tmp1 <= 0) {
+          t = 1;
+          sqr = c + 2 * e + f;
+        } else if (e >= 0) {
+          t = 0;
+          sqr = f;
+        } else {
+          t = -e / c;
+          sqr = e * t + f;
+        }
+      }
+    } else if (t < 0) {
+      double tmp0 = b + e;
+      double tmp1 = a + d;
+      if (tmp1 > tmp0) {
+        double numer = tmp1 - tmp0;
+        double denom = a - 2 * b + c;
+        if (numer >= denom) {
+          t = 1;
+          s = 0;
+          sqr = c + 2 * e + f;
+        } else {
+          t = numer / denom;
+          s = 1 - t;
+          sqr = QUAD_FORM;
+        }
+      } else {
+        t = 0;
+        if (tmp1 <= 0) {
+          s = 1;
+          sqr = a + 2 * d + f;
+        } else if (d >= 0) {
+          s = 0;
+          sqr = f;
+        } else {
+          s = -d / a;
+          sqr = d * s + f;
+        }
+      }
+    } else {
+      double numer = c + e - b - d;
+      if (numer <= 0) {
+        s = 0;
+        t = 1;
+        sqr = c + 2 * e + f;
+      } else {
+        double denom = a - 2 * b + c;
+        if (numer >= denom) {
+          s = 1;
+          t = 0;
+          sqr = a + 2 * d + f;
+        } else {
+          s = numer / denom;
+          t = 1 - s;
+          sqr = QUAD_FORM;
+        }
+      }
+    }
+  }
+#undef QUAD_FORM
+  if (sqr <= 0.0) sqr = 0.0;
+  *s_out = s;
+  *t_out = t;
+  return sqr;
+}
+
+int or_bf_solver(const double* xA, const double* yA, const double* zA,
+                 const double* xB, const double* yB, const double* zB,
+                 double P[3], double Q[3]) {
+  double va[3][3], vb[3][3];
+  for (int k = 0; k < 3; ++k) {
+    va[k][0] = xA[k]; va[k][1] = yA[k]; va[k][2] = zA[k];
+    vb[k][0] = xB[k]; vb[k][1] = yB[k]; vb[k][2] = zB[k];
+  }
+  /* The plane/segment pre-pass at bf.cpp:270-407 writes the outputs and is
+   * always overwritten below; it has no observable effect and is omitted. */
+
+  /* edge-edge */
+  double ssmin = 1E+30, ttc = 0, ssc = 0;
+  int ssid = 0;
+  for (int k = 0; k < 3; ++k) {
+    double u[3];
+    sub3(va[k], va[(k + 1) % 3], u);
+    const double a = OR_DOT(u, u);
+    for (int m = 0; m < 3; ++m) {
+      double v[3], w[3];
+      sub3(vb[m], vb[(m + 1) % 3], v);
+      sub3(va[(k + 1) % 3], vb[(m + 1) % 3], w);
+      double b = OR_DOT(u, v), c = OR_DOT(v, v), d = OR_DOT(u, w), e = OR_DOT(v, w);
+      double sc, tc;
+      segseg_params(a, b, c, d, e, &sc, &tc);
+      double dP[3];
+      for (int q = 0; q < 3; ++q) dP[q] = w[q] + (sc * u[q]) - (tc * v[q]);
+      double dist = sqrt(dP[0] * dP[0] + dP[1] * dP[1] + dP[2] * dP[2]);
+      if (dist < ssmin) {
+        ssmin = dist;
+        ttc = tc;
+        ssc = sc;
+        ssid = 3 * k + m;
+      }
+    }
+  }
+  {
+    int k = ssid / 3, m = ssid % 3;
+    const double* oa = va[(k + 1) % 3];
+    const double* ob = vb[(m + 1) % 3];
+    double u[3], v[3];
+    sub3(va[k], oa, u);
+    sub3(vb[m], ob, v);
+    for (int q = 0; q < 3; ++q) {
+      P[q] = oa[q] + (u[q] * ssc);
+      Q[q] = ob[q] + (v[q] * ttc);
+    }
+  }
+
+  /* vertex-face: j = 0..2 vertex j of B against face A; j = 3..5 vertex j-3
+   * of A against face B. */
+  double ptmin = 1E30, ss = 0.0, tt = 0.0;
+  int id = 0;
+  for (int j = 0; j < 6; ++j) {
+    const double(*face)[3] = (j < 3) ? va : vb;
+    const double* pt = (j < 3) ? vb[j] : va[j - 3];
+    double e0[3], e1[3], dd[3];
+    sub3(face[1], face[0], e0);
+    sub3(face[2], face[0], e1);
+    sub3(face[0], pt, dd);
+    double s, t;
+    double sqr = pt_tri_sqr(OR_DOT(e0, e0), OR_DOT(e0, e1), OR_DOT(e1, e1), OR_DOT(e0, dd),
+                            OR_DOT(e1, dd), OR_DOT(dd, dd), &s, &t);
+    double dist = sqrt(sqr);
+    if (dist < ptmin) {
+      ptmin = dist;
+      ss = s;
+      tt = t;
+      id = j;
+    }
+  }
+  if (ptmin > ssmin) return 1;
+
+  const double(*face)[3] = (id < 3) ? va : vb;
+  double onface[3];
+  for (int q = 0; q < 3; ++q)
+    onface[q] = face[0][q] + ((face[2][q] - face[0][q]) * tt) + ((face[1][q] - face[0][q]) * ss);
+  if (id < 3) {
+    for (int q = 0; q < 3; ++q) { P[q] = onface[q]; Q[q] = vb[id][q]; }
+  } else {
+    for (int q = 0; q < 3; ++q) { P[q] = va[id - 3][q]; Q[q] = onface[q]; }
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Penalty solver (penalty.cpp:278-431 and 434-581).  Unknowns x = (alpha,
+ * beta, gamma, delta) of P = A0 + alpha BA + beta CA, Q = D0 + gamma ED +
+ * delta FD; the 4x4 regularised Newton system is solved by the reference's
+ * unrolled elimination, in its operation order.
+ * ------------------------------------------------------------------------- */
+typedef struct pen_ctx {
+  double A0[3], B0[3], BA[3], CA[3], ED[3], FD[3];
+  double h[16];
+  double r;
+  double reg0; /* delta before the first step */
+} pen_ctx;
+
+static void pen_setup(const double* xA, const double* yA, const double* zA,
+                      const double* xB, const double* yB, const double* zB, pen_ctx* c) {
+  c->A0[0] = xA[0]; c->A0[1] = yA[0]; c->A0[2] = zA[0];
+  c->B0[0] = xB[0]; c->B0[1] = yB[0]; c->B0[2] = zB[0];
+  c->BA[0] = xA[1] - xA[0]; c->BA[1] = yA[1] - yA[0]; c->BA[2] = zA[1] - zA[0];
+  c->CA[0] = xA[2] - xA[0]; c->CA[1] = yA[2] - yA[0]; c->CA[2] = zA[2] - zA[0];
+  c->ED[0] = xB[1] - xB[0]; c->ED[1] = yB[1] - yB[0]; c->ED[2] = zB[1] - zB[0];
+  c->FD[0] = xB[2] - xB[0]; c->FD[1] = yB[2] - yB[0]; c->FD[2] = zB[2] - zB[0];
+  double* h = c->h;
+  h[0] = 2. * OR_DOT(c->BA, c->BA);
+  h[1] = 2. * OR_DOT(c->CA, c->BA);
+  h[2] = -2. * OR_DOT(c->ED, c->BA);
+  h[3] = -2. * OR_DOT(c->FD, c->BA);
+  h[4] = h[1];
+  h[5] = 2. * OR_DOT(c->CA, c->CA);
+  h[6] = -2. * OR_DOT(c->ED, c->CA);
+  h[7] = -2. * OR_DOT(c->FD, c->CA);
+  h[8] = h[2];
+  h[9] = h[6];
+  h[10] = 2. * OR_DOT(c->ED, c->ED);
+  h[11] = 2. * OR_DOT(c->FD, c->ED);
+  h[12] = h[3];
+  h[13] = h[7];
+  h[14] = h[11];
+  h[15] = 2. * OR_DOT(c->FD, c->FD);
+  double trace = h[0] + h[5] + h[10] + h[15];
+  c->reg0 = trace * 1E-2;
+  c->r = sqrt(0.0125 * trace) * 1E5;
+}
+
+/* One Newton step: returns dx for the current x and regularisation `reg`. */
+static void pen_step(const pen_ctx* c, const double x[4], double reg, double dx[4]) {
+  const double* h = c->h;
+  const double r = c->r;
+  double dh[8], mx[6], sub[3], a[16], b[4];
+  int c0 = (-x[0] <= 0), c1 = (-x[1] <= 0), c2 = (x[0] + x[1] - 1 <= 0);
+  int c3 = (-x[2] <= 0), c4 = (-x[3] <= 0), c5 = (x[2] + x[3] - 1 <= 0);
+  dh[0] = c0 ? 0.0 : -1;
+  mx[0] = c0 ? 0.0 : -x[0];
+  dh[2] = c1 ? 0.0 : -1;
+  mx[1] = c1 ? 0.0 : -x[1];
+  dh[1] = dh[3] = c2 ? 0.0 : 1;
+  mx[2] = c2 ? 0.0 : x[0] + x[1] - 1;
+  dh[4] = c3 ? 0.0 : -1;
+  mx[3] = c3 ? 0.0 : -x[2];
+  dh[6] = c4 ? 0.0 : -1;
+  mx[4] = c4 ? 0.0 : -x[3];
+  dh[5] = dh[7] = c5 ? 0.0 : 1;
+  mx[5] = c5 ? 0.0 : x[2] + x[3] - 1;
+  for (int q = 0; q < 3; ++q)
+    sub[q] = (c->A0[q] + (c->BA[q] * x[0]) + (c->CA[q] * x[1])) -
+             (c->B0[q] + (c->ED[q] * x[2]) + (c->FD[q] * x[3]));
+  double tmp1, tmp2, tmp3, tmp4, tmp5, tmp6;
+  b[0] = 2 * OR_DOT(sub, c->BA) + r * (dh[0] * mx[0] + dh[1] * mx[2]);
+  a[0] = h[0] + r * (dh[0] * dh[0] + dh[1] * dh[1]) + reg;
+  a[4] = h[4] + r * (dh[3] * dh[1]);
+  tmp1 = (h[1] + r * (dh[1] * dh[3])) / a[0];
+  a[13] = h[13] - h[12] * tmp1;
+  a[9] = h[9] - h[8] * tmp1;
+  a[5] = (h[5] + r * (dh[2] * dh[2] + dh[3] * dh[3]) + reg) - a[4] * tmp1;
+  b[1] = (2 * OR_DOT(sub, c->CA) + r * (dh[2] * mx[1] + dh[3] * mx[2])) - b[0] * tmp1;
+  tmp2 = h[2] / a[0];
+  tmp3 = h[3] / a[0];
+  tmp4 = ((h[6]) - a[4] * tmp2) / a[5];
+  a[14] = ((h[14] + r * (dh[7] * dh[5])) - h[12] * tmp2) - a[13] * tmp4;
+  a[10] = ((h[10] + r * (dh[4] * dh[4] + dh[5] * dh[5]) + reg) - h[8] * tmp2) - a[9] * tmp4;
+  b[2] = ((-2 * OR_DOT(sub, c->ED) + r * (dh[4] * mx[3] + dh[5] * mx[5])) - b[0] * tmp2) -
+         b[1] * tmp4;
+  tmp5 = (h[7] - a[4] * tmp3) / a[5];
+  tmp6 = (((h[11] + r * (dh[5] * dh[7])) - h[8] * tmp3) - a[9] * tmp5) / a[10];
+  dx[3] = ((((-2 * OR_DOT(sub, c->FD) + r * (dh[6] * mx[4] + dh[7] * mx[5])) - b[2] * tmp6) -
+            b[0] * tmp3) -
+           b[1] * tmp5) /
+          ((((h[15] + r * (dh[6] * dh[6] + dh[7] * dh[7]) + reg) - h[12] * tmp3) - a[13] * tmp5) -
+           a[14] * tmp6);
+  dx[2] = (b[2] - (a[14] * dx[3])) / a[10];
+  dx[1] = (b[1] - (a[9] * dx[2] + a[13] * dx[3])) / a[5];
+  dx[0] = (b[0] - (a[4] * dx[1] + h[8] * dx[2] + h[12] * dx[3])) / a[0];
+}
+
+static double dot4(const double a[4], const double b[4]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+
+static void pen_points(const pen_ctx* c, const double x[4], double P[3], double Q[3]) {
+  for (int q = 0; q < 3; ++q) {
+    P[q] = c->A0[q] + (c->BA[q] * x[0]) + (c->CA[q] * x[1]);
+    Q[q] = c->B0[q] + (c->ED[q] * x[2]) + (c->FD[q] * x[3]);
+  }
+}
+
+void or_penalty_solver_from(const double* xA, const double* yA, const double* zA,
+                            const double* xB, const double* yB, const double* zB,
+                            const double x0[4], int iterations,
+                            double P[3], double Q[3], double max_err, int* failed,
+                            double x_out[4]) {
+  pen_ctx c;
+  pen_setup(xA, yA, zA, xB, yB, zB, &c);
+  double x[4] = {x0[0], x0[1], x0[2], x0[3]};
+  double reg = c.reg0;
+  int f = 0;
+  for (int i = 0; i < iterations; ++i) {
+    double dx[4];
+    pen_step(&c, x, reg, dx);
+    reg = 1E4 * reg;
+    f = (dot4(dx, dx) / dot4(x, x) > max_err * max_err) ? 1 : 0;
+    for (int q = 0; q < 4; ++q) x[q] -= dx[q];
+  }
+  if (failed) *failed = f;
+  if (x_out) memcpy(x_out, x, sizeof(x));
+  pen_points(&c, x, P, Q);
+}
+
+void or_penalty_solver(const double* xA, const double* yA, const double* zA,
+                       const double* xB, const double* yB, const double* zB,
+                       double P[3], double Q[3], double max_err, int* failed) {
+  static const double start[4] = {0.33, 0.33, 0.33, 0.33};
+  or_penalty_solver_from(xA, yA, zA, xB, yB, zB, start, 4, P, Q, max_err, failed, NULL);
+}
+
+void or_penalty_solver_adaptive(const double* xA, const double* yA, const double* zA,
+                                const double* xB, const double* yB, const double* zB,
+                                double P[3], double Q[3], double max_err, int* its) {
+  pen_ctx c;
+  pen_setup(xA, yA, zA, xB, yB, zB, &c);
+  double x[4] = {0.33, 0.33, 0.33, 0.33};
+  double reg = c.reg0;
+  for (int i = 0; i < 16; ++i) {
+    double dx[4];
+    reg = i < 3 ? reg : 1E5 * reg;
+    pen_step(&c, x, reg, dx);
+    double err = dot4(dx, dx) / dot4(x, x);
+    if (err < max_err * max_err) {
+      *its = i;
+      break;
+    }
+    for (int q = 0; q < 4; ++q) x[q] = x[q] - dx[q];
+  }
+  pen_points(&c, x, P, Q);
+}
+
+/* ------------------------------------------------------------------ contacts */
+static double dist_pq(const double P[3], const double Q[3]) {
+  return sqrt(((Q[0] - P[0]) * (Q[0] - P[0])) + ((Q[1] - P[1]) * (Q[1] - P[1])) +
+              ((Q[2] - P[2]) * (Q[2] - P[2])));
+}
+
+void or_contact_make(double xPA, double yPA, double zPA, double epsA, int idA,
+                     double xQB, double yQB, double zQB, double epsB, int idB,
+                     int friction, dh_contact* c) {
+  memset(c, 0, sizeof(*c));
+  c->x[0] = (xPA + xQB) / 2.0;
+  c->x[1] = (yPA + yQB) / 2.0;
+  c->x[2] = (zPA + zQB) / 2.0;
+  c->P[0] = xPA; c->P[1] = yPA; c->P[2] = zPA;
+  c->Q[0] = xQB; c->Q[1] = yQB; c->Q[2] = zQB;
+  c->eps_total = epsA + epsB;
+  double dist = dist_pq(c->P, c->Q);
+  c->depth = c->eps_total - dist;
+  c->normal[0] = (xPA - xQB) / dist;
+  c->normal[1] = (yPA - yQB) / dist;
+  c->normal[2] = (zPA - zQB) / dist;
+  c->friction = friction ? 1 : 0;
+  c->master = idA;
+  c->slave = idB;
+  c->penetration = 1;
+}
+
+/* One row of triangle pairs (triangle iA of A against all of B). */
+typedef struct row_buf {
+  double* P; /* 3*nB */
+  double* Q;
+  double* d;
+  int* failed;
+} row_buf;
+
+static void row_bf(const double* xA, const double* yA, const double* zA, int iA,
+                   const double* xB, const double* yB, const double* zB, int nB, row_buf* rb) {
+  for (int iB = 0; iB < nB; ++iB) {
+    double* P = rb->P + 3 * iB;
+    double* Q = rb->Q + 3 * iB;
+    or_bf_solver(xA + 3 * iA, yA + 3 * iA, zA + 3 * iA, xB + 3 * iB, yB + 3 * iB, zB + 3 * iB, P, Q);
+    rb->d[iB] = dist_pq(P, Q);
+  }
+}
+
+static void row_penalty(const double* xA, const double* yA, const double* zA, int iA,
+                        const double* xB, const double* yB, const double* zB, int nB,
+                        double max_err, row_buf* rb) {
+  for (int iB = 0; iB < nB; ++iB) {
+    double* P = rb->P + 3 * iB;
+    double* Q = rb->Q + 3 * iB;
+    int f = 0;
+    or_penalty_solver(xA + 3 * iA, yA + 3 * iA, zA + 3 * iA, xB + 3 * iB, yB + 3 * iB,
+                      zB + 3 * iB, P, Q, max_err, &f);
+    rb->failed[iB] = f;
+    rb->d[iB] = dist_pq(P, Q);
+  }
+}
+
+typedef struct emit_ctx {
+  dh_contact* out;
+  int cap;
+  int n;
+  double epsA, epsB;
+  int idA, idB, fric;
+} emit_ctx;
+
+static void emit(emit_ctx* e, const double P[3], const double Q[3], int iA, int iB) {
+  if (e->n < e->cap && e->out) {
+    dh_contact* c = e->out + e->n;
+    or_contact_make(P[0], P[1], P[2], e->epsA, e->idA, Q[0], Q[1], Q[2], e->epsB, e->idB,
+                    e->fric, c);
+    c->triA = iA;
+    c->triB = iB;
+    c->pair = 0;
+  }
+  e->n++;
+}
+
+int or_detect(int method,
+              const double* xA, const double* yA, const double* zA, int nA,
+              double epsA, int fricA, int idA,
+              const double* xB, const double* yB, const double* zB, int nB,
+              double epsB, int fricB, int idB,
+              dh_contact* out, int cap, int* n_out, dh_stats* stats) {
+  if (nA < 0 || nB < 0 || !n_out) return DH_ERR_ARG;
+  if (method == DH_HYBRID_TRIANGLE && nB > nA) return DH_ERR_REF_UB;
+  dh_stats dummy;
+  if (!stats) {
+    memset(&dummy, 0, sizeof(dummy));
+    stats = &dummy;
+  }
+  emit_ctx e = {out, cap, 0, epsA, epsB, idA, idB, (fricA && fricB) ? 1 : 0};
+  const double margin = epsA + epsB;
+  row_buf rb;
+  rb.P = (double*)malloc(sizeof(double) * 3 * (nB + 1));
+  rb.Q = (double*)malloc(sizeof(double) * 3 * (nB + 1));
+  rb.d = (double*)malloc(sizeof(double) * (nB + 1));
+  rb.failed = (int*)calloc(nB + 1, sizeof(int));
+  stats->triangle_checks += (int64_t)nA * nB;
+
+  switch (method) {
+    case DH_BF:
+      for (int iA = 0; iA < nA; ++iA) {
+        row_bf(xA, yA, zA, iA, xB, yB, zB, nB, &rb);
+        stats->pairs_evaluated += nB;
+        for (int iB = 0; iB < nB; ++iB)
+          if (rb.d[iB] <= margin) emit(&e, rb.P + 3 * iB, rb.Q + 3 * iB, iA, iB);
+      }
+      break;
+    case DH_PENALTY: {
+      const double max_err = (epsA + epsB) / 16.0;
+      for (int iA = 0; iA < nA; ++iA) {
+        row_penalty(xA, yA, zA, iA, xB, yB, zB, nB, max_err, &rb);
+        stats->pairs_evaluated += nB;
+        for (int iB = 0; iB < nB; ++iB)
+          if (rb.d[iB] < margin) emit(&e, rb.P + 3 * iB, rb.Q + 3 * iB, iA, iB);
+      }
+    } break;
+    case DH_PENALTY_STAT: {
+      /* penalty.cpp:50-111: adaptive solver, iteration histogram, and an
+       * emission loop that only visits iB < nB in point units (no *3). */
+      for (int iA = 0; iA < nA; ++iA) {
+        const double max_err = (epsA + epsB) / 16.0;
+        for (int iB = 0; iB < nB; ++iB) {
+          int its = 0;
+          or_penalty_solver_adaptive(xA + 3 * iA, yA + 3 * iA, zA + 3 * iA, xB + 3 * iB,
+                                     yB + 3 * iB, zB + 3 * iB, rb.P + 3 * iB, rb.Q + 3 * iB,
+                                     max_err, &its);
+          stats->newton_histogram[its]++;
+          rb.d[iB] = dist_pq(rb.P + 3 * iB, rb.Q + 3 * iB);
+        }
+        stats->pairs_evaluated += nB;
+        for (int iBp = 0; iBp < nB; iBp += 3) {
+          int iB = iBp / 3;
+          if (rb.d[iB] <= margin) emit(&e, rb.P + 3 * iB, rb.Q + 3 * iB, iA, iB);
+        }
+      }
+    } break;
+    case DH_HYBRID_TRIANGLE: {
+      /* hybrid.cpp:59-200: rows run to nB (the reference's loop bound). */
+      const double max_err = (epsA + epsB) / 16;
+      for (int iA = 0; iA < nB; ++iA) {
+        row_penalty(xA, yA, zA, iA, xB, yB, zB, nB, max_err, &rb);
+        stats->pairs_evaluated += nB;
+        int any = 0;
+        for (int iB = 0; iB < nB; ++iB) any |= rb.failed[iB];
+        if (any) {
+          row_bf(xA, yA, zA, iA, xB, yB, zB, nB, &rb);
+          stats->pairs_evaluated += nB;
+          stats->fallback_rows++;
+        }
+        for (int iB = 0; iB < nB; ++iB)
+          if (rb.d[iB] < margin) emit(&e, rb.P + 3 * iB, rb.Q + 3 * iB, iA, iB);
+      }
+    } break;
+    case DH_HYBRID_BATCH: {
+      const double max_err = (epsA + epsB) / 16;
+      for (int iA = 0; iA < nA; ++iA) {
+        row_penalty(xA, yA, zA, iA, xB, yB, zB, nB, max_err, &rb);
+        stats->pairs_evaluated += nB;
+        int counter = 0;
+        for (int iB = 0; iB < nB; ++iB)
+          if (rb.failed[iB]) counter++;
+        double local_err = counter * max_err;
+        if (local_err > ((double)max_err * (double)(3 * nB)) / 32.0) {
+          row_bf(xA, yA, zA, iA, xB, yB, zB, nB, &rb);
+          stats->pairs_evaluated += nB;
+          stats->fallback_rows++;
+        }
+        for (int iB = 0; iB < nB; ++iB)
+          if (rb.d[iB] < margin) emit(&e, rb.P + 3 * iB, rb.Q + 3 * iB, iA, iB);
+      }
+    } break;
+    case DH_HYBRID_BATCH_STAT:
+    case DH_HYBRID_TRIANGLE_STAT: {
+      /* hybrid.cpp:202-336 and 489-627 (identical arithmetic): fallback when
+       * count*maxErr > maxErr*nB/32; only the row minimum is emitted. */
+      for (int iA = 0; iA < nA; ++iA) {
+        const double max_err = (epsA + epsB) / 16;
+        stats->batch_size += nB;
+        stats->batch_error += max_err;
+        row_penalty(xA, yA, zA, iA, xB, yB, zB, nB, max_err, &rb);
+        stats->pairs_evaluated += nB;
+        int counter = 0;
+        for (int iB = 0; iB < nB; ++iB)
+          if (rb.failed[iB]) counter++;
+        double local_err = counter * max_err;
+        stats->penalty_fails += counter;
+        if (local_err > ((double)max_err * (double)nB) / 32) {
+          stats->batch_fails += 1;
+          row_bf(xA, yA, zA, iA, xB, yB, zB, nB, &rb);
+          stats->pairs_evaluated += nB;
+          stats->fallback_rows++;
+        }
+        double min_d = 1E99;
+        for (int iB = 0; iB < nB; ++iB) min_d = (min_d < rb.d[iB]) ? min_d : rb.d[iB];
+        for (int iB = 0; iB < nB; ++iB)
+          if (rb.d[iB] <= min_d && rb.d[iB] < margin)
+            emit(&e, rb.P + 3 * iB, rb.Q + 3 * iB, iA, iB);
+      }
+    } break;
+    default:
+      free(rb.P); free(rb.Q); free(rb.d); free(rb.failed);
+      return DH_ERR_ARG;
+  }
+  free(rb.P); free(rb.Q); free(rb.d); free(rb.failed);
+  *n_out = e.n;
+  return (e.n > cap) ? DH_ERR_CAPACITY : DH_OK;
+}
+
+int or_tri_pairs(int method, const double* soa, int64_t n, double eps_total,
+                 double* P, double* Q, double* d, uint8_t* flags) {
+  const double max_err = eps_total / 16.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double xa[3], ya[3], za[3], xb[3], yb[3], zb[3], p[3], q[3];
+    for (int k = 0; k < 3; ++k) {
+      xa[k] = soa[(0 + k) * n + i];
+      ya[k] = soa[(3 + k) * n + i];
+      za[k] = soa[(6 + k) * n + i];
+      xb[k] = soa[(9 + k) * n + i];
+      yb[k] = soa[(12 + k) * n + i];
+      zb[k] = soa[(15 + k) * n + i];
+    }
+    uint8_t fl = 0;
+    switch (method) {
+      case DH_PAIR_BF64:
+        if (or_bf_solver(xa, ya, za, xb, yb, zb, p, q)) fl |= DH_FLAG_EDGE;
+        break;
+      case DH_PAIR_PENALTY64: {
+        int f = 0;
+        or_penalty_solver(xa, ya, za, xb, yb, zb, p, q, max_err, &f);
+        if (f) fl |= DH_FLAG_FAILED;
+      } break;
+      case DH_PAIR_HYBRID64: {
+        int f = 0;
+        or_penalty_solver(xa, ya, za, xb, yb, zb, p, q, max_err, &f);
+        if (f) {
+          fl |= DH_FLAG_FAILED | DH_FLAG_RECHECK;
+          if (or_bf_solver(xa, ya, za, xb, yb, zb, p, q)) fl |= DH_FLAG_EDGE;
+        }
+      } break;
+      case DH_PAIR_PENALTY_ADAPT64: {
+        int its = 0;
+        or_penalty_solver_adaptive(xa, ya, za, xb, yb, zb, p, q, max_err, &its);
+      } break;
+      default:
+        return DH_ERR_ARG;
+    }
+    for (int c = 0; c < 3; ++c) {
+      P[c * n + i] = p[c];
+      Q[c * n + i] = q[c];
+    }
+    d[i] = dist_pq(p, q);
+    if (flags) flags[i] = fl;
+  }
+  return DH_OK;
+}
+
+/* -------------------------------------------------------------------- forces */
+void or_contacts_forces(const dh_contact* cs, int nc, int negate_normal,
+                        const double posA[3], const double angA[3], const double linA[3],
+                        double massA,
+                        const double posB[3], const double angB[3], const double linB[3],
+                        double massB,
+                        const dh_force_params* fp, double force[3], double torque[3]) {
+  for (int k = 0; k < nc; ++k) {
+    const dh_contact* c = cs + k;
+    double n[3];
+    for (int q = 0; q < 3; ++q) n[q] = negate_normal ? -1.0 * c->normal[q] : c->normal[q];
+    double z[3], vi[3], vj[3], vij[3];
+    sub3(c->x, posA, z);
+    vi[0] = angA[1] * z[2] - angA[2] * z[1] + linA[0];
+    vi[1] = angA[2] * z[0] - angA[0] * z[2] + linA[1];
+    vi[2] = angA[0] * z[1] - angA[1] * z[0] + linA[2];
+    sub3(c->x, posB, z);
+    vj[0] = angB[1] * z[2] - angB[2] * z[1] + linB[0];
+    vj[1] = angB[2] * z[0] - angB[0] * z[2] + linB[1];
+    vj[2] = angB[0] * z[1] - angB[1] * z[0] + linB[2];
+    sub3(vj, vi, vij);
+
+    /* spring (forces.cpp:208-220) / springSphere (227-249) */
+    double ma = 1.0 / ((1.0 / massA) + (1.0 / massB));
+    double vel = (vij[0] * n[0]) + (vij[1] * n[1]) + (vij[2] * n[2]);
+    double damp = fp->damper * 2.0 * sqrt(ma * fp->spring) * vel;
+    double fn = fp->spring * c->depth + damp;
+    double f[3] = {fn * n[0], fn * n[1], fn * n[2]};
+    double fr[3] = {0.0, 0.0, 0.0};
+    if (c->friction) {
+      /* friction (forces.cpp:255-289): tangential part of v_i, not v_ij. */
+      double vt[3];
+      for (int q = 0; q < 3; ++q)
+        vt[q] = vi[q] - n[q] * ((vi[0] * n[0]) + (vi[1] * n[1]) + (vi[2] * n[2]));
+      for (int q = 0; q < 3; ++q) fr[q] = -vt[q] * fp->friction * fn;
+    }
+    for (int q = 0; q < 3; ++q) force[q] += f[q] + fr[q];
+    double arm[3];
+    sub3(c->x, posA, arm);
+    torque[0] += arm[1] * (f[2]) - arm[2] * (f[1]);
+    torque[1] += arm[2] * (f[0]) - arm[0] * (f[2]);
+    torque[2] += arm[0] * (f[1]) - arm[1] * (f[0]);
+  }
+}
+
+/* ------------------------------------------------------------------ dynamics */
+void or_expmap(double o1, double o2, double o3, double L[9]) {
+  double v0 = o1 * o1, v1 = o2 * o2, v2 = o3 * o3;
+  double angsq = v0 + v1 + v2;
+  double sx, cx;
+  if (angsq < 3.0461741978671E-02) {
+    /* Taylor series of sin(x)/x and (1-cos x)/x^2 in angsq = x^2 (1/(2k+1)!
+     * and 1/(2k+2)!, rounded to 16 digits as in dynamics.cpp:41-64). */
+    sx = 1.0 + (-1.666666666666667E-1 +
+                (8.333333333333333E-3 +
+                 (-1.984126984126984E-4 +
+                  (2.755731922398589E-6 +
+                   (-2.505210838544172E-8 + 1.605904383682161E-10 * angsq) * angsq) *
+                      angsq) *
+                     angsq) *
+                    angsq) *
+                   angsq;
+    cx = 0.5 + (-4.166666666666667E-2 +
+                (1.388888888888889E-3 +
+                 (-2.480158730158730E-5 +
+                  (2.755731922398589E-7 +
+                   (-2.087675698786810E-9 + 1.147074559772972E-11 * angsq) * angsq) *
+                      angsq) *
+                     angsq) *
+                    angsq) *
+                   angsq;
+  } else {
+    double t = angsq;
+    double ang = sqrt(angsq);
+    sx = sin(ang) / ang;
+    cx = (1.0 - cos(ang)) / t;
+  }
+  double v01 = o1 * o2;
+  double v02 = o2 * o3; /* the reference's product (dynamics.cpp:78) */
+  double v12 = o2 * o3;
+  double s0 = sx * o1, s1 = sx * o2, s2 = sx * o3;
+  L[0] = -cx * (v2 + v1);
+  L[1] = cx * v01;
+  L[2] = cx * v02;
+  L[3] = L[1];
+  L[4] = -cx * (v2 + v0);
+  L[5] = cx * v12;
+  L[6] = L[2];
+  L[7] = L[5];
+  L[8] = -cx * (v1 + v0);
+  L[0] += 1.0;
+  L[1] += s2;
+  L[2] -= s1;
+  L[3] -= s2;
+  L[4] += 1.0;
+  L[5] += s0;
+  L[6] += s1;
+  L[7] -= s0;
+  L[8] += 1.0;
+}
+
+void or_update_rotation_matrix(double angular[3], const double refAngular[3],
+                               double rotation[9], double step) {
+  double DL[9], r0[9];
+  or_expmap(step * refAngular[0], step * refAngular[1], step * refAngular[2], DL);
+  memcpy(r0, rotation, sizeof(r0));
+  for (int col = 0; col < 3; ++col)
+    for (int row = 0; row < 3; ++row)
+      rotation[3 * col + row] =
+          r0[row] * DL[3 * col] + r0[3 + row] * DL[3 * col + 1] + r0[6 + row] * DL[3 * col + 2];
+  for (int row = 0; row < 3; ++row)
+    angular[row] = rotation[row] * refAngular[0] + rotation[3 + row] * refAngular[1] +
+                   rotation[6 + row] * refAngular[2];
+}
+
+void or_update_angular(double refAngular[3], const double rotation[9],
+                       const double inertia[9], const double inverse[9],
+                       const double torque[3], double step) {
+  double half = 0.5 * step;
+  double T[3], DL[9], A[3], B[3];
+  for (int row = 0; row < 3; ++row)
+    T[row] = rotation[3 * row] * torque[0] + rotation[3 * row + 1] * torque[1] +
+             rotation[3 * row + 2] * torque[2];
+  or_expmap(-half * refAngular[0], -half * refAngular[1], -half * refAngular[2], DL);
+  for (int r = 0; r < 3; ++r)
+    A[r] = inertia[r] * refAngular[0] + inertia[3 + r] * refAngular[1] + inertia[6 + r] * refAngular[2];
+  for (int r = 0; r < 3; ++r) B[r] = DL[r] * A[0] + DL[3 + r] * A[1] + DL[6 + r] * A[2];
+  for (int r = 0; r < 3; ++r) B[r] = B[r] + half * T[r];
+  for (int r = 0; r < 3; ++r) A[r] = inverse[r] * B[0] + inverse[3 + r] * B[1] + inverse[6 + r] * B[2];
+  for (int r = 0; r < 3; ++r) B[r] = inertia[r] * A[0] + inertia[3 + r] * A[1] + inertia[6 + r] * A[2];
+  T[0] -= A[1] * B[2] - A[2] * B[1];
+  T[1] -= A[2] * B[0] - A[0] * B[2];
+  T[2] -= A[0] * B[1] - A[1] * B[0];
+  for (int r = 0; r < 3; ++r) T[r] *= step;
+  for (int r = 0; r < 3; ++r)
+    refAngular[r] = refAngular[r] + inverse[r] * T[0] + inverse[3 + r] * T[1] + inverse[6 + r] * T[2];
+}
+
+void or_update_vertex(double* x, double* y, double* z, double rx, double ry, double rz,
+                      const double R[9], const double pos[3], const double refpos[3]) {
+  double C[3] = {rx - refpos[0], ry - refpos[1], rz - refpos[2]};
+  *x = pos[0] + (R[0] * C[0] + R[3] * C[1] + R[6] * C[2]);
+  *y = pos[1] + (R[1] * C[0] + R[4] * C[1] + R[7] * C[2]);
+  *z = pos[2] + (R[2] * C[0] + R[5] * C[1] + R[8] * C[2]);
+}
+
+/* -------------------------------------------------------------------- step */
+static void particle_aabb(const or_scene* s, int p, double lo[3], double hi[3]) {
+  lo[0] = lo[1] = lo[2] = INFINITY;
+  hi[0] = hi[1] = hi[2] = -INFINITY;
+  for (int v = 3 * s->tri_offset[p]; v < 3 * s->tri_offset[p + 1]; ++v) {
+    double c[3] = {s->x[v], s->y[v], s->z[v]};
+    for (int q = 0; q < 3; ++q) {
+      if (c[q] < lo[q]) lo[q] = c[q];
+      if (c[q] > hi[q]) hi[q] = c[q];
+    }
+  }
+  /* inflate by the margin plus a 1e-9 m band, so that a pair whose true
+   * distance is within eps_total is never culled */
+  for (int q = 0; q < 3; ++q) {
+    lo[q] -= s->eps[p] + 0.5e-9;
+    hi[q] += s->eps[p] + 0.5e-9;
+  }
+}
+
+int64_t or_step(or_scene* s, int method, const dh_force_params* fp, double dt,
+                const double gravity[3], int64_t* checks, int* npairs_out) {
+  const int n = s->n;
+  double* lo = (double*)malloc(sizeof(double) * 3 * n);
+  double* hi = (double*)malloc(sizeof(double) * 3 * n);
+  for (int p = 0; p < n; ++p) particle_aabb(s, p, lo + 3 * p, hi + 3 * p);
+  double* force = (double*)calloc(3 * n, sizeof(double));
+  double* torque = (double*)calloc(3 * n, sizeof(double));
+  int64_t total_contacts = 0;
+  int npairs = 0;
+  int cap = 4096;
+  dh_contact* buf = (dh_contact*)malloc(sizeof(dh_contact) * cap);
+  /* Per-pair force sums are stored first and then added to each particle in
+   * ascending partner index. */
+  typedef struct pr { int i, j; double fA[3], tA[3], fB[3], tB[3]; } pr;
+  int prcap = 1024, npr = 0;
+  pr* prs = (pr*)malloc(sizeof(pr) * prcap);
+  for (int i = 0; i < n; ++i) {
+    for (int j = i + 1; j < n; ++j) {
+      if (s->b.obstacle[i] && s->b.obstacle[j]) continue;
+      int ov = 1;
+      for (int q = 0; q < 3; ++q)
+        if (lo[3 * i + q] > hi[3 * j + q] || lo[3 * j + q] > hi[3 * i + q]) ov = 0;
+      if (!ov) continue;
+      npairs++;
+      int oi = s->tri_offset[i], oj = s->tri_offset[j];
+      int ni = s->tri_offset[i + 1] - oi, nj = s->tri_offset[j + 1] - oj;
+      if (checks) *checks += (int64_t)ni * nj;
+      int nc = 0;
+      int rc;
+      for (;;) {
+        rc = or_detect(method, s->x + 3 * oi, s->y + 3 * oi, s->z + 3 * oi, ni, s->eps[i],
+                       s->friction[i], s->id[i], s->x + 3 * oj, s->y + 3 * oj, s->z + 3 * oj,
+                       nj, s->eps[j], s->friction[j], s->id[j], buf, cap, &nc, NULL);
+        if (rc != DH_ERR_CAPACITY) break;
+        cap = nc * 2;
+        buf = (dh_contact*)realloc(buf, sizeof(dh_contact) * cap);
+      }
+      if (nc == 0) continue;
+      total_contacts += nc;
+      if (npr == prcap) {
+        prcap *= 2;
+        prs = (pr*)realloc(prs, sizeof(pr) * prcap);
+      }
+      pr* r = prs + npr++;
+      memset(r, 0, sizeof(*r));
+      r->i = i;
+      r->j = j;
+      const dh_bodies* b = &s->b;
+      or_contacts_forces(buf, nc, 0, b->pos + 3 * i, b->angvel + 3 * i, b->linvel + 3 * i,
+                         b->mass[i], b->pos + 3 * j, b->angvel + 3 * j, b->linvel + 3 * j,
+                         b->mass[j], fp, r->fA, r->tA);
+      or_contacts_forces(buf, nc, 1, b->pos + 3 * j, b->angvel + 3 * j, b->linvel + 3 * j,
+                         b->mass[j], b->pos + 3 * i, b->angvel + 3 * i, b->linvel + 3 * i,
+                         b->mass[i], fp, r->fB, r->tB);
+    }
+  }
+  /* particle p: partners ascending.  Pairs with p as the second member
+   * (partner i < p) come before pairs with p first (partner j > p).
+   * A particle without any contact is skipped by the force/angular update
+   * (Engine.cpp:464, Collision.cpp:1056): updateAngular is not a no-op for
+   * zero torque, so the skip is observable. */
+  int* touched = (int*)calloc(n, sizeof(int));
+  for (int k = 0; k < npr; ++k) touched[prs[k].i] = touched[prs[k].j] = 1;
+  for (int p = 0; p < n; ++p) {
+    for (int k = 0; k < npr; ++k)
+      if (prs[k].j == p)
+        for (int q = 0; q < 3; ++q) {
+          force[3 * p + q] += prs[k].fB[q];
+          torque[3 * p + q] += prs[k].tB[q];
+        }
+    for (int k = 0; k < npr; ++k)
+      if (prs[k].i == p)
+        for (int q = 0; q < 3; ++q) {
+          force[3 * p + q] += prs[k].fA[q];
+          torque[3 * p + q] += prs[k].tA[q];
+        }
+  }
+  dh_bodies* b = &s->b;
+  for (int p = 0; p < n; ++p) {
+    if (b->obstacle[p] || !touched[p]) continue;
+    for (int q = 0; q < 3; ++q) b->linvel[3 * p + q] += dt * (force[3 * p + q] / b->mass[p]);
+    or_update_angular(b->refangvel + 3 * p, b->rot + 9 * p, b->inertia + 9 * p,
+                      b->inverse + 9 * p, torque + 3 * p, dt);
+  }
+  for (int p = 0; p < n; ++p) {
+    if (b->obstacle[p]) continue;
+    for (int q = 0; q < 3; ++q)
+      if (gravity[q] != 0.0) b->linvel[3 * p + q] += dt * gravity[q];
+    for (int q = 0; q < 3; ++q) b->pos[3 * p + q] += dt * b->linvel[3 * p + q];
+    or_update_rotation_matrix(b->angvel + 3 * p, b->refangvel + 3 * p, b->rot + 9 * p, dt);
+    for (int v = 3 * s->tri_offset[p]; v < 3 * s->tri_offset[p + 1]; ++v)
+      or_update_vertex(s->x + v, s->y + v, s->z + v, s->rx[v], s->ry[v], s->rz[v],
+                       b->rot + 9 * p, b->pos + 3 * p, b->refpos + 3 * p);
+  }
+  free(lo); free(hi); free(force); free(torque); free(buf); free(prs); free(touched);
+  if (npairs_out) *npairs_out = npairs;
+  return total_contacts;
+}
