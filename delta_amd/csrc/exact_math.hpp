@@ -73,24 +73,18 @@ DH_HD void segseg(T a, T b, T c, T d, T e, T& sc, T& tc) {
     tN = e + b;
     tD = c;
   }
-  // clamp t and recompute s on the visible edge
-  const T md = -d;
-  const T mdb = (-d + b);
-  if (tN < T(0)) {
-    tN = T(0);
-    const bool lo = md < T(0);
-    const bool hi = !lo && (md > a);
-    const T nsN = lo ? T(0) : (hi ? sD : md);
-    sD = (lo || hi) ? sD : a;
-    sN = nsN;
-  } else if (tN > tD) {
-    tN = tD;
-    const bool lo = mdb < T(0);
-    const bool hi = !lo && (mdb > a);
-    const T nsN = lo ? T(0) : (hi ? sD : mdb);
-    sD = (lo || hi) ? sD : a;
-    sN = nsN;
-  }
+  // clamp t and recompute s on the visible edge (selects, no branches)
+  const bool tlo = tN < T(0);
+  const bool thi = !tlo && (tN > tD);
+  const T v = tlo ? -d : (-d + b);
+  const bool lo = v < T(0);
+  const bool hi = !lo && (v > a);
+  const T nsN = lo ? T(0) : (hi ? sD : v);
+  const T nsD = (lo || hi) ? sD : a;
+  const bool clamp = tlo || thi;
+  sN = clamp ? nsN : sN;
+  sD = clamp ? nsD : sD;
+  tN = tlo ? T(0) : (thi ? tD : tN);
   using std::fabs;
   const T qs = sN / sD;
   const T qt = tN / tD;
@@ -98,104 +92,61 @@ DH_HD void segseg(T a, T b, T c, T d, T e, T& sc, T& tc) {
   tc = (fabs(tN) < small_num) ? T(0) : qt;
 }
 
-// Point-triangle squared distance (bf.cpp:922-1091) with region kinds.
-enum PtKind : int { K_F = 0, K_S1, K_T1, K_SE, K_TE, K_IN, K_D2, K_D6 };
+// Point-triangle squared distance (bf.cpp:922-1091).  The eight outcomes of
+// the region analysis: F (s=t=0), S1 (s=1), T1 (t=1), SE (t=0, s=-d/a),
+// TE (s=0, t=-e/c), IN (interior), D2 (s+t=1, s=numer/denom), D6 (s+t=1,
+// t=numer/denom).
 
 template <typename T>
 DH_HD T pt_tri(T a, T b, T c, T d, T e, T f, T& s_out, T& t_out) {
   const T det = a * c - b * b;
   const T s0 = b * e - c * d;
   const T t0 = b * d - a * e;
-  int kind;
-  T numer = T(0), denom = T(1);
-  if ((s0 + t0) <= det) {
-    if (s0 < T(0)) {
-      if (t0 < T(0) && d < T(0)) {
-        kind = (-d >= a) ? K_S1 : K_SE;
-      } else {
-        kind = (e >= T(0)) ? K_F : ((-e >= c) ? K_T1 : K_TE);
-      }
-    } else if (t0 < T(0)) {
-      kind = (d >= T(0)) ? K_F : ((-d >= a) ? K_S1 : K_SE);
-    } else {
-      kind = K_IN;
-    }
-  } else {
-    if (s0 < T(0)) {
-      const T tmp0 = b + d, tmp1 = c + e;
-      if (tmp1 > tmp0) {
-        numer = tmp1 - tmp0;
-        denom = a - T(2) * b + c;
-        kind = (numer >= denom) ? K_S1 : K_D2;
-      } else {
-        kind = (tmp1 <= T(0)) ? K_T1 : ((e >= T(0)) ? K_F : K_TE);
-      }
-    } else if (t0 < T(0)) {
-      const T tmp0 = b + e, tmp1 = a + d;
-      if (tmp1 > tmp0) {
-        numer = tmp1 - tmp0;
-        denom = a - T(2) * b + c;
-        kind = (numer >= denom) ? K_T1 : K_D6;
-      } else {
-        kind = (tmp1 <= T(0)) ? K_S1 : ((d >= T(0)) ? K_F : K_SE);
-      }
-    } else {
-      numer = c + e - b - d;
-      if (numer <= T(0)) {
-        kind = K_T1;
-      } else {
-        denom = a - T(2) * b + c;
-        kind = (numer >= denom) ? K_S1 : K_D2;
-      }
-    }
-  }
+  // region of (s0, t0) (Eberly's numbering) and the resulting case, as
+  // booleans so that the wave selects instead of branching
+  const bool c_st = (s0 + t0) <= det;
+  const bool sneg = s0 < T(0), tneg = t0 < T(0);
+  const bool r0 = c_st && !sneg && !tneg;
+  const bool r4 = c_st && sneg && tneg;
+  const bool r3 = c_st && sneg && !tneg;
+  const bool r5 = c_st && !sneg && tneg;
+  const bool r2 = !c_st && sneg;
+  const bool r6 = !c_st && !sneg && tneg;
+  const bool r1 = !c_st && !sneg && !tneg;
+  const T tmp0 = sneg ? (b + d) : (b + e);  // region 2 : region 6
+  const T tmp1 = sneg ? (c + e) : (a + d);
+  const bool gt = tmp1 > tmp0;
+  const T denom = a - T(2) * b + c;
+  const T numer26 = tmp1 - tmp0;
+  const T numer1 = c + e - b - d;
+  const bool nge = numer26 >= denom, t1le = tmp1 <= T(0);
+  const bool n1le = numer1 <= T(0), n1ge = numer1 >= denom;
+  const bool e_pos = e >= T(0), e_big = -e >= c;
+  const bool d_pos = d >= T(0), d_big = -d >= a, d_neg = d < T(0);
+  const bool useE = r3 || (r4 && !d_neg) || (r2 && !gt && !t1le);  // s = 0 edge
+  const bool useD = r5 || (r6 && !gt && !t1le);                    // t = 0 edge
+  const bool bIN = r0;
+  const bool bT1 = (useE && !e_pos && e_big) || (r2 && !gt && t1le) || (r6 && gt && nge) ||
+                   (r1 && n1le);
+  const bool bTE = useE && !e_pos && !e_big;
+  const bool bS1 = (useD && !d_pos && d_big) || (r4 && d_neg && d_big) || (r2 && gt && nge) ||
+                   (r6 && !gt && t1le) || (r1 && !n1le && n1ge);
+  const bool bSE = (useD && !d_pos && !d_big) || (r4 && d_neg && !d_big);
+  const bool bD2 = (r2 && gt && !nge) || (r1 && !n1le && !n1ge);
+  const bool bD6 = r6 && gt && !nge;
   // one division: 1/det, -d/a, -e/c or numer/denom
-  T num = numer, den = denom;
-  if (kind == K_IN) {
-    num = T(1);
-    den = det;
-  } else if (kind == K_SE) {
-    num = -d;
-    den = a;
-  } else if (kind == K_TE) {
-    num = -e;
-    den = c;
-  }
+  const T num = bIN ? T(1) : bSE ? -d : bTE ? -e : (r1 ? numer1 : numer26);
+  const T den = bIN ? det : bSE ? a : bTE ? c : denom;
   const T q = num / den;
-  T s = T(0), t = T(0);
-  if (kind == K_IN) {
-    s = s0 * q;
-    t = t0 * q;
-  } else if (kind == K_D2) {
-    s = q;
-    t = T(1) - q;
-  } else if (kind == K_D6) {
-    t = q;
-    s = T(1) - q;
-  } else if (kind == K_SE) {
-    s = q;
-  } else if (kind == K_TE) {
-    t = q;
-  } else if (kind == K_S1) {
-    s = T(1);
-  } else if (kind == K_T1) {
-    t = T(1);
-  }
-  T sqr;
-  if (kind == K_IN || kind == K_D2 || kind == K_D6) {
-    sqr = s * (a * s + b * t + T(2) * d) + t * (b * s + c * t + T(2) * e) + f;
-  } else if (kind == K_SE) {
-    sqr = d * s + f;
-  } else if (kind == K_TE) {
-    sqr = e * t + f;
-  } else if (kind == K_S1) {
-    sqr = a + T(2) * d + f;
-  } else if (kind == K_T1) {
-    sqr = c + T(2) * e + f;
-  } else {
-    sqr = f;
-  }
-  if (sqr <= T(0)) sqr = T(0);
+  const T s = bIN ? s0 * q : (bD2 || bSE) ? q : bD6 ? T(1) - q : bS1 ? T(1) : T(0);
+  const T t = bIN ? t0 * q : bD2 ? T(1) - q : (bD6 || bTE) ? q : bT1 ? T(1) : T(0);
+  const T quad = s * (a * s + b * t + T(2) * d) + t * (b * s + c * t + T(2) * e) + f;
+  T sqr = (bIN || bD2 || bD6) ? quad
+        : bSE ? d * s + f
+        : bTE ? e * t + f
+        : bS1 ? a + T(2) * d + f
+        : bT1 ? c + T(2) * e + f : f;
+  sqr = (sqr <= T(0)) ? T(0) : sqr;
   s_out = s;
   t_out = t;
   return sqr;
@@ -209,10 +160,55 @@ struct Tri {
 // t.v[i] for a run-time i in 0..2, as selects
 template <typename T>
 DH_HD V3<T> pick3(const Tri<T>& t, int i) {
-  V3<T> r = t.v[0];
-  r = (i == 1) ? t.v[1] : r;
-  r = (i == 2) ? t.v[2] : r;
-  return r;
+  const bool i1 = (i == 1), i2 = (i == 2);
+  const T x0 = t.v[0].x, x1 = t.v[1].x, x2 = t.v[2].x;
+  const T y0 = t.v[0].y, y1 = t.v[1].y, y2 = t.v[2].y;
+  const T z0 = t.v[0].z, z1 = t.v[1].z, z2 = t.v[2].z;
+  return {i2 ? x2 : (i1 ? x1 : x0), i2 ? y2 : (i1 ? y1 : y0), i2 ? z2 : (i1 ? z1 : z0)};
+}
+
+template <typename T>
+DH_HD V3<T> sel3(bool c, const V3<T>& a, const V3<T>& b) {
+  return {c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
+}
+
+// One segment-segment feature of bfSolver (bf.cpp:409-713), folded into the
+// running minimum (strict <, first index wins).
+template <typename T>
+DH_HD void edge_edge(T& ssmin, T& ttc, T& ssc, int& ssid, const V3<T>& ha, const V3<T>& oa,
+                     const V3<T>& hb, const V3<T>& ob, int j) {
+  const V3<T> u = sub(ha, oa);
+  const V3<T> v = sub(hb, ob);
+  const V3<T> w = sub(oa, ob);
+  const T a = dot(u, u), b = dot(u, v), c = dot(v, v), d = dot(u, w), e = dot(v, w);
+  T sc, tc;
+  segseg(a, b, c, d, e, sc, tc);
+  const V3<T> dP = {w.x + (sc * u.x) - (tc * v.x), w.y + (sc * u.y) - (tc * v.y),
+                    w.z + (sc * u.z) - (tc * v.z)};
+  const T dist = dsqrt(dP.x * dP.x + dP.y * dP.y + dP.z * dP.z);
+  const bool better = dist < ssmin;
+  ssmin = better ? dist : ssmin;
+  ttc = better ? tc : ttc;
+  ssc = better ? sc : ssc;
+  ssid = better ? j : ssid;
+}
+
+// One vertex-face feature (bf.cpp:875-1100): point pt against face f0 f1 f2.
+template <typename T>
+DH_HD void vertex_face(T& ptmin, T& ss, T& tt, int& id, const V3<T>& f0, const V3<T>& f1,
+                       const V3<T>& f2, const V3<T>& pt, int j) {
+  const V3<T> e0 = sub(f1, f0);
+  const V3<T> e1 = sub(f2, f0);
+  const V3<T> dd = sub(f0, pt);
+  T s, t;
+  const T sqr = pt_tri(dot(e0, e0), dot(e0, e1), dot(e1, e1), dot(e0, dd), dot(e1, dd),
+                       dot(dd, dd), s, t);
+  const T dist = dsqrt(sqr);
+  const bool better = dist < ptmin;
+  ptmin = better ? dist : ptmin;
+  ss = better ? s : ss;
+  tt = better ? t : tt;
+  id = better ? j : id;
 }
 
 // bfSolver: closest points P on A and Q on B.  Returns true when the
@@ -221,54 +217,52 @@ template <typename T>
 DH_HD bool bf_solver(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q) {
   T ssmin = T(1E+30), ttc = T(0), ssc = T(0);
   int ssid = 0;
-#pragma unroll
+  // Edge k of a triangle runs from its origin v[(k+1)%3] towards v[k].  The
+  // features are visited in the reference's order by rotating the vertex
+  // triples; the loops are kept rolled so that only one feature is live at a
+  // time (register pressure decides the occupancy of this kernel).
+  V3<T> a0 = A.v[0], a1 = A.v[1], a2 = A.v[2];
+#pragma unroll 1
   for (int k = 0; k < 3; ++k) {
-    const V3<T> oa = A.v[(k + 1) % 3];
-    const V3<T> u = sub(A.v[k], oa);
-    const T a = dot(u, u);
-#pragma unroll
+    V3<T> b0 = B.v[0], b1 = B.v[1], b2 = B.v[2];
+#pragma unroll 1
     for (int m = 0; m < 3; ++m) {
-      const V3<T> ob = B.v[(m + 1) % 3];
-      const V3<T> v = sub(B.v[m], ob);
-      const V3<T> w = sub(oa, ob);
-      const T b = dot(u, v), c = dot(v, v), d = dot(u, w), e = dot(v, w);
-      T sc, tc;
-      segseg(a, b, c, d, e, sc, tc);
-      const V3<T> dP = {w.x + (sc * u.x) - (tc * v.x), w.y + (sc * u.y) - (tc * v.y),
-                        w.z + (sc * u.z) - (tc * v.z)};
-      const T dist = dsqrt(dP.x * dP.x + dP.y * dP.y + dP.z * dP.z);
-      if (dist < ssmin) {
-        ssmin = dist;
-        ttc = tc;
-        ssc = sc;
-        ssid = 3 * k + m;
-      }
+      edge_edge(ssmin, ttc, ssc, ssid, a0, a1, b0, b1, 3 * k + m);
+      const V3<T> tb = b0;
+      b0 = b1;
+      b1 = b2;
+      b2 = tb;
     }
+    const V3<T> ta = a0;
+    a0 = a1;
+    a1 = a2;
+    a2 = ta;
   }
   T ptmin = T(1E30), ss = T(0), tt = T(0);
   int id = 0;
   // j = 0..2: vertex j of B against face A; j = 3..5: vertex j-3 of A against B
-  auto vf = [&](const V3<T>& f0, const V3<T>& f1, const V3<T>& f2, const V3<T>& pt, int j) {
-    const V3<T> e0 = sub(f1, f0);
-    const V3<T> e1 = sub(f2, f0);
-    const V3<T> dd = sub(f0, pt);
-    T s, t;
-    const T sqr = pt_tri(dot(e0, e0), dot(e0, e1), dot(e1, e1), dot(e0, dd), dot(e1, dd),
-                         dot(dd, dd), s, t);
-    const T dist = dsqrt(sqr);
-    if (dist < ptmin) {
-      ptmin = dist;
-      ss = s;
-      tt = t;
-      id = j;
+  {
+    V3<T> p0 = B.v[0], p1 = B.v[1], p2 = B.v[2];
+#pragma unroll 1
+    for (int j = 0; j < 3; ++j) {
+      vertex_face(ptmin, ss, tt, id, A.v[0], A.v[1], A.v[2], p0, j);
+      const V3<T> tp = p0;
+      p0 = p1;
+      p1 = p2;
+      p2 = tp;
     }
-  };
-  vf(A.v[0], A.v[1], A.v[2], B.v[0], 0);
-  vf(A.v[0], A.v[1], A.v[2], B.v[1], 1);
-  vf(A.v[0], A.v[1], A.v[2], B.v[2], 2);
-  vf(B.v[0], B.v[1], B.v[2], A.v[0], 3);
-  vf(B.v[0], B.v[1], B.v[2], A.v[1], 4);
-  vf(B.v[0], B.v[1], B.v[2], A.v[2], 5);
+  }
+  {
+    V3<T> p0 = A.v[0], p1 = A.v[1], p2 = A.v[2];
+#pragma unroll 1
+    for (int j = 3; j < 6; ++j) {
+      vertex_face(ptmin, ss, tt, id, B.v[0], B.v[1], B.v[2], p0, j);
+      const V3<T> tp = p0;
+      p0 = p1;
+      p1 = p2;
+      p2 = tp;
+    }
+  }
   if (ptmin > ssmin) {
     const int k = ssid / 3, m = ssid - 3 * (ssid / 3);
     // register selects instead of dynamic indexing (keeps Tri out of scratch)
@@ -280,15 +274,15 @@ DH_HD bool bf_solver(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q) {
     return true;
   }
   const bool onA = id < 3;
-  const V3<T> f0 = onA ? A.v[0] : B.v[0];
-  const V3<T> f1 = onA ? A.v[1] : B.v[1];
-  const V3<T> f2 = onA ? A.v[2] : B.v[2];
+  const V3<T> f0 = sel3(onA, A.v[0], B.v[0]);
+  const V3<T> f1 = sel3(onA, A.v[1], B.v[1]);
+  const V3<T> f2 = sel3(onA, A.v[2], B.v[2]);
   const V3<T> onf = {f0.x + ((f2.x - f0.x) * tt) + ((f1.x - f0.x) * ss),
                      f0.y + ((f2.y - f0.y) * tt) + ((f1.y - f0.y) * ss),
                      f0.z + ((f2.z - f0.z) * tt) + ((f1.z - f0.z) * ss)};
-  const V3<T> vtx = onA ? pick3(B, id) : pick3(A, id - 3);
-  P = onA ? onf : vtx;
-  Q = onA ? vtx : onf;
+  const V3<T> vtx = sel3(onA, pick3(B, id), pick3(A, id - 3));
+  P = sel3(onA, onf, vtx);
+  Q = sel3(onA, vtx, onf);
   return false;
 }
 
