@@ -10,7 +10,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdelta_hip.so")
 HOSTCHECK = os.path.join(HERE, "libdh_hostcheck.so")
 
-HIP_SOURCES = ["runtime.hip", "tri_pairs.hip", "mixed.hip"]
+HIP_SOURCES = ["runtime.hip", "tri_pairs.hip", "mixed.hip", "detect.hip", "dynamics.hip",
+               "broadphase.hip", "engine.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1,0 +1,269 @@
+// broadphase.hip — candidate particle pairs from a uniform hash grid.
+//
+// The reference has no broad phase on its engine path (Engine.cpp:272-295 is
+// an O(N^2) loop with an optional bounding-sphere test; Peano supplies a grid
+// in the app).  Here the candidate set is defined exactly: every pair (i<j),
+// not both obstacles, whose particle AABBs inflated by their margins (plus a
+// 0.5e-9 m band each) overlap.  The grid is a search structure only: cell
+// size >= the largest inflated extent of a non-obstacle particle guarantees
+// that overlapping boxes sit in neighbouring cells, so the grid result equals
+// the all-pairs test.  Obstacles (floors, funnels, boxes: few, large) are
+// tested against every particle.  Output pairs are sorted (i ascending, then
+// j ascending) regardless of hash-bucket order.
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace dh {
+
+struct Aabb {
+  double lo[3], hi[3];
+};
+
+__global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      const int* __restrict__ tri_offset,
+                                                      const double* __restrict__ eps,
+                                                      Aabb* __restrict__ box) {
+  const int i = blockIdx.x;
+  const int v0 = 3 * tri_offset[i], v1 = 3 * tri_offset[i + 1];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int v = v0 + threadIdx.x; v < v1; v += 64) {
+    const double c[3] = {x[v], y[v], z[v]};
+    for (int q = 0; q < 3; ++q) {
+      lo[q] = fmin(lo[q], c[q]);
+      hi[q] = fmax(hi[q], c[q]);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int q = 0; q < 3; ++q) {
+      lo[q] = fmin(lo[q], __shfl_xor(lo[q], o));
+      hi[q] = fmax(hi[q], __shfl_xor(hi[q], o));
+    }
+  if (threadIdx.x == 0) {
+    const double m = eps[i] + 0.5e-9;
+    Aabb b;
+    for (int q = 0; q < 3; ++q) {
+      b.lo[q] = lo[q] - m;
+      b.hi[q] = hi[q] + m;
+    }
+    box[i] = b;
+  }
+}
+
+__device__ __forceinline__ bool overlap(const Aabb& a, const Aabb& b) {
+  for (int q = 0; q < 3; ++q)
+    if (a.lo[q] > b.hi[q] || b.lo[q] > a.hi[q]) return false;
+  return true;
+}
+
+__device__ __forceinline__ void cell_of(const Aabb& b, double inv_h, long long c[3]) {
+  for (int q = 0; q < 3; ++q) c[q] = static_cast<long long>(floor(0.5 * (b.lo[q] + b.hi[q]) * inv_h));
+}
+
+__device__ __forceinline__ unsigned hash_cell(long long cx, long long cy, long long cz,
+                                              unsigned mask) {
+  unsigned long long h = static_cast<unsigned long long>(cx) * 0x9E3779B97F4A7C15ull ^
+                         static_cast<unsigned long long>(cy) * 0xC2B2AE3D27D4EB4Full ^
+                         static_cast<unsigned long long>(cz) * 0x165667B19E3779F9ull;
+  h ^= h >> 29;
+  return static_cast<unsigned>(h) & mask;
+}
+
+struct GridArgs {
+  const Aabb* box;
+  const int* obstacle;  // may be null
+  int n;
+  double h, inv_h;
+  int* head;  // table of `mask+1` buckets
+  int* next;
+  unsigned mask;
+  const int* obst_list;  // indices of obstacles, ascending
+  int nobst;
+  int* error;
+};
+
+__global__ __launch_bounds__(256) void k_grid_insert(GridArgs g) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.n) return;
+  if (g.obstacle && g.obstacle[i]) return;
+  const Aabb b = g.box[i];
+  for (int q = 0; q < 3; ++q)
+    if (b.hi[q] - b.lo[q] > g.h) atomicCAS(g.error, 0, 1);
+  long long c[3];
+  cell_of(b, g.inv_h, c);
+  const unsigned hsh = hash_cell(c[0], c[1], c[2], g.mask);
+  g.next[i] = atomicExch(&g.head[hsh], i);
+}
+
+// Visits the partners j > i of particle i; `emit(j)` is called once each.
+template <typename F>
+__device__ void for_partners(const GridArgs& g, int i, F emit) {
+  const Aabb bi = g.box[i];
+  const bool obst_i = g.obstacle && g.obstacle[i];
+  if (obst_i) {
+    for (int j = i + 1; j < g.n; ++j) {
+      if (g.obstacle[j]) continue;
+      if (overlap(bi, g.box[j])) emit(j);
+    }
+    return;
+  }
+  long long c[3];
+  cell_of(bi, g.inv_h, c);
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const long long cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
+        const unsigned hsh = hash_cell(cx, cy, cz, g.mask);
+        // distinct neighbour cells may share a bucket: visit a bucket once
+        bool seen = false;
+        for (int ex = -1; ex <= 1 && !seen; ++ex)
+          for (int ey = -1; ey <= 1 && !seen; ++ey)
+            for (int ez = -1; ez <= 1 && !seen; ++ez) {
+              const int before = (ex + 1) * 9 + (ey + 1) * 3 + (ez + 1);
+              const int cur = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+              if (before >= cur) continue;
+              if (hash_cell(c[0] + ex, c[1] + ey, c[2] + ez, g.mask) == hsh) seen = true;
+            }
+        if (seen) continue;
+        for (int j = g.head[hsh]; j >= 0; j = g.next[j]) {
+          if (j <= i) continue;
+          const Aabb bj = g.box[j];
+          long long cj[3];
+          cell_of(bj, g.inv_h, cj);
+          if (cj[0] - c[0] < -1 || cj[0] - c[0] > 1 || cj[1] - c[1] < -1 || cj[1] - c[1] > 1 ||
+              cj[2] - c[2] < -1 || cj[2] - c[2] > 1)
+            continue;
+          if (overlap(bi, bj)) emit(j);
+        }
+      }
+  for (int o = 0; o < g.nobst; ++o) {
+    const int j = g.obst_list[o];
+    if (j > i && overlap(bi, g.box[j])) emit(j);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_grid_count(GridArgs g, int* __restrict__ count) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.n) return;
+  int c = 0;
+  for_partners(g, i, [&](int) { ++c; });
+  count[i] = c;
+}
+
+__global__ __launch_bounds__(256) void k_grid_fill(GridArgs g, const long long* __restrict__ start,
+                                                   long long cap, int* __restrict__ pairs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.n) return;
+  const long long s = start[i];
+  long long w = s;
+  for_partners(g, i, [&](int j) {
+    if (w < cap) {
+      pairs[2 * w] = i;
+      pairs[2 * w + 1] = j;
+    }
+    ++w;
+  });
+  // insertion sort of this particle's partners (short lists)
+  const long long e = w < cap ? w : cap;
+  for (long long a = s + 1; a < e; ++a) {
+    const int v = pairs[2 * a + 1];
+    long long b = a - 1;
+    while (b >= s && pairs[2 * b + 1] > v) {
+      pairs[2 * (b + 1) + 1] = pairs[2 * b + 1];
+      --b;
+    }
+    pairs[2 * (b + 1) + 1] = v;
+  }
+}
+
+int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
+
+struct BpScratch {
+  DevBuf box, head, next, count, start, err, obst;
+  int nobst_cached = -1;
+};
+static BpScratch g_bp;
+
+int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
+                    long long cap, long long* npairs, hipStream_t st) {
+  const int n = geom->nparticles;
+  int rc;
+  if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
+  unsigned tbl = 1;
+  while (tbl < 2u * static_cast<unsigned>(n)) tbl <<= 1;
+  if ((rc = g_bp.head.grow(sizeof(int) * tbl))) return rc;
+  if ((rc = g_bp.next.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_bp.count.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_bp.start.grow(sizeof(long long) * (n + 1)))) return rc;
+  if ((rc = g_bp.err.grow(sizeof(int)))) return rc;
+  // obstacle list (host-side, small)
+  std::vector<int> obst;
+  if (obstacle_dev) {
+    std::vector<int> flags(n);
+    DH_HIP_CHECK(hipMemcpyAsync(flags.data(), obstacle_dev, sizeof(int) * n,
+                                hipMemcpyDeviceToHost, st));
+    DH_HIP_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i)
+      if (flags[i]) obst.push_back(i);
+  }
+  if ((rc = g_bp.obst.grow(sizeof(int) * (obst.size() + 1)))) return rc;
+  if (!obst.empty())
+    DH_HIP_CHECK(hipMemcpyAsync(g_bp.obst.ptr, obst.data(), sizeof(int) * obst.size(),
+                                hipMemcpyHostToDevice, st));
+  DH_HIP_CHECK(hipMemsetAsync(g_bp.head.ptr, 0xFF, sizeof(int) * tbl, st));
+  DH_HIP_CHECK(hipMemsetAsync(g_bp.err.ptr, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
+                     geom->eps, g_bp.box.as<Aabb>());
+  GridArgs g;
+  g.box = g_bp.box.as<Aabb>();
+  g.obstacle = obstacle_dev;
+  g.n = n;
+  g.h = cell;
+  g.inv_h = 1.0 / cell;
+  g.head = g_bp.head.as<int>();
+  g.next = g_bp.next.as<int>();
+  g.mask = tbl - 1;
+  g.obst_list = g_bp.obst.as<int>();
+  g.nobst = static_cast<int>(obst.size());
+  g.error = g_bp.err.as<int>();
+  hipLaunchKernelGGL(k_grid_insert, grid_for(n, 256), 256, 0, st, g);
+  hipLaunchKernelGGL(k_grid_count, grid_for(n, 256), 256, 0, st, g, g_bp.count.as<int>());
+  DH_HIP_CHECK(hipGetLastError());
+  if ((rc = scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st))) return rc;
+  long long last_start = 0;
+  int last_count = 0, err = 0;
+  DH_HIP_CHECK(hipMemcpyAsync(&last_start, g_bp.start.as<long long>() + (n - 1), sizeof(long long),
+                              hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipMemcpyAsync(&last_count, g_bp.count.as<int>() + (n - 1), sizeof(int),
+                              hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipMemcpyAsync(&err, g_bp.err.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipStreamSynchronize(st));
+  if (err)
+    return set_error(DH_ERR_ARG, "dh_broadphase: a particle's inflated AABB exceeds the cell size");
+  *npairs = last_start + last_count;
+  if (*npairs > cap) return DH_ERR_CAPACITY;
+  hipLaunchKernelGGL(k_grid_fill, grid_for(n, 256), 256, 0, st, g, g_bp.start.as<long long>(), cap,
+                     pairs);
+  DH_HIP_CHECK(hipGetLastError());
+  return DH_OK;
+}
+
+}  // namespace dh
+
+using namespace dh;
+
+extern "C" int dh_broadphase(const dh_geometry* geom, const int* obstacle_dev, double cell,
+                             int* pairs_dev, int cap, int* npairs, void* stream) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!geom || !npairs || !(cell > 0) || cap < 0 || (cap > 0 && !pairs_dev))
+    return set_error(DH_ERR_ARG, "dh_broadphase: bad arguments");
+  *npairs = 0;
+  if (geom->nparticles < 2) return DH_OK;
+  long long np = 0;
+  rc = broadphase_impl(geom, obstacle_dev, cell, pairs_dev, cap, &np, as_stream(stream));
+  *npairs = static_cast<int>(np);
+  return rc;
+}

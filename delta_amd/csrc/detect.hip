@@ -1,0 +1,607 @@
+// detect.hip — particle-pair narrow phase: the reference's drivers bf,
+// penalty, hybridWithPer{Batch,Triangle}FallBack, penaltyStat and
+// hybrid{Batch,Triangle}Stat (bf.cpp:28-149, penalty.cpp:50-241,
+// hybrid.cpp:59-627) over a batch of particle pairs.
+//
+// One workgroup (4 waves) owns one particle pair and walks its nA x nB
+// triangle pairs in row-major chunks of 256.  Row decisions of the hybrid
+// drivers (how many penalty solves failed in row iA) are accumulated in LDS
+// in a first pass; the emitting pass then evaluates each pair with the
+// method its row requires.  Contacts are compacted per chunk with wave
+// ballots and one global atomic per chunk, tagged with (pair, rank in pair),
+// and scattered into pair order afterwards, so the output order equals the
+// reference's serial emission order and does not depend on scheduling.
+#include <cstring>
+#include <vector>
+
+#include "exact_math.hpp"
+#include "runtime.hpp"
+
+namespace dh {
+
+// Longest row the reference's drivers support: per-row stack arrays of
+// 10000 entries indexed by 3*iB (bf.cpp:98, penalty.cpp:186, hybrid.cpp:402).
+constexpr int kMaxRowTriangles = 3333;
+
+enum DetectError : int { DE_NONE = 0, DE_REF_UB = 1, DE_TOO_LONG = 2 };
+
+enum StatSlot : int {
+  ST_HIST = 0,            // 17 slots
+  ST_PENALTY_FAILS = 17,
+  ST_BATCH_FAILS = 18,
+  ST_BATCH_SIZE = 19,
+  ST_CHECKS = 20,
+  ST_EVALUATED = 21,
+  ST_FALLBACK_ROWS = 22,
+  ST_COUNT = 23
+};
+
+struct DetectArgs {
+  const double* x;
+  const double* y;
+  const double* z;
+  const int* tri_offset;
+  const double* eps;
+  const int* friction;
+  const int* id;
+  const int* pairs;
+  int npairs;
+  dh_contact* temp;  // unordered contacts, pad_ = rank within the pair
+  long long cap;
+  unsigned long long* counter;  // [0] contacts reserved
+  int* pair_count;
+  unsigned long long* stats;  // ST_COUNT counters
+  int* error;                 // DetectError of the first failing pair
+};
+
+__device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
+  Tri<double> r;
+  const int b = 3 * t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) r.v[k] = {g.x[b + k], g.y[b + k], g.z[b + k]};
+  return r;
+}
+
+// Lanes below this one whose predicate is set (wave64).
+__device__ __forceinline__ int lane_prefix(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0));
+}
+
+struct Pair {
+  int pa, pb, oA, oB, nA, nB;
+  double epsA, epsB, margin, max_err;
+  int fric, idA, idB;
+};
+
+__device__ __forceinline__ Pair load_pair(const DetectArgs& g, int k) {
+  Pair p;
+  p.pa = g.pairs[2 * k];
+  p.pb = g.pairs[2 * k + 1];
+  p.oA = g.tri_offset[p.pa];
+  p.oB = g.tri_offset[p.pb];
+  p.nA = g.tri_offset[p.pa + 1] - p.oA;
+  p.nB = g.tri_offset[p.pb + 1] - p.oB;
+  p.epsA = g.eps[p.pa];
+  p.epsB = g.eps[p.pb];
+  p.margin = p.epsA + p.epsB;
+  p.max_err = (p.epsA + p.epsB) / 16.0;
+  p.fric = (g.friction[p.pa] && g.friction[p.pb]) ? 1 : 0;
+  p.idA = g.id[p.pa];
+  p.idB = g.id[p.pb];
+  return p;
+}
+
+__device__ __forceinline__ bool penalty_pair(const Tri<double>& A, const Tri<double>& B,
+                                             double max_err, V3<double>& P, V3<double>& Q) {
+  const double x0[4] = {0.33, 0.33, 0.33, 0.33};
+  return penalty_fixed(A, B, x0, 4, max_err, P, Q, (double*)nullptr);
+}
+
+// Row fallback rules.
+template <int M>
+__device__ __forceinline__ bool row_falls_back(int fails, int nB, double max_err) {
+  if (M == DH_HYBRID_TRIANGLE) return fails > 0;
+  const double local = fails * max_err;
+  if (M == DH_HYBRID_BATCH) return local > (max_err * static_cast<double>(3 * nB)) / 32.0;
+  return local > (max_err * static_cast<double>(nB)) / 32;  // *Stat variants
+}
+
+template <int M>
+__device__ __forceinline__ double final_pair(const DetectArgs& g, const Pair& p, int iA, int iB,
+                                             const int* row_fail, V3<double>& P, V3<double>& Q) {
+  const Tri<double> A = load_tri(g, p.oA + iA);
+  const Tri<double> B = load_tri(g, p.oB + iB);
+  if (M == DH_BF) {
+    bf_solver(A, B, P, Q);
+  } else if (M == DH_PENALTY) {
+    penalty_pair(A, B, p.max_err, P, Q);
+  } else if (M == DH_PENALTY_STAT) {
+    penalty_adaptive(A, B, p.max_err, P, Q);
+  } else {
+    if (row_falls_back<M>(row_fail[iA], p.nB, p.max_err))
+      bf_solver(A, B, P, Q);
+    else
+      penalty_pair(A, B, p.max_err, P, Q);
+  }
+  return pq_dist(P, Q);
+}
+
+__device__ __forceinline__ void make_contact(const Pair& p, int k, int iA, int iB,
+                                             const V3<double>& P, const V3<double>& Q,
+                                             dh_contact* c) {
+  // contactpoint.cpp:116-155
+  c->x[0] = (P.x + Q.x) / 2.0;
+  c->x[1] = (P.y + Q.y) / 2.0;
+  c->x[2] = (P.z + Q.z) / 2.0;
+  c->P[0] = P.x; c->P[1] = P.y; c->P[2] = P.z;
+  c->Q[0] = Q.x; c->Q[1] = Q.y; c->Q[2] = Q.z;
+  c->eps_total = p.epsA + p.epsB;
+  const double dist = pq_dist(P, Q);
+  c->depth = c->eps_total - dist;
+  c->normal[0] = (P.x - Q.x) / dist;
+  c->normal[1] = (P.y - Q.y) / dist;
+  c->normal[2] = (P.z - Q.z) / dist;
+  c->master = p.idA;
+  c->slave = p.idB;
+  c->penetration = 1;
+  c->friction = p.fric;
+  c->triA = iA;
+  c->triB = iB;
+  c->pair = k;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
+  extern __shared__ unsigned char smem[];
+  __shared__ int wave_cnt[4];
+  __shared__ unsigned long long chunk_base;
+  __shared__ unsigned long long local_stats[6];
+
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const Pair p = load_pair(g, k);
+  const bool is_hybrid = (M == DH_HYBRID_BATCH || M == DH_HYBRID_TRIANGLE ||
+                          M == DH_HYBRID_BATCH_STAT || M == DH_HYBRID_TRIANGLE_STAT);
+  const bool is_rowstat = (M == DH_HYBRID_BATCH_STAT || M == DH_HYBRID_TRIANGLE_STAT);
+  const int rows = (M == DH_HYBRID_TRIANGLE) ? p.nB : p.nA;
+
+  if (M == DH_HYBRID_TRIANGLE && p.nB > p.nA) {
+    if (tid == 0) atomicCAS(g.error, DE_NONE, static_cast<int>(DE_REF_UB));
+    if (tid == 0) g.pair_count[k] = 0;
+    return;
+  }
+  if ((is_hybrid || is_rowstat) && (rows > kMaxRowTriangles || p.nB > kMaxRowTriangles)) {
+    if (tid == 0) atomicCAS(g.error, DE_NONE, static_cast<int>(DE_TOO_LONG));
+    if (tid == 0) g.pair_count[k] = 0;
+    return;
+  }
+  if (tid < 6) local_stats[tid] = 0;
+
+  int* row_fail = reinterpret_cast<int*>(smem);
+  int* row_lastnan = row_fail + kMaxRowTriangles;
+  unsigned long long* row_min =
+      reinterpret_cast<unsigned long long*>(smem + 2 * kMaxRowTriangles * sizeof(int) + 8);
+  const int total = rows * p.nB;
+
+  if (is_hybrid) {
+    for (int r = tid; r < rows; r += 256) {
+      row_fail[r] = 0;
+      if (is_rowstat) {
+        row_lastnan[r] = -1;
+        row_min[r] = ~0ull;
+      }
+    }
+    __syncthreads();
+    // pass 1: penalty on every pair, failures per row
+    for (int f = tid; f < total; f += 256) {
+      const int iA = f / p.nB, iB = f - iA * p.nB;
+      const Tri<double> A = load_tri(g, p.oA + iA);
+      const Tri<double> B = load_tri(g, p.oB + iB);
+      V3<double> P, Q;
+      if (penalty_pair(A, B, p.max_err, P, Q)) atomicAdd(&row_fail[iA], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long fb = 0, fails = 0, bfails = 0;
+      for (int r = 0; r < rows; ++r) {
+        fails += row_fail[r];
+        if (row_falls_back<M>(row_fail[r], p.nB, p.max_err)) {
+          fb++;
+          bfails++;
+        }
+      }
+      local_stats[0] = fb;
+      local_stats[1] = fails;
+      local_stats[2] = bfails;
+    }
+  }
+  if (is_rowstat) {
+    // pass 2: last NaN distance per row; pass 3: minimum after it
+    // (std::min(d, minD) semantics of hybrid.cpp:316-318)
+    for (int f = tid; f < total; f += 256) {
+      const int iA = f / p.nB, iB = f - iA * p.nB;
+      V3<double> P, Q;
+      const double d = final_pair<M>(g, p, iA, iB, row_fail, P, Q);
+      if (d != d) atomicMax(&row_lastnan[iA], iB);
+    }
+    __syncthreads();
+    for (int f = tid; f < total; f += 256) {
+      const int iA = f / p.nB, iB = f - iA * p.nB;
+      if (iB <= row_lastnan[iA]) continue;
+      V3<double> P, Q;
+      const double d = final_pair<M>(g, p, iA, iB, row_fail, P, Q);
+      atomicMin(&row_min[iA], static_cast<unsigned long long>(__double_as_longlong(d)));
+    }
+    __syncthreads();
+  }
+
+  // emitting pass, chunk by chunk in row-major order
+  unsigned long long running = 0;
+  for (int base = 0; base < total; base += 256) {
+    const int f = base + tid;
+    bool emit = false;
+    V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
+    int iA = 0, iB = 0;
+    if (f < total) {
+      iA = f / p.nB;
+      iB = f - iA * p.nB;
+      if (M == DH_PENALTY_STAT) {
+        const Tri<double> A = load_tri(g, p.oA + iA);
+        const Tri<double> B = load_tri(g, p.oB + iB);
+        const int its = penalty_adaptive(A, B, p.max_err, P, Q);
+        atomicAdd(&g.stats[ST_HIST + (its < 0 ? 0 : its)], 1ull);
+        const double d = pq_dist(P, Q);
+        emit = (3 * iB < p.nB) && (d <= p.margin);
+      } else {
+        const double d = final_pair<M>(g, p, iA, iB, row_fail, P, Q);
+        if (M == DH_BF) {
+          emit = d <= p.margin;
+        } else if (is_rowstat) {
+          double mn;
+          if (row_lastnan[iA] == p.nB - 1) {
+            mn = __longlong_as_double(0x7ff8000000000000ll);
+          } else {
+            mn = __longlong_as_double(static_cast<long long>(row_min[iA]));
+            if (row_lastnan[iA] < 0) mn = (1E99 < mn) ? 1E99 : mn;
+          }
+          emit = (d <= mn) && (d < p.margin);
+        } else {
+          emit = d < p.margin;
+        }
+      }
+    }
+    const unsigned long long mask = __ballot(emit);
+    const int pre = lane_prefix(mask);
+    if (lane == 0) wave_cnt[wave] = __popcll(mask);
+    __syncthreads();
+    int off = 0, cnt = 0;
+    for (int w = 0; w < 4; ++w) {
+      off += (w < wave) ? wave_cnt[w] : 0;
+      cnt += wave_cnt[w];
+    }
+    if (tid == 0 && cnt > 0) chunk_base = atomicAdd(g.counter, static_cast<unsigned long long>(cnt));
+    __syncthreads();
+    if (emit) {
+      const unsigned long long slot = chunk_base + off + pre;
+      if (slot < static_cast<unsigned long long>(g.cap)) {
+        dh_contact c;
+        make_contact(p, k, iA, iB, P, Q, &c);
+        c.pad_ = static_cast<int>(running + off + pre);
+        g.temp[slot] = c;
+      }
+    }
+    running += cnt;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    g.pair_count[k] = static_cast<int>(running);
+    // evaluations the reference performs: every pair once, plus bfSolver
+    // again over each fallback row
+    const unsigned long long evals =
+        static_cast<unsigned long long>(total) +
+        (is_hybrid ? local_stats[0] * static_cast<unsigned long long>(p.nB) : 0ull);
+    atomicAdd(&g.stats[ST_CHECKS], static_cast<unsigned long long>(p.nA) * p.nB);
+    atomicAdd(&g.stats[ST_EVALUATED], evals);
+    if (is_hybrid) {
+      atomicAdd(&g.stats[ST_FALLBACK_ROWS], local_stats[0]);
+      if (is_rowstat) {
+        atomicAdd(&g.stats[ST_PENALTY_FAILS], local_stats[1]);
+        atomicAdd(&g.stats[ST_BATCH_FAILS], local_stats[2]);
+        atomicAdd(&g.stats[ST_BATCH_SIZE], static_cast<unsigned long long>(rows) * p.nB);
+      }
+    }
+  }
+}
+
+// exclusive scan of per-pair counts (two levels, 1024 per block)
+__global__ __launch_bounds__(1024) void k_scan_blocks(const int* __restrict__ in, int n,
+                                                      long long* __restrict__ out,
+                                                      long long* __restrict__ block_sums) {
+  __shared__ long long buf[1024];
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const long long v = (i < n) ? in[i] : 0;
+  buf[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 1; s < 1024; s <<= 1) {
+    const long long t = (threadIdx.x >= s) ? buf[threadIdx.x - s] : 0;
+    __syncthreads();
+    buf[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = buf[threadIdx.x] - v;
+  if (threadIdx.x == 1023) block_sums[blockIdx.x] = buf[1023];
+}
+
+__global__ void k_scan_sums(long long* __restrict__ sums, int nb) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    long long acc = 0;
+    for (int b = 0; b < nb; ++b) {
+      const long long v = sums[b];
+      sums[b] = acc;
+      acc += v;
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(long long* __restrict__ out, int n,
+                                                   const long long* __restrict__ sums) {
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) out[i] += sums[blockIdx.x];
+}
+
+__global__ void k_scan_total(const int* __restrict__ in, int n, long long* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = (n > 0) ? out[n - 1] + in[n - 1] : 0;
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const dh_contact* __restrict__ temp, long long n,
+                                                 const long long* __restrict__ offs,
+                                                 dh_contact* __restrict__ out) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  dh_contact c = temp[i];
+  const long long dst = offs[c.pair] + c.pad_;
+  c.pad_ = 0;
+  out[dst] = c;
+}
+
+struct DetectScratch {
+  DevBuf temp, counter, stats, error, counts, offs, sums;
+};
+static DetectScratch g_scratch;
+
+// offs[0..n]: exclusive prefix sums of counts, offs[n] = total
+int scan_counts(const int* counts, int n, long long* offs, hipStream_t st) {
+  const int nb = (n + 1023) / 1024 > 0 ? (n + 1023) / 1024 : 1;
+  int rc = g_scratch.sums.grow(sizeof(long long) * (nb + 1));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scan_blocks, nb, 1024, 0, st, counts, n, offs,
+                     g_scratch.sums.as<long long>());
+  hipLaunchKernelGGL(k_scan_sums, 1, 64, 0, st, g_scratch.sums.as<long long>(), nb);
+  hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, g_scratch.sums.as<long long>());
+  hipLaunchKernelGGL(k_scan_total, 1, 64, 0, st, counts, n, offs);
+  DH_HIP_CHECK(hipGetLastError());
+  return DH_OK;
+}
+
+template <int M>
+static void launch_detect(unsigned grid, size_t shm, hipStream_t st, const DetectArgs& a) {
+  hipLaunchKernelGGL(k_detect<M>, grid, 256, shm, st, a);
+}
+
+int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
+                      dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
+                      long long* offs_out, dh_stats* stats, hipStream_t st) {
+  int rc;
+  const long long temp_cap = cap > 0 ? cap : 1;
+  if ((rc = g_scratch.temp.grow(sizeof(dh_contact) * temp_cap))) return rc;
+  if ((rc = g_scratch.counter.grow(sizeof(unsigned long long)))) return rc;
+  if ((rc = g_scratch.stats.grow(sizeof(unsigned long long) * ST_COUNT))) return rc;
+  if ((rc = g_scratch.error.grow(sizeof(int)))) return rc;
+  if ((rc = g_scratch.counts.grow(sizeof(int) * (npairs + 1)))) return rc;
+  if ((rc = g_scratch.offs.grow(sizeof(long long) * (npairs + 1)))) return rc;
+  DH_HIP_CHECK(hipMemsetAsync(g_scratch.counter.ptr, 0, sizeof(unsigned long long), st));
+  DH_HIP_CHECK(hipMemsetAsync(g_scratch.stats.ptr, 0, sizeof(unsigned long long) * ST_COUNT, st));
+  DH_HIP_CHECK(hipMemsetAsync(g_scratch.error.ptr, 0, sizeof(int), st));
+  int* counts = pair_count_user ? pair_count_user : g_scratch.counts.as<int>();
+  DetectArgs a;
+  a.x = geom->x;
+  a.y = geom->y;
+  a.z = geom->z;
+  a.tri_offset = geom->tri_offset;
+  a.eps = geom->eps;
+  a.friction = geom->friction;
+  a.id = geom->id;
+  a.pairs = pairs;
+  a.npairs = npairs;
+  a.temp = g_scratch.temp.as<dh_contact>();
+  a.cap = cap;
+  a.counter = g_scratch.counter.as<unsigned long long>();
+  a.pair_count = counts;
+  a.stats = g_scratch.stats.as<unsigned long long>();
+  a.error = g_scratch.error.as<int>();
+  const size_t shm_rows = 2 * kMaxRowTriangles * sizeof(int) + 8 + kMaxRowTriangles * 8;
+  const unsigned grid = static_cast<unsigned>(npairs);
+  switch (method) {
+    case DH_BF: launch_detect<DH_BF>(grid, 0, st, a); break;
+    case DH_PENALTY: launch_detect<DH_PENALTY>(grid, 0, st, a); break;
+    case DH_PENALTY_STAT: launch_detect<DH_PENALTY_STAT>(grid, 0, st, a); break;
+    case DH_HYBRID_BATCH: launch_detect<DH_HYBRID_BATCH>(grid, shm_rows, st, a); break;
+    case DH_HYBRID_TRIANGLE: launch_detect<DH_HYBRID_TRIANGLE>(grid, shm_rows, st, a); break;
+    case DH_HYBRID_BATCH_STAT: launch_detect<DH_HYBRID_BATCH_STAT>(grid, shm_rows, st, a); break;
+    case DH_HYBRID_TRIANGLE_STAT:
+      launch_detect<DH_HYBRID_TRIANGLE_STAT>(grid, shm_rows, st, a);
+      break;
+    default:
+      return set_error(DH_ERR_UNSUPPORTED, "dh_detect_pairs: method %d", method);
+  }
+  DH_HIP_CHECK(hipGetLastError());
+  unsigned long long total = 0;
+  int err = 0;
+  unsigned long long st_host[ST_COUNT];
+  DH_HIP_CHECK(hipMemcpyAsync(&total, g_scratch.counter.ptr, sizeof(total),
+                              hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipMemcpyAsync(&err, g_scratch.error.ptr, sizeof(err), hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipMemcpyAsync(st_host, g_scratch.stats.ptr, sizeof(st_host),
+                              hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipStreamSynchronize(st));
+  if (err == DE_REF_UB)
+    return set_error(DH_ERR_REF_UB,
+                     "hybridWithPerTriangleFallBack reads row iA < nTriB of A: nTriB > nTriA is "
+                     "out of bounds in the reference (hybrid.cpp:130)");
+  if (err == DE_TOO_LONG)
+    return set_error(DH_ERR_ARG, "more than %d triangles in a particle (reference row limit)",
+                     kMaxRowTriangles);
+  *n_total = static_cast<long long>(total);
+  if (stats) {
+    for (int i = 0; i < 17; ++i) stats->newton_histogram[i] += st_host[ST_HIST + i];
+    stats->penalty_fails += st_host[ST_PENALTY_FAILS];
+    stats->batch_fails += st_host[ST_BATCH_FAILS];
+    stats->batch_size += st_host[ST_BATCH_SIZE];
+    stats->triangle_checks += st_host[ST_CHECKS];
+    stats->pairs_evaluated += st_host[ST_EVALUATED];
+    stats->fallback_rows += st_host[ST_FALLBACK_ROWS];
+  }
+  if (static_cast<long long>(total) > cap) return DH_ERR_CAPACITY;
+  long long* offs = offs_out ? offs_out : g_scratch.offs.as<long long>();
+  if ((rc = scan_counts(counts, npairs, offs, st))) return rc;
+  if (total > 0) {
+    hipLaunchKernelGGL(k_scatter, grid_for(static_cast<int64_t>(total), 256), 256, 0, st,
+                       g_scratch.temp.as<dh_contact>(), static_cast<long long>(total), offs, out);
+    DH_HIP_CHECK(hipGetLastError());
+  }
+  return DH_OK;
+}
+
+// _batchError of the *Stat drivers: maxErr added once per row, in row order
+// and pair order (hybrid.cpp:240,530).
+static double batch_error_sum(double acc, double max_err, int rows) {
+  for (int r = 0; r < rows; ++r) acc += max_err;
+  return acc;
+}
+
+}  // namespace dh
+
+using namespace dh;
+
+extern "C" {
+
+int dh_exclusive_scan(const int* counts_dev, int n, int64_t* offs_dev, void* stream) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (n < 0 || !offs_dev || (n > 0 && !counts_dev))
+    return set_error(DH_ERR_ARG, "dh_exclusive_scan: bad arguments");
+  return scan_counts(counts_dev, n, reinterpret_cast<long long*>(offs_dev), as_stream(stream));
+}
+
+int dh_detect_pairs(int method, const dh_geometry* geom, const int* pairs_dev, int npairs,
+                    dh_contact* out_dev, int cap, int* n_out, int* pair_count_dev,
+                    dh_stats* stats, void* stream) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!geom || !n_out || npairs < 0 || cap < 0 || (npairs > 0 && !pairs_dev))
+    return set_error(DH_ERR_ARG, "dh_detect_pairs: bad arguments");
+  *n_out = 0;
+  if (npairs == 0) return DH_OK;
+  if (method == DH_GJK) return set_error(DH_ERR_UNSUPPORTED, "dh_detect_pairs: use dh_gjk");
+  long long total = 0;
+  rc = detect_pairs_impl(method, geom, pairs_dev, npairs, out_dev, cap, &total, pair_count_dev,
+                         nullptr, stats, as_stream(stream));
+  *n_out = static_cast<int>(total);
+  if (stats && (method == DH_HYBRID_BATCH_STAT || method == DH_HYBRID_TRIANGLE_STAT) &&
+      (rc == DH_OK || rc == DH_ERR_CAPACITY)) {
+    // sequential _batchError accumulation needs eps and row counts per pair
+    std::vector<int> pr(2 * static_cast<size_t>(npairs));
+    std::vector<int> off(geom->nparticles + 1);
+    std::vector<double> eps(geom->nparticles);
+    DH_HIP_CHECK(hipMemcpy(pr.data(), pairs_dev, sizeof(int) * pr.size(), hipMemcpyDeviceToHost));
+    DH_HIP_CHECK(hipMemcpy(off.data(), geom->tri_offset, sizeof(int) * off.size(),
+                           hipMemcpyDeviceToHost));
+    DH_HIP_CHECK(hipMemcpy(eps.data(), geom->eps, sizeof(double) * eps.size(),
+                           hipMemcpyDeviceToHost));
+    double acc = stats->batch_error;
+    for (int k = 0; k < npairs; ++k) {
+      const int a = pr[2 * k], b = pr[2 * k + 1];
+      acc = batch_error_sum(acc, (eps[a] + eps[b]) / 16, off[a + 1] - off[a]);
+      (void)b;
+    }
+    stats->batch_error = acc;
+  }
+  return rc;
+}
+
+int dh_detect_pair(int method, const double* xA, const double* yA, const double* zA, int nTriA,
+                   double epsA, int frictionA, int idA, const double* xB, const double* yB,
+                   const double* zB, int nTriB, double epsB, int frictionB, int idB,
+                   dh_contact* out, int cap, int* n_out, dh_stats* stats) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!n_out || nTriA < 0 || nTriB < 0 || cap < 0 || (cap > 0 && !out))
+    return set_error(DH_ERR_ARG, "dh_detect_pair: bad arguments");
+  *n_out = 0;
+  if (method == DH_HYBRID_TRIANGLE && nTriB > nTriA)
+    return set_error(DH_ERR_REF_UB,
+                     "hybridWithPerTriangleFallBack reads row iA < nTriB of A: nTriB > nTriA is "
+                     "out of bounds in the reference (hybrid.cpp:130)");
+  if (nTriA == 0 || nTriB == 0) return DH_OK;
+  static DevBuf geo, meta, pairs_buf, outbuf;
+  const size_t nv = 3 * static_cast<size_t>(nTriA + nTriB);
+  if ((rc = geo.grow(3 * nv * sizeof(double)))) return rc;
+  if ((rc = meta.grow(64 + 2 * sizeof(double) + 9 * sizeof(int)))) return rc;
+  if ((rc = pairs_buf.grow(2 * sizeof(int)))) return rc;
+  double* gx = geo.as<double>();
+  double* gy = gx + nv;
+  double* gz = gy + nv;
+  const size_t na = 3 * static_cast<size_t>(nTriA), nb = 3 * static_cast<size_t>(nTriB);
+  DH_HIP_CHECK(hipMemcpy(gx, xA, na * 8, hipMemcpyHostToDevice));
+  DH_HIP_CHECK(hipMemcpy(gx + na, xB, nb * 8, hipMemcpyHostToDevice));
+  DH_HIP_CHECK(hipMemcpy(gy, yA, na * 8, hipMemcpyHostToDevice));
+  DH_HIP_CHECK(hipMemcpy(gy + na, yB, nb * 8, hipMemcpyHostToDevice));
+  DH_HIP_CHECK(hipMemcpy(gz, zA, na * 8, hipMemcpyHostToDevice));
+  DH_HIP_CHECK(hipMemcpy(gz + na, zB, nb * 8, hipMemcpyHostToDevice));
+  struct Meta {
+    double eps[2];
+    int off[3];
+    int fric[2];
+    int id[2];
+    int pairs[2];
+  } m;
+  m.eps[0] = epsA;
+  m.eps[1] = epsB;
+  m.off[0] = 0;
+  m.off[1] = nTriA;
+  m.off[2] = nTriA + nTriB;
+  m.fric[0] = frictionA ? 1 : 0;
+  m.fric[1] = frictionB ? 1 : 0;
+  m.id[0] = idA;
+  m.id[1] = idB;
+  m.pairs[0] = 0;
+  m.pairs[1] = 1;
+  DH_HIP_CHECK(hipMemcpy(meta.ptr, &m, sizeof(m), hipMemcpyHostToDevice));
+  const Meta* dm = meta.as<Meta>();
+  dh_geometry g;
+  g.x = gx;
+  g.y = gy;
+  g.z = gz;
+  g.tri_offset = dm->off;
+  g.eps = dm->eps;
+  g.friction = dm->fric;
+  g.id = dm->id;
+  g.nparticles = 2;
+  g.ntri_total = nTriA + nTriB;
+  const long long dcap = cap > 0 ? cap : 1;
+  if ((rc = outbuf.grow(sizeof(dh_contact) * dcap))) return rc;
+  long long total = 0;
+  rc = detect_pairs_impl(method, &g, dm->pairs, 1, outbuf.as<dh_contact>(), cap, &total, nullptr,
+                         nullptr, stats, nullptr);
+  *n_out = static_cast<int>(total);
+  if (rc != DH_OK) return rc;
+  if (stats && (method == DH_HYBRID_BATCH_STAT || method == DH_HYBRID_TRIANGLE_STAT))
+    stats->batch_error = batch_error_sum(stats->batch_error, (epsA + epsB) / 16, nTriA);
+  if (total > 0)
+    DH_HIP_CHECK(hipMemcpy(out, outbuf.ptr, sizeof(dh_contact) * total, hipMemcpyDeviceToHost));
+  return DH_OK;
+}
+
+}  // extern "C"

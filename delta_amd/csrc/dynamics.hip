@@ -1,0 +1,313 @@
+// dynamics.hip — contact forces and the fused explicit update.
+//
+// k_pair_forces:  one lane per particle pair, the pair's contacts in order,
+//                 both sides (A with the stored normal, B with it negated).
+// k_particle_sum: one lane per particle, its pairs in ascending partner index
+//                 (deterministic, so the step is bit-reproducible).
+// k_integrate:    one wave per particle; every lane recomputes the particle's
+//                 velocity/rotation update (identical arithmetic), lane 0 stores
+//                 it, and the wave rewrites the particle's vertex replicas.
+#include "dyn_math.hpp"
+#include "runtime.hpp"
+
+namespace dh {
+
+struct BodiesDev {
+  double* pos;
+  double* refpos;
+  double* linvel;
+  double* angvel;
+  double* refangvel;
+  double* rot;
+  double* inertia;
+  double* inverse;
+  double* mass;
+  int* obstacle;
+  int n;
+};
+
+static BodiesDev to_dev(const dh_bodies* b) {
+  return {b->pos,  b->refpos,  b->linvel, b->angvel,   b->refangvel, b->rot,
+          b->inertia, b->inverse, b->mass, b->obstacle, b->n};
+}
+
+__device__ __forceinline__ void load3(const double* p, int i, double v[3]) {
+  v[0] = p[3 * i];
+  v[1] = p[3 * i + 1];
+  v[2] = p[3 * i + 2];
+}
+
+__global__ __launch_bounds__(256) void k_pair_forces(BodiesDev b, const int* __restrict__ pairs,
+                                                     int npairs,
+                                                     const dh_contact* __restrict__ cs,
+                                                     const long long* __restrict__ off, ForceConsts k,
+                                                     double* __restrict__ pf,
+                                                     int* __restrict__ incidence) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= npairs) return;
+  const int a = pairs[2 * p], c = pairs[2 * p + 1];
+  double posA[3], angA[3], linA[3], posB[3], angB[3], linB[3];
+  load3(b.pos, a, posA);
+  load3(b.angvel, a, angA);
+  load3(b.linvel, a, linA);
+  load3(b.pos, c, posB);
+  load3(b.angvel, c, angB);
+  load3(b.linvel, c, linB);
+  const double mA = b.mass[a], mB = b.mass[c];
+  double fA[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, fB[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
+  const long long c0 = off[p], c1 = off[p + 1];
+  for (long long i = c0; i < c1; ++i) {
+    const dh_contact& ct = cs[i];
+    const double n[3] = {ct.normal[0], ct.normal[1], ct.normal[2]};
+    const double nn[3] = {-1.0 * ct.normal[0], -1.0 * ct.normal[1], -1.0 * ct.normal[2]};
+    contact_force(ct.x, n, ct.depth, ct.friction != 0, posA, angA, linA, mA, posB, angB, linB, mB,
+                  k, fA, tA);
+    contact_force(ct.x, nn, ct.depth, ct.friction != 0, posB, angB, linB, mB, posA, angA, linA,
+                  mA, k, fB, tB);
+  }
+  double* o = pf + 12 * static_cast<long long>(p);
+  for (int q = 0; q < 3; ++q) {
+    o[q] = fA[q];
+    o[3 + q] = tA[q];
+    o[6 + q] = fB[q];
+    o[9 + q] = tB[q];
+  }
+  if (c1 > c0) {
+    atomicAdd(&incidence[a], 1);
+    atomicAdd(&incidence[c], 1);
+  }
+}
+
+// CSR of contact-carrying pairs per particle: entries (pair, side) appended in
+// arbitrary order, ordered by partner in k_particle_sum.
+__global__ __launch_bounds__(256) void k_fill_incidence(const int* __restrict__ pairs, int npairs,
+                                                        const long long* __restrict__ off,
+                                                        const long long* __restrict__ start,
+                                                        int* __restrict__ fill,
+                                                        int* __restrict__ entries) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= npairs || off[p + 1] == off[p]) return;
+  const int a = pairs[2 * p], c = pairs[2 * p + 1];
+  const int ia = atomicAdd(&fill[a], 1);
+  entries[start[a] + ia] = 2 * p;  // side A
+  const int ic = atomicAdd(&fill[c], 1);
+  entries[start[c] + ic] = 2 * p + 1;  // side B
+}
+
+__global__ __launch_bounds__(256) void k_particle_sum(int n, const int* __restrict__ pairs,
+                                                      const long long* __restrict__ start,
+                                                      const int* __restrict__ count,
+                                                      const int* __restrict__ entries,
+                                                      const double* __restrict__ pf,
+                                                      double* __restrict__ force,
+                                                      double* __restrict__ torque,
+                                                      int* __restrict__ touched) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const long long s = start[i];
+  const int m = count[i];
+  double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+  // ascending partner index: repeated selection of the next larger partner
+  int last = -1;
+  for (int done = 0; done < m; ++done) {
+    int best = -1, bestp = 0x7fffffff;
+    for (int e = 0; e < m; ++e) {
+      const int en = entries[s + e];
+      const int pr = en >> 1;
+      const int partner = (en & 1) ? pairs[2 * pr] : pairs[2 * pr + 1];
+      if (partner > last && partner < bestp) {
+        bestp = partner;
+        best = en;
+      }
+    }
+    last = bestp;
+    const double* o = pf + 12 * static_cast<long long>(best >> 1) + ((best & 1) ? 6 : 0);
+    for (int q = 0; q < 3; ++q) {
+      f[q] += o[q];
+      t[q] += o[3 + q];
+    }
+  }
+  for (int q = 0; q < 3; ++q) {
+    force[3 * i + q] = f[q];
+    torque[3 * i + q] = t[q];
+  }
+  if (touched) touched[i] = m > 0 ? 1 : 0;
+}
+
+struct Gravity {
+  double g[3];
+};
+
+__global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __restrict__ force,
+                                                  const double* __restrict__ torque,
+                                                  const int* __restrict__ touched, double h,
+                                                  Gravity grav, double* __restrict__ x,
+                                                  double* __restrict__ y, double* __restrict__ z,
+                                                  const double* __restrict__ rx,
+                                                  const double* __restrict__ ry,
+                                                  const double* __restrict__ rz,
+                                                  const int* __restrict__ tri_offset) {
+  const int i = blockIdx.x;
+  if (b.obstacle[i]) return;
+  double v[3], W[3], R[9], pos[3], refpos[3], w[3];
+  load3(b.linvel, i, v);
+  load3(b.refangvel, i, W);
+  load3(b.pos, i, pos);
+  load3(b.refpos, i, refpos);
+  for (int q = 0; q < 9; ++q) R[q] = b.rot[9 * i + q];
+  if (!touched || touched[i]) {
+    const double m = b.mass[i];
+    double F[3], T[3], J[9], Ji[9];
+    load3(force, i, F);
+    load3(torque, i, T);
+    for (int q = 0; q < 9; ++q) {
+      J[q] = b.inertia[9 * i + q];
+      Ji[q] = b.inverse[9 * i + q];
+    }
+    for (int q = 0; q < 3; ++q) v[q] += h * (F[q] / m);
+    update_angular(W, R, J, Ji, T, h);
+  }
+  for (int q = 0; q < 3; ++q)
+    if (grav.g[q] != 0.0) v[q] += h * grav.g[q];
+  for (int q = 0; q < 3; ++q) pos[q] += h * v[q];
+  update_rotation(w, W, R, h);
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 3; ++q) {
+      b.linvel[3 * i + q] = v[q];
+      b.refangvel[3 * i + q] = W[q];
+      b.angvel[3 * i + q] = w[q];
+      b.pos[3 * i + q] = pos[q];
+    }
+    for (int q = 0; q < 9; ++q) b.rot[9 * i + q] = R[q];
+  }
+  const int v0 = 3 * tri_offset[i], v1 = 3 * tri_offset[i + 1];
+  for (int j = v0 + threadIdx.x; j < v1; j += 64)
+    update_vertex(x[j], y[j], z[j], rx[j], ry[j], rz[j], R, pos, refpos);
+}
+
+__global__ void k_update_angular_one(double* W, const double* R, const double* J,
+                                     const double* Ji, const double* t, double h) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) update_angular(W, R, J, Ji, t, h);
+}
+
+__global__ void k_update_rotation_one(double* w, const double* W, double* R, double h) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) update_rotation(w, W, R, h);
+}
+
+int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
+
+struct ForceScratch {
+  DevBuf pf, incid, start, fill, entries;
+};
+static ForceScratch g_fs;
+
+}  // namespace dh
+
+using namespace dh;
+
+extern "C" {
+
+int dh_contacts_forces(const dh_bodies* bodies, const int* pairs_dev, int npairs,
+                       const dh_contact* contacts_dev, const int64_t* pair_offset_dev,
+                       const dh_force_params* fp, double* force_dev, double* torque_dev,
+                       int* touched_dev, void* stream) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!bodies || !fp || !force_dev || !torque_dev || npairs < 0 ||
+      (npairs > 0 && (!pairs_dev || !pair_offset_dev)))
+    return set_error(DH_ERR_ARG, "dh_contacts_forces: bad arguments");
+  const hipStream_t st = as_stream(stream);
+  const int n = bodies->n;
+  if ((rc = g_fs.pf.grow(sizeof(double) * 12 * (npairs + 1)))) return rc;
+  if ((rc = g_fs.incid.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_fs.fill.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_fs.start.grow(sizeof(long long) * (n + 1)))) return rc;
+  if ((rc = g_fs.entries.grow(sizeof(int) * (2 * static_cast<size_t>(npairs) + 1)))) return rc;
+  DH_HIP_CHECK(hipMemsetAsync(g_fs.incid.ptr, 0, sizeof(int) * (n + 1), st));
+  DH_HIP_CHECK(hipMemsetAsync(g_fs.fill.ptr, 0, sizeof(int) * (n + 1), st));
+  const BodiesDev b = to_dev(bodies);
+  const ForceConsts k{fp->spring, fp->damper, fp->friction};
+  if (npairs > 0) {
+    hipLaunchKernelGGL(k_pair_forces, grid_for(npairs, 256), 256, 0, st, b, pairs_dev, npairs,
+                       contacts_dev, reinterpret_cast<const long long*>(pair_offset_dev), k,
+                       g_fs.pf.as<double>(),
+                       g_fs.incid.as<int>());
+    DH_HIP_CHECK(hipGetLastError());
+  }
+  if ((rc = scan_counts(g_fs.incid.as<int>(), n, g_fs.start.as<long long>(), st))) return rc;
+  if (npairs > 0) {
+    hipLaunchKernelGGL(k_fill_incidence, grid_for(npairs, 256), 256, 0, st, pairs_dev, npairs,
+                       reinterpret_cast<const long long*>(pair_offset_dev),
+                       g_fs.start.as<long long>(), g_fs.fill.as<int>(),
+                       g_fs.entries.as<int>());
+    DH_HIP_CHECK(hipGetLastError());
+  }
+  if (n > 0) {
+    hipLaunchKernelGGL(k_particle_sum, grid_for(n, 256), 256, 0, st, n, pairs_dev,
+                       g_fs.start.as<long long>(), g_fs.incid.as<int>(), g_fs.entries.as<int>(),
+                       g_fs.pf.as<double>(), force_dev, torque_dev, touched_dev);
+    DH_HIP_CHECK(hipGetLastError());
+  }
+  return DH_OK;
+}
+
+int dh_integrate(dh_bodies* bodies, const double* force_dev, const double* torque_dev,
+                 const int* touched_dev, double dt, const double gravity[3], double* x_dev,
+                 double* y_dev, double* z_dev, const double* refx_dev, const double* refy_dev,
+                 const double* refz_dev, const int* tri_offset_dev, void* stream) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!bodies || !force_dev || !torque_dev || !gravity || !tri_offset_dev)
+    return set_error(DH_ERR_ARG, "dh_integrate: bad arguments");
+  if (bodies->n == 0) return DH_OK;
+  Gravity g{{gravity[0], gravity[1], gravity[2]}};
+  hipLaunchKernelGGL(k_integrate, bodies->n, 64, 0, as_stream(stream), to_dev(bodies), force_dev,
+                     torque_dev, touched_dev, dt, g, x_dev, y_dev, z_dev, refx_dev, refy_dev,
+                     refz_dev, tri_offset_dev);
+  DH_HIP_CHECK(hipGetLastError());
+  return DH_OK;
+}
+
+int dh_update_angular(double refAngular[3], const double rotation[9], const double inertia[9],
+                      const double inverse[9], const double torque[3], double step) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  static DevBuf buf;
+  if ((rc = buf.grow(sizeof(double) * 33))) return rc;
+  double h[33];
+  for (int i = 0; i < 3; ++i) h[i] = refAngular[i];
+  for (int i = 0; i < 9; ++i) {
+    h[3 + i] = rotation[i];
+    h[12 + i] = inertia[i];
+    h[21 + i] = inverse[i];
+  }
+  for (int i = 0; i < 3; ++i) h[30 + i] = torque[i];
+  DH_HIP_CHECK(hipMemcpy(buf.ptr, h, sizeof(h), hipMemcpyHostToDevice));
+  double* d = buf.as<double>();
+  hipLaunchKernelGGL(k_update_angular_one, 1, 1, 0, 0, d, d + 3, d + 12, d + 21, d + 30, step);
+  DH_HIP_CHECK(hipGetLastError());
+  DH_HIP_CHECK(hipMemcpy(refAngular, d, sizeof(double) * 3, hipMemcpyDeviceToHost));
+  return DH_OK;
+}
+
+int dh_update_rotation_matrix(double angular[3], const double refAngular[3], double rotation[9],
+                              double step) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  static DevBuf buf;
+  if ((rc = buf.grow(sizeof(double) * 15))) return rc;
+  double h[15];
+  for (int i = 0; i < 3; ++i) h[i] = 0.0;
+  for (int i = 0; i < 3; ++i) h[3 + i] = refAngular[i];
+  for (int i = 0; i < 9; ++i) h[6 + i] = rotation[i];
+  DH_HIP_CHECK(hipMemcpy(buf.ptr, h, sizeof(h), hipMemcpyHostToDevice));
+  double* d = buf.as<double>();
+  hipLaunchKernelGGL(k_update_rotation_one, 1, 1, 0, 0, d, d + 3, d + 6, step);
+  DH_HIP_CHECK(hipGetLastError());
+  DH_HIP_CHECK(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 3; ++i) angular[i] = h[i];
+  for (int i = 0; i < 9; ++i) rotation[i] = h[6 + i];
+  return DH_OK;
+}
+
+}  // extern "C"

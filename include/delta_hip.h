@@ -219,8 +219,11 @@ typedef struct dh_bodies {
  * applied to A = pairs[2k] (normal as stored) and to B = pairs[2k+1] (normal
  * negated, Collision.cpp:177-185).  Per-particle force/torque (device, 3n
  * each) are overwritten with the sum over the particle's pairs, in ascending
- * partner index, of the per-pair sums in contact order.  is_sphere selects
- * springSphere.  Constants: spring k, damping ratio zeta, friction coefficient. */
+ * partner index, of the per-pair sums in contact order.  touched_dev (n ints,
+ * may be NULL) receives 1 for particles with at least one contact: only those
+ * get the force/angular update (Engine.cpp:464).  Constants: spring k,
+ * damping ratio, friction coefficient (is_sphere: the caller passes the
+ * springSphere constants SSPRING, SDAMPER, SFRICTIONGOLD, forces.cpp:17-19). */
 typedef struct dh_force_params {
   double spring;      /* SPRING 1E4   (forces.cpp:11) */
   double damper;      /* DAMPER 0.5   (forces.cpp:12) */
@@ -229,15 +232,17 @@ typedef struct dh_force_params {
 } dh_force_params;
 
 int dh_contacts_forces(const dh_bodies* bodies, const int* pairs_dev, int npairs,
-                       const dh_contact* contacts_dev, const int* pair_offset_dev,
+                       const dh_contact* contacts_dev, const int64_t* pair_offset_dev,
                        const dh_force_params* fp,
-                       double* force_dev, double* torque_dev, void* stream);
+                       double* force_dev, double* torque_dev, int* touched_dev, void* stream);
 
 /* Engine::deriveForces tail + Engine::updatePosition (Engine.cpp:511-565):
- * v += dt F/m, updateAngular, v[gaxis] += dt*g, x += dt v, updateRotationMatrix,
- * and every vertex replica x = c + R (X - C_ref).  Obstacles are skipped. */
+ * for touched particles v += dt F/m and updateAngular; then for every
+ * non-obstacle particle v += dt*g (non-zero components), x += dt v,
+ * updateRotationMatrix, and every vertex replica x = c + R (X - C_ref).
+ * touched_dev may be NULL (all particles touched). */
 int dh_integrate(dh_bodies* bodies, const double* force_dev, const double* torque_dev,
-                 double dt, const double gravity[3],
+                 const int* touched_dev, double dt, const double gravity[3],
                  double* x_dev, double* y_dev, double* z_dev,
                  const double* refx_dev, const double* refy_dev, const double* refz_dev,
                  const int* tri_offset_dev, void* stream);
@@ -251,11 +256,54 @@ int dh_update_rotation_matrix(double angular[3], const double refAngular[3],
                               double rotation[9], double step);
 
 /* ------------------------------------------------------------ broad phase */
-/* Uniform-grid candidate search over particle AABBs (inflated by eps).
- * Writes candidate pairs (A<B) to pairs_dev (2*cap ints), sorted ascending;
- * *npairs receives the count (DH_ERR_CAPACITY if more than cap). */
-int dh_broadphase(const dh_geometry* geom, int* pairs_dev, int cap, int* npairs,
-                  void* stream);
+/* Candidate particle pairs (no counterpart function in the reference, whose
+ * engine loops over all pairs, Engine.cpp:272-295): every pair (i<j), not both
+ * obstacles, whose AABBs inflated by their margins (+0.5e-9 m each) overlap,
+ * found with a uniform hash grid of cell size `cell` (>= the largest inflated
+ * extent of a non-obstacle particle; DH_ERR_ARG otherwise).  obstacle_dev
+ * (n ints, may be NULL) marks particles tested against all others.  Writes
+ * pairs_dev[2k], pairs_dev[2k+1], sorted ascending; *npairs receives the
+ * count (DH_ERR_CAPACITY if more than cap). */
+int dh_broadphase(const dh_geometry* geom, const int* obstacle_dev, double cell,
+                  int* pairs_dev, int cap, int* npairs, void* stream);
+
+/* Exclusive prefix sum of n ints into n+1 int64 offsets (offs[n] = total). */
+int dh_exclusive_scan(const int* counts_dev, int n, int64_t* offs_dev, void* stream);
+
+/* ------------------------------------------------------------- time step */
+/* One explicit DEM step in the order of Engine::iterate (Engine.cpp:155-163,
+ * 270-567): broad phase, narrow phase with `method` on the current geometry,
+ * getContactsForces per pair, v += dt F/m and updateAngular for particles
+ * with contacts, then gravity, position, rotation and vertex replicas for
+ * every non-obstacle particle.  Buffers of the scene are device memory owned
+ * by the caller; intermediate buffers are owned by the library and grow on
+ * demand.  info (host, may be NULL) receives the step's counters. */
+typedef struct dh_scene {
+  dh_geometry geom;         /* geom.x/y/z: spatial vertices, updated in place */
+  double* x;
+  double* y;
+  double* z;
+  const double* refx;       /* referential vertices (ParticleRecord._ref*)    */
+  const double* refy;
+  const double* refz;
+  dh_bodies bodies;
+} dh_scene;
+
+typedef struct dh_step_params {
+  int method;               /* DH_BF, DH_PENALTY, DH_HYBRID_*                 */
+  double dt;
+  double gravity[3];        /* reference: (0, -9.8, 0), Engine.cpp:534        */
+  dh_force_params forces;
+  double cell;              /* broad-phase cell size                          */
+} dh_step_params;
+
+typedef struct dh_step_info {
+  int64_t candidate_pairs;  /* broad-phase pairs                              */
+  int64_t contacts;         /* contact points emitted                         */
+  int64_t triangle_checks;  /* sum of nTriA*nTriB over candidates             */
+} dh_step_info;
+
+int dh_step(dh_scene* scene, const dh_step_params* params, dh_step_info* info, void* stream);
 
 #ifdef __cplusplus
 }

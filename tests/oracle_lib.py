@@ -180,3 +180,42 @@ def ref_detect(method, mA, mB, epsA, epsB, fricA=0, fricB=0, idA=0, idB=1):
     stats = dict(hist=list(h), penalty_fails=pf.value, batch_fails=bfl.value,
                  batch_size=bs.value, batch_error=be.value)
     return contacts_to_array(buf, n), stats
+
+
+class DhBodiesHost(C.Structure):
+    _fields_ = [("pos", DP), ("refpos", DP), ("linvel", DP), ("angvel", DP),
+                ("refangvel", DP), ("rot", DP), ("inertia", DP), ("inverse", DP),
+                ("mass", DP), ("obstacle", IP), ("material", IP), ("n", C.c_int)]
+
+
+class OrScene(C.Structure):
+    _fields_ = [("n", C.c_int), ("tri_offset", IP), ("x", DP), ("y", DP), ("z", DP),
+                ("rx", DP), ("ry", DP), ("rz", DP), ("eps", DP), ("friction", IP),
+                ("id", IP), ("b", DhBodiesHost)]
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(IP)
+
+
+def or_step(hs, method, spring, damper, friction, dt, gravity):
+    """One oracle step on a delta_amd.engine.HostScene, in place.  Returns
+    (contacts, checks, candidate pairs)."""
+    for f in ("x", "y", "z", "rx", "ry", "rz", "eps", "pos", "refpos", "linvel", "angvel",
+              "refangvel", "rot", "inertia", "inverse", "mass"):
+        setattr(hs, f, np.ascontiguousarray(getattr(hs, f), dtype=np.float64))
+    b = DhBodiesHost(dptr(hs.pos), dptr(hs.refpos), dptr(hs.linvel), dptr(hs.angvel),
+                     dptr(hs.refangvel), dptr(hs.rot), dptr(hs.inertia), dptr(hs.inverse),
+                     dptr(hs.mass), _ip(hs.obstacle), _ip(hs.material), hs.n)
+    sc = OrScene(hs.n, _ip(hs.tri_offset), dptr(hs.x), dptr(hs.y), dptr(hs.z), dptr(hs.rx),
+                 dptr(hs.ry), dptr(hs.rz), dptr(hs.eps), _ip(hs.friction), _ip(hs.id), b)
+    fp = DhForceParams(spring, damper, friction, 0)
+    g = (C.c_double * 3)(*gravity)
+    checks = C.c_int64(0)
+    npairs = C.c_int(0)
+    L = oracle()
+    L.or_step.restype = C.c_int64
+    nc = L.or_step(C.byref(sc), method, C.byref(fp), C.c_double(dt), g, C.byref(checks),
+                   C.byref(npairs))
+    return nc, checks.value, npairs.value
