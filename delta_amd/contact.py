@@ -169,6 +169,11 @@ def getBatchError():
     return float(_stats.batch_error)
 
 
+def reset_all_statistics():
+    """Zero every counter (the reference has no single call for this)."""
+    C.memset(C.addressof(_stats), 0, C.sizeof(_stats))
+
+
 def engine_counters():
     """Counters with no reference getter: nTriA*nTriB per call (the
     reference's tri-cmp, Engine.cpp:451), evaluations run, fallback rows."""

@@ -7,8 +7,10 @@
 // 0.5e-9 m band each) overlap.  The grid is a search structure only: cell
 // size >= the largest inflated extent of a non-obstacle particle guarantees
 // that overlapping boxes sit in neighbouring cells, so the grid result equals
-// the all-pairs test.  Obstacles (floors, funnels, boxes: few, large) are
-// tested against every particle.  Output pairs are sorted (i ascending, then
+// the all-pairs test.  "Large" particles -- obstacles (floors, funnels, boxes)
+// and any particle whose inflated box outgrows the cell, which the reference's
+// non-orthogonal rotation update (dynamics.cpp:78) can cause -- are kept out
+// of the grid and tested against every particle.  Output pairs are sorted (i ascending, then
 // j ascending) regardless of hash-bucket order.
 #include <vector>
 
@@ -79,18 +81,25 @@ struct GridArgs {
   int* head;  // table of `mask+1` buckets
   int* next;
   unsigned mask;
-  const int* obst_list;  // indices of obstacles, ascending
-  int nobst;
-  int* error;
+  int* large;        // indices of large particles (any order)
+  int* nlarge;
 };
+
+__device__ __forceinline__ bool is_large(const GridArgs& g, int i, const Aabb& b) {
+  if (g.obstacle && g.obstacle[i]) return true;
+  for (int q = 0; q < 3; ++q)
+    if (!(b.hi[q] - b.lo[q] <= g.h)) return true;
+  return false;
+}
 
 __global__ __launch_bounds__(256) void k_grid_insert(GridArgs g) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= g.n) return;
-  if (g.obstacle && g.obstacle[i]) return;
   const Aabb b = g.box[i];
-  for (int q = 0; q < 3; ++q)
-    if (b.hi[q] - b.lo[q] > g.h) atomicCAS(g.error, 0, 1);
+  if (is_large(g, i, b)) {
+    g.large[atomicAdd(g.nlarge, 1)] = i;
+    return;
+  }
   long long c[3];
   cell_of(b, g.inv_h, c);
   const unsigned hsh = hash_cell(c[0], c[1], c[2], g.mask);
@@ -102,9 +111,9 @@ template <typename F>
 __device__ void for_partners(const GridArgs& g, int i, F emit) {
   const Aabb bi = g.box[i];
   const bool obst_i = g.obstacle && g.obstacle[i];
-  if (obst_i) {
+  if (is_large(g, i, bi)) {
     for (int j = i + 1; j < g.n; ++j) {
-      if (g.obstacle[j]) continue;
+      if (obst_i && g.obstacle[j]) continue;
       if (overlap(bi, g.box[j])) emit(j);
     }
     return;
@@ -138,8 +147,9 @@ __device__ void for_partners(const GridArgs& g, int i, F emit) {
           if (overlap(bi, bj)) emit(j);
         }
       }
-  for (int o = 0; o < g.nobst; ++o) {
-    const int j = g.obst_list[o];
+  const int nl = *g.nlarge;
+  for (int o = 0; o < nl; ++o) {
+    const int j = g.large[o];
     if (j > i && overlap(bi, g.box[j])) emit(j);
   }
 }
@@ -182,7 +192,6 @@ int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
 
 struct BpScratch {
   DevBuf box, head, next, count, start, err, obst;
-  int nobst_cached = -1;
 };
 static BpScratch g_bp;
 
@@ -198,22 +207,9 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
   if ((rc = g_bp.count.grow(sizeof(int) * (n + 1)))) return rc;
   if ((rc = g_bp.start.grow(sizeof(long long) * (n + 1)))) return rc;
   if ((rc = g_bp.err.grow(sizeof(int)))) return rc;
-  // obstacle list (host-side, small)
-  std::vector<int> obst;
-  if (obstacle_dev) {
-    std::vector<int> flags(n);
-    DH_HIP_CHECK(hipMemcpyAsync(flags.data(), obstacle_dev, sizeof(int) * n,
-                                hipMemcpyDeviceToHost, st));
-    DH_HIP_CHECK(hipStreamSynchronize(st));
-    for (int i = 0; i < n; ++i)
-      if (flags[i]) obst.push_back(i);
-  }
-  if ((rc = g_bp.obst.grow(sizeof(int) * (obst.size() + 1)))) return rc;
-  if (!obst.empty())
-    DH_HIP_CHECK(hipMemcpyAsync(g_bp.obst.ptr, obst.data(), sizeof(int) * obst.size(),
-                                hipMemcpyHostToDevice, st));
-  DH_HIP_CHECK(hipMemsetAsync(g_bp.head.ptr, 0xFF, sizeof(int) * tbl, st));
+  if ((rc = g_bp.obst.grow(sizeof(int) * (n + 1)))) return rc;
   DH_HIP_CHECK(hipMemsetAsync(g_bp.err.ptr, 0, sizeof(int), st));
+  DH_HIP_CHECK(hipMemsetAsync(g_bp.head.ptr, 0xFF, sizeof(int) * tbl, st));
   hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
                      geom->eps, g_bp.box.as<Aabb>());
   GridArgs g;
@@ -225,23 +221,19 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
   g.head = g_bp.head.as<int>();
   g.next = g_bp.next.as<int>();
   g.mask = tbl - 1;
-  g.obst_list = g_bp.obst.as<int>();
-  g.nobst = static_cast<int>(obst.size());
-  g.error = g_bp.err.as<int>();
+  g.large = g_bp.obst.as<int>();
+  g.nlarge = g_bp.err.as<int>();
   hipLaunchKernelGGL(k_grid_insert, grid_for(n, 256), 256, 0, st, g);
   hipLaunchKernelGGL(k_grid_count, grid_for(n, 256), 256, 0, st, g, g_bp.count.as<int>());
   DH_HIP_CHECK(hipGetLastError());
   if ((rc = scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st))) return rc;
   long long last_start = 0;
-  int last_count = 0, err = 0;
+  int last_count = 0;
   DH_HIP_CHECK(hipMemcpyAsync(&last_start, g_bp.start.as<long long>() + (n - 1), sizeof(long long),
                               hipMemcpyDeviceToHost, st));
   DH_HIP_CHECK(hipMemcpyAsync(&last_count, g_bp.count.as<int>() + (n - 1), sizeof(int),
                               hipMemcpyDeviceToHost, st));
-  DH_HIP_CHECK(hipMemcpyAsync(&err, g_bp.err.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
   DH_HIP_CHECK(hipStreamSynchronize(st));
-  if (err)
-    return set_error(DH_ERR_ARG, "dh_broadphase: a particle's inflated AABB exceeds the cell size");
   *npairs = last_start + last_count;
   if (*npairs > cap) return DH_ERR_CAPACITY;
   hipLaunchKernelGGL(k_grid_fill, grid_for(n, 256), 256, 0, st, g, g_bp.start.as<long long>(), cap,

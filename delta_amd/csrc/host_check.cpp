@@ -42,3 +42,27 @@ extern "C" void hc_tri_pairs(int method, const double* soa, long n, double eps_t
     if (flags) flags[i] = fl;
   }
 }
+
+#include "dyn_math.hpp"
+
+extern "C" void hc_expmap(double a, double b, double c, double* L) { dh::expmap(a, b, c, L); }
+extern "C" void hc_update_rotation(double* w, const double* W, double* R, double h) {
+  dh::update_rotation(w, W, R, h);
+}
+extern "C" void hc_update_angular(double* W, const double* R, const double* J, const double* Ji,
+                                  const double* t, double h) {
+  dh::update_angular(W, R, J, Ji, t, h);
+}
+extern "C" void hc_update_vertex(double* x, double* y, double* z, double rx, double ry, double rz,
+                                 const double* R, const double* c, const double* C) {
+  dh::update_vertex(*x, *y, *z, rx, ry, rz, R, c, C);
+}
+extern "C" void hc_contact_force(const double* x, const double* n, double depth, int fric,
+                                 const double* posA, const double* angA, const double* linA,
+                                 double mA, const double* posB, const double* angB,
+                                 const double* linB, double mB, double spring, double damper,
+                                 double friction, double* force, double* torque) {
+  dh::ForceConsts k{spring, damper, friction};
+  dh::contact_force(x, n, depth, fric != 0, posA, angA, linA, mA, posB, angB, linB, mB, k, force,
+                    torque);
+}

@@ -170,6 +170,12 @@ def ref_detect(method, mA, mB, epsA, epsB, fricA=0, fricB=0, idA=0, idB=1):
     buf = (DhContact * cap)()
     r = ref()
     r.ref_clean_stats()
+    h0 = (C.c_longlong * 17)()
+    pf0, bf0, bs0 = C.c_longlong(), C.c_longlong(), C.c_longlong()
+    be0 = C.c_double()
+    # cleanHybridStatistics leaves _batchSize and _batchError untouched
+    # (hybrid.cpp:36-41): record them before the call
+    r.ref_get_stats(h0, C.byref(pf0), C.byref(bf0), C.byref(bs0), C.byref(be0))
     n = r.ref_detect(method, dptr(xa[0]), dptr(xa[1]), dptr(xa[2]), nA, C.c_double(epsA),
                      fricA, idA, dptr(xb[0]), dptr(xb[1]), dptr(xb[2]), nB, C.c_double(epsB),
                      fricB, idB, buf, cap)
@@ -178,7 +184,8 @@ def ref_detect(method, mA, mB, epsA, epsB, fricA=0, fricB=0, idA=0, idB=1):
     be = C.c_double()
     r.ref_get_stats(h, C.byref(pf), C.byref(bfl), C.byref(bs), C.byref(be))
     stats = dict(hist=list(h), penalty_fails=pf.value, batch_fails=bfl.value,
-                 batch_size=bs.value, batch_error=be.value)
+                 batch_size=bs.value, batch_error=be.value, batch_size_before=bs0.value,
+                 batch_error_before=be0.value)
     return contacts_to_array(buf, n), stats
 
 
@@ -219,3 +226,40 @@ def or_step(hs, method, spring, damper, friction, dt, gravity):
     nc = L.or_step(C.byref(sc), method, C.byref(fp), C.c_double(dt), g, C.byref(checks),
                    C.byref(npairs))
     return nc, checks.value, npairs.value
+
+
+def or_forces_scene(hs, pairs, contacts_by_pair, spring, damper, friction):
+    """Per-particle force/torque and touched flags from per-pair contact
+    lists, ascending partner order (the step's accumulation rule)."""
+    n = hs.n
+    fp = DhForceParams(spring, damper, friction, 0)
+    L = oracle()
+    per = []
+    for k, (a, b) in enumerate(pairs):
+        cs = contacts_by_pair[k]
+        if len(cs) == 0:
+            continue
+        buf = (DhContact * len(cs)).from_buffer_copy(np.ascontiguousarray(cs).tobytes())
+        fA = np.zeros(3); tA = np.zeros(3); fB = np.zeros(3); tB = np.zeros(3)
+        pos, ang, lin = hs.pos, hs.angvel, hs.linvel
+        L.or_contacts_forces(buf, len(cs), 0, dptr(np.ascontiguousarray(pos[a])),
+                             dptr(np.ascontiguousarray(ang[a])), dptr(np.ascontiguousarray(lin[a])),
+                             C.c_double(hs.mass[a]), dptr(np.ascontiguousarray(pos[b])),
+                             dptr(np.ascontiguousarray(ang[b])), dptr(np.ascontiguousarray(lin[b])),
+                             C.c_double(hs.mass[b]), C.byref(fp), dptr(fA), dptr(tA))
+        L.or_contacts_forces(buf, len(cs), 1, dptr(np.ascontiguousarray(pos[b])),
+                             dptr(np.ascontiguousarray(ang[b])), dptr(np.ascontiguousarray(lin[b])),
+                             C.c_double(hs.mass[b]), dptr(np.ascontiguousarray(pos[a])),
+                             dptr(np.ascontiguousarray(ang[a])), dptr(np.ascontiguousarray(lin[a])),
+                             C.c_double(hs.mass[a]), C.byref(fp), dptr(fB), dptr(tB))
+        per.append((a, b, fA, tA, fB, tB))
+    force = np.zeros((n, 3)); torque = np.zeros((n, 3)); touched = np.zeros(n, np.int32)
+    for p in range(n):
+        contrib = [(a, fB, tB) for (a, b, fA, tA, fB, tB) in per if b == p]
+        contrib += [(b, fA, tA) for (a, b, fA, tA, fB, tB) in per if a == p]
+        contrib.sort(key=lambda c: c[0])
+        for _, f, t in contrib:
+            force[p] = force[p] + f
+            torque[p] = torque[p] + t
+        touched[p] = 1 if contrib else 0
+    return force, torque, touched

@@ -44,8 +44,7 @@ def test_pair_driver_matches_oracle(method, case):
     if code == _lib.DH_HYBRID_TRIANGLE and len(B) > len(A):
         A, B = B, A
     xa, xb = shapes.to_soa(A), shapes.to_soa(B)
-    contact.cleanPenaltyStatistics()
-    contact._stats.__init__()
+    contact.reset_all_statistics()
     got = fn(*xa, len(A), EPS, True, 7, *xb, len(B), EPS, True, 9)
     st = contact._stats
     rc, want, ost = ol.or_detect(code, A, B, EPS, EPS, 1, 1, 7, 9)

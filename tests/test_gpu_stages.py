@@ -1,0 +1,125 @@
+"""The step's stages one at a time on the GPU against the oracle: broad
+phase candidate set, batched detection on it, contact forces, and the fused
+update given identical forces.  Each stage starts from identical inputs, so a
+mismatch names its stage."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from delta_amd import _lib, contact, engine
+from delta_amd.device import DeviceArray
+from test_gpu_engine import spinning_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def host_candidates(hs):
+    lo = np.zeros((hs.n, 3)); hi = np.zeros((hs.n, 3))
+    for p in range(hs.n):
+        a, b = hs.tri_offset[p], hs.tri_offset[p + 1]
+        v = np.stack([hs.x[3 * a:3 * b], hs.y[3 * a:3 * b], hs.z[3 * a:3 * b]], 1)
+        m = hs.eps[p] + 0.5e-9
+        lo[p] = v.min(0) - m
+        hi[p] = v.max(0) + m
+    out = []
+    for i in range(hs.n):
+        for j in range(i + 1, hs.n):
+            if hs.obstacle[i] and hs.obstacle[j]:
+                continue
+            if np.all(lo[i] <= hi[j]) and np.all(lo[j] <= hi[i]):
+                out.append((i, j))
+    return out
+
+
+def gpu_pairs(ds, hs, cell):
+    L = _lib.lib()
+    L.dh_broadphase.argtypes = [C.POINTER(_lib.DhGeometry), C.c_void_p, C.c_double, C.c_void_p,
+                                C.c_int, C.POINTER(C.c_int), C.c_void_p]
+    cap = hs.n * 64
+    out = DeviceArray((2 * cap,), np.int32)
+    n = C.c_int(0)
+    _lib.check(L.dh_broadphase(C.byref(ds.scene.geom), ds.buf["obstacle"].ptr, cell, out.ptr,
+                               cap, C.byref(n), None), "broadphase")
+    return [tuple(p) for p in out.download()[:2 * n.value].reshape(-1, 2)], out
+
+
+@pytest.mark.parametrize("scene", ["spinning", "c1"])
+def test_broadphase_equals_all_pairs(scene):
+    hs = spinning_scene() if scene == "spinning" else engine.c1_scene()
+    ds = engine.DeviceScene(hs)
+    got, _ = gpu_pairs(ds, hs, hs.cell_size())
+    assert got == host_candidates(hs)
+    # a cell smaller than the particles still yields the same set (all large)
+    got2, _ = gpu_pairs(ds, hs, hs.cell_size() * 0.3)
+    assert got2 == got
+
+
+def test_forces_and_update_match_oracle():
+    hs = spinning_scene()
+    ds = engine.DeviceScene(hs)
+    pairs, d_pairs = gpu_pairs(ds, hs, hs.cell_size())
+    L = _lib.lib()
+    # detection on the GPU
+    cap = 100000
+    out = DeviceArray((cap * 144,), np.uint8)
+    cnt = DeviceArray((len(pairs),), np.int32)
+    nout = C.c_int(0)
+    st = _lib.DhStats()
+    L.dh_detect_pairs.argtypes = [C.c_int, C.POINTER(_lib.DhGeometry), C.c_void_p, C.c_int,
+                                  C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p,
+                                  C.POINTER(_lib.DhStats), C.c_void_p]
+    _lib.check(L.dh_detect_pairs(_lib.DH_BF, C.byref(ds.scene.geom), d_pairs.ptr, len(pairs),
+                                 out.ptr, cap, C.byref(nout), cnt.ptr, C.byref(st), None), "det")
+    counts = cnt.download()
+    allc = np.frombuffer(out.download().tobytes()[:144 * nout.value], dtype=contact.CONTACT_DTYPE)
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    by_pair = [allc[offs[k]:offs[k + 1]] for k in range(len(pairs))]
+    # forces on the GPU
+    d_offs = DeviceArray.from_host(offs)
+    F = DeviceArray((hs.n, 3), np.float64)
+    T = DeviceArray((hs.n, 3), np.float64)
+    tch = DeviceArray((hs.n,), np.int32)
+    fp = _lib.DhForceParams(1e5, 0.1, 0.5, 0)
+    L.dh_contacts_forces.argtypes = [C.POINTER(_lib.DhBodies), C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_void_p, C.POINTER(_lib.DhForceParams), C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]
+    _lib.check(L.dh_contacts_forces(C.byref(ds.scene.bodies), d_pairs.ptr, len(pairs), out.ptr,
+                                    d_offs.ptr, C.byref(fp), F.ptr, T.ptr, tch.ptr, None), "forces")
+    fo, to, tcho = ol.or_forces_scene(hs, pairs, by_pair, 1e5, 0.1, 0.5)
+    assert np.array_equal(tch.download(), tcho)
+    assert np.array_equal(bits(F.download()), bits(fo)), "force"
+    assert np.array_equal(bits(T.download()), bits(to)), "torque"
+    # fused update on the GPU with the oracle's forces
+    F.upload(fo)
+    T.upload(to)
+    g = (C.c_double * 3)(0.0, -9.81, 0.0)
+    b = ds.buf
+    L.dh_integrate.argtypes = [C.POINTER(_lib.DhBodies), C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_double, C.c_double * 3] + [C.c_void_p] * 7 + [C.c_void_p]
+    _lib.check(L.dh_integrate(C.byref(ds.scene.bodies), F.ptr, T.ptr, tch.ptr, 1e-6, g,
+                              b["x"].ptr, b["y"].ptr, b["z"].ptr, b["rx"].ptr, b["ry"].ptr,
+                              b["rz"].ptr, b["tri_offset"].ptr, None), "integrate")
+    orc = ol.oracle()
+    for p in range(hs.n):
+        if hs.obstacle[p]:
+            continue
+        lv = hs.linvel[p].copy(); W = hs.refangvel[p].copy(); R = hs.rot[p].copy()
+        pos = hs.pos[p].copy()
+        if tcho[p]:
+            lv = lv + 1e-6 * (fo[p] / hs.mass[p])
+            orc.or_update_angular(ol.dptr(W), ol.dptr(R), ol.dptr(np.ascontiguousarray(hs.inertia[p])),
+                                  ol.dptr(np.ascontiguousarray(hs.inverse[p])),
+                                  ol.dptr(np.ascontiguousarray(to[p])), C.c_double(1e-6))
+        lv[1] = lv[1] + 1e-6 * -9.81
+        pos = pos + 1e-6 * lv
+        w = np.zeros(3)
+        orc.or_update_rotation_matrix(ol.dptr(w), ol.dptr(W), ol.dptr(R), C.c_double(1e-6))
+        for name, want in (("linvel", lv), ("refangvel", W), ("rot", R), ("pos", pos), ("angvel", w)):
+            got = b[name].download().reshape(hs.n, -1)[p]
+            assert np.array_equal(bits(got), bits(want)), (name, p)

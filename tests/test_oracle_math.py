@@ -1,0 +1,215 @@
+"""Pins of the oracle that do not come from the reference: closed forms,
+invariants, and an independent brute-force triangle distance in extended
+precision.  A dropped term, a swapped index or a wrong sign in the oracle
+fails at least one of these."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+LD = np.longdouble
+
+
+def _pt_seg(p, a, b):
+    ab = b - a
+    t = np.clip(np.dot(p - a, ab) / np.dot(ab, ab), 0, 1)
+    return np.linalg.norm(p - (a + t * ab))
+
+
+def _pt_tri(p, a, b, c):
+    n = np.cross(b - a, c - a)
+    n = n / np.linalg.norm(n)
+    q = p - np.dot(p - a, n) * n
+    # barycentric inside test
+    def same(u, v, w, x):
+        return np.dot(np.cross(v - u, x - u), n) >= 0
+    if same(a, b, c, q) and same(b, c, a, q) and same(c, a, b, q):
+        return abs(np.dot(p - a, n))
+    return min(_pt_seg(p, a, b), _pt_seg(p, b, c), _pt_seg(p, c, a))
+
+
+def _seg_seg(p0, p1, q0, q1):
+    cands = [_pt_seg(p0, q0, q1), _pt_seg(p1, q0, q1), _pt_seg(q0, p0, p1), _pt_seg(q1, p0, p1)]
+    u, v, w = p1 - p0, q1 - q0, p0 - q0
+    a, b, c, d, e = np.dot(u, u), np.dot(u, v), np.dot(v, v), np.dot(u, w), np.dot(v, w)
+    D = a * c - b * b
+    if D > 1e-30 * a * c:
+        s = (b * e - c * d) / D
+        t = (a * e - b * d) / D
+        if 0 <= s <= 1 and 0 <= t <= 1:
+            cands.append(np.linalg.norm(w + s * u - t * v))
+    return min(cands)
+
+
+def brute_distance(A, B):
+    A = A.astype(LD)
+    B = B.astype(LD)
+    d = [_pt_tri(B[j], *A) for j in range(3)] + [_pt_tri(A[j], *B) for j in range(3)]
+    for k in range(3):
+        for m in range(3):
+            d.append(_seg_seg(A[k], A[(k + 1) % 3], B[m], B[(m + 1) % 3]))
+    return min(d)
+
+
+def bf_d(A, B):
+    P, Q, _ = ol.or_bf_solver(A, B)
+    return np.linalg.norm(Q - P), P, Q
+
+
+def test_collision_test_geometry():
+    """CollisionTest.cpp:35-83: A and A shifted by 0.2 along x."""
+    A = np.array([[0, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    B = A + [0.2, 0, 0]
+    d, P, Q = bf_d(A, B)
+    assert d == pytest.approx(0.2, rel=1e-15)
+    rc, cs, _ = ol.or_detect(0, A[None], B[None], 0.05, 0.05)
+    assert rc == 0 and len(cs) == 0
+    rc, cs, _ = ol.or_detect(0, A[None], B[None], 0.125, 0.125)
+    assert len(cs) == 1
+    assert cs[0]["x"][0] == pytest.approx(0.1)
+    assert cs[0]["depth"] == pytest.approx(0.25 - 0.2)
+    assert np.allclose(cs[0]["normal"], [-1, 0, 0])
+
+
+def test_vertex_over_face_interior():
+    A = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float)
+    h = 0.037
+    B = np.array([[0.2, 0.3, h], [0.25, 0.3, 1.0], [0.2, 0.5, 0.8]])
+    d, P, Q = bf_d(A, B)
+    assert d == pytest.approx(h, rel=1e-14)
+    assert np.allclose(P, [0.2, 0.3, 0]) and np.allclose(Q, B[0])
+
+
+def test_skew_edges():
+    h = 0.013
+    A = np.array([[0, 0, 0], [1, 0, 0], [0.5, -1, 0]], float)
+    B = np.array([[0.5, h, -1], [0.5, h, 1], [0.5, h + 1, 0]])
+    d, P, Q = bf_d(A, B)
+    assert d == pytest.approx(h, rel=1e-13)
+    assert np.allclose(P, [0.5, 0, 0]) and np.allclose(Q, [0.5, h, 0])
+
+
+def random_separated(rng, n):
+    out = []
+    while len(out) < n:
+        A = rng.uniform(0, 1, (3, 3))
+        B = rng.uniform(0, 1, (3, 3)) + rng.normal(size=3) * 1.5
+        # keep pairs separated by a plane along the centroid axis
+        ax = B.mean(0) - A.mean(0)
+        if (A @ ax).max() < (B @ ax).min():
+            out.append((A, B))
+    return out
+
+
+def test_against_independent_brute_force():
+    rng = np.random.default_rng(5)
+    for A, B in random_separated(rng, 400):
+        d, _, _ = bf_d(A, B)
+        ref = float(brute_distance(A, B))
+        assert d == pytest.approx(ref, rel=1e-11, abs=1e-14)
+
+
+def test_symmetry_and_invariance():
+    rng = np.random.default_rng(6)
+    Rm = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    for A, B in random_separated(rng, 200):
+        d, P, Q = bf_d(A, B)
+        d2, P2, Q2 = bf_d(B, A)
+        assert d2 == pytest.approx(d, rel=1e-12)
+        d3, _, _ = bf_d(A @ Rm.T + 0.3, B @ Rm.T + 0.3)
+        assert d3 == pytest.approx(d, rel=1e-10)
+        d4, _, _ = bf_d(A[[1, 2, 0]], B[[2, 0, 1]])
+        assert d4 == pytest.approx(d, rel=1e-12)
+        assert np.linalg.norm(Q - P) == pytest.approx(d)
+
+
+def test_penalty_converges_to_bf_when_well_conditioned():
+    """The adaptive solver's optimum equals the brute-force distance for unit-
+    scale, clearly separated pairs (the penalty/regularisation bias is small
+    at that scale)."""
+    rng = np.random.default_rng(8)
+    errs = []
+    for A, B in random_separated(rng, 100):
+        d, _, _ = bf_d(A, B)
+        P, Q, its = ol.or_penalty_adaptive(A, B, 1e-6)
+        errs.append(abs(np.linalg.norm(Q - P) - d) / d)
+    assert np.median(errs) < 1e-2
+
+
+def rodrigues(w):
+    th = np.linalg.norm(w)
+    k = w / th
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
+
+
+def expmap(w):
+    L = np.zeros(9)
+    ol.oracle().or_expmap(C.c_double(w[0]), C.c_double(w[1]), C.c_double(w[2]), ol.dptr(L))
+    return L.reshape(3, 3).T  # column-major storage
+
+
+@pytest.mark.parametrize("ang", [1e-3, 0.1, 0.9])
+def test_expmap_axis_aligned_is_rodrigues(ang):
+    for ax in range(3):
+        w = np.zeros(3)
+        w[ax] = ang
+        assert np.allclose(expmap(w), rodrigues(w), atol=1e-15)
+
+
+def test_expmap_reference_product_in_13_term():
+    """dynamics.cpp:78 uses Omega2*Omega3 where Rodrigues has Omega1*Omega3:
+    entries (0,2) and (2,0) differ from Rodrigues by cx*(O2*O3 - O1*O3),
+    everything else agrees."""
+    w = np.array([0.01, -0.02, 0.015])
+    E, R = expmap(w), rodrigues(w)
+    th2 = w @ w
+    cx = (1 - np.cos(np.sqrt(th2))) / th2
+    delta = cx * (w[1] * w[2] - w[0] * w[2])
+    D = E - R
+    assert D[2, 0] == pytest.approx(delta, rel=1e-6) and D[0, 2] == pytest.approx(delta, rel=1e-6)
+    D[2, 0] = D[0, 2] = 0
+    assert np.abs(D).max() < 1e-15
+
+
+def test_free_flight_closed_form():
+    """One particle far from anything: v_n = v0 + n dt g and
+    x_n = x0 + n dt v0 + dt^2 g n(n+1)/2 (gravity before the position update)."""
+    from delta_amd import engine, shapes
+    m = shapes.jittered_icosphere(1, 0.5e-3, 0.1, shapes.Rng(3))
+    hs = engine.HostScene([m], 1e-4, 0, [0], 2500.0, [0], linvel=[[0.1, 0.2, -0.3]])
+    x0 = hs.pos.copy()
+    dt, n = 1e-4, 50
+    for _ in range(n):
+        ol.or_step(hs, 0, 1e5, 0.1, 0.5, dt, (0.0, -9.81, 0.0))
+    v0 = np.array([0.1, 0.2, -0.3])
+    g = np.array([0, -9.81, 0])
+    assert np.allclose(hs.linvel[0], v0 + n * dt * g, rtol=1e-13)
+    assert np.allclose(hs.pos[0], x0[0] + n * dt * v0 + dt * dt * g * n * (n + 1) / 2,
+                       rtol=0, atol=1e-15)
+
+
+def test_head_on_spring_dashpot():
+    """One contact, normal along x, approaching velocities: the force on A is
+    (K depth + zeta 2 sqrt(m* K) v_n) n, torque zero for a central contact."""
+    c = ol.DhContact()
+    c.x[:] = [0.5, 0, 0]
+    c.normal[:] = [-1.0, 0, 0]
+    c.depth = 1e-4
+    c.friction = 0
+    K, z, mA, mB = 1e5, 0.1, 2e-6, 3e-6
+    f = np.zeros(3); t = np.zeros(3)
+    fp = ol.DhForceParams(K, z, 0.5, 0)
+    pA, pB = np.array([0.0, 0, 0]), np.array([1.0, 0, 0])
+    vA, vB = np.array([1.0, 0, 0]), np.array([-2.0, 0, 0])
+    zero = np.zeros(3)
+    ol.oracle().or_contacts_forces(C.byref(c), 1, 0, ol.dptr(pA), ol.dptr(zero), ol.dptr(vA),
+                                   C.c_double(mA), ol.dptr(pB), ol.dptr(zero.copy()),
+                                   ol.dptr(vB), C.c_double(mB), C.byref(fp), ol.dptr(f), ol.dptr(t))
+    ms = 1 / (1 / mA + 1 / mB)
+    vn = np.dot(vB - vA, [-1.0, 0, 0])
+    want = (K * 1e-4 + z * 2 * np.sqrt(ms * K) * vn) * np.array([-1.0, 0, 0])
+    assert np.allclose(f, want, rtol=1e-14)
+    assert np.allclose(t, 0)
