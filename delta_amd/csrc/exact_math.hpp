@@ -214,7 +214,8 @@ DH_HD void vertex_face(T& ptmin, T& ss, T& tt, int& id, const V3<T>& f0, const V
 // bfSolver: closest points P on A and Q on B.  Returns true when the
 // edge-edge result is kept (ptmin > ssmin, bf.cpp:1102).
 template <typename T>
-DH_HD bool bf_solver(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q) {
+DH_HD bool bf_solver_ids(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q, int& ssid_out,
+                         int& id_out) {
   T ssmin = T(1E+30), ttc = T(0), ssc = T(0);
   int ssid = 0;
   // Edge k of a triangle runs from its origin v[(k+1)%3] towards v[k].  The
@@ -263,6 +264,8 @@ DH_HD bool bf_solver(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q) {
       p2 = tp;
     }
   }
+  ssid_out = ssid;
+  id_out = id;
   if (ptmin > ssmin) {
     const int k = ssid / 3, m = ssid - 3 * (ssid / 3);
     // register selects instead of dynamic indexing (keeps Tri out of scratch)
@@ -284,6 +287,12 @@ DH_HD bool bf_solver(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q) {
   P = sel3(onA, onf, vtx);
   Q = sel3(onA, vtx, onf);
   return false;
+}
+
+template <typename T>
+DH_HD bool bf_solver(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q) {
+  int ssid, id;
+  return bf_solver_ids(A, B, P, Q, ssid, id);
 }
 
 // |Q-P| in the caller's order (bf.cpp:117, contactpoint.cpp:157-159).

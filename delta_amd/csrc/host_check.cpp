@@ -66,3 +66,28 @@ extern "C" void hc_contact_force(const double* x, const double* n, double depth,
   dh::contact_force(x, n, depth, fric != 0, posA, angA, linA, mA, posB, angB, linB, mB, k, force,
                     torque);
 }
+
+#include "pair_features.hpp"
+
+// Per pair: bfSolver's winning feature code, whether eval_feature(code) gives
+// bfSolver's points bit for bit, whether that result certifies, and for every
+// code 0..14 whether it certifies plus its distance.
+extern "C" void hc_learned_probe(const double* soa, long n, int* code, int* same, int* cert_bf,
+                                 int* cert_all, double* dist_all, double* d_bf) {
+  for (long i = 0; i < n; ++i) {
+    Tri<double> A = load(soa, n, i, 0), B = load(soa, n, i, 9);
+    V3<double> P, Q, P2, Q2;
+    const int c = dh::bf_feature_code(A, B, P, Q);
+    code[i] = c;
+    dh::eval_feature(A, B, c, P2, Q2);
+    same[i] = (P.x == P2.x && P.y == P2.y && P.z == P2.z && Q.x == Q2.x && Q.y == Q2.y &&
+               Q.z == Q2.z);
+    cert_bf[i] = dh::certify(A, B, P, Q);
+    d_bf[i] = dh::pq_dist(P, Q);
+    for (int f = 0; f < 15; ++f) {
+      dh::eval_feature(A, B, f, P2, Q2);
+      cert_all[15 * i + f] = dh::certify(A, B, P2, Q2);
+      dist_all[15 * i + f] = dh::pq_dist(P2, Q2);
+    }
+  }
+}

@@ -1,0 +1,132 @@
+// pair_features.hpp — single-feature evaluation, optimality certificate and
+// canonical frame for the learned narrow phase (host + device, binary64).
+//
+// A triangle pair's closest points are attained on one of bfSolver's 15
+// feature pairs: codes 0..8 are edge k of A against edge m of B (code 3k+m,
+// edge k runs from vertex (k+1)%3 towards vertex k, bf.cpp:409-713), codes
+// 9..14 are vertex j of B against face A (9+j) and vertex j of A against face
+// B (12+j) (bf.cpp:875-1100).  eval_feature reproduces bfSolver's arithmetic
+// for one feature, so when the predicted feature is bfSolver's winner the
+// closest points are bit-identical to bfSolver's.
+//
+// certify(): for triangles A, B and candidate points P in A, Q in B with
+// n = Q - P, the pair is accepted when n.(a - P) <= tol for every vertex a of
+// A and n.(Q - b) <= tol for every vertex b of B, tol = kCertRel * |n|^2.
+// Then every x in A and y in B satisfy |y - x| >= |n| - 2 tol / |n|, so
+// |n| is within 2*kCertRel of the true distance (and of bfSolver's, which
+// equals the true distance for non-intersecting triangles); intersecting
+// triangles can never pass.
+#pragma once
+#include "exact_math.hpp"
+
+namespace dh {
+
+constexpr double kCertRel = 4e-7;  // distance error bound 8e-7 relative
+
+template <typename T>
+DH_HD V3<T> cross3(const V3<T>& a, const V3<T>& b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Closest points of feature `code` (0..14).
+DH_HD void eval_feature(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+                        V3<double>& Q) {
+  if (code < 9) {
+    const int k = code / 3, m = code - 3 * (code / 3);
+    const V3<double> ha = pick3(A, k), oa = pick3(A, k == 2 ? 0 : k + 1);
+    const V3<double> hb = pick3(B, m), ob = pick3(B, m == 2 ? 0 : m + 1);
+    const V3<double> u = sub(ha, oa), v = sub(hb, ob), w = sub(oa, ob);
+    const double a = dot(u, u), b = dot(u, v), c = dot(v, v), d = dot(u, w), e = dot(v, w);
+    double sc, tc;
+    segseg(a, b, c, d, e, sc, tc);
+    P = {oa.x + (u.x * sc), oa.y + (u.y * sc), oa.z + (u.z * sc)};
+    Q = {ob.x + (v.x * tc), ob.y + (v.y * tc), ob.z + (v.z * tc)};
+    return;
+  }
+  const int j = code - 9;
+  const bool onA = j < 3;
+  const V3<double> f0 = sel3(onA, A.v[0], B.v[0]);
+  const V3<double> f1 = sel3(onA, A.v[1], B.v[1]);
+  const V3<double> f2 = sel3(onA, A.v[2], B.v[2]);
+  const V3<double> pt = sel3(onA, pick3(B, j), pick3(A, j - 3));
+  const V3<double> e0 = sub(f1, f0), e1 = sub(f2, f0), dd = sub(f0, pt);
+  double s, t;
+  pt_tri(dot(e0, e0), dot(e0, e1), dot(e1, e1), dot(e0, dd), dot(e1, dd), dot(dd, dd), s, t);
+  const V3<double> onf = {f0.x + ((f2.x - f0.x) * t) + ((f1.x - f0.x) * s),
+                          f0.y + ((f2.y - f0.y) * t) + ((f1.y - f0.y) * s),
+                          f0.z + ((f2.z - f0.z) * t) + ((f1.z - f0.z) * s)};
+  P = sel3(onA, onf, pt);
+  Q = sel3(onA, pt, onf);
+}
+
+DH_HD bool certify(const Tri<double>& A, const Tri<double>& B, const V3<double>& P,
+                   const V3<double>& Q) {
+  const V3<double> n = sub(Q, P);
+  const double d2 = dot(n, n);
+  const double tol = kCertRel * d2;
+  bool ok = d2 > 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ok = ok && (dot(n, sub(A.v[k], P)) <= tol);
+    ok = ok && (dot(n, sub(Q, B.v[k])) <= tol);
+  }
+  return ok;
+}
+
+// Feature code of bfSolver's winner (same selection as bf_solver).
+DH_HD int bf_feature_code(const Tri<double>& A, const Tri<double>& B, V3<double>& P,
+                          V3<double>& Q) {
+  int ssid = 0, id = 0;
+  const bool edge = bf_solver_ids(A, B, P, Q, ssid, id);
+  return edge ? ssid : 9 + id;
+}
+
+// Canonical frame of a pair (origin at A's centroid, x along A's edge 0->1,
+// z along A's normal, lengths in units of A's longest edge), as 18 floats:
+// A0, A1, A2, B0, B1, B2 (xyz each), clamped to [-32, 32].  Returns false for
+// a degenerate A (no frame).
+DH_HD bool canonical_inputs(const Tri<double>& A, const Tri<double>& B, float out[18]) {
+  const V3<double> c = {(A.v[0].x + A.v[1].x + A.v[2].x) / 3.0,
+                        (A.v[0].y + A.v[1].y + A.v[2].y) / 3.0,
+                        (A.v[0].z + A.v[1].z + A.v[2].z) / 3.0};
+  const V3<double> e01 = sub(A.v[1], A.v[0]), e12 = sub(A.v[2], A.v[1]),
+                   e20 = sub(A.v[0], A.v[2]);
+  const double l01 = dsqrt(dot(e01, e01));
+  double lmax = l01;
+  const double l12 = dsqrt(dot(e12, e12)), l20 = dsqrt(dot(e20, e20));
+  lmax = l12 > lmax ? l12 : lmax;
+  lmax = l20 > lmax ? l20 : lmax;
+  const V3<double> nz = cross3(e01, sub(A.v[2], A.v[0]));
+  const double ln = dsqrt(dot(nz, nz));
+  const bool ok = (l01 > 0.0) && (ln > 0.0) && (lmax > 0.0);
+  const double il = ok ? 1.0 / l01 : 0.0, in = ok ? 1.0 / ln : 0.0;
+  const V3<double> ex = {e01.x * il, e01.y * il, e01.z * il};
+  const V3<double> ez = {nz.x * in, nz.y * in, nz.z * in};
+  const V3<double> ey = cross3(ez, ex);
+  const double s = ok ? 1.0 / lmax : 0.0;
+  for (int v = 0; v < 6; ++v) {
+    const V3<double> p = (v < 3) ? A.v[v] : B.v[v - 3];
+    const V3<double> q = {(p.x - c.x) * s, (p.y - c.y) * s, (p.z - c.z) * s};
+    const double r[3] = {dot(q, ex), dot(q, ey), dot(q, ez)};
+    for (int a = 0; a < 3; ++a) {
+      double x = r[a];
+      x = x > 32.0 ? 32.0 : (x < -32.0 ? -32.0 : x);
+      out[3 * v + a] = static_cast<float>(x);
+    }
+  }
+  return ok;
+}
+
+// Barycentric parameters of P in A (P = A0 + al (A1-A0) + be (A2-A0)) by the
+// 2x2 normal equations.
+DH_HD void barycentric(const Tri<double>& A, const V3<double>& P, double& al, double& be) {
+  const V3<double> e0 = sub(A.v[1], A.v[0]), e1 = sub(A.v[2], A.v[0]), r = sub(P, A.v[0]);
+  const double a = dot(e0, e0), b = dot(e0, e1), c = dot(e1, e1);
+  const double d = dot(e0, r), e = dot(e1, r);
+  const double det = a * c - b * b;
+  const bool ok = det > 0.0;
+  al = ok ? (c * d - b * e) / det : 0.0;
+  be = ok ? (a * e - b * d) / det : 0.0;
+}
+
+}  // namespace dh

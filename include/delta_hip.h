@@ -187,6 +187,30 @@ int dh_gen_tri_pairs(uint64_t seed, int64_t n, const dh_gen_params* p, double* s
 /* Element-wise binary64 -> binary32 rounding of n values (device). */
 int dh_convert_f32(const double* src_dev, float* dst_dev, int64_t n, void* stream);
 
+/* ------------------------------------------------- learned narrow phase */
+/* Feature network (no counterpart in the reference): 18 canonical-frame
+ * coordinates -> 64 -> 64 -> 32 outputs (15 closest-feature logits, an
+ * exact-path logit, 4 barycentric parameters), GELU, bf16 MFMA with FP32
+ * accumulation.  Parameters are dh_mlp_param_count() floats (device).
+ * dh_mlp_labels: for n pairs (dh_tri_pairs layout) writes the network input
+ * X (n x 32 bf16), bfSolver's winning feature (0..14), the barycentric
+ * parameters of its P and Q (4n floats) and the exact-path label (1 when
+ * bfSolver's own result cannot be certified: intersecting or degenerate).
+ * dh_mlp_grad: loss and gradient over pairs [offset, offset+batch) (grad
+ * overwritten).  dh_mlp_adam: one Adam step (step >= 1).
+ * dh_learned_set_model: the parameters DH_PAIR_LEARNED uses (kept by pointer). */
+int dh_mlp_param_count(void);
+int dh_mlp_labels(const double* soa_dev, int64_t n, uint16_t* X_dev, uint8_t* feat_dev,
+                  float* bary_dev, float* flag_dev, void* stream);
+int dh_mlp_forward(const float* params_dev, const uint16_t* X_dev, int64_t n, float* out_dev,
+                   void* stream);
+int dh_mlp_grad(const float* params_dev, const uint16_t* X_dev, const uint8_t* feat_dev,
+                const float* bary_dev, const float* flag_dev, int64_t offset, int batch,
+                float* grad_dev, float* loss_dev, void* stream);
+int dh_mlp_adam(float* params_dev, float* m_dev, float* v_dev, const float* grad_dev, float lr,
+                int step, void* stream);
+int dh_learned_set_model(const float* params_dev);
+
 /* Number of pairs of the last dh_tri_pairs call per path (host). */
 typedef struct dh_pair_counters {
   int64_t screened;     /* pairs that went through the FP32 screen          */
