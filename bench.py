@@ -37,7 +37,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="bf64,penalty64,hybrid64,mixed,learned")
+    ap.add_argument("--methods", default="bf64,penalty64,hybrid64,learned")
+    ap.add_argument("--train-pairs-log2", type=int, default=20)
+    ap.add_argument("--train-epochs", type=int, default=3)
     return ap.parse_args()
 
 
@@ -62,6 +64,56 @@ def time_method(batch, method, steps, warmup):
     L.dh_event_destroy(e0)
     L.dh_event_destroy(e1)
     return ms.value / steps, wall
+
+
+def train_network(a):
+    """Train the feature network on pairs of the same distribution drawn with
+    another seed (untimed setup)."""
+    from delta_amd import learn, narrow
+    t0 = time.perf_counter()
+    tb = narrow.PairBatch(1 << a.train_pairs_log2, seed=101)
+    ds = learn.Dataset(tb.soa, tb.n)
+    tr = learn.Trainer(seed=0)
+    losses = tr.fit(ds, epochs=a.train_epochs, batch=8192, lr=3e-3)
+    top1, top2 = tr.accuracy(ds, min(ds.n, 1 << 18))
+    tr.install()
+    info = {"train_pairs": tb.n, "epochs": a.train_epochs, "final_loss": losses[-1],
+            "top1": float(top1), "top2": float(top2), "seconds": time.perf_counter() - t0}
+    print(f"[bench] trained network: {info}", file=sys.stderr, flush=True)
+    return tr, ds, info
+
+
+def learned_accuracy(batch):
+    """Learned results against the FP64 brute force over every pair."""
+    import ctypes as C
+    from delta_amd import _lib
+    batch.run("bf64")
+    P0, Q0, d0, _ = batch.results()
+    batch.run("learned")
+    P, Q, d, fl = batch.results()
+    cnt = _lib.DhPairCounters()
+    _lib.check(_lib.lib().dh_tri_pairs_counters(C.byref(cnt)), "counters")
+    rel = np.abs(d - d0) / np.where(d0 > 0, d0, 1.0)
+    margin = 2e-4
+    band = np.abs(d0 - margin) <= 1e-9
+    mism = int(np.count_nonzero((d <= margin)[~band] != (d0 <= margin)[~band]))
+    contact = (d0 <= margin) & (d <= margin)
+    # contact-record errors (contactpoint.cpp:116-155) on contacts
+    x0, x1 = (P0 + Q0) / 2, (P + Q) / 2
+    n0 = (P0 - Q0) / np.where(d0 > 0, d0, 1.0)
+    n1 = (P - Q) / np.where(d > 0, d, 1.0)
+    cert = (fl & 8) != 0
+    return {"pairs": int(batch.n), "certified_fraction": float(cert.mean()),
+            "brute_force_fraction": float(1 - cert.mean()),
+            "features_evaluated_per_pair": cnt.features_fp64 / batch.n,
+            "max_rel_distance_error": float(rel.max()),
+            "contact_set_mismatches_outside_1e-9_band": mism,
+            "pairs_in_band": int(band.sum()),
+            "contacts": int(contact.sum()),
+            "max_contact_point_error_m": float(np.abs(x1 - x0)[:, contact].max()) if contact.any() else 0.0,
+            "max_normal_error": float(np.abs(n1 - n0)[:, contact].max()) if contact.any() else 0.0,
+            "max_depth_error_m": float(np.abs(d - d0)[contact].max()) if contact.any() else 0.0,
+            "bitwise_equal_fraction": float(np.mean(d.view(np.uint64) == d0.view(np.uint64)))}
 
 
 def cpu_baseline(batch, target_s):
@@ -114,8 +166,11 @@ def main():
     from delta_amd import narrow
     n = 1 << a.pairs_log2
     print(f"[bench] generating {n} pairs in HBM", file=sys.stderr, flush=True)
-    batch = narrow.PairBatch(n, seed=3, with_f32=True)
+    batch = narrow.PairBatch(n, seed=3)
     results = {}
+    train_info = None
+    if "learned" in a.methods.split(","):
+        trainer, _, train_info = train_network(a)
     for method in a.methods.split(","):
         try:
             ms, wall = time_method(batch, method, a.steps, a.warmup)
@@ -125,14 +180,12 @@ def main():
         results[method] = {"ms_per_step": ms, "checks_per_s": n / (ms * 1e-3), "wall_s": wall}
         print(f"[bench] {method}: {ms:.3f} ms/pass, {n / (ms * 1e-3):.3e} pairs/s",
               file=sys.stderr, flush=True)
-    head = "learned" if "learned" in results else ("mixed" if "mixed" in results else "bf64")
+    head = "learned" if "learned" in results else "bf64"
     r = results[head]
     bf = results.get("bf64")
-    # roofline of the dominant kernel (one launch per pass)
-    if head == "bf64":
-        bytes_per = narrow.BYTES_F64
-    else:
-        bytes_per = narrow.BYTES_F32_BULK
+    # roofline of the pass: algorithmic bytes per pair (18 FP64 inputs,
+    # P, Q, d, flag) over the measured pass time
+    bytes_per = narrow.BYTES_F64
     achieved = bytes_per * n / (r["ms_per_step"] * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS, "traffic": None,
@@ -151,7 +204,7 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if head == "bf64" else "f32+f64",
+        "dtype": "f64" if head == "bf64" else "bf16+f64",
         "data": "synthetic",
         "config": {"workload": "isolated narrow-phase microbench: 2^24 random triangle pairs, "
                                "1 mm cube, separation log-uniform 1e-7..1e-2 m, 5% near-parallel, "
@@ -162,6 +215,9 @@ def main():
                     for k, v in results.items()},
     }
     out.update(extra)
+    if train_info:
+        out["network"] = train_info
+        out["learned_vs_bf64"] = learned_accuracy(batch)
     if not a.no_cpu:
         cb, chk = cpu_baseline(batch, a.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -386,11 +386,14 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learned(LearnedArgs a) {
     const int64_t i = ch * 64 + lane;
     const bool valid = i < a.n;
     const int64_t ii = valid ? i : 0;
-    const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
-    const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
-    float in[18];
-    const bool frame = canonical_inputs(A, B, in);
-    for (int k = 0; k < kIn; ++k) x_s[wave][lane][k] = (k < kInUsed) ? to_bf16(in[k]) : 0;
+    bool frame;
+    {
+      const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
+      const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
+      float in[18];
+      frame = canonical_inputs(A, B, in);
+      for (int k = 0; k < kIn; ++k) x_s[wave][lane][k] = (k < kInUsed) ? to_bf16(in[k]) : 0;
+    }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
 #pragma unroll 1
@@ -420,6 +423,9 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learned(LearnedArgs a) {
       }
     }
     const bool exact_path = !frame || (o_s[wave][lane][kFlagOut] > 0.f);
+    // the pair again (cache-resident): keeps it out of registers during the network
+    const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
+    const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
     V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
     bool done = false;
     if (valid && !exact_path) {
@@ -493,8 +499,8 @@ __global__ __launch_bounds__(256) void k_queue_bf(const double* __restrict__ soa
 }
 
 struct LearnedState {
-  DevBuf queue, qlen, counters;
-  const float* model = nullptr;
+  DevBuf queue, qlen, counters, model_copy;
+  const float* model = nullptr;  // points into model_copy once set
   unsigned long long last_counters[3] = {0, 0, 0};
 };
 static LearnedState g_learned;
@@ -607,7 +613,13 @@ int dh_mlp_adam(float* params_dev, float* m_dev, float* v_dev, const float* grad
 }
 
 int dh_learned_set_model(const float* params_dev) {
-  g_learned.model = params_dev;
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!params_dev) return set_error(DH_ERR_ARG, "dh_learned_set_model: null");
+  if ((rc = g_learned.model_copy.grow(sizeof(float) * kNumParams))) return rc;
+  DH_HIP_CHECK(hipMemcpy(g_learned.model_copy.ptr, params_dev, sizeof(float) * kNumParams,
+                         hipMemcpyDeviceToDevice));
+  g_learned.model = g_learned.model_copy.as<float>();
   return DH_OK;
 }
 

@@ -1,7 +1,7 @@
 // mlp.hpp — the feature-prediction network of the learned narrow phase.
 //
 // 18 canonical-frame coordinates (padded to 32) -> 64 -> 64 -> 32 outputs,
-// GELU (tanh form) between layers, bf16 weights and activations, FP32
+// GELU (sigmoid form) between layers, bf16 weights and activations, FP32
 // accumulation on MFMA (v_mfma_f32_16x16x16_bf16).  Outputs: [0,15) logits
 // of the closest feature pair (0..8 edge k of A x edge m of B, 9..14 vertex-
 // face), 15 the exact-path logit, [16,20) barycentric parameters of P and Q.
@@ -54,19 +54,15 @@ __host__ __device__ inline float from_bf16(uint16_t h) {
   return f;
 }
 
-// GELU, tanh form, and its derivative.
-__device__ inline float gelu(float x) {
-  const float c = 0.7978845608028654f;  // sqrt(2/pi)
-  const float t = tanhf(c * (x + 0.044715f * x * x * x));
-  return 0.5f * x * (1.0f + t);
+// GELU in its sigmoid form, x * sigmoid(1.702 x) (Hendrycks & Gimpel 2016),
+// with the hardware exp2 and reciprocal, and its derivative.
+__device__ inline float gelu_sig(float x) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-1.702f * x));
 }
-
+__device__ inline float gelu(float x) { return x * gelu_sig(x); }
 __device__ inline float gelu_grad(float x) {
-  const float c = 0.7978845608028654f;
-  const float u = c * (x + 0.044715f * x * x * x);
-  const float t = tanhf(u);
-  const float du = c * (1.0f + 3.0f * 0.044715f * x * x);
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
+  const float s = gelu_sig(x);
+  return s + 1.702f * x * s * (1.0f - s);
 }
 
 __device__ inline v4f mfma16(v4s a, v4s b, v4f c) {

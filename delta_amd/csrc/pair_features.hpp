@@ -83,35 +83,38 @@ DH_HD int bf_feature_code(const Tri<double>& A, const Tri<double>& B, V3<double>
 
 // Canonical frame of a pair (origin at A's centroid, x along A's edge 0->1,
 // z along A's normal, lengths in units of A's longest edge), as 18 floats:
-// A0, A1, A2, B0, B1, B2 (xyz each), clamped to [-32, 32].  Returns false for
-// a degenerate A (no frame).
+// A0, A1, A2, B0, B1, B2 (xyz each), clamped to [-32, 32].  The vertices are
+// taken relative to A0 in binary64 and the frame is built in binary32 (the
+// network reads bf16).  Returns false for a degenerate A (no frame).
 DH_HD bool canonical_inputs(const Tri<double>& A, const Tri<double>& B, float out[18]) {
-  const V3<double> c = {(A.v[0].x + A.v[1].x + A.v[2].x) / 3.0,
-                        (A.v[0].y + A.v[1].y + A.v[2].y) / 3.0,
-                        (A.v[0].z + A.v[1].z + A.v[2].z) / 3.0};
-  const V3<double> e01 = sub(A.v[1], A.v[0]), e12 = sub(A.v[2], A.v[1]),
-                   e20 = sub(A.v[0], A.v[2]);
-  const double l01 = dsqrt(dot(e01, e01));
-  double lmax = l01;
-  const double l12 = dsqrt(dot(e12, e12)), l20 = dsqrt(dot(e20, e20));
-  lmax = l12 > lmax ? l12 : lmax;
-  lmax = l20 > lmax ? l20 : lmax;
-  const V3<double> nz = cross3(e01, sub(A.v[2], A.v[0]));
-  const double ln = dsqrt(dot(nz, nz));
-  const bool ok = (l01 > 0.0) && (ln > 0.0) && (lmax > 0.0);
-  const double il = ok ? 1.0 / l01 : 0.0, in = ok ? 1.0 / ln : 0.0;
-  const V3<double> ex = {e01.x * il, e01.y * il, e01.z * il};
-  const V3<double> ez = {nz.x * in, nz.y * in, nz.z * in};
-  const V3<double> ey = cross3(ez, ex);
-  const double s = ok ? 1.0 / lmax : 0.0;
+  V3<float> r[6];
   for (int v = 0; v < 6; ++v) {
     const V3<double> p = (v < 3) ? A.v[v] : B.v[v - 3];
-    const V3<double> q = {(p.x - c.x) * s, (p.y - c.y) * s, (p.z - c.z) * s};
-    const double r[3] = {dot(q, ex), dot(q, ey), dot(q, ez)};
+    r[v] = {static_cast<float>(p.x - A.v[0].x), static_cast<float>(p.y - A.v[0].y),
+            static_cast<float>(p.z - A.v[0].z)};
+  }
+  const V3<float> c = {(r[1].x + r[2].x) / 3.0f, (r[1].y + r[2].y) / 3.0f,
+                       (r[1].z + r[2].z) / 3.0f};
+  const V3<float> e01 = r[1], e02 = r[2], e12 = sub(r[2], r[1]);
+  const float l01 = dot(e01, e01), l12 = dot(e12, e12), l02 = dot(e02, e02);
+  float lmax = l01 > l12 ? l01 : l12;
+  lmax = lmax > l02 ? lmax : l02;
+  const V3<float> nz = cross3(e01, e02);
+  const float ln = dot(nz, nz);
+  const bool ok = (l01 > 0.f) && (ln > 0.f) && (lmax > 0.f);
+  using std::sqrt;
+  const float il = ok ? 1.0f / sqrt(l01) : 0.f, in = ok ? 1.0f / sqrt(ln) : 0.f;
+  const V3<float> ex = {e01.x * il, e01.y * il, e01.z * il};
+  const V3<float> ez = {nz.x * in, nz.y * in, nz.z * in};
+  const V3<float> ey = cross3(ez, ex);
+  const float s = ok ? 1.0f / sqrt(lmax) : 0.f;
+  for (int v = 0; v < 6; ++v) {
+    const V3<float> q = {(r[v].x - c.x) * s, (r[v].y - c.y) * s, (r[v].z - c.z) * s};
+    const float w[3] = {dot(q, ex), dot(q, ey), dot(q, ez)};
     for (int a = 0; a < 3; ++a) {
-      double x = r[a];
-      x = x > 32.0 ? 32.0 : (x < -32.0 ? -32.0 : x);
-      out[3 * v + a] = static_cast<float>(x);
+      float x = w[a];
+      x = x > 32.f ? 32.f : (x < -32.f ? -32.f : x);
+      out[3 * v + a] = x;
     }
   }
   return ok;

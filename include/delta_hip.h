@@ -198,7 +198,7 @@ int dh_convert_f32(const double* src_dev, float* dst_dev, int64_t n, void* strea
  * bfSolver's own result cannot be certified: intersecting or degenerate).
  * dh_mlp_grad: loss and gradient over pairs [offset, offset+batch) (grad
  * overwritten).  dh_mlp_adam: one Adam step (step >= 1).
- * dh_learned_set_model: the parameters DH_PAIR_LEARNED uses (kept by pointer). */
+ * dh_learned_set_model: copies the parameters DH_PAIR_LEARNED uses. */
 int dh_mlp_param_count(void);
 int dh_mlp_labels(const double* soa_dev, int64_t n, uint16_t* X_dev, uint8_t* feat_dev,
                   float* bary_dev, float* flag_dev, void* stream);

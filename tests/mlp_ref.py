@@ -14,16 +14,18 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
+def _sig(x):
+    return 1.0 / (1.0 + np.exp(-1.702 * x))
+
+
 def gelu(x):
-    c = np.float32(0.7978845608028654)
-    return 0.5 * x * (1 + np.tanh(c * (x + 0.044715 * x ** 3)))
+    """x * sigmoid(1.702 x), the kernel's GELU."""
+    return x * _sig(x)
 
 
 def gelu_grad(x):
-    c = 0.7978845608028654
-    u = c * (x + 0.044715 * x ** 3)
-    t = np.tanh(u)
-    return 0.5 * (1 + t) + 0.5 * x * (1 - t * t) * c * (1 + 3 * 0.044715 * x * x)
+    s = _sig(x)
+    return s + 1.702 * x * s * (1 - s)
 
 
 def unpack(p):
