@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_labels(const double* __restrict__ soa, 
   float in[18];
   const bool frame = canonical_inputs(A, B, in);
   uint16_t* x = X + i * kIn;
-  for (int k = 0; k < kIn; ++k) x[k] = (k < kInUsed) ? to_bf16(in[k]) : 0;
+  for (int k = 0; k < kIn; ++k) x[k] = (k < kInUsed) ? bf16_bits(in[k]) : 0;
   V3<double> P, Q;
   const int code = bf_feature_code(A, B, P, Q);
   feat[i] = static_cast<uint8_t>(code);
@@ -216,8 +216,8 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs a) {
       for (int to = 0; to < 4; ++to)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          a1_s[to * 16 + 4 * g + j][pl] = to_bf16(gelu(z1[to][j]));
-          a2_s[to * 16 + 4 * g + j][pl] = to_bf16(gelu(z2[to][j]));
+          a1_s[to * 16 + 4 * g + j][pl] = bf16_bits(gelu(z1[to][j]));
+          a2_s[to * 16 + 4 * g + j][pl] = bf16_bits(gelu(z2[to][j]));
         }
       // loss and dL/dz3 (tile 0: logits 0..14 and the flag 15; tile 1: 16..19)
       const int64_t gp = a.offset + (valid ? p : 0);
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs a) {
       for (int tn = 0; tn < 2; ++tn) {
         dz3b[tn] = pack4(dz3[tn][0], dz3[tn][1], dz3[tn][2], dz3[tn][3]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) d3_s[tn * 16 + 4 * g + j][pl] = to_bf16(dz3[tn][j]);
+        for (int j = 0; j < 4; ++j) d3_s[tn * 16 + 4 * g + j][pl] = bf16_bits(dz3[tn][j]);
       }
       v4s dz2b[4];
 #pragma unroll
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           d[j] = acc[j] * gelu_grad(z2[tu][j]);
-          d2_s[tu * 16 + 4 * g + j][pl] = to_bf16(d[j]);
+          d2_s[tu * 16 + 4 * g + j][pl] = bf16_bits(d[j]);
         }
         dz2b[tu] = pack4(d[0], d[1], d[2], d[3]);
       }
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs a) {
         for (int tn = 0; tn < 4; ++tn) acc = mfma16(w2b[tu][tn], dz2b[tn], acc);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          d1_s[tu * 16 + 4 * g + j][pl] = to_bf16(acc[j] * gelu_grad(z1[tu][j]));
+          d1_s[tu * 16 + 4 * g + j][pl] = bf16_bits(acc[j] * gelu_grad(z1[tu][j]));
       }
     }
     __syncthreads();
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learned(LearnedArgs a) {
       const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
       float in[18];
       frame = canonical_inputs(A, B, in);
-      for (int k = 0; k < kIn; ++k) x_s[wave][lane][k] = (k < kInUsed) ? to_bf16(in[k]) : 0;
+      for (int k = 0; k < kIn; ++k) x_s[wave][lane][k] = (k < kInUsed) ? bf16_bits(in[k]) : 0;
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();

@@ -1,7 +1,7 @@
 // mlp.hpp — the feature-prediction network of the learned narrow phase.
 //
 // 18 canonical-frame coordinates (padded to 32) -> 64 -> 64 -> 32 outputs,
-// GELU (sigmoid form) between layers, bf16 weights and activations, FP32
+// GELU (hard-sigmoid form) between layers, bf16 weights and activations, FP32
 // accumulation on MFMA (v_mfma_f32_16x16x16_bf16).  Outputs: [0,15) logits
 // of the closest feature pair (0..8 edge k of A x edge m of B, 9..14 vertex-
 // face), 15 the exact-path logit, [16,20) barycentric parameters of P and Q.
@@ -54,28 +54,34 @@ __host__ __device__ inline float from_bf16(uint16_t h) {
   return f;
 }
 
-// GELU in its sigmoid form, x * sigmoid(1.702 x) (Hendrycks & Gimpel 2016),
-// with the hardware exp2 and reciprocal, and its derivative.
-__device__ inline float gelu_sig(float x) {
-  return __builtin_amdgcn_rcpf(1.0f + __expf(-1.702f * x));
+// GELU in the hard-sigmoid form x * clamp(1.702 x / 6 + 1/2, 0, 1): the
+// sigmoid of x * sigmoid(1.702 x) (Hendrycks & Gimpel 2016) replaced by its
+// piecewise-linear hard-sigmoid, one fma, one med3 and one mul per unit.
+constexpr float kHardA = 1.702f / 6.0f;
+__device__ inline float gelu(float x) {
+  return x * __builtin_amdgcn_fmed3f(kHardA * x + 0.5f, 0.0f, 1.0f);
 }
-__device__ inline float gelu(float x) { return x * gelu_sig(x); }
 __device__ inline float gelu_grad(float x) {
-  const float s = gelu_sig(x);
-  return s + 1.702f * x * s * (1.0f - s);
+  const float t = kHardA * x + 0.5f;
+  return (t <= 0.0f) ? 0.0f : ((t >= 1.0f) ? 1.0f : 2.0f * kHardA * x + 0.5f);
 }
 
 __device__ inline v4f mfma16(v4s a, v4s b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+// four FP32 -> bf16 (round to nearest even, v_cvt_pk_bf16_f32)
 __device__ inline v4s pack4(float a, float b, float c, float d) {
-  v4s r;
-  r[0] = static_cast<short>(to_bf16(a));
-  r[1] = static_cast<short>(to_bf16(b));
-  r[2] = static_cast<short>(to_bf16(c));
-  r[3] = static_cast<short>(to_bf16(d));
-  return r;
+  const v4bf t = {static_cast<__bf16>(a), static_cast<__bf16>(b), static_cast<__bf16>(c),
+                  static_cast<__bf16>(d)};
+  return __builtin_bit_cast(v4s, t);
+}
+
+__device__ inline uint16_t bf16_bits(float f) {
+  const __bf16 h = static_cast<__bf16>(f);
+  return __builtin_bit_cast(uint16_t, h);
 }
 
 // A-operand tile of W^T for out-tile `to`, k-tile `tk` of a [K][N] weight:

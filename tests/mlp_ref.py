@@ -14,18 +14,17 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
-def _sig(x):
-    return 1.0 / (1.0 + np.exp(-1.702 * x))
+HARD_A = np.float32(1.702) / np.float32(6.0)
 
 
 def gelu(x):
-    """x * sigmoid(1.702 x), the kernel's GELU."""
-    return x * _sig(x)
+    """x * clamp(1.702 x / 6 + 1/2, 0, 1), the kernel's GELU."""
+    return x * np.clip(HARD_A * x + 0.5, 0.0, 1.0)
 
 
 def gelu_grad(x):
-    s = _sig(x)
-    return s + 1.702 * x * s * (1 - s)
+    t = HARD_A * x + 0.5
+    return np.where(t <= 0, 0.0, np.where(t >= 1, 1.0, 2 * HARD_A * x + 0.5))
 
 
 def unpack(p):
