@@ -1,17 +1,19 @@
 // learned.hip — the learned narrow phase: labels from the FP64 brute force,
 // MFMA training of the feature network, and the fused inference kernel.
 //
-// Inference per pair: canonical-frame inputs -> network (bf16 MFMA) -> the two
-// most likely closest features -> each evaluated in binary64 with bfSolver's
-// arithmetic and checked by the optimality certificate (pair_features.hpp).
-// Certified pairs are finished; the others (and those the network marks for
-// the exact path) are compacted into a queue that the FP64 brute-force kernel
-// drains.  The network therefore decides only how much work a pair costs;
+// Inference per pair: canonical-frame inputs -> network (bf16 MFMA) -> the
+// most likely edge-edge and the most likely vertex-face feature -> both
+// evaluated in binary64 with bfSolver's arithmetic, the nearer checked by the
+// optimality certificate (pair_features.hpp).  Certified pairs are finished;
+// the others (and those the network marks for the exact path) are compacted
+// into per-workgroup queues that the FP64 brute-force kernel drains.  The network therefore decides only how much work a pair costs;
 // every distance is either certified within 8e-7 relative or bfSolver's own.
 #include "exact_math.hpp"
 #include "mlp.hpp"
 #include "pair_features.hpp"
 #include "runtime.hpp"
+
+#include <vector>
 
 namespace dh {
 
@@ -357,7 +359,12 @@ __global__ void k_adam(float* __restrict__ W, float* __restrict__ m, float* __re
 }
 
 // ------------------------------------------------------------- inference
-constexpr int kInferWaves = 4;
+// Workgroup of kInferWaves waves; the network's A-operand tiles and biases
+// live once per workgroup in LDS, so registers hold only the pair being
+// certified and the layer accumulators.
+constexpr int kInferWaves = 8;
+constexpr int kTiles = 8 + 16 + 8;  // W1, W2, W3 A-operand tiles
+constexpr int kCol = 68;             // row stride (dwords) of the per-wave staging
 
 struct LearnedArgs {
   const double* soa;
@@ -367,19 +374,48 @@ struct LearnedArgs {
   double* Q;
   double* d;
   uint8_t* flags;
-  int* queue;               // indices for the FP64 brute force
-  unsigned long long* qlen;
-  unsigned long long* counters;  // [0] certified, [1] features evaluated
+  // Pairs for the brute force: workgroup b appends to its own region
+  // [b * region, (b + 1) * region) of the queue (original index) and of qsoa
+  // (coordinates, 18 x qcap, region-major) through a counter in LDS; a
+  // single device-wide append counter would serialise every wave on one
+  // address across the eight XCDs.
+  int* queue;
+  double* qsoa;
+  int64_t qcap;
+  int64_t region;
+  unsigned* wg_counts;  // per workgroup: certified, features evaluated, queued
 };
 
-__global__ __launch_bounds__(64 * kInferWaves) void k_learned(LearnedArgs a) {
-  __shared__ uint16_t x_s[kInferWaves][64][kIn];
-  __shared__ float o_s[kInferWaves][64][16];
+__device__ inline v4s lds_tile(const v4s* tiles, int t, int lane) { return tiles[t * 64 + lane]; }
+
+__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_learned(LearnedArgs a) {
+  __shared__ v4s w_s[kTiles * 64];
+  __shared__ float b_s[2 * kHid + kOut];
+  __shared__ unsigned wg_tally[3];  // certified, features evaluated, queued
+  // Per wave, one [16][kCol] dword array in column layout (column = pair):
+  // first the inputs as 9 rows of bf16 pairs (inputs 2r, 2r+1), then, for
+  // the same columns, the 16 outputs as FP32 rows.  A column's inputs are
+  // read by the sub-tile that later writes its outputs.  kCol = 64 + 4 spreads
+  // per-lane rows and the MFMA operand/result patterns over all 32 banks
+  // (two lanes per bank, the minimum for 64 lanes).
+  __shared__ uint32_t io_s[kInferWaves][16 * kCol];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  FwdWeights fw;
-  load_fwd(a.W, fw, lane);
-  unsigned long long cert_count = 0, evals = 0;
+  // stage the network once per workgroup
+  for (int t = wave; t < kTiles; t += kInferWaves) {
+    v4s v;
+    if (t < 8) v = load_wt(a.W + kW1, kHid, t / 2, t % 2, lane);
+    else if (t < 24) v = load_wt(a.W + kW2, kHid, (t - 8) / 4, (t - 8) % 4, lane);
+    else v = load_wt(a.W + kW3, kOut, (t - 24) / 4, (t - 24) % 4, lane);
+    w_s[t * 64 + lane] = v;
+  }
+  for (int i = threadIdx.x; i < 2 * kHid + kOut; i += 64 * kInferWaves)
+    b_s[i] = (i < kHid) ? a.W[kB1 + i] : (i < 2 * kHid) ? a.W[kB2 + i - kHid]
+                                                          : a.W[kB3 + i - 2 * kHid];
+  if (threadIdx.x < 3) wg_tally[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned cert_count = 0, evals = 0;  // per lane
+  const int64_t qbase = static_cast<int64_t>(blockIdx.x) * a.region;
   const int64_t nchunks = (a.n + 63) / 64;
   for (int64_t ch = static_cast<int64_t>(blockIdx.x) * kInferWaves + wave; ch < nchunks;
        ch += static_cast<int64_t>(gridDim.x) * kInferWaves) {
@@ -392,53 +428,104 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learned(LearnedArgs a) {
       const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
       float in[18];
       frame = canonical_inputs(A, B, in);
-      for (int k = 0; k < kIn; ++k) x_s[wave][lane][k] = (k < kInUsed) ? bf16_bits(in[k]) : 0;
+      uint32_t* col = &io_s[wave][lane];
+#pragma unroll
+      for (int r = 0; r < 9; ++r)
+        col[r * kCol] = static_cast<uint32_t>(bf16_bits(in[2 * r])) |
+                        (static_cast<uint32_t>(bf16_bits(in[2 * r + 1])) << 16);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
 #pragma unroll 1
     for (int s = 0; s < 4; ++s) {
       const int pl = s * 16 + c;
-      v4s xb[2] = {x_operand(&x_s[wave][pl][0], 0, lane), x_operand(&x_s[wave][pl][0], 1, lane)};
-      v4f z1[4], z2[4], z3[2];
-      forward16(fw, a.W, xb, lane, z1, z2, z3);
+      // opaque per sub-tile: keeps the compiler from hoisting the weight and
+      // bias loads out of the loops into ~100 registers
+      int wl = lane, bl = 4 * g;
+      asm volatile("" : "+v"(wl), "+v"(bl));
+      // B operands: lane (c, g) needs inputs 4g..4g+3 (k-tile 0) and
+      // 16+4g..19+4g (k-tile 1: only inputs 16, 17 are nonzero)
+      v4s x0, x1 = {0, 0, 0, 0};
+      {
+        const uint32_t d0 = io_s[wave][(2 * g) * kCol + pl];
+        const uint32_t d1 = io_s[wave][(2 * g + 1) * kCol + pl];
+        const uint32_t d8 = io_s[wave][8 * kCol + pl];
+        x0 = {static_cast<short>(d0 & 0xffffu), static_cast<short>(d0 >> 16),
+              static_cast<short>(d1 & 0xffffu), static_cast<short>(d1 >> 16)};
+        if (g == 0) {
+          x1[0] = static_cast<short>(d8 & 0xffffu);
+          x1[1] = static_cast<short>(d8 >> 16);
+        }
+      }
+      v4s act1[4], act2[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) o_s[wave][pl][4 * g + j] = z3[0][j];
+      for (int to = 0; to < 4; ++to) {
+        v4f acc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = b_s[to * 16 + bl + j];
+        acc = mfma16(lds_tile(w_s, 2 * to, wl), x0, acc);
+        acc = mfma16(lds_tile(w_s, 2 * to + 1, wl), x1, acc);
+        act1[to] = pack4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
+      }
+#pragma unroll
+      for (int to = 0; to < 4; ++to) {
+        v4f acc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = b_s[kHid + to * 16 + bl + j];
+#pragma unroll
+        for (int tk = 0; tk < 4; ++tk) acc = mfma16(lds_tile(w_s, 8 + 4 * to + tk, wl), act1[tk], acc);
+        act2[to] = pack4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
+      }
+      {  // output tile 0: feature logits and the exact-path logit
+        v4f acc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = b_s[2 * kHid + bl + j];
+#pragma unroll
+        for (int tk = 0; tk < 4; ++tk) acc = mfma16(lds_tile(w_s, 24 + tk, wl), act2[tk], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) io_s[wave][(4 * g + j) * kCol + pl] = __float_as_uint(acc[j]);
+      }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    // the lane's own pair: two best features, then the certificate
-    int f1 = 0, f2 = 1;
-    float l1 = -3.0e38f, l2 = -3.0e38f;
+    // the lane's own pair: the best-ranked edge-edge and the best-ranked
+    // vertex-face feature, both evaluated (no divergence between the two
+    // kinds), the nearer one certified
+    int fe = 0, fv = 9;
+    float le = -3.0e38f, lv = -3.0e38f;
+#pragma unroll
     for (int f = 0; f < kNumFeat; ++f) {
-      const float v = o_s[wave][lane][f];
-      if (v > l1) {
-        l2 = l1;
-        f2 = f1;
-        l1 = v;
-        f1 = f;
-      } else if (v > l2) {
-        l2 = v;
-        f2 = f;
+      const float v = __uint_as_float(io_s[wave][f * kCol + lane]);
+      if (f < 9) {
+        fe = v > le ? f : fe;
+        le = v > le ? v : le;
+      } else {
+        fv = v > lv ? f : fv;
+        lv = v > lv ? v : lv;
       }
     }
-    const bool exact_path = !frame || (o_s[wave][lane][kFlagOut] > 0.f);
+    const bool exact_path = !frame || (__uint_as_float(io_s[wave][kFlagOut * kCol + lane]) > 0.f);
     // the pair again (cache-resident): keeps it out of registers during the network
     const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
     const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
-    V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
+    V3<double> P, Q;
     bool done = false;
-    if (valid && !exact_path) {
-      eval_feature(A, B, f1, P, Q);
-      evals++;
-      done = certify(A, B, P, Q);
-    }
-    if (__ballot(valid && !exact_path && !done)) {
-      if (valid && !exact_path && !done) {
-        eval_feature(A, B, f2, P, Q);
-        evals++;
-        done = certify(A, B, P, Q);
-      }
+    {
+      V3<double> Pv, Qv;
+      eval_ee(A, B, fe, P, Q);
+      const V3<double> ne = sub(Q, P);
+      const double de = dot(ne, ne);
+      // one feature at a time: an empty asm makes the vertex-face code depend
+      // on the edge-edge result, so the two evaluations are not interleaved
+      // (which would need ~150 registers instead of 128)
+      asm volatile("" : "+v"(fv) : "v"(de));
+      eval_vf(A, B, fv, Pv, Qv);
+      const V3<double> nv = sub(Qv, Pv);
+      const bool take_v = dot(nv, nv) < de;
+      P = sel3(take_v, Pv, P);
+      Q = sel3(take_v, Qv, Q);
+      done = valid && !exact_path && certify(A, B, P, Q);
+      evals += (valid && !exact_path) ? 2 : 0;
     }
     if (valid && done) {
       a.P[i] = P.x;
@@ -451,16 +538,30 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learned(LearnedArgs a) {
       if (a.flags) a.flags[i] = DH_FLAG_CERTIFIED;
       cert_count++;
     }
-    // queue the rest for the FP64 brute force (wave-aggregated append)
+    // queue the rest for the FP64 brute force: wave-aggregated append of the
+    // index and the coordinates (contiguous, so the brute force reads them
+    // coalesced)
     const bool need = valid && !done;
     const unsigned long long mask = __ballot(need);
     if (mask) {
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(a.qlen, static_cast<unsigned long long>(__popcll(mask)));
+      unsigned base = 0;
+      if (lane == 0) base = atomicAdd(&wg_tally[2], static_cast<unsigned>(__popcll(mask)));
       base = __shfl(base, 0);
       const int pre = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
                                                 __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0));
-      if (need) a.queue[base + pre] = static_cast<int>(i);
+      if (need) {
+        const int64_t q = qbase + base + pre;
+        a.queue[q] = static_cast<int>(i);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          a.qsoa[(0 + k) * a.qcap + q] = A.v[k].x;
+          a.qsoa[(3 + k) * a.qcap + q] = A.v[k].y;
+          a.qsoa[(6 + k) * a.qcap + q] = A.v[k].z;
+          a.qsoa[(9 + k) * a.qcap + q] = B.v[k].x;
+          a.qsoa[(12 + k) * a.qcap + q] = B.v[k].y;
+          a.qsoa[(15 + k) * a.qcap + q] = B.v[k].z;
+        }
+      }
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -468,40 +569,46 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learned(LearnedArgs a) {
     evals += __shfl_xor(evals, o);
   }
   if (lane == 0) {
-    atomicAdd(&a.counters[0], cert_count);
-    atomicAdd(&a.counters[1], evals);
+    atomicAdd(&wg_tally[0], cert_count);
+    atomicAdd(&wg_tally[1], evals);
   }
+  __syncthreads();
+  if (threadIdx.x < 3) a.wg_counts[3 * blockIdx.x + threadIdx.x] = wg_tally[threadIdx.x];
 }
 
-// FP64 brute force over the queued pairs (full lane occupancy).
-__global__ __launch_bounds__(256) void k_queue_bf(const double* __restrict__ soa, int64_t n,
-                                                  const int* __restrict__ queue,
-                                                  const unsigned long long* __restrict__ qlen,
-                                                  double* __restrict__ P, double* __restrict__ Q,
-                                                  double* __restrict__ d,
+// FP64 brute force over the queued pairs: workgroup b takes the region that
+// k_learned's workgroup b filled, reading the compacted coordinates
+// (coalesced, all lanes busy but the region's tail).
+__global__ __launch_bounds__(256) void k_queue_bf(const double* __restrict__ qsoa, int64_t qcap,
+                                                  int64_t region, const int* __restrict__ queue,
+                                                  const unsigned* __restrict__ wg_counts,
+                                                  int64_t n, double* __restrict__ P,
+                                                  double* __restrict__ Q, double* __restrict__ d,
                                                   uint8_t* __restrict__ flags) {
-  const unsigned long long len = *qlen;
-  for (unsigned long long t = static_cast<unsigned long long>(blockIdx.x) * 256 + threadIdx.x;
-       t < len; t += static_cast<unsigned long long>(gridDim.x) * 256) {
-    const int64_t i = queue[t];
-    const Tri<double> A = load_tri_soa(soa, n, i, 0), B = load_tri_soa(soa, n, i, 9);
-    V3<double> p, q;
-    const bool edge = bf_solver(A, B, p, q);
+  const unsigned len = wg_counts[3 * blockIdx.x + 2];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * region;
+  for (unsigned t = threadIdx.x; t < len; t += 256) {
+    const int64_t q = base + t;
+    const int64_t i = queue[q];
+    const Tri<double> A = load_tri_soa(qsoa, qcap, q, 0);
+    const Tri<double> B = load_tri_soa(qsoa, qcap, q, 9);
+    V3<double> p, qq;
+    const bool edge = bf_solver(A, B, p, qq);
     P[i] = p.x;
     P[n + i] = p.y;
     P[2 * n + i] = p.z;
-    Q[i] = q.x;
-    Q[n + i] = q.y;
-    Q[2 * n + i] = q.z;
-    d[i] = pq_dist(p, q);
+    Q[i] = qq.x;
+    Q[n + i] = qq.y;
+    Q[2 * n + i] = qq.z;
+    d[i] = pq_dist(p, qq);
     if (flags) flags[i] = DH_FLAG_RECHECK | (edge ? DH_FLAG_EDGE : 0);
   }
 }
 
 struct LearnedState {
-  DevBuf queue, qlen, counters, model_copy;
+  DevBuf queue, qsoa, wg_counts, model_copy;
   const float* model = nullptr;  // points into model_copy once set
-  unsigned long long last_counters[3] = {0, 0, 0};
+  unsigned grid = 0;              // workgroups of the last launch
 };
 static LearnedState g_learned;
 
@@ -509,37 +616,40 @@ int learned_pairs(const double* soa, int64_t n, double* P, double* Q, double* d,
                   hipStream_t st) {
   if (!g_learned.model)
     return set_error(DH_ERR_ARG, "learned narrow phase: no network set (dh_learned_set_model)");
-  int rc;
-  if ((rc = g_learned.queue.grow(sizeof(int) * (n + 1)))) return rc;
-  if ((rc = g_learned.qlen.grow(sizeof(unsigned long long)))) return rc;
-  if ((rc = g_learned.counters.grow(sizeof(unsigned long long) * 4))) return rc;
-  DH_HIP_CHECK(hipMemsetAsync(g_learned.qlen.ptr, 0, sizeof(unsigned long long), st));
-  DH_HIP_CHECK(hipMemsetAsync(g_learned.counters.ptr, 0, sizeof(unsigned long long) * 4, st));
-  LearnedArgs a{soa, n, g_learned.model, P, Q, d, flags, g_learned.queue.as<int>(),
-                g_learned.qlen.as<unsigned long long>(),
-                g_learned.counters.as<unsigned long long>()};
   const int64_t chunks = (n + 63) / 64;
   int64_t grid = (chunks + kInferWaves - 1) / kInferWaves;
-  if (grid > 256 * 16) grid = 256 * 16;  // persistent: 16 workgroups per CU
-  hipLaunchKernelGGL(k_learned, static_cast<unsigned>(grid), 64 * kInferWaves, 0, st, a);
-  int64_t qgrid = (n + 255) / 256;
-  if (qgrid > 256 * 16) qgrid = 256 * 16;
-  hipLaunchKernelGGL(k_queue_bf, static_cast<unsigned>(qgrid), 256, 0, st, soa, n,
-                     g_learned.queue.as<int>(), g_learned.qlen.as<unsigned long long>(), P, Q, d,
-                     flags);
+  if (grid > 256 * 8) grid = 256 * 8;  // persistent: 8 workgroups per CU at most
+  if (grid < 1) grid = 1;
+  // every wave of a workgroup takes at most `per_wave` chunks
+  const int64_t per_wave = (chunks + grid * kInferWaves - 1) / (grid * kInferWaves);
+  const int64_t region = per_wave * kInferWaves * 64;
+  const int64_t qcap = grid * region;
+  int rc;
+  if ((rc = g_learned.queue.grow(sizeof(int) * qcap))) return rc;
+  if ((rc = g_learned.qsoa.grow(sizeof(double) * 18 * qcap))) return rc;
+  if ((rc = g_learned.wg_counts.grow(sizeof(unsigned) * 3 * grid))) return rc;
+  g_learned.grid = static_cast<unsigned>(grid);
+  LearnedArgs a{soa, n, g_learned.model, P, Q, d, flags, g_learned.queue.as<int>(),
+                g_learned.qsoa.as<double>(), qcap, region, g_learned.wg_counts.as<unsigned>()};
+  if (n > 0) {
+    hipLaunchKernelGGL(k_learned, static_cast<unsigned>(grid), 64 * kInferWaves, 0, st, a);
+    hipLaunchKernelGGL(k_queue_bf, static_cast<unsigned>(grid), 256, 0, st,
+                       g_learned.qsoa.as<double>(), qcap, region, g_learned.queue.as<int>(),
+                       g_learned.wg_counts.as<unsigned>(), n, P, Q, d, flags);
+  }
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
 }
 
+// Totals of the last learned pass: certified, features evaluated, queued.
 int learned_counters(unsigned long long out[3]) {
-  unsigned long long h[2] = {0, 0}, q = 0;
-  if (g_learned.counters.ptr) {
-    DH_HIP_CHECK(hipMemcpy(h, g_learned.counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
-    DH_HIP_CHECK(hipMemcpy(&q, g_learned.qlen.ptr, sizeof(q), hipMemcpyDeviceToHost));
-  }
-  out[0] = h[0];
-  out[1] = h[1];
-  out[2] = q;
+  out[0] = out[1] = out[2] = 0;
+  if (!g_learned.wg_counts.ptr || g_learned.grid == 0) return DH_OK;
+  std::vector<unsigned> h(3 * static_cast<size_t>(g_learned.grid));
+  DH_HIP_CHECK(hipMemcpy(h.data(), g_learned.wg_counts.ptr, sizeof(unsigned) * h.size(),
+                         hipMemcpyDeviceToHost));
+  for (size_t b = 0; b < g_learned.grid; ++b)
+    for (int k = 0; k < 3; ++k) out[k] += h[3 * b + k];
   return DH_OK;
 }
 

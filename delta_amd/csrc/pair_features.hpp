@@ -28,10 +28,10 @@ DH_HD V3<T> cross3(const V3<T>& a, const V3<T>& b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-// Closest points of feature `code` (0..14).
-DH_HD void eval_feature(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
-                        V3<double>& Q) {
-  if (code < 9) {
+// Closest points of edge-edge feature `code` (0..8).
+DH_HD void eval_ee(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+                   V3<double>& Q) {
+  {
     const int k = code / 3, m = code - 3 * (code / 3);
     const V3<double> ha = pick3(A, k), oa = pick3(A, k == 2 ? 0 : k + 1);
     const V3<double> hb = pick3(B, m), ob = pick3(B, m == 2 ? 0 : m + 1);
@@ -41,8 +41,12 @@ DH_HD void eval_feature(const Tri<double>& A, const Tri<double>& B, int code, V3
     segseg(a, b, c, d, e, sc, tc);
     P = {oa.x + (u.x * sc), oa.y + (u.y * sc), oa.z + (u.z * sc)};
     Q = {ob.x + (v.x * tc), ob.y + (v.y * tc), ob.z + (v.z * tc)};
-    return;
   }
+}
+
+// Closest points of vertex-face feature `code` (9..14).
+DH_HD void eval_vf(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+                   V3<double>& Q) {
   const int j = code - 9;
   const bool onA = j < 3;
   const V3<double> f0 = sel3(onA, A.v[0], B.v[0]);
@@ -57,6 +61,15 @@ DH_HD void eval_feature(const Tri<double>& A, const Tri<double>& B, int code, V3
                           f0.z + ((f2.z - f0.z) * t) + ((f1.z - f0.z) * s)};
   P = sel3(onA, onf, pt);
   Q = sel3(onA, pt, onf);
+}
+
+// Closest points of feature `code` (0..14).
+DH_HD void eval_feature(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+                        V3<double>& Q) {
+  if (code < 9)
+    eval_ee(A, B, code, P, Q);
+  else
+    eval_vf(A, B, code, P, Q);
 }
 
 DH_HD bool certify(const Tri<double>& A, const Tri<double>& B, const V3<double>& P,
@@ -81,6 +94,16 @@ DH_HD int bf_feature_code(const Tri<double>& A, const Tri<double>& B, V3<double>
   return edge ? ssid : 9 + id;
 }
 
+// 1/sqrt(x) for the canonical frame, whose coordinates are rounded to bf16:
+// the hardware approximation (v_rsq_f32, about 1 ulp) on the GPU.
+DH_HD float rsqrt_frame(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / std::sqrt(x);
+#endif
+}
+
 // Canonical frame of a pair (origin at A's centroid, x along A's edge 0->1,
 // z along A's normal, lengths in units of A's longest edge), as 18 floats:
 // A0, A1, A2, B0, B1, B2 (xyz each), clamped to [-32, 32].  The vertices are
@@ -102,12 +125,11 @@ DH_HD bool canonical_inputs(const Tri<double>& A, const Tri<double>& B, float ou
   const V3<float> nz = cross3(e01, e02);
   const float ln = dot(nz, nz);
   const bool ok = (l01 > 0.f) && (ln > 0.f) && (lmax > 0.f);
-  using std::sqrt;
-  const float il = ok ? 1.0f / sqrt(l01) : 0.f, in = ok ? 1.0f / sqrt(ln) : 0.f;
+  const float il = ok ? rsqrt_frame(l01) : 0.f, in = ok ? rsqrt_frame(ln) : 0.f;
   const V3<float> ex = {e01.x * il, e01.y * il, e01.z * il};
   const V3<float> ez = {nz.x * in, nz.y * in, nz.z * in};
   const V3<float> ey = cross3(ez, ex);
-  const float s = ok ? 1.0f / sqrt(lmax) : 0.f;
+  const float s = ok ? rsqrt_frame(lmax) : 0.f;
   for (int v = 0; v < 6; ++v) {
     const V3<float> q = {(r[v].x - c.x) * s, (r[v].y - c.y) * s, (r[v].z - c.z) * s};
     const float w[3] = {dot(q, ex), dot(q, ey), dot(q, ez)};

@@ -63,3 +63,81 @@ def test_dynamics_match_oracle(hc):
         hc.hc_update_rotation(ol.dptr(w1), ol.dptr(W), ol.dptr(R1), h)
         orc.or_update_rotation_matrix(ol.dptr(w2), ol.dptr(W), ol.dptr(R2), h)
         assert np.array_equal(bits(R1), bits(R2)) and np.array_equal(bits(w1), bits(w2))
+
+
+def _separated_and_crossing(rng, m):
+    """Pairs separated by A's plane (B lifted along A's normal to a minimum
+    height log-uniform in [1e-7, 1e-3]; a quarter with B's edge nearly
+    parallel to one of A's) and pairs where an edge of B pierces A's
+    interior."""
+    A = rng.uniform(0, 1e-3, (m, 3, 3))
+    nA = np.cross(A[:, 1] - A[:, 0], A[:, 2] - A[:, 0])
+    nA /= np.linalg.norm(nA, axis=1)[:, None]
+    B = rng.uniform(0, 1e-3, (m, 3, 3))
+    k = m // 4
+    B[:k, 1] = A[:k, 1] + rng.normal(size=(k, 3)) * 1e-10
+    B[:k, 2] = A[:k, 2]
+    h = np.einsum("mvk,mk->mv", B - A[:, :1], nA)
+    lift = 10 ** rng.uniform(-7, -3, m) - h.min(axis=1)
+    B = B + (lift[:, None] * nA)[:, None, :]
+    c = A.mean(axis=1)
+    X = np.empty_like(A)
+    hh = 10 ** rng.uniform(-7, -4, m)
+    X[:, 0] = c + hh[:, None] * nA
+    X[:, 1] = c - hh[:, None] * nA
+    X[:, 2] = rng.uniform(0, 1e-3, (m, 3))
+    return A, B, X
+
+
+def test_feature_evaluation_and_certificate(hc):
+    """pair_features.hpp (the learned path's FP64 stage), host build, against
+    the oracle's bfSolver (bf.cpp:409-1100): evaluating bfSolver's winning
+    feature reproduces its closest points bit for bit; the certificate accepts
+    a feature only when its distance is within 8e-7 of the true distance,
+    accepts bfSolver's own answer on separated pairs and never accepts a pair
+    of intersecting triangles."""
+    g = np.load(os.path.join(ol.ROOT, "tests", "golden", "pair_solvers.npz"))
+    rng = np.random.default_rng(5)
+    m = 8000
+    As, Bs, Xs = _separated_and_crossing(rng, m)
+    A = np.concatenate([g["A"], As, As])
+    B = np.concatenate([g["B"], Bs, Xs])
+    ng = len(g["A"])
+    sep = np.zeros(len(A), bool)
+    sep[ng:ng + m] = True
+    cross = np.zeros(len(A), bool)
+    cross[ng + m:] = True
+    s = soa_of(A, B)
+    n = len(A)
+    code = np.zeros(n, np.int32); same = np.zeros(n, np.int32); cbf = np.zeros(n, np.int32)
+    call = np.zeros(15 * n, np.int32); dall = np.zeros(15 * n); dbf = np.zeros(n)
+    hc.hc_learned_probe(ol.dptr(s), C.c_long(n), *(x.ctypes.data_as(C.c_void_p) for x in
+                                                   (code, same, cbf, call, dall, dbf)))
+    P = np.zeros(3 * n); Q = np.zeros(3 * n); d = np.zeros(n); f = np.zeros(n, np.uint8)
+    ol.oracle().or_tri_pairs(0, ol.dptr(s), C.c_long(n), C.c_double(2e-4), ol.dptr(P),
+                             ol.dptr(Q), ol.dptr(d), f.ctypes.data_as(C.c_void_p))
+    call = call.reshape(n, 15).astype(bool)
+    dall = dall.reshape(n, 15)
+    assert same.all()
+    assert np.array_equal(bits(dbf), bits(d))
+    assert ((code >= 0) & (code < 15)).all()
+    # a feature bfSolver never picks can be undefined (NaN on degenerate
+    # parallel edges); it never certifies
+    nan = np.isnan(dall)
+    assert not call[nan].any()
+    # every feature's distance is a distance between points of the triangles
+    # (bfSolver ranks features by its own rounded intermediates: on near-ties
+    # another feature can come out smaller, by 1.8e-10 relative at most here,
+    # on a nearly parallel edge pair)
+    assert (dall[~nan] >= np.broadcast_to(d[:, None], dall.shape)[~nan] * (1 - 1e-9)).all()
+    # soundness on separated pairs (bfSolver's distance is the true one there
+    # up to the near-ties above)
+    rel = (dall - d[:, None]) / np.where(d > 0, d, 1.0)[:, None]
+    assert (rel[sep][call[sep]] <= 8e-7).all()
+    # completeness: bfSolver's own answer certifies on separated pairs except
+    # on some near-ties of nearly parallel edges, where a neighbouring feature
+    # does
+    assert cbf[sep].mean() > 0.99
+    assert call[sep].any(axis=1).mean() > 0.999
+    # intersecting triangles: no feature ever certifies
+    assert not call[cross].any()
