@@ -94,7 +94,7 @@ __device__ inline void forward16(const FwdWeights& fw, const float* __restrict__
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] += W[kB1 + to * 16 + 4 * g + j];
     z1[to] = acc;
-    a1[to] = pack4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
+    a1[to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
   }
 #pragma unroll
   for (int to = 0; to < 4; ++to) {
@@ -104,7 +104,7 @@ __device__ inline void forward16(const FwdWeights& fw, const float* __restrict__
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] += W[kB2 + to * 16 + 4 * g + j];
     z2[to] = acc;
-    a2[to] = pack4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
+    a2[to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
   }
 #pragma unroll
   for (int to = 0; to < 2; ++to) {
@@ -218,8 +218,8 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs a) {
       for (int to = 0; to < 4; ++to)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          a1_s[to * 16 + 4 * g + j][pl] = bf16_bits(gelu(z1[to][j]));
-          a2_s[to * 16 + 4 * g + j][pl] = bf16_bits(gelu(z2[to][j]));
+          a1_s[to * 16 + 4 * g + j][pl] = bf16_bits(relu(z1[to][j]));
+          a2_s[to * 16 + 4 * g + j][pl] = bf16_bits(relu(z2[to][j]));
         }
       // loss and dL/dz3 (tile 0: logits 0..14 and the flag 15; tile 1: 16..19)
       const int64_t gp = a.offset + (valid ? p : 0);
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs a) {
         float d[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          d[j] = acc[j] * gelu_grad(z2[tu][j]);
+          d[j] = acc[j] * relu_grad(z2[tu][j]);
           d2_s[tu * 16 + 4 * g + j][pl] = bf16_bits(d[j]);
         }
         dz2b[tu] = pack4(d[0], d[1], d[2], d[3]);
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(TrainArgs a) {
         for (int tn = 0; tn < 4; ++tn) acc = mfma16(w2b[tu][tn], dz2b[tn], acc);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          d1_s[tu * 16 + 4 * g + j][pl] = bf16_bits(acc[j] * gelu_grad(z1[tu][j]));
+          d1_s[tu * 16 + 4 * g + j][pl] = bf16_bits(acc[j] * relu_grad(z1[tu][j]));
       }
     }
     __syncthreads();
@@ -363,7 +363,9 @@ __global__ void k_adam(float* __restrict__ W, float* __restrict__ m, float* __re
 // live once per workgroup in LDS, so registers hold only the pair being
 // certified and the layer accumulators.
 constexpr int kInferWaves = 8;
-constexpr int kTiles = 8 + 16 + 8;  // W1, W2, W3 A-operand tiles
+// A-operand tiles (16 x 32) of the three layers for v_mfma_f32_16x16x32:
+// W1 4 output tiles x 1 k-block, W2 4 x 2, W3 (first output tile only) 1 x 2.
+constexpr int kTiles = 4 + 8 + 2;
 constexpr int kCol = 68;             // row stride (dwords) of the per-wave staging
 
 struct LearnedArgs {
@@ -386,10 +388,38 @@ struct LearnedArgs {
   unsigned* wg_counts;  // per workgroup: certified, features evaluated, queued
 };
 
-__device__ inline v4s lds_tile(const v4s* tiles, int t, int lane) { return tiles[t * 64 + lane]; }
+// Wait for this wave's LDS operations only (lgkmcnt(0); vmcnt and expcnt at
+// their maxima), so the previous chunk's global stores stay in flight.
+__device__ inline void lds_wait() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
+__device__ inline v8s lds_tile(const v8s* tiles, int t, int lane) { return tiles[t * 64 + lane]; }
+
+// Slot (g, j) of k-block b of a hidden layer's input: the previous layer's
+// accumulator layout puts units 16t + 4g + j' (j' < 4) of output tile t in
+// lane group g, so block b pairs tiles 2b (slots 0..3) and 2b+1 (slots 4..7).
+__device__ inline int hidden_k(int b, int g, int j) {
+  return 32 * b + 4 * g + (j < 4 ? j : 12 + j);
+}
+
+// Tile t of the staged network: lane (r = l%16, g = l/16) holds the 8
+// weights W[k][16*to + r] for the k of its slots.  Layer 1's k-block holds
+// inputs 8g + j.
+__device__ inline v8s stage_tile(const float* W, int t, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  float w[8];
+  for (int j = 0; j < 8; ++j) {
+    if (t < 4)
+      w[j] = W[kW1 + (8 * g + j) * kHid + 16 * t + r];
+    else if (t < 12)
+      w[j] = W[kW2 + hidden_k((t - 4) % 2, g, j) * kHid + 16 * ((t - 4) / 2) + r];
+    else
+      w[j] = W[kW3 + hidden_k(t - 12, g, j) * kOut + r];
+  }
+  return cat8(pack4(w[0], w[1], w[2], w[3]), pack4(w[4], w[5], w[6], w[7]));
+}
 
 __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_learned(LearnedArgs a) {
-  __shared__ v4s w_s[kTiles * 64];
+  __shared__ v8s w_s[kTiles * 64];
   __shared__ float b_s[2 * kHid + kOut];
   __shared__ unsigned wg_tally[3];  // certified, features evaluated, queued
   // Per wave, one [16][kCol] dword array in column layout (column = pair):
@@ -402,13 +432,7 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   // stage the network once per workgroup
-  for (int t = wave; t < kTiles; t += kInferWaves) {
-    v4s v;
-    if (t < 8) v = load_wt(a.W + kW1, kHid, t / 2, t % 2, lane);
-    else if (t < 24) v = load_wt(a.W + kW2, kHid, (t - 8) / 4, (t - 8) % 4, lane);
-    else v = load_wt(a.W + kW3, kOut, (t - 24) / 4, (t - 24) % 4, lane);
-    w_s[t * 64 + lane] = v;
-  }
+  for (int t = wave; t < kTiles; t += kInferWaves) w_s[t * 64 + lane] = stage_tile(a.W, t, lane);
   for (int i = threadIdx.x; i < 2 * kHid + kOut; i += 64 * kInferWaves)
     b_s[i] = (i < kHid) ? a.W[kB1 + i] : (i < 2 * kHid) ? a.W[kB2 + i - kHid]
                                                           : a.W[kB3 + i - 2 * kHid];
@@ -423,9 +447,9 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
     const bool valid = i < a.n;
     const int64_t ii = valid ? i : 0;
     bool frame;
+    const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
+    const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
     {
-      const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
-      const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
       float in[18];
       frame = canonical_inputs(A, B, in);
       uint32_t* col = &io_s[wave][lane];
@@ -434,7 +458,7 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
         col[r * kCol] = static_cast<uint32_t>(bf16_bits(in[2 * r])) |
                         (static_cast<uint32_t>(bf16_bits(in[2 * r + 1])) << 16);
     }
-    __builtin_amdgcn_s_waitcnt(0);
+    lds_wait();
     __builtin_amdgcn_wave_barrier();
 #pragma unroll 1
     for (int s = 0; s < 4; ++s) {
@@ -443,19 +467,20 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
       // bias loads out of the loops into ~100 registers
       int wl = lane, bl = 4 * g;
       asm volatile("" : "+v"(wl), "+v"(bl));
-      // B operands: lane (c, g) needs inputs 4g..4g+3 (k-tile 0) and
-      // 16+4g..19+4g (k-tile 1: only inputs 16, 17 are nonzero)
-      v4s x0, x1 = {0, 0, 0, 0};
-      {
-        const uint32_t d0 = io_s[wave][(2 * g) * kCol + pl];
-        const uint32_t d1 = io_s[wave][(2 * g + 1) * kCol + pl];
-        const uint32_t d8 = io_s[wave][8 * kCol + pl];
-        x0 = {static_cast<short>(d0 & 0xffffu), static_cast<short>(d0 >> 16),
-              static_cast<short>(d1 & 0xffffu), static_cast<short>(d1 >> 16)};
-        if (g == 0) {
-          x1[0] = static_cast<short>(d8 & 0xffffu);
-          x1[1] = static_cast<short>(d8 >> 16);
+      // B operand of layer 1: lane (c, g) holds inputs 8g..8g+7 of pair pl
+      // (dword rows 4g..4g+3; only inputs 0..17 are nonzero)
+      v8s x = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (g < 2) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t w = io_s[wave][(4 * g + r) * kCol + pl];
+          x[2 * r] = static_cast<short>(w & 0xffffu);
+          x[2 * r + 1] = static_cast<short>(w >> 16);
         }
+      } else if (g == 2) {
+        const uint32_t w = io_s[wave][8 * kCol + pl];
+        x[0] = static_cast<short>(w & 0xffffu);
+        x[1] = static_cast<short>(w >> 16);
       }
       v4s act1[4], act2[4];
 #pragma unroll
@@ -463,30 +488,29 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
         v4f acc;
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] = b_s[to * 16 + bl + j];
-        acc = mfma16(lds_tile(w_s, 2 * to, wl), x0, acc);
-        acc = mfma16(lds_tile(w_s, 2 * to + 1, wl), x1, acc);
-        act1[to] = pack4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
+        acc = mfma32(lds_tile(w_s, to, wl), x, acc);
+        act1[to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
       }
 #pragma unroll
       for (int to = 0; to < 4; ++to) {
         v4f acc;
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] = b_s[kHid + to * 16 + bl + j];
-#pragma unroll
-        for (int tk = 0; tk < 4; ++tk) acc = mfma16(lds_tile(w_s, 8 + 4 * to + tk, wl), act1[tk], acc);
-        act2[to] = pack4(gelu(acc[0]), gelu(acc[1]), gelu(acc[2]), gelu(acc[3]));
+        acc = mfma32(lds_tile(w_s, 4 + 2 * to, wl), cat8(act1[0], act1[1]), acc);
+        acc = mfma32(lds_tile(w_s, 5 + 2 * to, wl), cat8(act1[2], act1[3]), acc);
+        act2[to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
       }
       {  // output tile 0: feature logits and the exact-path logit
         v4f acc;
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] = b_s[2 * kHid + bl + j];
-#pragma unroll
-        for (int tk = 0; tk < 4; ++tk) acc = mfma16(lds_tile(w_s, 24 + tk, wl), act2[tk], acc);
+        acc = mfma32(lds_tile(w_s, 12, wl), cat8(act2[0], act2[1]), acc);
+        acc = mfma32(lds_tile(w_s, 13, wl), cat8(act2[2], act2[3]), acc);
 #pragma unroll
         for (int j = 0; j < 4; ++j) io_s[wave][(4 * g + j) * kCol + pl] = __float_as_uint(acc[j]);
       }
     }
-    __builtin_amdgcn_s_waitcnt(0);
+    lds_wait();
     __builtin_amdgcn_wave_barrier();
     // the lane's own pair: the best-ranked edge-edge and the best-ranked
     // vertex-face feature, both evaluated (no divergence between the two
@@ -505,9 +529,6 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
       }
     }
     const bool exact_path = !frame || (__uint_as_float(io_s[wave][kFlagOut * kCol + lane]) > 0.f);
-    // the pair again (cache-resident): keeps it out of registers during the network
-    const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
-    const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
     V3<double> P, Q;
     bool done = false;
     {
@@ -527,7 +548,9 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
       done = valid && !exact_path && certify(A, B, P, Q);
       evals += (valid && !exact_path) ? 2 : 0;
     }
-    if (valid && done) {
+    // every lane stores (the brute force overwrites the uncertified ones):
+    // whole cache lines, no partial-line writes
+    if (valid) {
       a.P[i] = P.x;
       a.P[a.n + i] = P.y;
       a.P[2 * a.n + i] = P.z;
@@ -535,8 +558,8 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
       a.Q[a.n + i] = Q.y;
       a.Q[2 * a.n + i] = Q.z;
       a.d[i] = pq_dist(P, Q);
-      if (a.flags) a.flags[i] = DH_FLAG_CERTIFIED;
-      cert_count++;
+      if (a.flags) a.flags[i] = done ? DH_FLAG_CERTIFIED : DH_FLAG_RECHECK;
+      cert_count += done ? 1 : 0;
     }
     // queue the rest for the FP64 brute force: wave-aggregated append of the
     // index and the coordinates (contiguous, so the brute force reads them
@@ -574,34 +597,28 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
   }
   __syncthreads();
   if (threadIdx.x < 3) a.wg_counts[3 * blockIdx.x + threadIdx.x] = wg_tally[threadIdx.x];
-}
-
-// FP64 brute force over the queued pairs: workgroup b takes the region that
-// k_learned's workgroup b filled, reading the compacted coordinates
-// (coalesced, all lanes busy but the region's tail).
-__global__ __launch_bounds__(256) void k_queue_bf(const double* __restrict__ qsoa, int64_t qcap,
-                                                  int64_t region, const int* __restrict__ queue,
-                                                  const unsigned* __restrict__ wg_counts,
-                                                  int64_t n, double* __restrict__ P,
-                                                  double* __restrict__ Q, double* __restrict__ d,
-                                                  uint8_t* __restrict__ flags) {
-  const unsigned len = wg_counts[3 * blockIdx.x + 2];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * region;
-  for (unsigned t = threadIdx.x; t < len; t += 256) {
-    const int64_t q = base + t;
-    const int64_t i = queue[q];
-    const Tri<double> A = load_tri_soa(qsoa, qcap, q, 0);
-    const Tri<double> B = load_tri_soa(qsoa, qcap, q, 9);
-    V3<double> p, qq;
-    const bool edge = bf_solver(A, B, p, qq);
-    P[i] = p.x;
-    P[n + i] = p.y;
-    P[2 * n + i] = p.z;
-    Q[i] = qq.x;
-    Q[n + i] = qq.y;
-    Q[2 * n + i] = qq.z;
-    d[i] = pq_dist(p, qq);
-    if (flags) flags[i] = DH_FLAG_RECHECK | (edge ? DH_FLAG_EDGE : 0);
+  // Drain the workgroup's queue with the FP64 brute force, every lane busy:
+  // the entries were written by this workgroup (same CU, L2-resident), and
+  // their placeholder results are overwritten here.
+  const unsigned qlen = wg_tally[2];
+  for (unsigned e0 = wave * 64; e0 < qlen; e0 += kInferWaves * 64) {
+    const unsigned e = e0 + lane;
+    if (e < qlen) {
+      const int64_t q = qbase + e;
+      const int64_t i = a.queue[q];
+      const Tri<double> A = load_tri_soa(a.qsoa, a.qcap, q, 0);
+      const Tri<double> B = load_tri_soa(a.qsoa, a.qcap, q, 9);
+      V3<double> p, qq;
+      const bool edge = bf_solver(A, B, p, qq);
+      a.P[i] = p.x;
+      a.P[a.n + i] = p.y;
+      a.P[2 * a.n + i] = p.z;
+      a.Q[i] = qq.x;
+      a.Q[a.n + i] = qq.y;
+      a.Q[2 * a.n + i] = qq.z;
+      a.d[i] = pq_dist(p, qq);
+      if (a.flags) a.flags[i] = DH_FLAG_RECHECK | (edge ? DH_FLAG_EDGE : 0);
+    }
   }
 }
 
@@ -633,9 +650,6 @@ int learned_pairs(const double* soa, int64_t n, double* P, double* Q, double* d,
                 g_learned.qsoa.as<double>(), qcap, region, g_learned.wg_counts.as<unsigned>()};
   if (n > 0) {
     hipLaunchKernelGGL(k_learned, static_cast<unsigned>(grid), 64 * kInferWaves, 0, st, a);
-    hipLaunchKernelGGL(k_queue_bf, static_cast<unsigned>(grid), 256, 0, st,
-                       g_learned.qsoa.as<double>(), qcap, region, g_learned.queue.as<int>(),
-                       g_learned.wg_counts.as<unsigned>(), n, P, Q, d, flags);
   }
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;

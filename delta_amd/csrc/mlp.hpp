@@ -1,7 +1,7 @@
 // mlp.hpp — the feature-prediction network of the learned narrow phase.
 //
 // 18 canonical-frame coordinates (padded to 32) -> 64 -> 64 -> 32 outputs,
-// GELU (hard-sigmoid form) between layers, bf16 weights and activations, FP32
+// ReLU between layers, bf16 weights and activations, FP32
 // accumulation on MFMA (v_mfma_f32_16x16x16_bf16).  Outputs: [0,15) logits
 // of the closest feature pair (0..8 edge k of A x edge m of B, 9..14 vertex-
 // face), 15 the exact-path logit, [16,20) barycentric parameters of P and Q.
@@ -54,20 +54,30 @@ __host__ __device__ inline float from_bf16(uint16_t h) {
   return f;
 }
 
-// GELU in the hard-sigmoid form x * clamp(1.702 x / 6 + 1/2, 0, 1): the
-// sigmoid of x * sigmoid(1.702 x) (Hendrycks & Gimpel 2016) replaced by its
-// piecewise-linear hard-sigmoid, one fma, one med3 and one mul per unit.
-constexpr float kHardA = 1.702f / 6.0f;
-__device__ inline float gelu(float x) {
-  return x * __builtin_amdgcn_fmed3f(kHardA * x + 0.5f, 0.0f, 1.0f);
-}
-__device__ inline float gelu_grad(float x) {
-  const float t = kHardA * x + 0.5f;
-  return (t <= 0.0f) ? 0.0f : ((t >= 1.0f) ? 1.0f : 2.0f * kHardA * x + 0.5f);
-}
+// ReLU between layers.  On bf16 values it is a 16-bit integer max against 0
+// (a negative bf16, -0 included, is a negative int16), so relu4 applies it to
+// four packed activations with two v_pk_max_i16.
+__device__ inline float relu(float x) { return x > 0.0f ? x : 0.0f; }
+__device__ inline float relu_grad(float x) { return x > 0.0f ? 1.0f : 0.0f; }
 
 __device__ inline v4f mfma16(v4s a, v4s b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// v_mfma_f32_16x16x32_bf16 (gfx950): lane l holds 8 A values of row l%16 and
+// 8 B values of column l%16; A slot (l/16, j) multiplies B slot (l/16, j), so
+// any assignment of the 32 k indices to (l/16, j) slots works as long as A
+// and B use the same one.
+__device__ inline v4f mfma32(v8s a, v8s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                 __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
+
+__device__ inline v8s cat8(v4s lo, v4s hi) {
+  return {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
 }
 
 typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
@@ -77,6 +87,11 @@ __device__ inline v4s pack4(float a, float b, float c, float d) {
   const v4bf t = {static_cast<__bf16>(a), static_cast<__bf16>(b), static_cast<__bf16>(c),
                   static_cast<__bf16>(d)};
   return __builtin_bit_cast(v4s, t);
+}
+
+__device__ inline v4s relu4(v4s x) {
+  const v4s zero = {0, 0, 0, 0};
+  return __builtin_elementwise_max(x, zero);
 }
 
 __device__ inline uint16_t bf16_bits(float f) {

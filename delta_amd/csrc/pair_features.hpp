@@ -107,37 +107,41 @@ DH_HD float rsqrt_frame(float x) {
 // Canonical frame of a pair (origin at A's centroid, x along A's edge 0->1,
 // z along A's normal, lengths in units of A's longest edge), as 18 floats:
 // A0, A1, A2, B0, B1, B2 (xyz each), clamped to [-32, 32].  The vertices are
-// taken relative to A0 in binary64 and the frame is built in binary32 (the
-// network reads bf16).  Returns false for a degenerate A (no frame).
+// taken relative to A0 in binary64; the frame is built in binary32 with
+// fused multiply-adds (the network reads bf16).  Returns false for a
+// degenerate A (no frame).
+DH_HD float fdot(const V3<float>& a, const V3<float>& b) {
+  return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z));
+}
+
 DH_HD bool canonical_inputs(const Tri<double>& A, const Tri<double>& B, float out[18]) {
   V3<float> r[6];
-  for (int v = 0; v < 6; ++v) {
+  r[0] = {0.f, 0.f, 0.f};
+  for (int v = 1; v < 6; ++v) {
     const V3<double> p = (v < 3) ? A.v[v] : B.v[v - 3];
     r[v] = {static_cast<float>(p.x - A.v[0].x), static_cast<float>(p.y - A.v[0].y),
             static_cast<float>(p.z - A.v[0].z)};
   }
-  const V3<float> c = {(r[1].x + r[2].x) / 3.0f, (r[1].y + r[2].y) / 3.0f,
-                       (r[1].z + r[2].z) / 3.0f};
   const V3<float> e01 = r[1], e02 = r[2], e12 = sub(r[2], r[1]);
-  const float l01 = dot(e01, e01), l12 = dot(e12, e12), l02 = dot(e02, e02);
-  float lmax = l01 > l12 ? l01 : l12;
-  lmax = lmax > l02 ? lmax : l02;
+  const float l01 = fdot(e01, e01), l12 = fdot(e12, e12), l02 = fdot(e02, e02);
+  const float lmax = fmaxf(fmaxf(l01, l12), l02);
   const V3<float> nz = cross3(e01, e02);
-  const float ln = dot(nz, nz);
-  const bool ok = (l01 > 0.f) && (ln > 0.f) && (lmax > 0.f);
-  const float il = ok ? rsqrt_frame(l01) : 0.f, in = ok ? rsqrt_frame(ln) : 0.f;
-  const V3<float> ex = {e01.x * il, e01.y * il, e01.z * il};
-  const V3<float> ez = {nz.x * in, nz.y * in, nz.z * in};
-  const V3<float> ey = cross3(ez, ex);
+  const float ln = fdot(nz, nz);
+  const bool ok = (l01 > 0.f) && (ln > 0.f);
   const float s = ok ? rsqrt_frame(lmax) : 0.f;
+  const float kx = ok ? rsqrt_frame(l01) : 0.f, kz = ok ? rsqrt_frame(ln) : 0.f;
+  const V3<float> ex = {e01.x * kx, e01.y * kx, e01.z * kx};
+  const V3<float> ez = {nz.x * kz, nz.y * kz, nz.z * kz};
+  const V3<float> ey = cross3(ez, ex);
+  // scaled axes, and the centroid (r1 + r2) / 3 projected on them
+  const V3<float> sx = {ex.x * s, ex.y * s, ex.z * s}, sy = {ey.x * s, ey.y * s, ey.z * s},
+                  sz = {ez.x * s, ez.y * s, ez.z * s};
+  const V3<float> c = {(r[1].x + r[2].x) * (1.0f / 3.0f), (r[1].y + r[2].y) * (1.0f / 3.0f),
+                       (r[1].z + r[2].z) * (1.0f / 3.0f)};
+  const float cx = fdot(c, sx), cy = fdot(c, sy), cz = fdot(c, sz);
   for (int v = 0; v < 6; ++v) {
-    const V3<float> q = {(r[v].x - c.x) * s, (r[v].y - c.y) * s, (r[v].z - c.z) * s};
-    const float w[3] = {dot(q, ex), dot(q, ey), dot(q, ez)};
-    for (int a = 0; a < 3; ++a) {
-      float x = w[a];
-      x = x > 32.f ? 32.f : (x < -32.f ? -32.f : x);
-      out[3 * v + a] = x;
-    }
+    const float w[3] = {fdot(r[v], sx) - cx, fdot(r[v], sy) - cy, fdot(r[v], sz) - cz};
+    for (int a = 0; a < 3; ++a) out[3 * v + a] = fminf(fmaxf(w[a], -32.f), 32.f);
   }
   return ok;
 }

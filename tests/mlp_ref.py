@@ -14,17 +14,12 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
-HARD_A = np.float32(1.702) / np.float32(6.0)
+def relu(x):
+    return np.maximum(x, 0.0)
 
 
-def gelu(x):
-    """x * clamp(1.702 x / 6 + 1/2, 0, 1), the kernel's GELU."""
-    return x * np.clip(HARD_A * x + 0.5, 0.0, 1.0)
-
-
-def gelu_grad(x):
-    t = HARD_A * x + 0.5
-    return np.where(t <= 0, 0.0, np.where(t >= 1, 1.0, 2 * HARD_A * x + 0.5))
+def relu_grad(x):
+    return (x > 0).astype(np.float64)
 
 
 def unpack(p):
@@ -45,9 +40,9 @@ def forward(p, Xbits):
     w1, b1, w2, b2, w3, b3 = unpack(p)
     x = x_float(Xbits).astype(np.float64)
     z1 = x @ bf16(w1).astype(np.float64) + b1
-    a1 = bf16(gelu(z1).astype(np.float32)).astype(np.float64)
+    a1 = bf16(relu(z1).astype(np.float32)).astype(np.float64)
     z2 = a1 @ bf16(w2).astype(np.float64) + b2
-    a2 = bf16(gelu(z2).astype(np.float32)).astype(np.float64)
+    a2 = bf16(relu(z2).astype(np.float32)).astype(np.float64)
     z3 = a2 @ bf16(w3).astype(np.float64) + b3
     return z1, a1, z2, a2, z3
 
@@ -76,9 +71,9 @@ def loss_and_grad(p, Xbits, feat, bary, flag):
     dz3[:, 16:20] = 0.5 * r
     dz3 = bf16((dz3 / B).astype(np.float32)).astype(np.float64)
     da2 = dz3 @ bf16(w3).astype(np.float64).T
-    dz2 = bf16((da2 * gelu_grad(z2)).astype(np.float32)).astype(np.float64)
+    dz2 = bf16((da2 * relu_grad(z2)).astype(np.float32)).astype(np.float64)
     da1 = dz2 @ bf16(w2).astype(np.float64).T
-    dz1 = bf16((da1 * gelu_grad(z1)).astype(np.float32)).astype(np.float64)
+    dz1 = bf16((da1 * relu_grad(z1)).astype(np.float32)).astype(np.float64)
     g = np.zeros(L.N_PARAMS)
     g[L.W1:L.B1] = (x.T @ dz1).reshape(-1)
     g[L.B1:L.W2] = dz1.sum(0)
