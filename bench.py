@@ -38,8 +38,11 @@ def parse():
                     help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--methods", default="bf64,penalty64,hybrid64,learned")
-    ap.add_argument("--train-pairs-log2", type=int, default=20)
-    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device-to-device copy kernel three times "
+                         "(reference for scaling HBM byte counters) and exit")
+    ap.add_argument("--train-pairs-log2", type=int, default=23)
+    ap.add_argument("--train-epochs", type=int, default=10)
     return ap.parse_args()
 
 
@@ -157,8 +160,37 @@ def cpu_baseline(batch, target_s):
             {"pairs": m, "bitwise_mismatches": mism})
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by a copy kernel of known size),
+    or None."""
+    path = os.path.join(ROOT, "profiles", "k_learned_tri_pairs_2p24_hbm_bytes.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
+def copy_calibration():
+    """A plain copy of known size (256 MiB read + 256 MiB written per
+    launch) for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": 1 << 28, "launches": 3}), flush=True)
+
+
 def main():
     a = parse()
+    if a.calibrate_copy:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a "
                                    "design note only (bench.py --gpus 1)"}))
@@ -187,9 +219,12 @@ def main():
     # P, Q, d, flag) over the measured pass time
     bytes_per = narrow.BYTES_F64
     achieved = bytes_per * n / (r["ms_per_step"] * 1e-3) / 1e9
+    kernel = "k_learned" if head == "learned" else "k_tri_pairs_exact<0>"
+    traffic = measured_traffic(kernel) if n == 1 << 24 else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-            "kernel": f"dh_tri_pairs[{head}]", "bytes_per_pair": bytes_per}
+            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+            "kernel": kernel, "bytes_per_pair": bytes_per,
+            "traffic_source": "profiles/k_learned_tri_pairs_2p24_hbm_bytes.md" if traffic else None}
     extra = {}
     if bf:
         fl = narrow.FLOP_BF * n / (bf["ms_per_step"] * 1e-3) / 1e12

@@ -367,6 +367,7 @@ constexpr int kInferWaves = 8;
 // W1 4 output tiles x 1 k-block, W2 4 x 2, W3 (first output tile only) 1 x 2.
 constexpr int kTiles = 4 + 8 + 2;
 constexpr int kCol = 68;             // row stride (dwords) of the per-wave staging
+constexpr int kSubs = 2;             // 16-pair sub-tiles evaluated together
 
 struct LearnedArgs {
   const double* soa;
@@ -460,54 +461,73 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
     }
     lds_wait();
     __builtin_amdgcn_wave_barrier();
+    // kSubs sub-tiles of 16 pairs at a time, layer by layer: independent
+    // MFMA chains share each weight tile read from LDS.
 #pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
-      const int pl = s * 16 + c;
-      // opaque per sub-tile: keeps the compiler from hoisting the weight and
+    for (int s0 = 0; s0 < 4; s0 += kSubs) {
+      // opaque per group: keeps the compiler from hoisting the weight and
       // bias loads out of the loops into ~100 registers
       int wl = lane, bl = 4 * g;
       asm volatile("" : "+v"(wl), "+v"(bl));
-      // B operand of layer 1: lane (c, g) holds inputs 8g..8g+7 of pair pl
-      // (dword rows 4g..4g+3; only inputs 0..17 are nonzero)
-      v8s x = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (g < 2) {
+      // B operands of layer 1: lane (c, g) holds inputs 8g..8g+7 of its
+      // pair (dword rows 4g..4g+3; only inputs 0..17 are nonzero)
+      v8s x[kSubs];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const uint32_t w = io_s[wave][(4 * g + r) * kCol + pl];
-          x[2 * r] = static_cast<short>(w & 0xffffu);
-          x[2 * r + 1] = static_cast<short>(w >> 16);
+      for (int u = 0; u < kSubs; ++u) {
+        const int pl = (s0 + u) * 16 + c;
+        x[u] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (g < 2) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const uint32_t w = io_s[wave][(4 * g + r) * kCol + pl];
+            x[u][2 * r] = static_cast<short>(w & 0xffffu);
+            x[u][2 * r + 1] = static_cast<short>(w >> 16);
+          }
+        } else if (g == 2) {
+          const uint32_t w = io_s[wave][8 * kCol + pl];
+          x[u][0] = static_cast<short>(w & 0xffffu);
+          x[u][1] = static_cast<short>(w >> 16);
         }
-      } else if (g == 2) {
-        const uint32_t w = io_s[wave][8 * kCol + pl];
-        x[0] = static_cast<short>(w & 0xffffu);
-        x[1] = static_cast<short>(w >> 16);
       }
-      v4s act1[4], act2[4];
+      v4s act1[kSubs][4], act2[kSubs][4];
 #pragma unroll
       for (int to = 0; to < 4; ++to) {
-        v4f acc;
+        const v8s w = lds_tile(w_s, to, wl);
+        v4f b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] = b_s[to * 16 + bl + j];
-        acc = mfma32(lds_tile(w_s, to, wl), x, acc);
-        act1[to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
+        for (int j = 0; j < 4; ++j) b[j] = b_s[to * 16 + bl + j];
+#pragma unroll
+        for (int u = 0; u < kSubs; ++u) {
+          const v4f acc = mfma32(w, x[u], b);
+          act1[u][to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
+        }
       }
 #pragma unroll
       for (int to = 0; to < 4; ++to) {
-        v4f acc;
+        const v8s w0 = lds_tile(w_s, 4 + 2 * to, wl), w1 = lds_tile(w_s, 5 + 2 * to, wl);
+        v4f b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] = b_s[kHid + to * 16 + bl + j];
-        acc = mfma32(lds_tile(w_s, 4 + 2 * to, wl), cat8(act1[0], act1[1]), acc);
-        acc = mfma32(lds_tile(w_s, 5 + 2 * to, wl), cat8(act1[2], act1[3]), acc);
-        act2[to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
+        for (int j = 0; j < 4; ++j) b[j] = b_s[kHid + to * 16 + bl + j];
+#pragma unroll
+        for (int u = 0; u < kSubs; ++u) {
+          v4f acc = mfma32(w0, cat8(act1[u][0], act1[u][1]), b);
+          acc = mfma32(w1, cat8(act1[u][2], act1[u][3]), acc);
+          act2[u][to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
+        }
       }
       {  // output tile 0: feature logits and the exact-path logit
-        v4f acc;
+        const v8s w0 = lds_tile(w_s, 12, wl), w1 = lds_tile(w_s, 13, wl);
+        v4f b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] = b_s[2 * kHid + bl + j];
-        acc = mfma32(lds_tile(w_s, 12, wl), cat8(act2[0], act2[1]), acc);
-        acc = mfma32(lds_tile(w_s, 13, wl), cat8(act2[2], act2[3]), acc);
+        for (int j = 0; j < 4; ++j) b[j] = b_s[2 * kHid + bl + j];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) io_s[wave][(4 * g + j) * kCol + pl] = __float_as_uint(acc[j]);
+        for (int u = 0; u < kSubs; ++u) {
+          v4f acc = mfma32(w0, cat8(act2[u][0], act2[u][1]), b);
+          acc = mfma32(w1, cat8(act2[u][2], act2[u][3]), acc);
+          const int pl = (s0 + u) * 16 + c;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) io_s[wave][(4 * g + j) * kCol + pl] = __float_as_uint(acc[j]);
+        }
       }
     }
     lds_wait();
