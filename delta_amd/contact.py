@@ -139,6 +139,19 @@ def hybridTriangleStat(xA, yA, zA, numberOfTrianglesOfGeometryA, epsilonA, frict
 
 
 # statistics getters (penalty.h:47-53, hybrid.h:130-170)
+def gjk(numberOfTrianglesOfGeometryA, xA, yA, zA, epsilonA, frictionA, particleA,
+        numberOfTrianglesOfGeometryB, xB, yB, zB, epsilonB, frictionB, particleB):
+    """gjk.cpp:25-100 (count-first argument order of gjk.h:32-48): GJK over
+    every vertex of both particles, at most one contact, when |P-Q| <=
+    epsA+epsB; its triangle indices are -1.  The reference's wrapper copies
+    only nTri vertices and then reads 3*nTri points from that buffer, past its
+    end; this one uses all 3*nTri vertices."""
+    _bind()
+    return _detect(_lib.DH_GJK, xA, yA, zA, numberOfTrianglesOfGeometryA, epsilonA, frictionA,
+                   particleA, xB, yB, zB, numberOfTrianglesOfGeometryB, epsilonB, frictionB,
+                   particleB)
+
+
 def cleanPenaltyStatistics():
     for i in range(17):
         _stats.newton_histogram[i] = 0

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "exact_math.hpp"
+#include "gjk.hpp"
 #include "runtime.hpp"
 
 namespace dh {
@@ -315,6 +316,79 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
   }
 }
 
+// GJK over the particles' vertex replicas (gjk.cpp:25-100 with the core of
+// gjk.cpp:418-460), one wave per particle pair: the support searches are
+// lane-parallel (first index of the strict extremum after a wave
+// reduction); the simplex arithmetic is uniform across the wave.  At most one
+// contact per pair, when |P-Q| <= epsA + epsB; triA = triB = -1.
+__device__ __forceinline__ int wave_extremum(double bd, int bi, bool maximal) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double od = __shfl_xor(bd, o);
+    const int oi = __shfl_xor(bi, o);
+    const bool better = oi >= 0 && (bi < 0 || (maximal ? od > bd : od < bd) ||
+                                    (od == bd && oi < bi));
+    bd = better ? od : bd;
+    bi = better ? oi : bi;
+  }
+  return bi;
+}
+
+__global__ __launch_bounds__(256) void k_gjk(DetectArgs g) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= g.npairs) return;
+  const Pair p = load_pair(g, k);
+  const int na = 3 * p.nA, nb = 3 * p.nB, ba = 3 * p.oA, bb = 3 * p.oB;
+  auto pa = [&](int i) { return V3<double>{g.x[ba + i], g.y[ba + i], g.z[ba + i]}; };
+  auto pb = [&](int i) { return V3<double>{g.x[bb + i], g.y[bb + i], g.z[bb + i]}; };
+  auto min_a = [&](const double* v) {
+    double bd = 1.7976931348623157e308;
+    int bi = -1;
+    for (int i = lane; i < na; i += 64) {
+      const double c[3] = {g.x[ba + i], g.y[ba + i], g.z[ba + i]};
+      const double d = gdot(c, v);
+      if (d < bd) {
+        bd = d;
+        bi = i;
+      }
+    }
+    return wave_extremum(bd, bi, false);
+  };
+  auto max_b = [&](const double* v) {
+    double bd = -1.7976931348623157e308;
+    int bi = -1;
+    for (int i = lane; i < nb; i += 64) {
+      const double c[3] = {g.x[bb + i], g.y[bb + i], g.z[bb + i]};
+      const double d = gdot(c, v);
+      if (d > bd) {
+        bd = d;
+        bi = i;
+      }
+    }
+    return wave_extremum(bd, bi, true);
+  };
+  double P[3] = {0, 0, 0}, Q[3] = {0, 0, 0};
+  bool ok = na > 0 && nb > 0;
+  if (ok) gjk_core(na, nb, pa, pb, min_a, max_b, P, Q, &ok);
+  const V3<double> vP = {P[0], P[1], P[2]}, vQ = {Q[0], Q[1], Q[2]};
+  const bool emit = ok && pq_dist(vP, vQ) <= p.margin;
+  if (lane == 0) {
+    if (emit) {
+      const unsigned long long slot = atomicAdd(g.counter, 1ull);
+      if (static_cast<long long>(slot) < g.cap) {
+        dh_contact c;
+        make_contact(p, k, -1, -1, vP, vQ, &c);
+        c.pad_ = 0;
+        g.temp[slot] = c;
+      }
+    }
+    g.pair_count[k] = emit ? 1 : 0;
+    atomicAdd(&g.stats[ST_CHECKS], static_cast<unsigned long long>(p.nA) * p.nB);
+    atomicAdd(&g.stats[ST_EVALUATED], 1ull);
+  }
+}
+
 // exclusive scan of per-pair counts (two levels, 1024 per block)
 __global__ __launch_bounds__(1024) void k_scan_blocks(const int* __restrict__ in, int n,
                                                       long long* __restrict__ out,
@@ -433,6 +507,9 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
     case DH_HYBRID_TRIANGLE_STAT:
       launch_detect<DH_HYBRID_TRIANGLE_STAT>(grid, shm_rows, st, a);
       break;
+    case DH_GJK:
+      hipLaunchKernelGGL(k_gjk, (grid + 3) / 4, 256, 0, st, a);
+      break;
     default:
       return set_error(DH_ERR_UNSUPPORTED, "dh_detect_pairs: method %d", method);
   }
@@ -504,7 +581,6 @@ int dh_detect_pairs(int method, const dh_geometry* geom, const int* pairs_dev, i
     return set_error(DH_ERR_ARG, "dh_detect_pairs: bad arguments");
   *n_out = 0;
   if (npairs == 0) return DH_OK;
-  if (method == DH_GJK) return set_error(DH_ERR_UNSUPPORTED, "dh_detect_pairs: use dh_gjk");
   long long total = 0;
   rc = detect_pairs_impl(method, geom, pairs_dev, npairs, out_dev, cap, &total, pair_count_dev,
                          nullptr, stats, as_stream(stream));

@@ -91,3 +91,40 @@ extern "C" void hc_learned_probe(const double* soa, long n, int* code, int* same
     }
   }
 }
+
+#include "gjk.hpp"
+
+// gjk.hpp's iteration with sequential support searches over AoS point sets
+// (the kernel replaces only the searches with a wave reduction).
+extern "C" double hc_gjk(const double* a, int na, const double* b, int nb, double* p,
+                         double* q) {
+  auto pa = [&](int i) { return dh::V3<double>{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; };
+  auto pb = [&](int i) { return dh::V3<double>{b[3 * i], b[3 * i + 1], b[3 * i + 2]}; };
+  auto min_a = [&](const double* v) {
+    int best = -1;
+    double bd = 1.7976931348623157e308;
+    for (int i = 0; i < na; ++i) {
+      const double d = dh::gdot(a + 3 * i, v);
+      if (d < bd) {
+        bd = d;
+        best = i;
+      }
+    }
+    return best;
+  };
+  auto max_b = [&](const double* v) {
+    int best = -1;
+    double bd = -1.7976931348623157e308;
+    for (int i = 0; i < nb; ++i) {
+      const double d = dh::gdot(b + 3 * i, v);
+      if (d > bd) {
+        bd = d;
+        best = i;
+      }
+    }
+    return best;
+  };
+  bool ok = true;
+  const double d = dh::gjk_core(na, nb, pa, pb, min_a, max_b, p, q, &ok);
+  return ok ? d : -1.0;
+}

@@ -1,24 +1,28 @@
 // delta_shim.cpp — drop-in replacement for the reference's
 //   delta/contact/detection/{bf,penalty,hybrid}.cpp   (detection drivers and
 //                                                      pair solvers)
+//   delta/contact/detection/gjk.cpp                   (GJK wrapper and core)
 //   delta/dynamics/dynamics.cpp                       (rigid-body update)
-// A Delta build compiles this file instead of those four, against Delta's own
+// A Delta build compiles this file instead of those five, against Delta's own
 // headers (bf.h, penalty.h, hybrid.h, dynamics.h, contactpoint.h), and links
 // libdelta_hip.so.  (forces.cpp stays: per call it is host arithmetic over a
 // few contacts; the batched GPU form is dh_contacts_forces inside dh_step.)  Every function keeps the reference signature
 // and meaning; the work runs in the HIP library through include/delta_hip.h.
 // Statistics keep the reference's global-state semantics (not thread-safe;
 // cleanHybridStatistics leaves the batch size and error, hybrid.cpp:36-41).
+#include <cfloat>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "contact/contactpoint.h"
 #include "contact/detection/bf.h"
+#include "contact/detection/gjk.h"
 #include "contact/detection/hybrid.h"
 #include "contact/detection/penalty.h"
 #include "dynamics/dynamics.h"
 #include "delta_hip.h"
+#include "gjk.hpp"
 
 namespace {
 
@@ -75,6 +79,41 @@ std::vector<delta::contact::contactpoint> run(int method, const iREAL* xA, const
 namespace delta {
 namespace contact {
 namespace detection {
+
+// gjk.cpp:25-100, on the GPU.  Uses every vertex of both particles: the
+// reference copies nTri vertices and then reads 3*nTri points from that
+// buffer, past its end.
+std::vector<contactpoint> gjk(int nA, iREAL* xA, iREAL* yA, iREAL* zA, iREAL epsA, bool fA,
+                              int pA, int nB, iREAL* xB, iREAL* yB, iREAL* zB, iREAL epsB,
+                              bool fB, int pB) {
+  return run(DH_GJK, xA, yA, zA, nA, epsA, fA, pA, xB, yB, zB, nB, epsB, fB, pB);
+}
+
+// gjk.cpp:418-460 on two AoS point sets, on the host: the kernel's own
+// iteration (gjk.hpp) with sequential support searches, so results equal
+// the GPU wrapper's for the same points.
+iREAL gjk(iREAL* a, int na, iREAL* b, int nb, iREAL* p, iREAL* q) {
+  auto pa = [&](int i) { return dh::V3<double>{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; };
+  auto pb = [&](int i) { return dh::V3<double>{b[3 * i], b[3 * i + 1], b[3 * i + 2]}; };
+  auto search = [](const double* c, int n, const double* v, bool maximal) {
+    int best = -1;
+    double bd = maximal ? -DBL_MAX : DBL_MAX;
+    for (int i = 0; i < n; ++i) {
+      const double d = dh::gdot(c + 3 * i, v);
+      if (maximal ? d > bd : d < bd) {
+        bd = d;
+        best = i;
+      }
+    }
+    return best;
+  };
+  auto min_a = [&](const double* v) { return search(a, na, v, false); };
+  auto max_b = [&](const double* v) { return search(b, nb, v, true); };
+  bool ok = true;
+  const double d = dh::gjk_core(na, nb, pa, pb, min_a, max_b, p, q, &ok);
+  if (!ok) throw std::runtime_error("gjk: no support point (NaN coordinates)");
+  return d;
+}
 
 std::vector<contactpoint> bf(const iREAL* xA, const iREAL* yA, const iREAL* zA, int nA,
                              iREAL epsA, bool fA, int pA, const iREAL* xB, const iREAL* yB,

@@ -8,6 +8,7 @@
  */
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -670,6 +671,24 @@ int or_detect(int method,
             emit(&e, rb.P + 3 * iB, rb.Q + 3 * iB, iA, iB);
       }
     } break;
+    case DH_GJK: {
+      /* gjk.cpp:25-100 over every vertex of both particles (the reference
+       * copies nTri of them and then reads 3*nTri points from that buffer,
+       * past its end); one contact when |Q-P| <= epsA + epsB */
+      double* a = (double*)malloc(sizeof(double) * 9 * (nA > 0 ? nA : 1));
+      double* b = (double*)malloc(sizeof(double) * 9 * (nB > 0 ? nB : 1));
+      double P[3], Q[3];
+      for (int i = 0; i < 3 * nA; ++i) { a[3 * i] = xA[i]; a[3 * i + 1] = yA[i]; a[3 * i + 2] = zA[i]; }
+      for (int i = 0; i < 3 * nB; ++i) { b[3 * i] = xB[i]; b[3 * i + 1] = yB[i]; b[3 * i + 2] = zB[i]; }
+      if (nA > 0 && nB > 0) {
+        const double g = or_gjk(a, 3 * nA, b, 3 * nB, P, Q);
+        stats->pairs_evaluated += 1;
+        const double dist = sqrt(((Q[0] - P[0]) * (Q[0] - P[0])) + ((Q[1] - P[1]) * (Q[1] - P[1])) +
+                                 ((Q[2] - P[2]) * (Q[2] - P[2])));
+        if (g == g && dist <= margin) emit(&e, P, Q, -1, -1);
+      }
+      free(a); free(b);
+    } break;
     default:
       free(rb.P); free(rb.Q); free(rb.d); free(rb.failed);
       return DH_ERR_ARG;
@@ -990,4 +1009,127 @@ int64_t or_step(or_scene* s, int method, const dh_force_params* fp, double dt,
   free(lo); free(hi); free(force); free(torque); free(buf); free(prs); free(touched);
   if (npairs_out) *npairs_out = npairs;
   return total_contacts;
+}
+
+/* ------------------------------------------------------------------ GJK */
+/* gjk.cpp:102-460: Gilbert-Johnson-Keerthi distance of two convex point sets
+ * (AoS, 3 doubles per point), Johnson's recursive sub-simplex projection. */
+#define GJK_EPS 1e-6 /* GEOMETRIC_EPSILON, gjk.cpp:102 */
+
+static double g_dot(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* sub-simplex bit sets in trial order and the trial counts (gjk.cpp:126-148) */
+static const int g_sets[15] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 11, 13, 14, 15};
+static const int g_tries[5] = {0, 0, 3, 7, 15};
+
+typedef struct { double w[3]; const double* a; const double* b; } g_vert;
+
+/* gjk.cpp:302-414 */
+static int g_project(g_vert* s, int n, double* l, double* v) {
+  double dt[4][4], cof[16][4];
+  if (n == 1) {
+    l[0] = 1.0;
+    v[0] = s[0].w[0]; v[1] = s[0].w[1]; v[2] = s[0].w[2];
+    return 1;
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = i; j < n; ++j) dt[i][j] = dt[j][i] = g_dot(s[i].w, s[j].w);
+  for (int m = 0; m < g_tries[n]; ++m) {
+    const int set = g_sets[m];
+    int good = 1, first = -1;
+    int members = 0;
+    for (int i = 0; i < n; ++i) members += (set >> i) & 1;
+    if (members == 1) {
+      for (int i = 0; i < n; ++i)
+        if (set & (1 << i)) { first = i; break; }
+      cof[set][first] = 1.0;
+      for (int j = 0; j < n; ++j) {
+        if (set & (1 << j)) continue;
+        cof[set][j] = dt[first][first] - dt[first][j];
+        if (cof[set][j] > 0.0) good = 0;
+      }
+    } else {
+      for (int i = 0; i < n; ++i) {
+        if (!(set & (1 << i))) continue;
+        if (first < 0) first = i;
+        cof[set][i] = cof[set & ~(1 << i)][i];
+        if (cof[set][i] <= 0.0) good = 0;
+      }
+      for (int j = 0; j < n; ++j) {
+        if (set & (1 << j)) continue;
+        cof[set][j] = 0.0;
+        for (int i = 0; i < n; ++i)
+          if (set & (1 << i)) cof[set][j] += cof[set][i] * (dt[i][first] - dt[i][j]);
+        if (cof[set][j] > 0.0) good = 0;
+      }
+    }
+    if (good) {
+      double sum = 0.0;
+      int o = 0;
+      for (int i = 0; i < n; ++i) {
+        if (!(set & (1 << i))) continue;
+        s[o] = s[i];
+        l[o] = cof[set][i];
+        sum += l[o];
+        ++o;
+      }
+      v[0] = v[1] = v[2] = 0.0;
+      for (int i = 0; i < o; ++i) {
+        l[i] /= sum;
+        v[0] = v[0] + l[i] * s[i].w[0];
+        v[1] = v[1] + l[i] * s[i].w[1];
+        v[2] = v[2] + l[i] * s[i].w[2];
+      }
+      return o;
+    }
+  }
+  return 0;
+}
+
+static const double* g_support(const double* c, int n, const double* v, int maximal) {
+  const double* best = NULL;
+  double bd = maximal ? -DBL_MAX : DBL_MAX;
+  for (int i = 0; i < n; ++i) {
+    const double d = g_dot(c + 3 * i, v);
+    if (maximal ? (d > bd) : (d < bd)) { bd = d; best = c + 3 * i; }
+  }
+  return best;
+}
+
+double or_gjk(const double* a, int na, const double* b, int nb, double p[3], double q[3]) {
+  g_vert s[4];
+  double v[3], l[4], mi = 0.0, vlen;
+  int far = 1, n = 0, it = 0;
+  const long long lim = (long long)(na + nb) * (na + nb);
+  v[0] = a[0] - b[0]; v[1] = a[1] - b[1]; v[2] = a[2] - b[2];
+  vlen = sqrt(g_dot(v, v));
+  while (far && vlen > GJK_EPS && n < 4 && it++ < lim) {
+    s[n].a = g_support(a, na, v, 0);
+    s[n].b = g_support(b, nb, v, 1);
+    if (!s[n].a || !s[n].b) return NAN;
+    for (int c = 0; c < 3; ++c) s[n].w[c] = s[n].a[c] - s[n].b[c];
+    {
+      const double delta = g_dot(v, s[n].w) / vlen;
+      mi = mi > delta ? mi : delta;
+    }
+    far = (vlen - mi) > GJK_EPS;
+    if (far) {
+      n = g_project(s, n + 1, l, v);
+      vlen = sqrt(g_dot(v, v));
+    }
+  }
+  if (n) {
+    p[0] = p[1] = p[2] = 0.0;
+    q[0] = q[1] = q[2] = 0.0;
+    for (--n; n >= 0; --n)
+      for (int c = 0; c < 3; ++c) {
+        p[c] = p[c] + l[n] * s[n].a[c];
+        q[c] = q[c] + l[n] * s[n].b[c];
+      }
+  } else {
+    for (int c = 0; c < 3; ++c) { p[c] = a[c]; q[c] = b[c]; }
+  }
+  return vlen;
 }

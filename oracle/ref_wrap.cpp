@@ -128,6 +128,14 @@ int ref_detect(int method, const double* xA, const double* yA, const double* zA,
   return (int)r.size();
 }
 
+// The reference's GJK core on AoS point sets (gjk.cpp:418-460).  Its
+// detection wrapper is not exposed: it passes 3*nTri as the point count of a
+// buffer holding nTri points (gjk.cpp:58-77) and so reads past its allocation.
+double ref_gjk_core(const double* a, int na, const double* b, int nb, double* p, double* q) {
+  std::vector<double> ca(a, a + 3 * na), cb(b, b + 3 * nb);
+  return det::gjk(ca.data(), na, cb.data(), nb, p, q);
+}
+
 void ref_clean_stats(void) {
   det::cleanPenaltyStatistics();
   det::cleanHybridStatistics();

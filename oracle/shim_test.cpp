@@ -8,6 +8,7 @@
 
 #include "contact/contactpoint.h"
 #include "contact/detection/bf.h"
+#include "contact/detection/gjk.h"
 #include "contact/detection/hybrid.h"
 #include "contact/detection/penalty.h"
 
@@ -18,14 +19,15 @@ int main(int argc, char** argv) {
   std::vector<double> a(9 * nA), b(9 * nB);
   for (auto& v : a) if (scanf("%lf", &v) != 1) return 2;
   for (auto& v : b) if (scanf("%lf", &v) != 1) return 2;
-  const double* xA = a.data(); const double* yA = xA + 3 * nA; const double* zA = yA + 3 * nA;
-  const double* xB = b.data(); const double* yB = xB + 3 * nB; const double* zB = yB + 3 * nB;
+  double* xA = a.data(); double* yA = xA + 3 * nA; double* zA = yA + 3 * nA;
+  double* xB = b.data(); double* yB = xB + 3 * nB; double* zB = yB + 3 * nB;
   namespace d = delta::contact::detection;
   std::vector<delta::contact::contactpoint> r;
   switch (method) {
     case 0: r = d::bf(xA, yA, zA, nA, 1e-4, true, 7, xB, yB, zB, nB, 1e-4, true, 9); break;
     case 1: r = d::penalty(xA, yA, zA, nA, 1e-4, true, 7, xB, yB, zB, nB, 1e-4, true, 9); break;
     case 2: r = d::hybridWithPerBatchFallBack(xA, yA, zA, nA, 1e-4, true, 7, xB, yB, zB, nB, 1e-4, true, 9); break;
+    case 7: r = d::gjk(nA, xA, yA, zA, 1e-4, true, 7, nB, xB, yB, zB, 1e-4, true, 9); break;
     default: return 3;
   }
   printf("%zu\n", r.size());

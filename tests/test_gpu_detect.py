@@ -20,6 +20,8 @@ METHODS = {
     "penalty_stat": (contact.penaltyStat, _lib.DH_PENALTY_STAT),
     "hybrid_batch_stat": (contact.hybridBatchStat, _lib.DH_HYBRID_BATCH_STAT),
     "hybrid_triangle_stat": (contact.hybridTriangleStat, _lib.DH_HYBRID_TRIANGLE_STAT),
+    "gjk": (lambda xA, yA, zA, nA, eA, fA, iA, xB, yB, zB, nB, eB, fB, iB:
+            contact.gjk(nA, xA, yA, zA, eA, fA, iA, nB, xB, yB, zB, eB, fB, iB), _lib.DH_GJK),
 }
 FIELDS = ["x", "normal", "depth", "P", "Q", "eps_total", "master", "slave", "penetration",
           "friction", "triA", "triB"]
@@ -66,7 +68,7 @@ def test_hybrid_triangle_rejects_reference_out_of_bounds():
     assert e.value.code == 4
 
 
-@pytest.mark.parametrize("method", ["bf", "penalty", "hybrid_batch", "hybrid_batch_stat"])
+@pytest.mark.parametrize("method", ["bf", "penalty", "hybrid_batch", "hybrid_batch_stat", "gjk"])
 def test_batched_pairs_match_per_pair_oracle(method):
     """dh_detect_pairs over several particle pairs: contacts grouped by pair
     in pair order, each group in reference order."""

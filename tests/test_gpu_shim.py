@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(ol.ROOT, "oracle", "_ref", "shim_test")
 
 
-@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("method", [0, 1, 2, 7])
 def test_shim_matches_oracle(method):
     if not os.path.exists(BIN):
         pytest.skip("oracle/_ref/shim_test not built (needs the reference headers)")
