@@ -212,8 +212,12 @@ DH_HD void vertex_face(T& ptmin, T& ss, T& tt, int& id, const V3<T>& f0, const V
 }
 
 // bfSolver: closest points P on A and Q on B.  Returns true when the
-// edge-edge result is kept (ptmin > ssmin, bf.cpp:1102).
-template <typename T>
+// edge-edge result is kept (ptmin > ssmin, bf.cpp:1102).  With kZoltan the
+// output stage is that of zoltan/bf.cpp, whose switch cases 1 fall through
+// into cases 2 (zoltan/bf.cpp:527-559, 907-925): edge-edge feature 1 takes
+// its Q from B's edge 2->0 with feature 1's parameter, vertex-face feature 1
+// reports B's vertex 2.
+template <typename T, bool kZoltan = false>
 DH_HD bool bf_solver_ids(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q, int& ssid_out,
                          int& id_out) {
   T ssmin = T(1E+30), ttc = T(0), ssc = T(0);
@@ -267,7 +271,7 @@ DH_HD bool bf_solver_ids(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q, i
   ssid_out = ssid;
   id_out = id;
   if (ptmin > ssmin) {
-    const int k = ssid / 3, m = ssid - 3 * (ssid / 3);
+    const int k = ssid / 3, m = (kZoltan && ssid == 1) ? 2 : ssid - 3 * (ssid / 3);
     // register selects instead of dynamic indexing (keeps Tri out of scratch)
     const V3<T> ha = pick3(A, k), oa = pick3(A, k == 2 ? 0 : k + 1);
     const V3<T> hb = pick3(B, m), ob = pick3(B, m == 2 ? 0 : m + 1);
@@ -283,7 +287,7 @@ DH_HD bool bf_solver_ids(const Tri<T>& A, const Tri<T>& B, V3<T>& P, V3<T>& Q, i
   const V3<T> onf = {f0.x + ((f2.x - f0.x) * tt) + ((f1.x - f0.x) * ss),
                      f0.y + ((f2.y - f0.y) * tt) + ((f1.y - f0.y) * ss),
                      f0.z + ((f2.z - f0.z) * tt) + ((f1.z - f0.z) * ss)};
-  const V3<T> vtx = sel3(onA, pick3(B, id), pick3(A, id - 3));
+  const V3<T> vtx = sel3(onA, pick3(B, (kZoltan && id == 1) ? 2 : id), pick3(A, id - 3));
   P = sel3(onA, onf, vtx);
   Q = sel3(onA, vtx, onf);
   return false;

@@ -128,3 +128,79 @@ extern "C" double hc_gjk(const double* a, int na, const double* b, int nb, doubl
   const double d = dh::gjk_core(na, nb, pa, pb, min_a, max_b, p, q, &ok);
   return ok ? d : -1.0;
 }
+
+#include "zoltan_math.hpp"
+
+// zoltan_math.hpp built for the host, with the kernels' drivers (one pair per
+// lane for bf and penalty, one batch of four per lane for hybrid), on
+// vertex-major rows D[k*n + i].
+extern "C" void hc_zoltan_bf(const double* soa, long n, double* P, double* Q, int* rc) {
+  for (long i = 0; i < n; ++i) {
+    const Tri<double> A = load(soa, n, i, 0), B = load(soa, n, i, 9);
+    dh::V3<double> p, q;
+    rc[i] = dh::zoltan_bf(A, B, p, q);
+    if (rc[i] >= 0) {
+      P[i] = p.x;
+      P[n + i] = p.y;
+      P[2 * n + i] = p.z;
+      Q[i] = q.x;
+      Q[n + i] = q.y;
+      Q[2 * n + i] = q.z;
+    }
+  }
+}
+
+static Tri<double> hc_tri(const double* abc) {
+  Tri<double> t;
+  for (int v = 0; v < 3; ++v) t.v[v] = {abc[3 * v], abc[3 * v + 1], abc[3 * v + 2]};
+  return t;
+}
+
+static Tri<double> hc_row_tri(const double* d, const double* e, const double* f, int n, long i) {
+  Tri<double> t;
+  t.v[0] = {d[i], d[n + i], d[2 * n + i]};
+  t.v[1] = {e[i], e[n + i], e[2 * n + i]};
+  t.v[2] = {f[i], f[n + i], f[2 * n + i]};
+  return t;
+}
+
+static void hc_put(double* p, int n, long i, const dh::V3<double>& v) {
+  p[i] = v.x;
+  p[n + i] = v.y;
+  p[2 * n + i] = v.z;
+}
+
+extern "C" void hc_zoltan_penalty(unsigned it, unsigned nt, const double* abc, const double* d,
+                                  const double* e, const double* f, double* p, double* q, int n) {
+  const Tri<double> A = hc_tri(abc);
+  for (unsigned i = it; i < nt; ++i) {
+    dh::V3<double> P, Q;
+    dh::zoltan_penalty(A, hc_row_tri(d, e, f, n, i), i, P, Q);
+    hc_put(p, n, i, P);
+    hc_put(q, n, i, Q);
+  }
+}
+
+extern "C" void hc_zoltan_hybrid(int it, int nt, const double* abc, const double* d,
+                                 const double* e, const double* f, double* p, double* q, int n) {
+  const Tri<double> A = hc_tri(abc);
+  for (unsigned k = static_cast<unsigned>(it); k < static_cast<unsigned>(nt) / 4u; ++k) {
+    double err[4];
+    Tri<double> B;
+    dh::V3<double> P, Q;
+    for (unsigned l = 0; l < 4; ++l) {
+      B = hc_row_tri(d, e, f, n, k * 4 + l);
+      err[l] = dh::zh_penalty(A, B, P, Q);
+      hc_put(p, n, k * 4 + l, P);
+      hc_put(q, n, k * 4 + l, Q);
+    }
+    double batch = 0.0;
+    for (int l = 0; l < 4; ++l) batch += err[l];
+    if (batch / 4u > 1E-8)
+      for (unsigned l = 0; l < 4; ++l) {
+        dh::zh_bf(A.v[0], A.v[1], A.v[2], B.v[0], B.v[1], B.v[2], P, Q);
+        hc_put(p, n, k * 4 + l, P);
+        hc_put(q, n, k * 4 + l, Q);
+      }
+  }
+}

@@ -187,6 +187,67 @@ int dh_gen_tri_pairs(uint64_t seed, int64_t n, const dh_gen_params* p, double* s
 /* Element-wise binary64 -> binary32 rounding of n values (device). */
 int dh_convert_f32(const double* src_dev, float* dst_dev, int64_t n, void* stream);
 
+/* ------------------------------------------- zoltan application kernels */
+/* zoltan/bf.cpp:60-952 (`bf(A3x, A3y, A3z, B3x, B3y, B3z, &P, &Q)`) for n unique
+ * pairs in the dh_tri_pairs layout.  bfSolver's arithmetic with the zoltan
+ * differences: a pair whose normal test returns early (zoltan/bf.cpp:111-119,
+ * 165-169) keeps its P/Q/d untouched and gets DH_FLAG_FAILED; the switch
+ * fall-throughs of cases 1 are kept.  d (may be NULL) = |Q-P|. */
+int dh_zoltan_bf(const double* soa_dev, int64_t n, double* P_dev, double* Q_dev,
+                 double* d_dev, uint8_t* flags_dev, void* stream);
+
+/* zoltan/penalty.cpp:27-208 (`penalty(it, nt, a, b, c, d, e, f, p, q)`):
+ * triangle (a, b, c) (host) against triangles i in [it, nt) of the
+ * vertex-major device arrays d[k][i], e[k][i], f[k][i] (host tables of three
+ * device pointers); writes p[k][i], q[k][i].  Five Newton steps; from i = 4 on
+ * the regularisation is multiplied by r before every step, as in the
+ * reference (penalty.cpp:153, which tests the pair index). */
+int dh_zoltan_penalty(unsigned it, unsigned nt, const double a[3], const double b[3],
+                      const double c[3], double* const d[3], double* const e[3],
+                      double* const f[3], double* const p[3], double* const q[3],
+                      void* stream);
+
+/* zoltan/hybrid.cpp:37-132 (`hybrid(it, nt, ...)`), same arrays: batches k
+ * in [it, nt/4) of the four pairs 4k..4k+3; seven-step penalty per pair; when
+ * the batch's mean error exceeds 1e-8, the brute force of
+ * zoltan/hybrid.cpp:646-885 on the batch's last pair is written to all four
+ * (the reference's serial semantics: its fallback reuses the last pair's
+ * D, E, F). */
+int dh_zoltan_hybrid(int it, int nt, const double a[3], const double b[3], const double c[3],
+                     double* const d[3], double* const e[3], double* const f[3],
+                     double* const p[3], double* const q[3], void* stream);
+
+/* zoltan/contact.h:34-43 contact record. */
+typedef struct dh_zoltan_contact {
+  int pid[2];
+  int color[2];
+  double point[3];
+  double normal[3];   /* (Q - P) / depth: not unit length (contact.cpp:120-122) */
+  double depth;       /* 0.15 - |Q - P|                                      */
+  double pp[3];
+  double qq[3];
+} dh_zoltan_contact;
+
+/* zoltan/contact.cpp:56-146 (`contact::detection(s, e, t, tid, pid, ...)`):
+ * every triangle pair i < j in [s, e) of different particles, t[v][axis][i]
+ * the spatial vertices (host table of device pointers; t[3..5] unused), pid
+ * (device) in [0, npid).  A pair is a contact when zoltan's bf gives
+ * |Q - P| < 0.15; it is dropped when any coordinate of its P or Q equals that
+ * of a contact its particle pid[i] already holds from this call.  out (device)
+ * receives the kept contacts in discovery order (i, then j); the reference's
+ * per-particle list conpnt[p] is the subsequence with pid[0] == p.  Differs
+ * from the reference where that is undefined: triangle i's y and z are read
+ * (the reference reads them into By, Bz and leaves Ay, Az uninitialised,
+ * contact.cpp:66-72), and pairs for which bf returns early are skipped. */
+int dh_zoltan_detection(int s, int e, double* const t[6][3], const int* tid_dev,
+                        const int* pid_dev, int npid, dh_zoltan_contact* out_dev, int cap,
+                        int* n_out, void* stream);
+
+/* zoltan/contact.cpp:149-240: the same for i in [s1, e1) against j in [s2, e2). */
+int dh_zoltan_detection2(int s1, int e1, int s2, int e2, double* const t[6][3],
+                         const int* tid_dev, const int* pid_dev, int npid,
+                         dh_zoltan_contact* out_dev, int cap, int* n_out, void* stream);
+
 /* ------------------------------------------------- learned narrow phase */
 /* Feature network (no counterpart in the reference): 18 canonical-frame
  * coordinates -> 64 -> 64 -> 32 outputs (15 closest-feature logits, an

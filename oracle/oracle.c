@@ -230,6 +230,11 @@ static double pt_tri_sqr(double a, double b, double c, double d, double e, doubl
   return sqr;
 }
 
+/* bfSolver on vertex tables; `zoltan` selects the output stage of
+ * zoltan/bf.cpp, whose cases 1 fall through into cases 2 (527-559, 907-925). */
+static int bf_core(const double va[3][3], const double vb[3][3], double P[3], double Q[3],
+                   int zoltan);
+
 int or_bf_solver(const double* xA, const double* yA, const double* zA,
                  const double* xB, const double* yB, const double* zB,
                  double P[3], double Q[3]) {
@@ -238,6 +243,34 @@ int or_bf_solver(const double* xA, const double* yA, const double* zA,
     va[k][0] = xA[k]; va[k][1] = yA[k]; va[k][2] = zA[k];
     vb[k][0] = xB[k]; vb[k][1] = yB[k]; vb[k][2] = zB[k];
   }
+  return bf_core(va, vb, P, Q, 0);
+}
+
+/* zoltan/bf.cpp:60-952: -1 when it returns before writing P, Q (exactly zero
+ * normal of B, or the mixed test nB.x == nA.y == nA.z == 0 of 165-169). */
+int or_zoltan_bf(const double* xA, const double* yA, const double* zA,
+                 const double* xB, const double* yB, const double* zB,
+                 double P[3], double Q[3]) {
+  double va[3][3], vb[3][3];
+  for (int k = 0; k < 3; ++k) {
+    va[k][0] = xA[k]; va[k][1] = yA[k]; va[k][2] = zA[k];
+    vb[k][0] = xB[k]; vb[k][1] = yB[k]; vb[k][2] = zB[k];
+  }
+  double u[3], v[3];
+  sub3(vb[1], vb[0], u);
+  sub3(vb[2], vb[1], v);
+  const double nb0 = u[1] * v[2] - u[2] * v[1], nb1 = u[2] * v[0] - u[0] * v[2],
+               nb2 = u[0] * v[1] - u[1] * v[0];
+  if (nb0 == 0 && nb1 == 0 && nb2 == 0) return -1;
+  sub3(va[1], va[0], u);
+  sub3(va[2], va[0], v);
+  const double na1 = u[2] * v[0] - u[0] * v[2], na2 = u[0] * v[1] - u[1] * v[0];
+  if (nb0 == 0 && na1 == 0 && na2 == 0) return -1;
+  return bf_core(va, vb, P, Q, 1);
+}
+
+static int bf_core(const double va[3][3], const double vb[3][3], double P[3], double Q[3],
+                   int zoltan) {
   /* The plane/segment pre-pass at bf.cpp:270-407 writes the outputs and is
    * always overwritten below; it has no observable effect and is omitted. */
 
@@ -267,7 +300,7 @@ int or_bf_solver(const double* xA, const double* yA, const double* zA,
     }
   }
   {
-    int k = ssid / 3, m = ssid % 3;
+    int k = ssid / 3, m = (zoltan && ssid == 1) ? 2 : ssid % 3;
     const double* oa = va[(k + 1) % 3];
     const double* ob = vb[(m + 1) % 3];
     double u[3], v[3];
@@ -308,7 +341,8 @@ int or_bf_solver(const double* xA, const double* yA, const double* zA,
   for (int q = 0; q < 3; ++q)
     onface[q] = face[0][q] + ((face[2][q] - face[0][q]) * tt) + ((face[1][q] - face[0][q]) * ss);
   if (id < 3) {
-    for (int q = 0; q < 3; ++q) { P[q] = onface[q]; Q[q] = vb[id][q]; }
+    const int vid = (zoltan && id == 1) ? 2 : id;
+    for (int q = 0; q < 3; ++q) { P[q] = onface[q]; Q[q] = vb[vid][q]; }
   } else {
     for (int q = 0; q < 3; ++q) { P[q] = va[id - 3][q]; Q[q] = onface[q]; }
   }
@@ -1132,4 +1166,272 @@ double or_gjk(const double* a, int na, const double* b, int nb, double p[3], dou
     for (int c = 0; c < 3; ++c) { p[c] = a[c]; q[c] = b[c]; }
   }
   return vlen;
+}
+
+/* ---------------------------------------------------------------------------
+ * zoltan application kernels (zoltan/penalty.cpp, zoltan/hybrid.cpp).
+ * ------------------------------------------------------------------------- */
+static void tri_tables(const double a[3], const double b[3], const double c[3],
+                       double x[3], double y[3], double z[3]) {
+  x[0] = a[0]; y[0] = a[1]; z[0] = a[2];
+  x[1] = b[0]; y[1] = b[1]; z[1] = b[2];
+  x[2] = c[0]; y[2] = c[1]; z[2] = c[2];
+}
+
+/* zoltan/penalty.cpp:27-208 */
+void or_zoltan_penalty(unsigned it, unsigned nt, const double a[3], const double b[3],
+                       const double c[3], double* const d[3], double* const e[3],
+                       double* const f[3], double* const p[3], double* const q[3]) {
+  double xa[3], ya[3], za[3];
+  tri_tables(a, b, c, xa, ya, za);
+  for (unsigned i = it; i < nt; ++i) {
+    double xb[3] = {d[0][i], e[0][i], f[0][i]}, yb[3] = {d[1][i], e[1][i], f[1][i]},
+           zb[3] = {d[2][i], e[2][i], f[2][i]};
+    pen_ctx cx;
+    pen_setup(xa, ya, za, xb, yb, zb, &cx);
+    const double trace = cx.h[0] + cx.h[5] + cx.h[10] + cx.h[15];
+    double delta = trace * 0.1;
+    const double lambda = sqrt(0.0125 * trace);
+    cx.r = lambda * pow(10, log(lambda) + 14);
+    double x[4] = {0.33, 0.33, 0.33, 0.33};
+    for (int j = 0; j < 5; ++j) {
+      double dx[4];
+      delta = i < 4 ? delta : cx.r * delta;
+      pen_step(&cx, x, delta, dx);
+      for (int k = 0; k < 4; ++k) x[k] = x[k] - dx[k];
+    }
+    double P[3], Q[3];
+    pen_points(&cx, x, P, Q);
+    for (int k = 0; k < 3; ++k) { p[k][i] = P[k]; q[k][i] = Q[k]; }
+  }
+}
+
+/* zoltan/hybrid.cpp:134-278: 7 steps, error of the last one */
+static void zh_penalty(const double A[3], const double B[3], const double C[3],
+                       const double D[3], const double E[3], const double F[3],
+                       double P[3], double Q[3], double* error) {
+  double xa[3], ya[3], za[3], xb[3], yb[3], zb[3];
+  tri_tables(A, B, C, xa, ya, za);
+  tri_tables(D, E, F, xb, yb, zb);
+  pen_ctx cx;
+  pen_setup(xa, ya, za, xb, yb, zb, &cx);
+  const double trace = cx.h[0] + cx.h[5] + cx.h[10] + cx.h[15];
+  const double delta = trace * 1E-16;
+  const double lambda = sqrt(0.0125 * trace);
+  cx.r = lambda * pow(10, log10(lambda) + 8);
+  double x[4] = {0.33, 0.33, 0.33, 0.33}, dx[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 7; ++i) {
+    pen_step(&cx, x, delta, dx);
+    for (int k = 0; k < 4; ++k) x[k] = x[k] - dx[k];
+  }
+  *error = sqrt(dot4(dx, dx) / dot4(x, x));
+  pen_points(&cx, x, P, Q);
+}
+
+/* zoltan/hybrid.cpp:280-342 */
+static int zh_segt(const double p1[3], const double p2[3], const double A[3], const double B[3],
+                   const double C[3], double P[3]) {
+  double u[3], v[3], n[3], dir[3], w0[3], w[3];
+  sub3(B, A, u);
+  sub3(C, A, v);
+  n[0] = u[1] * v[2] - u[2] * v[1];
+  n[1] = u[2] * v[0] - u[0] * v[2];
+  n[2] = u[0] * v[1] - u[1] * v[0];
+  if (n[0] == 0 && n[1] == 0 && n[2] == 0) return -1;
+  sub3(p2, p1, dir);
+  sub3(p1, A, w0);
+  const double a = -OR_DOT(n, w0);
+  const double b = OR_DOT(n, dir);
+  if (fabs(b) < 1E-99 && a == 0) return 2;
+  const double r = a / b;
+  if (r < 0.0 || r > 1.0) return 0;
+  for (int k = 0; k < 3; ++k) P[k] = p1[k] + r * dir[k];
+  const double uu = OR_DOT(u, u), uv = OR_DOT(u, v), vv = OR_DOT(v, v);
+  sub3(P, A, w);
+  const double wu = OR_DOT(w, u), wv = OR_DOT(w, v);
+  const double D = uv * uv - uu * vv;
+  const double s = (uv * wv - vv * wu) / D;
+  if (s < 0.0 || s > 1.0) return 0;
+  const double t = (uv * wu - uu * wv) / D;
+  if (t < 0.0 || (s + t) > 1.0) return 0;
+  return 1;
+}
+
+/* zoltan/hybrid.cpp:344-445, including Q's y and z taken from p4's x */
+static double zh_segseg(const double p1[3], const double p2[3], const double p3[3],
+                        const double p4[3], double P[3], double Q[3]) {
+  double u[3], v[3], w[3];
+  sub3(p1, p2, u);
+  sub3(p3, p4, v);
+  sub3(p2, p4, w);
+  const double a = OR_DOT(u, u), b = OR_DOT(u, v), c = OR_DOT(v, v), d = OR_DOT(u, w),
+               e = OR_DOT(v, w);
+  const double D = a * c - b * b;
+  double sD = D, tD = D, sN = 0, tN = 0;
+  const double small = 0.00000001;
+  if (D < small) {
+    sN = 0.0; sD = 1.0; tN = e; tD = c;
+  } else {
+    sN = (b * e - c * d);
+    tN = (a * e - b * d);
+    if (sN < 0.0) { sN = 0.0; tN = e; tD = c; }
+    else if (sN > sD) { sN = sD; tN = e + b; tD = c; }
+  }
+  if (tN < 0.0) {
+    tN = 0.0;
+    if (-d < 0.0) sN = 0.0;
+    else if (-d > a) sN = sD;
+    else { sN = -d; sD = a; }
+  } else if (tN > tD) {
+    tN = tD;
+    if ((-d + b) < 0.0) sN = 0;
+    else if ((-d + b) > a) sN = sD;
+    else { sN = (-d + b); sD = a; }
+  }
+  const double sc = (fabs(sN) < small) ? 0.0 : sN / sD;
+  const double tc = (fabs(tN) < small) ? 0.0 : tN / tD;
+  for (int k = 0; k < 3; ++k) P[k] = p2[k] + u[k] * sc;
+  Q[0] = p4[0] + v[0] * tc;
+  Q[1] = p4[0] + v[1] * tc;
+  Q[2] = p4[0] + v[2] * tc;
+  double dP[3];
+  for (int k = 0; k < 3; ++k) dP[k] = w[k] + (sc * u[k]) - (tc * v[k]);
+  return sqrt(OR_DOT(dP, dP));
+}
+
+/* zoltan/hybrid.cpp:447-644 */
+static double zh_pt(const double T1[3], const double T2[3], const double T3[3],
+                    const double cp[3], double tq[3]) {
+  double E0[3], E1[3], D[3], s, t;
+  sub3(T2, T1, E0);
+  sub3(T3, T1, E1);
+  sub3(T1, cp, D);
+  double sqr = pt_tri_sqr(OR_DOT(E0, E0), OR_DOT(E0, E1), OR_DOT(E1, E1), OR_DOT(E0, D),
+                          OR_DOT(E1, D), OR_DOT(D, D), &s, &t);
+  for (int k = 0; k < 3; ++k) tq[k] = T1[k] + (E1[k] * t) + (E0[k] * s);
+  return sqrt(sqr);
+}
+
+/* zoltan/hybrid.cpp:646-885; P and Q in/out */
+static void zh_bf(const double A[3], const double B[3], const double C[3], const double D[3],
+                  const double E[3], const double F[3], double P[3], double Q[3]) {
+  int hit = zh_segt(D, E, A, B, C, P) == 1;
+  if (!hit) hit = zh_segt(E, F, A, B, C, P) == 1;
+  if (!hit) hit = zh_segt(F, D, A, B, C, P) == 1;
+  if (!hit) hit = zh_segt(A, B, D, E, F, P) == 1;
+  if (!hit) hit = zh_segt(B, C, D, E, F, P) == 1;
+  if (!hit) hit = zh_segt(C, A, D, E, F, P) == 1;
+  if (hit) {
+    for (int k = 0; k < 3; ++k) Q[k] = P[k];
+    return;
+  }
+  const double* ta[3] = {A, B, C};
+  const double* tb[3] = {D, E, F};
+  double tp[3] = {0, 0, 0}, tq[3] = {0, 0, 0}, tmpP[3], tmpQ[3], ssP[3] = {0, 0, 0},
+         ssQ[3] = {0, 0, 0};
+  double ptmin = zh_pt(D, E, F, A, tmpP);
+  for (int k = 0; k < 3; ++k) { tp[k] = A[k]; tq[k] = tmpP[k]; }
+  for (int v = 1; v < 3; ++v) {
+    const double tmp = zh_pt(D, E, F, ta[v], tmpP);
+    if (ptmin > tmp) {
+      ptmin = tmp;
+      for (int k = 0; k < 3; ++k) { tp[k] = ta[v][k]; tq[k] = tmpP[k]; }
+    }
+  }
+  for (int v = 0; v < 3; ++v) {
+    const double tmp = zh_pt(A, B, C, tb[v], tmpP);
+    if (ptmin > tmp) {
+      ptmin = tmp;
+      for (int k = 0; k < 3; ++k) { tq[k] = tb[v][k]; tp[k] = tmpP[k]; }
+    }
+  }
+  double ssmin = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double tmp = zh_segseg(ta[i], ta[(i + 1) % 3], tb[j], tb[(j + 1) % 3], tmpP, tmpQ);
+      if ((i == 0 && j == 0) || ssmin > tmp) {
+        ssmin = tmp;
+        for (int k = 0; k < 3; ++k) { ssP[k] = tmpP[k]; ssQ[k] = tmpQ[k]; }
+      }
+    }
+  if (ssmin < ptmin) {
+    for (int k = 0; k < 3; ++k) { P[k] = ssP[k]; Q[k] = ssQ[k]; }
+  } else if (ssmin > ptmin) {
+    for (int k = 0; k < 3; ++k) { P[k] = tp[k]; Q[k] = tq[k]; }
+  } else if (ssmin == ptmin) {
+    for (int k = 0; k < 3; ++k) { P[k] = ssP[k]; Q[k] = tq[k]; }
+  }
+}
+
+/* zoltan/hybrid.cpp:37-132, serial semantics */
+void or_zoltan_hybrid(int it, int nt, const double a[3], const double b[3], const double c[3],
+                      double* const d[3], double* const e[3], double* const f[3],
+                      double* const p[3], double* const q[3]) {
+  double A[3] = {a[0], a[1], a[2]}, B[3] = {b[0], b[1], b[2]}, C[3] = {c[0], c[1], c[2]};
+  double D[3], E[3], F[3], P[3] = {0, 0, 0}, Q[3] = {0, 0, 0}, err[4];
+  for (unsigned k = (unsigned)it; k < (unsigned)nt / 4u; ++k) {
+    for (unsigned l = 0; l < 4; ++l) {
+      const unsigned i = k * 4 + l;
+      for (int m = 0; m < 3; ++m) { D[m] = d[m][i]; E[m] = e[m][i]; F[m] = f[m][i]; }
+      zh_penalty(A, B, C, D, E, F, P, Q, &err[l]);
+      for (int m = 0; m < 3; ++m) { p[m][i] = P[m]; q[m][i] = Q[m]; }
+    }
+    double batch = 0.0;
+    for (int l = 0; l < 4; ++l) batch += err[l];
+    if (batch / 4u > 1E-8) {
+      for (unsigned l = 0; l < 4; ++l) {
+        const unsigned i = k * 4 + l;
+        zh_bf(A, B, C, D, E, F, P, Q);
+        for (int m = 0; m < 3; ++m) { p[m][i] = P[m]; q[m][i] = Q[m]; }
+      }
+    }
+  }
+}
+
+/* zoltan/contact.cpp:56-240 with triangle i's y and z read (the reference
+ * leaves Ay, Az uninitialised) and early-returning bf pairs skipped.
+ * t[v][axis][i]; two-range form when all == 0.  Returns the number of kept
+ * contacts (written up to cap) in discovery order. */
+int or_zoltan_detection(int s1, int e1, int s2, int e2, int all, double* const t[6][3],
+                        const int* pid, dh_zoltan_contact* out, int cap) {
+  int n = 0;
+  for (int i = s1; i < e1; ++i) {
+    double Ax[3], Ay[3], Az[3];
+    for (int v = 0; v < 3; ++v) { Ax[v] = t[v][0][i]; Ay[v] = t[v][1][i]; Az[v] = t[v][2][i]; }
+    const int jb = all ? i + 1 : s2, je = all ? e1 : e2;
+    for (int j = jb; j < je; ++j) {
+      if (pid[i] == pid[j]) continue;
+      double Bx[3], By[3], Bz[3], P[3] = {0, 0, 0}, Q[3] = {0, 0, 0};
+      for (int v = 0; v < 3; ++v) { Bx[v] = t[v][0][j]; By[v] = t[v][1][j]; Bz[v] = t[v][2][j]; }
+      if (or_zoltan_bf(Ax, Ay, Az, Bx, By, Bz, P, Q) < 0) continue;
+      const double dist = sqrt((Q[0] - P[0]) * (Q[0] - P[0]) + (Q[1] - P[1]) * (Q[1] - P[1]) +
+                               (Q[2] - P[2]) * (Q[2] - P[2]));
+      const double margin = 15e-2;
+      if (!(dist < margin)) continue;
+      int found = 0;
+      for (int o = 0; o < n && o < cap; ++o) {
+        const dh_zoltan_contact* k = out + o;
+        if (k->pid[0] != pid[i]) continue;
+        if (k->pp[0] == P[0] || k->pp[1] == P[1] || k->pp[2] == P[2] || k->qq[0] == Q[0] ||
+            k->qq[1] == Q[1] || k->qq[2] == Q[2])
+          found = 1;
+      }
+      if (found) continue;
+      if (n < cap) {
+        dh_zoltan_contact* k = out + n;
+        const double depth = margin - dist;
+        k->pid[0] = pid[i]; k->pid[1] = pid[j];
+        k->color[0] = k->color[1] = 0;
+        for (int m = 0; m < 3; ++m) {
+          k->point[m] = (P[m] + Q[m]) / 2;
+          k->normal[m] = (Q[m] - P[m]) / depth;
+          k->pp[m] = P[m];
+          k->qq[m] = Q[m];
+        }
+        k->depth = depth;
+      }
+      ++n;
+    }
+  }
+  return n;
 }

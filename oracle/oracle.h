@@ -22,6 +22,13 @@
  *                           v02 term) + torque-free invariants.
  *   or_gjk                  oracle/_ref + equality with bfSolver minimum over
  *                           hull surfaces for separated convex sets.
+ *   or_zoltan_bf            zoltan reference golden vectors (oracle/_ref) +
+ *                           equality with or_bf_solver away from the
+ *                           fall-through cases.
+ *   or_zoltan_penalty/hybrid zoltan reference golden vectors (to libm rounding
+ *                           of pow/log/log10) + convergence to the brute force.
+ *   or_zoltan_detection     composition of or_zoltan_bf with the contact
+ *                           formulas; hand-built de-duplication cases.
  */
 #ifndef DELTA_ORACLE_H
 #define DELTA_ORACLE_H
@@ -109,6 +116,22 @@ typedef struct or_scene {
 
 int64_t or_step(or_scene* s, int method, const dh_force_params* fp, double dt,
                 const double gravity[3], int64_t* checks, int* npairs_out);
+
+/* zoltan/bf.cpp:60-952; -1 when the reference returns before writing P, Q. */
+int or_zoltan_bf(const double* xA, const double* yA, const double* zA,
+                 const double* xB, const double* yB, const double* zB,
+                 double P[3], double Q[3]);
+/* zoltan/penalty.cpp:27-208 and zoltan/hybrid.cpp:37-132 (serial semantics). */
+void or_zoltan_penalty(unsigned it, unsigned nt, const double a[3], const double b[3],
+                       const double c[3], double* const d[3], double* const e[3],
+                       double* const f[3], double* const p[3], double* const q[3]);
+void or_zoltan_hybrid(int it, int nt, const double a[3], const double b[3], const double c[3],
+                      double* const d[3], double* const e[3], double* const f[3],
+                      double* const p[3], double* const q[3]);
+/* zoltan/contact.cpp:56-240 (see dh_zoltan_detection); all != 0: j > i in
+ * [s1, e1), else j in [s2, e2).  Returns the number of kept contacts. */
+int or_zoltan_detection(int s1, int e1, int s2, int e2, int all, double* const t[6][3],
+                        const int* pid, dh_zoltan_contact* out, int cap);
 
 /* GJK distance between two convex point sets (gjk.cpp:302-460). */
 double or_gjk(const double* a, int na, const double* b, int nb, double p[3], double q[3]);
