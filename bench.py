@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="bf64,penalty64,hybrid64,mixed,learned")
+    ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,learned")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel three times "
                          "(reference for scaling HBM byte counters) and exit")
@@ -276,6 +276,19 @@ def main():
         out["learned_vs_bf64"] = learned_accuracy(batch)
     if "mixed" in results:
         out["mixed_vs_bf64"] = mixed_accuracy(batch)
+    if "bf64_exact" in results:
+        # the generator's intersecting pairs under both semantics: bfSolver
+        # (feature distance > 0) and the exact mode (d = 0)
+        kind = batch.kind.download()
+        batch.run("bf64_exact")
+        _, _, d_x, f_x = batch.results()
+        batch.run("bf64")
+        _, _, d_b, _ = batch.results()
+        hit = (f_x & 16) != 0
+        out["intersecting_pairs"] = {
+            "generated": int((kind == 2).sum()), "exact_mode_d0": int(hit.sum()),
+            "bfSolver_d_min_on_them": float(d_b[hit].min()) if hit.any() else None,
+            "bfSolver_d_median_on_them": float(np.median(d_b[hit])) if hit.any() else None}
     if not a.no_cpu:
         cb, chk = cpu_baseline(batch, a.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -6,6 +6,7 @@
 #include "exact_math.hpp"
 #include "runtime.hpp"
 #include "tri_pairs.hpp"
+#include "zoltan_math.hpp"
 
 namespace dh {
 
@@ -55,6 +56,20 @@ __global__ __launch_bounds__(256) void k_tri_pairs_exact(const double* __restric
     if (METHOD == DH_PAIR_HYBRID64 && failed) {
       fl |= DH_FLAG_RECHECK;
       if (bf_solver(A, B, p, q)) fl |= DH_FLAG_EDGE;
+    }
+  } else if (METHOD == DH_PAIR_BF64_EXACT) {
+    V3<double> x = {0, 0, 0};
+    const bool hit = zh_segt(B.v[0], B.v[1], A.v[0], A.v[1], A.v[2], x) == 1 ||
+                     zh_segt(B.v[1], B.v[2], A.v[0], A.v[1], A.v[2], x) == 1 ||
+                     zh_segt(B.v[2], B.v[0], A.v[0], A.v[1], A.v[2], x) == 1 ||
+                     zh_segt(A.v[0], A.v[1], B.v[0], B.v[1], B.v[2], x) == 1 ||
+                     zh_segt(A.v[1], A.v[2], B.v[0], B.v[1], B.v[2], x) == 1 ||
+                     zh_segt(A.v[2], A.v[0], B.v[0], B.v[1], B.v[2], x) == 1;
+    if (hit) {
+      p = q = x;
+      fl |= DH_FLAG_INTERSECT;
+    } else if (bf_solver(A, B, p, q)) {
+      fl |= DH_FLAG_EDGE;
     }
   } else {
     penalty_adaptive(A, B, max_err, p, q);
@@ -245,6 +260,10 @@ int dh_tri_pairs(int method, const double* soa, const float* soa32, int64_t n, d
     case DH_PAIR_PENALTY_ADAPT64:
       hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_PENALTY_ADAPT64>, grid, 256, 0, st, soa, n,
                          max_err, P, Q, d, flags);
+      break;
+    case DH_PAIR_BF64_EXACT:
+      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_BF64_EXACT>, grid, 256, 0, st, soa, n, max_err,
+                         P, Q, d, flags);
       break;
     case DH_PAIR_MIXED:
     case DH_PAIR_LEARNED:

@@ -18,6 +18,7 @@ METHODS = {
     "mixed": _lib.DH_PAIR_MIXED,
     "learned": _lib.DH_PAIR_LEARNED,
     "penalty_adapt64": _lib.DH_PAIR_PENALTY_ADAPT64,
+    "bf64_exact": _lib.DH_PAIR_BF64_EXACT,
 }
 
 # algorithmic cost per pair (operation counts of the reference's arithmetic,

@@ -55,7 +55,11 @@ enum {
   DH_PAIR_HYBRID64 = 2,    /* penalty, bfSolver where the pair's `failed` is set   */
   DH_PAIR_MIXED = 3,       /* FP32 feature screen + FP64 exact finalisation        */
   DH_PAIR_LEARNED = 4,     /* MLP warm start / feature guess + FP64 certificate    */
-  DH_PAIR_PENALTY_ADAPT64 = 5 /* penaltySolver(..., int& its), <=16 its, FP64       */
+  DH_PAIR_PENALTY_ADAPT64 = 5, /* penaltySolver(..., int& its), <=16 its, FP64      */
+  DH_PAIR_BF64_EXACT = 6   /* segment-triangle intersection test first (the zoltan
+                              segt of zoltan/hybrid.cpp:280-342, in its order of
+                              hybrid.cpp:652-670): intersecting pairs get P = Q =
+                              the crossing point, d = 0; the rest bfSolver      */
 };
 
 /* flags[] bits written by dh_tri_pairs */
@@ -63,7 +67,8 @@ enum {
   DH_FLAG_FAILED = 1,      /* penalty `failed` bit of the last Newton step         */
   DH_FLAG_RECHECK = 2,     /* FP64 path taken (screen could not certify)           */
   DH_FLAG_EDGE = 4,        /* edge-edge feature won (ptmin > ssmin)                */
-  DH_FLAG_CERTIFIED = 8    /* learned guess certified without the full BF          */
+  DH_FLAG_CERTIFIED = 8,   /* learned guess certified without the full BF          */
+  DH_FLAG_INTERSECT = 16   /* DH_PAIR_BF64_EXACT: the triangles intersect, d = 0   */
 };
 
 /* ------------------------------------------------------------ contact record */

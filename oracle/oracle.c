@@ -732,6 +732,21 @@ int or_detect(int method,
   return (e.n > cap) ? DH_ERR_CAPACITY : DH_OK;
 }
 
+static int zh_segt(const double p1[3], const double p2[3], const double A[3], const double B[3],
+                   const double C[3], double P[3]);
+
+/* The intersection pre-test of the exact mode: the six segment-triangle tests
+ * of zoltan/hybrid.cpp:652-670 in their order; P = the crossing point. */
+static int segt_chain(const double va[3][3], const double vb[3][3], double P[3]) {
+  if (zh_segt(vb[0], vb[1], va[0], va[1], va[2], P) == 1) return 1;
+  if (zh_segt(vb[1], vb[2], va[0], va[1], va[2], P) == 1) return 1;
+  if (zh_segt(vb[2], vb[0], va[0], va[1], va[2], P) == 1) return 1;
+  if (zh_segt(va[0], va[1], vb[0], vb[1], vb[2], P) == 1) return 1;
+  if (zh_segt(va[1], va[2], vb[0], vb[1], vb[2], P) == 1) return 1;
+  if (zh_segt(va[2], va[0], vb[0], vb[1], vb[2], P) == 1) return 1;
+  return 0;
+}
+
 int or_tri_pairs(int method, const double* soa, int64_t n, double eps_total,
                  double* P, double* Q, double* d, uint8_t* flags) {
   const double max_err = eps_total / 16.0;
@@ -766,6 +781,19 @@ int or_tri_pairs(int method, const double* soa, int64_t n, double eps_total,
       case DH_PAIR_PENALTY_ADAPT64: {
         int its = 0;
         or_penalty_solver_adaptive(xa, ya, za, xb, yb, zb, p, q, max_err, &its);
+      } break;
+      case DH_PAIR_BF64_EXACT: {
+        double va[3][3], vb[3][3], x[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) {
+          va[k][0] = xa[k]; va[k][1] = ya[k]; va[k][2] = za[k];
+          vb[k][0] = xb[k]; vb[k][1] = yb[k]; vb[k][2] = zb[k];
+        }
+        if (segt_chain(va, vb, x)) {
+          for (int c = 0; c < 3; ++c) p[c] = q[c] = x[c];
+          fl |= DH_FLAG_INTERSECT;
+        } else if (or_bf_solver(xa, ya, za, xb, yb, zb, p, q)) {
+          fl |= DH_FLAG_EDGE;
+        }
       } break;
       default:
         return DH_ERR_ARG;

@@ -53,7 +53,7 @@ def gen_batch():
     return narrow.PairBatch(1 << 16, seed=3)
 
 
-@pytest.mark.parametrize("method", ["bf64", "penalty64", "hybrid64", "penalty_adapt64"])
+@pytest.mark.parametrize("method", ["bf64", "penalty64", "hybrid64", "penalty_adapt64", "bf64_exact"])
 def test_generated_pairs_bitexact(gen_batch, method):
     soa = gen_batch.soa.download()
     gen_batch.run(method)
@@ -65,7 +65,7 @@ def test_generated_pairs_bitexact(gen_batch, method):
     assert np.array_equal(f, fo)
 
 
-@pytest.mark.parametrize("method", ["bf64", "penalty64", "hybrid64", "penalty_adapt64"])
+@pytest.mark.parametrize("method", ["bf64", "penalty64", "hybrid64", "penalty_adapt64", "bf64_exact"])
 def test_degenerate_pairs_bitexact(method):
     soa = degenerate_soa()
     b = narrow.PairBatch.from_host(soa)
@@ -131,3 +131,15 @@ def test_mixed_needs_binary32_copy():
     b = narrow.PairBatch(64, seed=1)
     with pytest.raises(_lib.DhError):
         b.run("mixed")
+
+
+def test_exact_mode_flags_generated_intersections(gen_batch):
+    """The generator pushes 1% of the B triangles through A (kind 2): the exact
+    mode reports them with d = 0, and none of the separated pairs."""
+    gen_batch.run("bf64_exact")
+    _, _, d, f = gen_batch.results()
+    kind = gen_batch.kind.download()
+    hit = (f & 16) != 0
+    assert hit[kind == 2].mean() > 0.99
+    assert not hit[kind != 2].any()
+    assert (d[hit] == 0).all()

@@ -213,3 +213,32 @@ def test_head_on_spring_dashpot():
     want = (K * 1e-4 + z * 2 * np.sqrt(ms * K) * vn) * np.array([-1.0, 0, 0])
     assert np.allclose(f, want, rtol=1e-14)
     assert np.allclose(t, 0)
+
+
+def test_exact_mode_reports_intersections():
+    """DH_PAIR_BF64_EXACT (oracle): a triangle pierced by an edge of the other
+    gets d = 0 with P = Q on both triangles; separated pairs equal bfSolver bit
+    for bit."""
+    import ctypes as C
+    A = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float)
+    pierce = np.array([[0.2, 0.2, -0.5], [0.25, 0.2, 0.5], [2.0, 2.0, 2.0]])
+    above = A + np.array([0.1, 0.1, 0.3])
+    n = 2
+    soa = np.empty((18, n))
+    for i, B in enumerate((pierce, above)):
+        for ax in range(3):
+            for v in range(3):
+                soa[3 * ax + v, i] = A[v, ax]
+                soa[9 + 3 * ax + v, i] = B[v, ax]
+    outs = []
+    for m in (6, 0):
+        P = np.zeros(3 * n); Q = np.zeros(3 * n); d = np.zeros(n); f = np.zeros(n, np.uint8)
+        rc = ol.oracle().or_tri_pairs(m, ol.dptr(soa), C.c_int64(n), C.c_double(1e-3), ol.dptr(P),
+                                      ol.dptr(Q), ol.dptr(d), f.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        outs.append((P.reshape(3, n), Q.reshape(3, n), d, f))
+    (P, Q, d, f), (P0, Q0, d0, f0) = outs
+    assert d[0] == 0.0 and f[0] & 16 and np.array_equal(P[:, 0], Q[:, 0])
+    assert abs(P[2, 0]) < 1e-15 and 0 < P[0, 0] < 1 and 0 < P[1, 0] < 1   # on A's plane z = 0
+    assert d0[0] > 0                                                        # bfSolver semantics
+    assert not f[1] & 16 and d[1] == d0[1] and np.array_equal(P[:, 1], P0[:, 1])
