@@ -38,6 +38,9 @@ def parse():
                     help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,learned")
+    ap.add_argument("--workload", default="pairs", choices=["pairs", "c1-dem"],
+                    help="pairs: 2^24 unique triangle pairs (the headline); c1-dem: the "
+                         "64-icosphere DEM step (particle-steps/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel three times "
                          "(reference for scaling HBM byte counters) and exit")
@@ -205,10 +208,73 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_each_way": 1 << 28, "launches": 3}), flush=True)
 
 
+def dem_c1(a):
+    """64 jittered 80-triangle icospheres on a 4x4x4 lattice (1.05 mm) above a
+    2-triangle floor, FP64 brute force, spring-dashpot k=1e5 N/m, damping 0.1,
+    margin 1e-4 m, dt = 1e-6 s; `--steps` steps timed after `--warmup`."""
+    from delta_amd import _lib, engine
+    hs = engine.c1_scene()
+    ds = engine.DeviceScene(hs)
+    prm = engine.step_params(method=_lib.DH_BF, dt=1e-6, spring=1e5, damper=0.1, friction=0.0,
+                             cell=hs.cell_size())
+    L = _lib.lib()
+    for _ in range(a.warmup):
+        engine.step(ds, prm)
+    _lib.check(L.dh_synchronize(), "sync")
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.dh_event_create(C.byref(e0))
+    L.dh_event_create(C.byref(e1))
+    checks = contacts = pairs = 0
+    L.dh_event_record(e0, None)
+    for _ in range(a.steps):
+        info = engine.step(ds, prm)
+        checks += info.triangle_checks
+        contacts += info.contacts
+        pairs += info.candidate_pairs
+    L.dh_event_record(e1, None)
+    _lib.check(L.dh_synchronize(), "sync")
+    ms = C.c_float()
+    L.dh_event_elapsed_ms(C.byref(ms), e0, e1)
+    per = ms.value / a.steps
+    nparticles = hs.n
+    out = {"metric": "particle-steps/s (64-icosphere DEM step, FP64 brute force)",
+           "value": nparticles * a.steps / (ms.value * 1e-3), "unit": "particle-steps/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": per,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "64 jittered 80-triangle icospheres, 4x4x4 lattice at 1.05 mm, "
+                                  "2-triangle floor, dt 1e-6 s, FP64 brute force",
+                      "particles": nparticles},
+           "triangle_checks_per_s": checks / (ms.value * 1e-3),
+           "mean_candidate_pairs": pairs / a.steps, "mean_contacts": contacts / a.steps}
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        hc = engine.c1_scene()
+        k = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < a.cpu_seconds and k < a.steps + a.warmup:
+            ol.or_step(hc, _lib.DH_BF, 1e5, 0.1, 0.0, 1e-6, (0.0, -9.81, 0.0))
+            k += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nparticles * k / dt, "unit": "particle-steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"the first {k} steps of the same scene, oracle step "
+                                         f"(FP64 brute force, one thread), {dt:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
         copy_calibration()
+        return
+    if a.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a "
+                                   "design note only (bench.py --gpus 1)"}))
+        sys.exit(2)
+    if a.workload == "c1-dem":
+        dem_c1(a)
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a "
