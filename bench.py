@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,learned")
+    ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,"
+                                         "penalty_cert,learned_penalty,learned")
     ap.add_argument("--workload", default="pairs", choices=["pairs", "c1-dem"],
                     help="pairs: 2^24 unique triangle pairs (the headline); c1-dem: the "
                          "64-icosphere DEM step (particle-steps/s)")
@@ -120,6 +121,21 @@ def learned_accuracy(batch):
             "max_normal_error": float(np.abs(n1 - n0)[:, contact].max()) if contact.any() else 0.0,
             "max_depth_error_m": float(np.abs(d - d0)[contact].max()) if contact.any() else 0.0,
             "bitwise_equal_fraction": float(np.mean(d.view(np.uint64) == d0.view(np.uint64)))}
+
+
+def warm_start_report(batch):
+    """Penalty Newton from the network's barycentric outputs against the same
+    from the reference's 0.33 start: Newton steps per pair and certified
+    fraction (the rest goes to the brute force)."""
+    from delta_amd import _lib
+    out = {}
+    for m in ("learned_penalty", "penalty_cert"):
+        batch.run(m)
+        cnt = _lib.DhPairCounters()
+        _lib.check(_lib.lib().dh_tri_pairs_counters(C.byref(cnt)), "counters")
+        out[m] = {"newton_steps_per_pair": cnt.features_fp64 / batch.n,
+                  "certified_fraction": cnt.certified / batch.n}
+    return out
 
 
 def mixed_accuracy(batch):
@@ -342,6 +358,8 @@ def main():
         out["learned_vs_bf64"] = learned_accuracy(batch)
     if "mixed" in results:
         out["mixed_vs_bf64"] = mixed_accuracy(batch)
+    if "learned_penalty" in results and "penalty_cert" in results:
+        out["network_warm_start"] = warm_start_report(batch)
     if "bf64_exact" in results:
         # the generator's intersecting pairs under both semantics: bfSolver
         # (feature distance > 0) and the exact mode (d = 0)

@@ -14,7 +14,7 @@ DH_PENALTY_STAT, DH_HYBRID_BATCH_STAT, DH_HYBRID_TRIANGLE_STAT, DH_GJK = 4, 5, 6
 # unique-pair methods
 DH_PAIR_BF64, DH_PAIR_PENALTY64, DH_PAIR_HYBRID64 = 0, 1, 2
 DH_PAIR_MIXED, DH_PAIR_LEARNED, DH_PAIR_PENALTY_ADAPT64 = 3, 4, 5
-DH_PAIR_BF64_EXACT = 6
+DH_PAIR_BF64_EXACT, DH_PAIR_LEARNED_PENALTY, DH_PAIR_PENALTY_CERT = 6, 7, 8
 DH_FLAG_FAILED, DH_FLAG_RECHECK, DH_FLAG_EDGE, DH_FLAG_CERTIFIED, DH_FLAG_INTERSECT = 1, 2, 4, 8, 16
 
 VP = C.c_void_p

@@ -365,7 +365,14 @@ __global__ void k_adam(float* __restrict__ W, float* __restrict__ m, float* __re
 constexpr int kInferWaves = 8;
 // A-operand tiles (16 x 32) of the three layers for v_mfma_f32_16x16x32:
 // W1 4 output tiles x 1 k-block, W2 4 x 2, W3 (first output tile only) 1 x 2.
-constexpr int kTiles = 4 + 8 + 2;
+constexpr int kTiles = 4 + 8 + 2 + 2;  // + W3 output tile 1 (barycentric outputs)
+
+// Inference modes of k_learned.
+enum LearnedMode : int {
+  kModeFeatures = 0,  // feature guesses + certificate (DH_PAIR_LEARNED)
+  kModeWarm = 1,      // network barycentrics as the penalty start (DH_PAIR_LEARNED_PENALTY)
+  kModeCold = 2       // the same penalty from the reference's 0.33 start (DH_PAIR_PENALTY_CERT)
+};
 constexpr int kCol = 68;             // row stride (dwords) of the per-wave staging
 constexpr int kSubs = 2;             // 16-pair sub-tiles evaluated together
 
@@ -413,12 +420,15 @@ __device__ inline v8s stage_tile(const float* W, int t, int lane) {
       w[j] = W[kW1 + (8 * g + j) * kHid + 16 * t + r];
     else if (t < 12)
       w[j] = W[kW2 + hidden_k((t - 4) % 2, g, j) * kHid + 16 * ((t - 4) / 2) + r];
-    else
+    else if (t < 14)
       w[j] = W[kW3 + hidden_k(t - 12, g, j) * kOut + r];
+    else
+      w[j] = W[kW3 + hidden_k(t - 14, g, j) * kOut + 16 + r];
   }
   return cat8(pack4(w[0], w[1], w[2], w[3]), pack4(w[4], w[5], w[6], w[7]));
 }
 
+template <int MODE>
 __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_learned(LearnedArgs a) {
   __shared__ v8s w_s[kTiles * 64];
   __shared__ float b_s[2 * kHid + kOut];
@@ -433,8 +443,10 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
   // stage the network once per workgroup
-  for (int t = wave; t < kTiles; t += kInferWaves) w_s[t * 64 + lane] = stage_tile(a.W, t, lane);
-  for (int i = threadIdx.x; i < 2 * kHid + kOut; i += 64 * kInferWaves)
+  // (no network in the cold mode: a.W may be null there)
+  for (int t = wave; t < (MODE == kModeCold ? 0 : kTiles); t += kInferWaves)
+    w_s[t * 64 + lane] = stage_tile(a.W, t, lane);
+  for (int i = threadIdx.x; i < (MODE == kModeCold ? 0 : 2 * kHid + kOut); i += 64 * kInferWaves)
     b_s[i] = (i < kHid) ? a.W[kB1 + i] : (i < 2 * kHid) ? a.W[kB2 + i - kHid]
                                                           : a.W[kB3 + i - 2 * kHid];
   if (threadIdx.x < 3) wg_tally[threadIdx.x] = 0;
@@ -447,10 +459,10 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
     const int64_t i = ch * 64 + lane;
     const bool valid = i < a.n;
     const int64_t ii = valid ? i : 0;
-    bool frame;
+    bool frame = true;
     const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
     const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
-    {
+    if (MODE != kModeCold) {
       float in[18];
       frame = canonical_inputs(A, B, in);
       uint32_t* col = &io_s[wave][lane];
@@ -464,7 +476,7 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
     // kSubs sub-tiles of 16 pairs at a time, layer by layer: independent
     // MFMA chains share each weight tile read from LDS.
 #pragma unroll 1
-    for (int s0 = 0; s0 < 4; s0 += kSubs) {
+    for (int s0 = 0; s0 < (MODE == kModeCold ? 0 : 4); s0 += kSubs) {
       // opaque per group: keeps the compiler from hoisting the weight and
       // bias loads out of the loops into ~100 registers
       int wl = lane, bl = 4 * g;
@@ -515,7 +527,7 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
           act2[u][to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
         }
       }
-      {  // output tile 0: feature logits and the exact-path logit
+      if (MODE == kModeFeatures) {  // output tile 0: feature logits and the exact-path logit
         const v8s w0 = lds_tile(w_s, 12, wl), w1 = lds_tile(w_s, 13, wl);
         v4f b;
 #pragma unroll
@@ -528,30 +540,46 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
 #pragma unroll
           for (int j = 0; j < 4; ++j) io_s[wave][(4 * g + j) * kCol + pl] = __float_as_uint(acc[j]);
         }
+      } else {  // output tile 1: outputs 16..19 (lane group 0) = the penalty start, to rows 0..3
+        const v8s w0 = lds_tile(w_s, 14, wl), w1 = lds_tile(w_s, 15, wl);
+        v4f b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = b_s[2 * kHid + 16 + bl + j];
+#pragma unroll
+        for (int u = 0; u < kSubs; ++u) {
+          v4f acc = mfma32(w0, cat8(act2[u][0], act2[u][1]), b);
+          acc = mfma32(w1, cat8(act2[u][2], act2[u][3]), acc);
+          const int pl = (s0 + u) * 16 + c;
+          if (g == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) io_s[wave][j * kCol + pl] = __float_as_uint(acc[j]);
+          }
+        }
       }
     }
     lds_wait();
     __builtin_amdgcn_wave_barrier();
-    // the lane's own pair: the best-ranked edge-edge and the best-ranked
-    // vertex-face feature, both evaluated (no divergence between the two
-    // kinds), the nearer one certified
-    int fe = 0, fv = 9;
-    float le = -3.0e38f, lv = -3.0e38f;
-#pragma unroll
-    for (int f = 0; f < kNumFeat; ++f) {
-      const float v = __uint_as_float(io_s[wave][f * kCol + lane]);
-      if (f < 9) {
-        fe = v > le ? f : fe;
-        le = v > le ? v : le;
-      } else {
-        fv = v > lv ? f : fv;
-        lv = v > lv ? v : lv;
-      }
-    }
-    const bool exact_path = !frame || (__uint_as_float(io_s[wave][kFlagOut * kCol + lane]) > 0.f);
     V3<double> P, Q;
     bool done = false;
-    {
+    if (MODE == kModeFeatures) {
+      // the lane's own pair: the best-ranked edge-edge and the best-ranked
+      // vertex-face feature, both evaluated (no divergence between the two
+      // kinds), the nearer one certified
+      int fe = 0, fv = 9;
+      float le = -3.0e38f, lv = -3.0e38f;
+#pragma unroll
+      for (int f = 0; f < kNumFeat; ++f) {
+        const float v = __uint_as_float(io_s[wave][f * kCol + lane]);
+        if (f < 9) {
+          fe = v > le ? f : fe;
+          le = v > le ? v : le;
+        } else {
+          fv = v > lv ? f : fv;
+          lv = v > lv ? v : lv;
+        }
+      }
+      const bool exact_path =
+          !frame || (__uint_as_float(io_s[wave][kFlagOut * kCol + lane]) > 0.f);
       V3<double> Pv, Qv;
       eval_ee(A, B, fe, P, Q);
       const V3<double> ne = sub(Q, P);
@@ -567,6 +595,15 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
       Q = sel3(take_v, Qv, Q);
       done = valid && !exact_path && certify(A, B, P, Q);
       evals += (valid && !exact_path) ? 2 : 0;
+    } else {
+      // penalty Newton from the network's barycentric guess (or 0.33), the
+      // result clamped onto the triangles and certified like a feature
+      double x0[4] = {0.33, 0.33, 0.33, 0.33};
+      if (MODE == kModeWarm)
+        for (int k = 0; k < 4; ++k) x0[k] = __uint_as_float(io_s[wave][k * kCol + lane]);
+      const int its = warm_penalty(A, B, x0, P, Q);
+      done = valid && certify(A, B, P, Q);
+      evals += valid ? its : 0;
     }
     // every lane stores (the brute force overwrites the uncertified ones):
     // whole cache lines, no partial-line writes
@@ -649,9 +686,9 @@ struct LearnedState {
 };
 static LearnedState g_learned;
 
-int learned_pairs(const double* soa, int64_t n, double* P, double* Q, double* d, uint8_t* flags,
-                  hipStream_t st) {
-  if (!g_learned.model)
+int learned_pairs(int mode, const double* soa, int64_t n, double* P, double* Q, double* d,
+                  uint8_t* flags, hipStream_t st) {
+  if (mode != kModeCold && !g_learned.model)
     return set_error(DH_ERR_ARG, "learned narrow phase: no network set (dh_learned_set_model)");
   const int64_t chunks = (n + 63) / 64;
   int64_t grid = (chunks + kInferWaves - 1) / kInferWaves;
@@ -669,7 +706,15 @@ int learned_pairs(const double* soa, int64_t n, double* P, double* Q, double* d,
   LearnedArgs a{soa, n, g_learned.model, P, Q, d, flags, g_learned.queue.as<int>(),
                 g_learned.qsoa.as<double>(), qcap, region, g_learned.wg_counts.as<unsigned>()};
   if (n > 0) {
-    hipLaunchKernelGGL(k_learned, static_cast<unsigned>(grid), 64 * kInferWaves, 0, st, a);
+    if (mode == kModeWarm)
+      hipLaunchKernelGGL(k_learned<kModeWarm>, static_cast<unsigned>(grid), 64 * kInferWaves, 0,
+                         st, a);
+    else if (mode == kModeCold)
+      hipLaunchKernelGGL(k_learned<kModeCold>, static_cast<unsigned>(grid), 64 * kInferWaves, 0,
+                         st, a);
+    else
+      hipLaunchKernelGGL(k_learned<kModeFeatures>, static_cast<unsigned>(grid), 64 * kInferWaves,
+                         0, st, a);
   }
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
@@ -770,8 +815,11 @@ int dh_learned_set_model(const float* params_dev) {
 }  // extern "C"
 
 namespace dh {
-int tri_pairs_learned(const double* soa, int64_t n, double* P, double* Q, double* d,
-                      uint8_t* flags, hipStream_t st) {
-  return learned_pairs(soa, n, P, Q, d, flags, st);
+int tri_pairs_learned(int method, const double* soa, int64_t n, double* P, double* Q,
+                      double* d, uint8_t* flags, hipStream_t st) {
+  const int mode = method == DH_PAIR_LEARNED_PENALTY ? kModeWarm
+                   : method == DH_PAIR_PENALTY_CERT  ? kModeCold
+                                                     : kModeFeatures;
+  return learned_pairs(mode, soa, n, P, Q, d, flags, st);
 }
 }  // namespace dh

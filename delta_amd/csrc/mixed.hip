@@ -16,8 +16,8 @@
 
 namespace dh {
 
-int tri_pairs_learned(const double* soa, int64_t n, double* P, double* Q, double* d,
-                      uint8_t* flags, hipStream_t st);
+int tri_pairs_learned(int method, const double* soa, int64_t n, double* P, double* Q,
+                      double* d, uint8_t* flags, hipStream_t st);
 int learned_counters(unsigned long long out[3]);
 
 constexpr int kMixedWaves = 4;
@@ -170,7 +170,9 @@ int tri_pairs_extra(int method, const double* soa, const float* soa32, int64_t n
                     double eps_total, double* P, double* Q, double* d, uint8_t* flags,
                     hipStream_t st) {
   g_last_extra = method;
-  if (method == DH_PAIR_LEARNED) return tri_pairs_learned(soa, n, P, Q, d, flags, st);
+  if (method == DH_PAIR_LEARNED || method == DH_PAIR_LEARNED_PENALTY ||
+      method == DH_PAIR_PENALTY_CERT)
+    return tri_pairs_learned(method, soa, n, P, Q, d, flags, st);
   if (method == DH_PAIR_MIXED)
     return tri_pairs_mixed(soa, soa32, n, eps_total, P, Q, d, flags, st);
   return set_error(DH_ERR_UNSUPPORTED, "dh_tri_pairs: method %d not available", method);

@@ -86,6 +86,49 @@ DH_HD bool certify(const Tri<double>& A, const Tri<double>& B, const V3<double>&
   return ok;
 }
 
+// (s, t) moved into the parameter triangle s, t >= 0, s + t <= 1.
+DH_HD void clamp_bary(double& s, double& t) {
+  s = s > 0.0 ? s : 0.0;
+  t = t > 0.0 ? t : 0.0;
+  const double u = s + t;
+  if (u > 1.0) {
+    s = s / u;
+    t = t / u;
+  }
+}
+
+// The reference's penalty Newton iteration (the arithmetic of
+// penaltySolver(..., int&), penalty.cpp:434-581: regularisation x1e5 from the
+// fourth step) started from x0, at most 16 steps, stopping once
+// |dx|^2 / |x|^2 < 1e-20 (a Newton residual of 1e-10); the result is moved
+// onto the two triangles (parameters clamped into their simplices) so that a
+// certificate can bound it.  Returns the number of steps taken.
+DH_HD int warm_penalty(const Tri<double>& A, const Tri<double>& B, const double x0[4],
+                       V3<double>& P, V3<double>& Q) {
+  Penalty<double> pen;
+  pen.setup(A, B);
+  double x[4] = {x0[0], x0[1], x0[2], x0[3]};
+  double reg = pen.reg0;
+  int its = 16;
+  for (int i = 0; i < 16; ++i) {
+    double dx[4];
+    reg = i < 3 ? reg : 1E5 * reg;
+    pen.step(x, reg, dx);
+    x[0] = x[0] - dx[0];
+    x[1] = x[1] - dx[1];
+    x[2] = x[2] - dx[2];
+    x[3] = x[3] - dx[3];
+    if (dot4(dx, dx) / dot4(x, x) < 1e-20) {
+      its = i + 1;
+      break;
+    }
+  }
+  clamp_bary(x[0], x[1]);
+  clamp_bary(x[2], x[3]);
+  pen.points(x, P, Q);
+  return its;
+}
+
 // Feature code of bfSolver's winner (same selection as bf_solver).
 DH_HD int bf_feature_code(const Tri<double>& A, const Tri<double>& B, V3<double>& P,
                           V3<double>& Q) {

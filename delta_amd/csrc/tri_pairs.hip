@@ -267,6 +267,8 @@ int dh_tri_pairs(int method, const double* soa, const float* soa32, int64_t n, d
       break;
     case DH_PAIR_MIXED:
     case DH_PAIR_LEARNED:
+    case DH_PAIR_LEARNED_PENALTY:
+    case DH_PAIR_PENALTY_CERT:
       return tri_pairs_extra(method, soa, soa32, n, eps_total, P, Q, d, flags, st);
     default:
       return set_error(DH_ERR_ARG, "dh_tri_pairs: unknown method %d", method);

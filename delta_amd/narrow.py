@@ -19,6 +19,8 @@ METHODS = {
     "learned": _lib.DH_PAIR_LEARNED,
     "penalty_adapt64": _lib.DH_PAIR_PENALTY_ADAPT64,
     "bf64_exact": _lib.DH_PAIR_BF64_EXACT,
+    "learned_penalty": _lib.DH_PAIR_LEARNED_PENALTY,
+    "penalty_cert": _lib.DH_PAIR_PENALTY_CERT,
 }
 
 # algorithmic cost per pair (operation counts of the reference's arithmetic,

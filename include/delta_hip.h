@@ -56,10 +56,17 @@ enum {
   DH_PAIR_MIXED = 3,       /* FP32 feature screen + FP64 exact finalisation        */
   DH_PAIR_LEARNED = 4,     /* MLP warm start / feature guess + FP64 certificate    */
   DH_PAIR_PENALTY_ADAPT64 = 5, /* penaltySolver(..., int& its), <=16 its, FP64      */
-  DH_PAIR_BF64_EXACT = 6   /* segment-triangle intersection test first (the zoltan
+  DH_PAIR_BF64_EXACT = 6,  /* segment-triangle intersection test first (the zoltan
                               segt of zoltan/hybrid.cpp:280-342, in its order of
                               hybrid.cpp:652-670): intersecting pairs get P = Q =
                               the crossing point, d = 0; the rest bfSolver      */
+  DH_PAIR_LEARNED_PENALTY = 7, /* the network's barycentric outputs start the penalty
+                                  Newton iteration (<= 16 steps, residual 1e-10), the
+                                  result is clamped onto the triangles and certified
+                                  like DH_PAIR_LEARNED; the rest bfSolver          */
+  DH_PAIR_PENALTY_CERT = 8     /* the same from the reference's start x = 0.33
+                                  (penalty.cpp:353-356), no network: what the
+                                  network's start gains                           */
 };
 
 /* flags[] bits written by dh_tri_pairs */
@@ -281,7 +288,8 @@ int dh_learned_set_model(const float* params_dev);
 typedef struct dh_pair_counters {
   int64_t screened;     /* pairs finished by the FP32 screen (MIXED)        */
   int64_t rechecked;    /* pairs finished by the full FP64 bfSolver         */
-  int64_t features_fp64;/* feature evaluations done in FP64 by finalisation */
+  int64_t features_fp64;/* FP64 feature evaluations (LEARNED) or Newton steps
+                           (LEARNED_PENALTY, PENALTY_CERT)                    */
   int64_t certified;    /* pairs certified from the learned guess           */
 } dh_pair_counters;
 int dh_tri_pairs_counters(dh_pair_counters* out);

@@ -107,3 +107,32 @@ def test_learned_ragged_sizes(train_set, n):
     assert cnt.certified + cnt.rechecked == n
     assert (np.abs(d - d0) / d0).max() <= 1e-6
     assert np.array_equal(d[~cert].view(np.uint64), d0[~cert].view(np.uint64))
+
+
+def test_network_warm_start_of_the_penalty_newton(train_set):
+    """DH_PAIR_LEARNED_PENALTY starts the penalty Newton iteration from the
+    network's barycentric outputs; DH_PAIR_PENALTY_CERT from the reference's
+    0.33.  Both certify or fall back, so both are correct; the trained start
+    needs fewer Newton steps and certifies more pairs."""
+    _, ds = train_set
+    tr = learn.Trainer(seed=0)
+    tr.fit(ds, epochs=6, batch=4096, lr=3e-3)
+    tr.install()
+    test = narrow.PairBatch(1 << 16, seed=3)
+    test.run("bf64")
+    _, _, d0, _ = test.results()
+    stats = {}
+    for m in ("learned_penalty", "penalty_cert"):
+        test.run(m)
+        L = _lib.lib()
+        cnt = _lib.DhPairCounters()
+        _lib.check(L.dh_tri_pairs_counters(C.byref(cnt)), "counters")
+        _, _, d, fl = test.results()
+        cert = (fl & 8) != 0
+        assert (np.abs(d - d0) / d0).max() <= 1e-6
+        assert np.array_equal(d[~cert].view(np.uint64), d0[~cert].view(np.uint64))
+        assert cnt.certified == cert.sum() and cnt.rechecked == (~cert).sum()
+        stats[m] = (cnt.features_fp64 / test.n, cert.mean())
+    (its_w, cert_w), (its_c, cert_c) = stats["learned_penalty"], stats["penalty_cert"]
+    assert its_w < its_c, stats
+    assert cert_w >= cert_c, stats
