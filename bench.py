@@ -234,8 +234,11 @@ def dem_c1(a):
     prm = engine.step_params(method=_lib.DH_BF, dt=1e-6, spring=1e5, damper=0.1, friction=0.0,
                              cell=hs.cell_size())
     L = _lib.lib()
+    max_pairs = max_contacts = 0
     for _ in range(a.warmup):
-        engine.step(ds, prm)
+        info = engine.step(ds, prm)
+        max_pairs = max(max_pairs, info.candidate_pairs)
+        max_contacts = max(max_contacts, info.contacts)
     _lib.check(L.dh_synchronize(), "sync")
     e0, e1 = C.c_void_p(), C.c_void_p()
     L.dh_event_create(C.byref(e0))
@@ -247,6 +250,8 @@ def dem_c1(a):
         checks += info.triangle_checks
         contacts += info.contacts
         pairs += info.candidate_pairs
+        max_pairs = max(max_pairs, info.candidate_pairs)
+        max_contacts = max(max_contacts, info.contacts)
     L.dh_event_record(e1, None)
     _lib.check(L.dh_synchronize(), "sync")
     ms = C.c_float()
@@ -263,6 +268,28 @@ def dem_c1(a):
                       "particles": nparticles},
            "triangle_checks_per_s": checks / (ms.value * 1e-3),
            "mean_candidate_pairs": pairs / a.steps, "mean_contacts": contacts / a.steps}
+    # the same steps as one captured graph per step (no host synchronisation
+    # between steps); the final state must equal the eager one bit for bit
+    dg = engine.DeviceScene(hs)
+    g = engine.StepGraph(dg, prm, pair_cap=2 * max_pairs + 64, contact_cap=2 * max_contacts + 1024)
+    try:
+        for _ in range(a.warmup):
+            g.launch()
+        _, overflow = g.status()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            g.launch()
+        ginfo, ov = g.status()
+        gdt = time.perf_counter() - t0
+        overflow |= ov
+    finally:
+        g.close()
+    same = all(np.array_equal(ds.download(f).view(np.uint64), dg.download(f).view(np.uint64))
+               for f in ("x", "y", "z", "pos", "linvel", "angvel", "rot"))
+    out["graph"] = {"ms_per_step": gdt * 1e3 / a.steps,
+                    "value": nparticles * a.steps / gdt, "overflow": overflow,
+                    "state_equals_eager": bool(same),
+                    "timing": "host clock from the first launch to the stream's end"}
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as ol
@@ -292,10 +319,6 @@ def main():
     if a.workload == "c1-dem":
         dem_c1(a)
         return
-    if a.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a "
-                                   "design note only (bench.py --gpus 1)"}))
-        sys.exit(2)
     from delta_amd import narrow
     n = 1 << a.pairs_log2
     print(f"[bench] generating {n} pairs in HBM", file=sys.stderr, flush=True)

@@ -242,6 +242,56 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
   return DH_OK;
 }
 
+// Buffers of broadphase_graph (before a capture).
+int broadphase_reserve(int n) {
+  int rc;
+  unsigned tbl = 1;
+  while (tbl < 2u * static_cast<unsigned>(n)) tbl <<= 1;
+  if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
+  if ((rc = g_bp.head.grow(sizeof(int) * tbl))) return rc;
+  if ((rc = g_bp.next.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_bp.count.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_bp.start.grow(sizeof(long long) * (n + 1)))) return rc;
+  if ((rc = g_bp.err.grow(sizeof(int)))) return rc;
+  if ((rc = g_bp.obst.grow(sizeof(int) * (n + 1)))) return rc;
+  return DH_OK;
+}
+
+// The broad phase for a captured step: the same kernels with fills instead of
+// memsets and no host read-back.  *np_dev (the total, which may exceed cap)
+// is left at the returned device address; pairs beyond cap are not written.
+int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
+                     long long cap, const long long** np_dev, hipStream_t st) {
+  const int n = geom->nparticles;
+  int rc;
+  if ((rc = broadphase_reserve(n))) return rc;
+  unsigned tbl = 1;
+  while (tbl < 2u * static_cast<unsigned>(n)) tbl <<= 1;
+  fill_async(g_bp.err.as<int>(), 0, 1, st);
+  fill_async(g_bp.head.as<int>(), -1, static_cast<long long>(tbl), st);
+  hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
+                     geom->eps, g_bp.box.as<Aabb>());
+  GridArgs g;
+  g.box = g_bp.box.as<Aabb>();
+  g.obstacle = obstacle_dev;
+  g.n = n;
+  g.h = cell;
+  g.inv_h = 1.0 / cell;
+  g.head = g_bp.head.as<int>();
+  g.next = g_bp.next.as<int>();
+  g.mask = tbl - 1;
+  g.large = g_bp.obst.as<int>();
+  g.nlarge = g_bp.err.as<int>();
+  hipLaunchKernelGGL(k_grid_insert, grid_for(n, 256), 256, 0, st, g);
+  hipLaunchKernelGGL(k_grid_count, grid_for(n, 256), 256, 0, st, g, g_bp.count.as<int>());
+  if ((rc = scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st))) return rc;
+  hipLaunchKernelGGL(k_grid_fill, grid_for(n, 256), 256, 0, st, g, g_bp.start.as<long long>(), cap,
+                     pairs);
+  DH_HIP_CHECK(hipGetLastError());
+  *np_dev = g_bp.start.as<long long>() + n;
+  return DH_OK;
+}
+
 }  // namespace dh
 
 using namespace dh;

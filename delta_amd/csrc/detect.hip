@@ -53,6 +53,9 @@ struct DetectArgs {
   int* pair_count;
   unsigned long long* stats;  // ST_COUNT counters
   int* error;                 // DetectError of the first failing pair
+  // When set, the number of pairs is read here (at most npairs, which is then
+  // the launch capacity); pairs beyond it record no contacts.
+  const long long* npairs_dev;
 };
 
 __device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
@@ -162,6 +165,10 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
   const int k = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  if (g.npairs_dev && k >= *g.npairs_dev) {
+    if (tid == 0) g.pair_count[k] = 0;
+    return;
+  }
   const Pair p = load_pair(g, k);
   const bool is_hybrid = (M == DH_HYBRID_BATCH || M == DH_HYBRID_TRIANGLE ||
                           M == DH_HYBRID_BATCH_STAT || M == DH_HYBRID_TRIANGLE_STAT);
@@ -338,6 +345,10 @@ __global__ __launch_bounds__(256) void k_gjk(DetectArgs g) {
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= g.npairs) return;
+  if (g.npairs_dev && k >= *g.npairs_dev) {
+    if (lane == 0) g.pair_count[k] = 0;
+    return;
+  }
   const Pair p = load_pair(g, k);
   const int na = 3 * p.nA, nb = 3 * p.nB, ba = 3 * p.oA, bb = 3 * p.oB;
   auto pa = [&](int i) { return V3<double>{g.x[ba + i], g.y[ba + i], g.z[ba + i]}; };
@@ -431,11 +442,14 @@ __global__ void k_scan_total(const int* __restrict__ in, int n, long long* __res
 
 __global__ __launch_bounds__(256) void k_scatter(const dh_contact* __restrict__ temp, long long n,
                                                  const long long* __restrict__ offs,
-                                                 dh_contact* __restrict__ out) {
+                                                 dh_contact* __restrict__ out,
+                                                 const unsigned long long* __restrict__ n_dev) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
+  // n is the capacity (of temp and out) when the count is read from n_dev
+  if (i >= n || (n_dev && static_cast<unsigned long long>(i) >= *n_dev)) return;
   dh_contact c = temp[i];
   const long long dst = offs[c.pair] + c.pad_;
+  if (dst >= n) return;  // more contacts than the capacity (reported by the caller)
   c.pad_ = 0;
   out[dst] = c;
 }
@@ -495,6 +509,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.pair_count = counts;
   a.stats = g_scratch.stats.as<unsigned long long>();
   a.error = g_scratch.error.as<int>();
+  a.npairs_dev = nullptr;
   const size_t shm_rows = 2 * kMaxRowTriangles * sizeof(int) + 8 + kMaxRowTriangles * 8;
   const unsigned grid = static_cast<unsigned>(npairs);
   switch (method) {
@@ -545,11 +560,78 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   if ((rc = scan_counts(counts, npairs, offs, st))) return rc;
   if (total > 0) {
     hipLaunchKernelGGL(k_scatter, grid_for(static_cast<int64_t>(total), 256), 256, 0, st,
-                       g_scratch.temp.as<dh_contact>(), static_cast<long long>(total), offs, out);
+                       g_scratch.temp.as<dh_contact>(), static_cast<long long>(total), offs, out,
+                       nullptr);
     DH_HIP_CHECK(hipGetLastError());
   }
   return DH_OK;
 }
+
+int detect_reserve(int pair_cap, long long contact_cap);
+
+// Detection for a captured step: launches for `pair_cap` pairs of which
+// *np_dev are live, contacts into `out` (capacity contact_cap), per-pair
+// counts and offsets; no host synchronisation.  Counters stay on the device:
+// dev_counters()[0] = contacts found (may exceed contact_cap), error flag.
+int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, int pair_cap,
+                       const long long* np_dev, dh_contact* out, long long contact_cap,
+                       int* counts, long long* offs, hipStream_t st) {
+  int rc;
+  if ((rc = detect_reserve(pair_cap, contact_cap))) return rc;
+  fill_async(g_scratch.counter.as<unsigned long long>(), 0ull, 1, st);
+  fill_async(g_scratch.stats.as<unsigned long long>(), 0ull, ST_COUNT, st);
+  fill_async(g_scratch.error.as<int>(), 0, 1, st);
+  DetectArgs a;
+  a.x = geom->x;
+  a.y = geom->y;
+  a.z = geom->z;
+  a.tri_offset = geom->tri_offset;
+  a.eps = geom->eps;
+  a.friction = geom->friction;
+  a.id = geom->id;
+  a.pairs = pairs;
+  a.npairs = pair_cap;
+  a.temp = g_scratch.temp.as<dh_contact>();
+  a.cap = contact_cap;
+  a.counter = g_scratch.counter.as<unsigned long long>();
+  a.pair_count = counts;
+  a.stats = g_scratch.stats.as<unsigned long long>();
+  a.error = g_scratch.error.as<int>();
+  a.npairs_dev = np_dev;
+  const unsigned grid = static_cast<unsigned>(pair_cap);
+  switch (method) {
+    case DH_BF: launch_detect<DH_BF>(grid, 0, st, a); break;
+    case DH_PENALTY: launch_detect<DH_PENALTY>(grid, 0, st, a); break;
+    default:
+      return set_error(DH_ERR_UNSUPPORTED, "captured step: method %d (use DH_BF or DH_PENALTY)",
+                       method);
+  }
+  if ((rc = scan_counts(counts, pair_cap, offs, st))) return rc;
+  hipLaunchKernelGGL(k_scatter, grid_for(contact_cap, 256), 256, 0, st,
+                     g_scratch.temp.as<dh_contact>(), contact_cap, offs, out,
+                     g_scratch.counter.as<unsigned long long>());
+  DH_HIP_CHECK(hipGetLastError());
+  return DH_OK;
+}
+
+// Buffers of detect_pairs_graph (call before a capture: no allocation may
+// happen inside one).
+int detect_reserve(int pair_cap, long long contact_cap) {
+  int rc;
+  if ((rc = g_scratch.temp.grow(sizeof(dh_contact) * (contact_cap > 0 ? contact_cap : 1)))) return rc;
+  if ((rc = g_scratch.counter.grow(sizeof(unsigned long long)))) return rc;
+  if ((rc = g_scratch.stats.grow(sizeof(unsigned long long) * ST_COUNT))) return rc;
+  if ((rc = g_scratch.error.grow(sizeof(int)))) return rc;
+  const int nb = (pair_cap + 1023) / 1024 > 0 ? (pair_cap + 1023) / 1024 : 1;
+  if ((rc = g_scratch.sums.grow(sizeof(long long) * (nb + 1)))) return rc;
+  return DH_OK;
+}
+
+// Device addresses of the detection counters: [0] contacts found, and the
+// error flag.
+const unsigned long long* detect_counter_dev() { return g_scratch.counter.as<unsigned long long>(); }
+const int* detect_error_dev() { return g_scratch.error.as<int>(); }
+const unsigned long long* detect_stats_dev() { return g_scratch.stats.as<unsigned long long>(); }
 
 // _batchError of the *Stat drivers: maxErr added once per row, in row order
 // and pair order (hybrid.cpp:240,530).

@@ -37,14 +37,21 @@ __device__ __forceinline__ void load3(const double* p, int i, double v[3]) {
   v[2] = p[3 * i + 2];
 }
 
+// np_dev (may be null): the live pair count when npairs is a capacity.
+__device__ __forceinline__ bool past_pairs(int p, int npairs, const long long* np_dev) {
+  return p >= npairs || (np_dev && p >= *np_dev);
+}
+
 __global__ __launch_bounds__(256) void k_pair_forces(BodiesDev b, const int* __restrict__ pairs,
                                                      int npairs,
                                                      const dh_contact* __restrict__ cs,
                                                      const long long* __restrict__ off, ForceConsts k,
                                                      double* __restrict__ pf,
-                                                     int* __restrict__ incidence) {
+                                                     int* __restrict__ incidence,
+                                                     const long long* __restrict__ np_dev,
+                                                     long long ncap) {
   const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= npairs) return;
+  if (past_pairs(p, npairs, np_dev)) return;
   const int a = pairs[2 * p], c = pairs[2 * p + 1];
   double posA[3], angA[3], linA[3], posB[3], angB[3], linB[3];
   load3(b.pos, a, posA);
@@ -55,7 +62,8 @@ __global__ __launch_bounds__(256) void k_pair_forces(BodiesDev b, const int* __r
   load3(b.linvel, c, linB);
   const double mA = b.mass[a], mB = b.mass[c];
   double fA[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, fB[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
-  const long long c0 = off[p], c1 = off[p + 1];
+  // contacts past the capacity ncap were dropped (reported by the caller)
+  const long long c0 = off[p] < ncap ? off[p] : ncap, c1 = off[p + 1] < ncap ? off[p + 1] : ncap;
   for (long long i = c0; i < c1; ++i) {
     const dh_contact& ct = cs[i];
     const double n[3] = {ct.normal[0], ct.normal[1], ct.normal[2]};
@@ -84,9 +92,14 @@ __global__ __launch_bounds__(256) void k_fill_incidence(const int* __restrict__ 
                                                         const long long* __restrict__ off,
                                                         const long long* __restrict__ start,
                                                         int* __restrict__ fill,
-                                                        int* __restrict__ entries) {
+                                                        int* __restrict__ entries,
+                                                        const long long* __restrict__ np_dev,
+                                                        long long ncap) {
   const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= npairs || off[p + 1] == off[p]) return;
+  if (past_pairs(p, npairs, np_dev)) return;
+  // the same test as k_pair_forces' incidence count (offsets clamped to ncap)
+  const long long c0 = off[p] < ncap ? off[p] : ncap, c1 = off[p + 1] < ncap ? off[p + 1] : ncap;
+  if (c1 <= c0) return;
   const int a = pairs[2 * p], c = pairs[2 * p + 1];
   const int ia = atomicAdd(&fill[a], 1);
   entries[start[a] + ia] = 2 * p;  // side A
@@ -194,12 +207,61 @@ __global__ void k_update_rotation_one(double* w, const double* W, double* R, dou
   if (threadIdx.x == 0 && blockIdx.x == 0) update_rotation(w, W, R, h);
 }
 
-int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
 
 struct ForceScratch {
   DevBuf pf, incid, start, fill, entries;
 };
 static ForceScratch g_fs;
+
+int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
+
+// Buffers of contacts_forces_impl for up to npairs pairs (before a capture).
+int forces_reserve(int n, int npairs) {
+  int rc;
+  if ((rc = g_fs.pf.grow(sizeof(double) * 12 * (npairs + 1)))) return rc;
+  if ((rc = g_fs.incid.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_fs.fill.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_fs.start.grow(sizeof(long long) * (n + 1)))) return rc;
+  if ((rc = g_fs.entries.grow(sizeof(int) * (2 * static_cast<size_t>(npairs) + 1)))) return rc;
+  return DH_OK;
+}
+
+// Per-pair force sums then per-particle sums in ascending partner order.
+// With np_dev, npairs is a capacity and *np_dev the live count; no host
+// synchronisation and no memset (capturable).
+int contacts_forces_impl(const dh_bodies* bodies, const int* pairs_dev, int npairs,
+                         const long long* np_dev, const dh_contact* contacts_dev,
+                         long long contact_cap, const long long* pair_offset_dev,
+                         const dh_force_params* fp, double* force_dev, double* torque_dev,
+                         int* touched_dev, hipStream_t st) {
+  const int n = bodies->n;
+  int rc;
+  if ((rc = forces_reserve(n, npairs))) return rc;
+  fill_async(g_fs.incid.as<int>(), 0, n + 1, st);
+  fill_async(g_fs.fill.as<int>(), 0, n + 1, st);
+  const BodiesDev b = to_dev(bodies);
+  const ForceConsts k{fp->spring, fp->damper, fp->friction};
+  if (npairs > 0) {
+    hipLaunchKernelGGL(k_pair_forces, grid_for(npairs, 256), 256, 0, st, b, pairs_dev, npairs,
+                       contacts_dev, pair_offset_dev, k, g_fs.pf.as<double>(),
+                       g_fs.incid.as<int>(), np_dev, contact_cap);
+    DH_HIP_CHECK(hipGetLastError());
+  }
+  if ((rc = scan_counts(g_fs.incid.as<int>(), n, g_fs.start.as<long long>(), st))) return rc;
+  if (npairs > 0) {
+    hipLaunchKernelGGL(k_fill_incidence, grid_for(npairs, 256), 256, 0, st, pairs_dev, npairs,
+                       pair_offset_dev, g_fs.start.as<long long>(), g_fs.fill.as<int>(),
+                       g_fs.entries.as<int>(), np_dev, contact_cap);
+    DH_HIP_CHECK(hipGetLastError());
+  }
+  if (n > 0) {
+    hipLaunchKernelGGL(k_particle_sum, grid_for(n, 256), 256, 0, st, n, pairs_dev,
+                       g_fs.start.as<long long>(), g_fs.incid.as<int>(), g_fs.entries.as<int>(),
+                       g_fs.pf.as<double>(), force_dev, torque_dev, touched_dev);
+    DH_HIP_CHECK(hipGetLastError());
+  }
+  return DH_OK;
+}
 
 }  // namespace dh
 
@@ -216,39 +278,10 @@ int dh_contacts_forces(const dh_bodies* bodies, const int* pairs_dev, int npairs
   if (!bodies || !fp || !force_dev || !torque_dev || npairs < 0 ||
       (npairs > 0 && (!pairs_dev || !pair_offset_dev)))
     return set_error(DH_ERR_ARG, "dh_contacts_forces: bad arguments");
-  const hipStream_t st = as_stream(stream);
-  const int n = bodies->n;
-  if ((rc = g_fs.pf.grow(sizeof(double) * 12 * (npairs + 1)))) return rc;
-  if ((rc = g_fs.incid.grow(sizeof(int) * (n + 1)))) return rc;
-  if ((rc = g_fs.fill.grow(sizeof(int) * (n + 1)))) return rc;
-  if ((rc = g_fs.start.grow(sizeof(long long) * (n + 1)))) return rc;
-  if ((rc = g_fs.entries.grow(sizeof(int) * (2 * static_cast<size_t>(npairs) + 1)))) return rc;
-  DH_HIP_CHECK(hipMemsetAsync(g_fs.incid.ptr, 0, sizeof(int) * (n + 1), st));
-  DH_HIP_CHECK(hipMemsetAsync(g_fs.fill.ptr, 0, sizeof(int) * (n + 1), st));
-  const BodiesDev b = to_dev(bodies);
-  const ForceConsts k{fp->spring, fp->damper, fp->friction};
-  if (npairs > 0) {
-    hipLaunchKernelGGL(k_pair_forces, grid_for(npairs, 256), 256, 0, st, b, pairs_dev, npairs,
-                       contacts_dev, reinterpret_cast<const long long*>(pair_offset_dev), k,
-                       g_fs.pf.as<double>(),
-                       g_fs.incid.as<int>());
-    DH_HIP_CHECK(hipGetLastError());
-  }
-  if ((rc = scan_counts(g_fs.incid.as<int>(), n, g_fs.start.as<long long>(), st))) return rc;
-  if (npairs > 0) {
-    hipLaunchKernelGGL(k_fill_incidence, grid_for(npairs, 256), 256, 0, st, pairs_dev, npairs,
-                       reinterpret_cast<const long long*>(pair_offset_dev),
-                       g_fs.start.as<long long>(), g_fs.fill.as<int>(),
-                       g_fs.entries.as<int>());
-    DH_HIP_CHECK(hipGetLastError());
-  }
-  if (n > 0) {
-    hipLaunchKernelGGL(k_particle_sum, grid_for(n, 256), 256, 0, st, n, pairs_dev,
-                       g_fs.start.as<long long>(), g_fs.incid.as<int>(), g_fs.entries.as<int>(),
-                       g_fs.pf.as<double>(), force_dev, torque_dev, touched_dev);
-    DH_HIP_CHECK(hipGetLastError());
-  }
-  return DH_OK;
+  return contacts_forces_impl(bodies, pairs_dev, npairs, nullptr, contacts_dev,
+                              0x7fffffffffffffffLL,
+                              reinterpret_cast<const long long*>(pair_offset_dev), fp, force_dev,
+                              torque_dev, touched_dev, as_stream(stream));
 }
 
 int dh_integrate(dh_bodies* bodies, const double* force_dev, const double* torque_dev,

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 namespace dh {
 
@@ -40,9 +41,26 @@ int ensure_init() {
   return g_init_status;
 }
 
+// While a captured step graph exists, its nodes hold the addresses of the
+// library's scratch buffers: a buffer outgrown meanwhile is retired, not
+// freed, until the last graph is destroyed.
+static int g_graphs_alive = 0;
+static std::vector<void*> g_retired;
+
+void graphs_alive_add(int delta) {
+  g_graphs_alive += delta;
+  if (g_graphs_alive == 0) {
+    for (void* p : g_retired) (void)hipFree(p);
+    g_retired.clear();
+  }
+}
+
 int DevBuf::grow(size_t need) {
   if (need <= bytes && ptr) return DH_OK;
-  if (ptr) DH_HIP_CHECK(hipFree(ptr));
+  if (ptr && g_graphs_alive > 0)
+    g_retired.push_back(ptr);
+  else if (ptr)
+    DH_HIP_CHECK(hipFree(ptr));
   ptr = nullptr;
   bytes = 0;
   size_t n = need < 256 ? 256 : need;

@@ -36,10 +36,26 @@ struct DevBuf {
   }
 };
 
+// Step graphs alive (+1 on creation, -1 on destruction); see DevBuf::grow.
+void graphs_alive_add(int delta);
+
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 inline unsigned grid_for(int64_t n, int block) {
   return static_cast<unsigned>((n + block - 1) / block);
+}
+
+// p[0..n) = v.  Used instead of hipMemsetAsync in work that may be captured
+// into a HIP graph.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T* __restrict__ p, T v, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename T>
+inline void fill_async(T* p, T v, long long n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<T>, grid_for(n, 256), 256, 0, st, p, v, n);
 }
 
 }  // namespace dh

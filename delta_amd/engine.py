@@ -148,3 +148,52 @@ def step(dscene, params, stream=None):
     _lib.check(L.dh_step(C.byref(dscene.scene), C.byref(params), C.byref(info), stream),
                "dh_step")
     return info
+
+
+class StepGraph:
+    """dh_step_graph: the whole explicit step (broad phase, detection,
+    forces, integration) captured once into a HIP graph with fixed pair and
+    contact capacities; each launch() replays it on the graph's own stream
+    without host synchronisation.  status() reads the last replay's counts
+    and the overflow bits of every replay since the previous status() (1
+    pairs, 2 contacts, 4 detection error): a replay that overflowed dropped
+    work, and the steps must be redone with larger capacities."""
+
+    def __init__(self, dscene, params, pair_cap, contact_cap):
+        L = _lib.lib()
+        L.dh_step_graph_create.argtypes = [C.POINTER(DhScene), C.POINTER(DhStepParams), C.c_int64,
+                                           C.c_int64, C.POINTER(C.c_void_p)]
+        L.dh_step_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+        L.dh_step_graph_status.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(DhStepInfo),
+                                           C.POINTER(C.c_int)]
+        L.dh_step_graph_destroy.argtypes = [C.c_void_p]
+        for f in ("dh_step_graph_create", "dh_step_graph_launch", "dh_step_graph_status",
+                  "dh_step_graph_destroy"):
+            getattr(L, f).restype = C.c_int
+        self._L = L
+        self._scene = dscene  # keeps the device arrays alive
+        h = C.c_void_p()
+        _lib.check(L.dh_step_graph_create(C.byref(dscene.scene), C.byref(params), pair_cap,
+                                          contact_cap, C.byref(h)), "dh_step_graph_create")
+        self._h = h
+
+    def launch(self, stream=None):
+        _lib.check(self._L.dh_step_graph_launch(self._h, stream), "dh_step_graph_launch")
+
+    def status(self, stream=None):
+        info = DhStepInfo()
+        ov = C.c_int(0)
+        _lib.check(self._L.dh_step_graph_status(self._h, stream, C.byref(info), C.byref(ov)),
+                   "dh_step_graph_status")
+        return info, ov.value
+
+    def close(self):
+        if self._h:
+            _lib.check(self._L.dh_step_graph_destroy(self._h), "dh_step_graph_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -404,6 +404,26 @@ typedef struct dh_step_info {
 
 int dh_step(dh_scene* scene, const dh_step_params* params, dh_step_info* info, void* stream);
 
+/* The same step captured once into a HIP graph and replayed (no counterpart
+ * in the reference: Engine::iterate, Engine.cpp:155-163, as one launch).
+ * Capacities are fixed at creation: kernels are launched for pair_cap pairs
+ * and contact_cap contacts and read the live counts from device memory, so a
+ * replay needs no host synchronisation.  Methods DH_BF and DH_PENALTY.
+ * dh_step_graph_status waits for the last replay (launch and status on the
+ * same stream) and reports its counts; *overflow != 0 (bit 0 pairs >
+ * pair_cap, bit 1 contacts > contact_cap, bit 2 a detection error) means
+ * that some replay since the previous status call dropped work: the caller
+ * restores its own copy of the state and recreates the graph with larger
+ * capacities.
+ * A graph keeps the scene pointers it was created with, and shares the
+ * library's scratch with the other entry points: serialise their use. */
+typedef struct dh_step_graph dh_step_graph;
+int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, int64_t pair_cap,
+                         int64_t contact_cap, dh_step_graph** out);
+int dh_step_graph_launch(dh_step_graph* graph, void* stream);
+int dh_step_graph_status(dh_step_graph* graph, void* stream, dh_step_info* info, int* overflow);
+int dh_step_graph_destroy(dh_step_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,3 +100,70 @@ def test_c1_lattice_steps():
         nc, checks, npairs = ol.or_step(hs, _lib.DH_BF, 1e5, 0.1, 0.5, 1e-6, (0.0, -9.81, 0.0))
         assert (info.contacts, info.candidate_pairs, info.triangle_checks) == (nc, npairs, checks)
         compare(ds, hs, s, exact=not sincos_branch_used(hs, 1e-6), rtol=1e-10)
+
+
+@pytest.mark.parametrize("method", [_lib.DH_BF, _lib.DH_PENALTY])
+def test_step_graph_replays_match_dh_step(method):
+    """The captured step (dh_step_graph) replayed 4 times against 4 dh_step
+    calls on a second copy of the C1 lattice: every state array bit for bit
+    and the same counts after each step, no overflow."""
+    hs = engine.c1_scene()
+    ds_eager = engine.DeviceScene(hs)
+    ds_graph = engine.DeviceScene(hs)
+    p = engine.step_params(method=method, cell=hs.cell_size())
+    g = engine.StepGraph(ds_graph, p, pair_cap=4096, contact_cap=1 << 16)
+    try:
+        for s in range(6):
+            info = engine.step(ds_eager, p)
+            g.launch()
+            ginfo, overflow = g.status()
+            assert overflow == 0
+            assert (ginfo.candidate_pairs, ginfo.contacts, ginfo.triangle_checks) == \
+                (info.candidate_pairs, info.contacts, info.triangle_checks), f"step {s}"
+            for f in STATE:
+                a = ds_eager.download(f).reshape(-1).view(np.uint64)
+                b = ds_graph.download(f).reshape(-1).view(np.uint64)
+                assert np.array_equal(a, b), f"step {s}: {f}"
+    finally:
+        g.close()
+
+
+def test_step_graph_reports_overflow():
+    """Capacities below the step's needs: the replay drops the excess and
+    sets the overflow bits (1 pairs, 2 contacts) instead of writing past its
+    buffers; the counts report what was needed."""
+    hs = engine.c1_scene()
+    ds = engine.DeviceScene(hs)
+    p = engine.step_params(method=_lib.DH_BF, cell=hs.cell_size())
+    info = engine.step(engine.DeviceScene(hs), p)
+    assert info.candidate_pairs > 8 and info.contacts > 16
+    g = engine.StepGraph(ds, p, pair_cap=8, contact_cap=4096)
+    try:
+        g.launch()
+        ginfo, overflow = g.status()
+        assert overflow & 1 and ginfo.candidate_pairs == info.candidate_pairs
+    finally:
+        g.close()
+    ds = engine.DeviceScene(hs)
+    g = engine.StepGraph(ds, p, pair_cap=4096, contact_cap=16)
+    try:
+        g.launch()
+        ginfo, overflow = g.status()
+        assert overflow == 2 and ginfo.contacts == info.contacts
+        assert np.all(np.isfinite(ds.download("pos")))
+    finally:
+        g.close()
+    # the bits are sticky: an overflowing first replay is still reported
+    # after a second replay that fits, and cleared by the status read
+    ds = engine.DeviceScene(hs)
+    g = engine.StepGraph(ds, p, pair_cap=4096, contact_cap=info.contacts - 1)
+    try:
+        g.launch()
+        g.launch()
+        ginfo, overflow = g.status()
+        assert ginfo.contacts < info.contacts and overflow == 2
+        g.launch()
+        _, overflow = g.status()
+        assert overflow == 0
+    finally:
+        g.close()
