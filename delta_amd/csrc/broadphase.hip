@@ -106,85 +106,154 @@ __global__ __launch_bounds__(256) void k_grid_insert(GridArgs g) {
   g.next[i] = atomicExch(&g.head[hsh], i);
 }
 
-// Visits the partners j > i of particle i; `emit(j)` is called once each.
+// One wave searches the partners j > i of one particle i.  A search through
+// the grid is latency-bound (bucket head, chain links and boxes are
+// dependent loads), so the 27 neighbour buckets are walked by 27 lanes at
+// once, one chain link per lane per round, and the large-particle list and
+// the all-particle scan of a large i are strided over the wave.  Finds are
+// compacted with ballots, so every lane sees the running count.
+constexpr int kBpWaves = 4;    // particles per workgroup
+constexpr int kBpList = 512;   // partners sorted in LDS per particle (else a serial sort)
+
+__device__ __forceinline__ int bp_lane_prefix(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0));
+}
+
+// Calls put(position, j) on the finding lane for every partner of a
+// particle i that is not large, positions 0.. in discovery order; returns
+// the number of partners (wave-uniform).
 template <typename F>
-__device__ void for_partners(const GridArgs& g, int i, F emit) {
-  const Aabb bi = g.box[i];
+__device__ int grid_partners(const GridArgs& g, int i, const Aabb& bi, int lane, F put) {
+  long long c[3];
+  cell_of(bi, g.inv_h, c);
+  const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+  const unsigned hsh = lane < 27 ? hash_cell(c[0] + dx, c[1] + dy, c[2] + dz, g.mask) : 0u;
+  // distinct neighbour cells may share a bucket: its lowest lane walks it
+  bool walk = lane < 27;
+  for (int l = 0; l < 26; ++l) {
+    const unsigned o = __shfl(hsh, l);
+    if (l < lane && o == hsh) walk = false;
+  }
+  int j = walk ? g.head[hsh] : -1;
+  int found = 0;
+  while (__ballot(j >= 0)) {
+    bool e = false;
+    const int cand = j;
+    if (j >= 0) {
+      if (j > i) {
+        const Aabb bj = g.box[j];
+        long long cj[3];
+        cell_of(bj, g.inv_h, cj);
+        e = cj[0] - c[0] >= -1 && cj[0] - c[0] <= 1 && cj[1] - c[1] >= -1 && cj[1] - c[1] <= 1 &&
+            cj[2] - c[2] >= -1 && cj[2] - c[2] <= 1 && overlap(bi, bj);
+      }
+      j = g.next[j];
+    }
+    const unsigned long long mask = __ballot(e);
+    if (e) put(found + bp_lane_prefix(mask), cand);
+    found += __popcll(mask);
+  }
+  const int nl = *g.nlarge;
+  for (int o0 = 0; o0 < nl; o0 += 64) {
+    const int o = o0 + lane;
+    bool e = false;
+    int cand = -1;
+    if (o < nl) {
+      cand = g.large[o];
+      e = cand > i && overlap(bi, g.box[cand]);
+    }
+    const unsigned long long mask = __ballot(e);
+    if (e) put(found + bp_lane_prefix(mask), cand);
+    found += __popcll(mask);
+  }
+  return found;
+}
+
+// Partners of a large particle i: every j > i (not both obstacles), in
+// ascending order; put(position, j) as above.
+template <typename F>
+__device__ int large_partners(const GridArgs& g, int i, const Aabb& bi, int lane, F put) {
   const bool obst_i = g.obstacle && g.obstacle[i];
+  int found = 0;
+  for (int j0 = i + 1; j0 < g.n; j0 += 64) {
+    const int j = j0 + lane;
+    bool e = false;
+    if (j < g.n) e = !(obst_i && g.obstacle && g.obstacle[j]) && overlap(bi, g.box[j]);
+    const unsigned long long mask = __ballot(e);
+    if (e) put(found + bp_lane_prefix(mask), j);
+    found += __popcll(mask);
+  }
+  return found;
+}
+
+__global__ __launch_bounds__(64 * kBpWaves) void k_grid_count(GridArgs g,
+                                                               int* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kBpWaves + (threadIdx.x >> 6);
+  if (i >= g.n) return;  // wave-uniform
+  const Aabb bi = g.box[i];
+  auto none = [](int, int) {};
+  const int m = is_large(g, i, bi) ? large_partners(g, i, bi, lane, none)
+                                   : grid_partners(g, i, bi, lane, none);
+  if (lane == 0) count[i] = m;
+}
+
+// Writes the partners of i at start[i].. in ascending order (pairs beyond
+// cap are not written).
+__global__ __launch_bounds__(64 * kBpWaves) void k_grid_fill(GridArgs g,
+                                                              const long long* __restrict__ start,
+                                                              long long cap,
+                                                              int* __restrict__ pairs) {
+  __shared__ int list[kBpWaves][kBpList];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * kBpWaves + w;
+  if (i >= g.n) return;  // wave-uniform
+  const long long s = start[i];
+  const long long m = start[i + 1] - s;
+  const Aabb bi = g.box[i];
+  auto direct = [&](int pos, int j) {
+    const long long q = s + pos;
+    if (q < cap) {
+      pairs[2 * q] = i;
+      pairs[2 * q + 1] = j;
+    }
+  };
   if (is_large(g, i, bi)) {
-    for (int j = i + 1; j < g.n; ++j) {
-      if (obst_i && g.obstacle[j]) continue;
-      if (overlap(bi, g.box[j])) emit(j);
+    large_partners(g, i, bi, lane, direct);  // already ascending
+    return;
+  }
+  if (m > kBpList) {
+    // more partners than the LDS list holds: unsorted, then a serial sort
+    grid_partners(g, i, bi, lane, direct);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    if (lane == 0) {
+      const long long e = s + m < cap ? s + m : cap;
+      for (long long a = s + 1; a < e; ++a) {
+        const int v = pairs[2 * a + 1];
+        long long b = a - 1;
+        while (b >= s && pairs[2 * b + 1] > v) {
+          pairs[2 * (b + 1) + 1] = pairs[2 * b + 1];
+          --b;
+        }
+        pairs[2 * (b + 1) + 1] = v;
+      }
     }
     return;
   }
-  long long c[3];
-  cell_of(bi, g.inv_h, c);
-  for (int dx = -1; dx <= 1; ++dx)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dz = -1; dz <= 1; ++dz) {
-        const long long cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
-        const unsigned hsh = hash_cell(cx, cy, cz, g.mask);
-        // distinct neighbour cells may share a bucket: visit a bucket once
-        bool seen = false;
-        for (int ex = -1; ex <= 1 && !seen; ++ex)
-          for (int ey = -1; ey <= 1 && !seen; ++ey)
-            for (int ez = -1; ez <= 1 && !seen; ++ez) {
-              const int before = (ex + 1) * 9 + (ey + 1) * 3 + (ez + 1);
-              const int cur = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-              if (before >= cur) continue;
-              if (hash_cell(c[0] + ex, c[1] + ey, c[2] + ez, g.mask) == hsh) seen = true;
-            }
-        if (seen) continue;
-        for (int j = g.head[hsh]; j >= 0; j = g.next[j]) {
-          if (j <= i) continue;
-          const Aabb bj = g.box[j];
-          long long cj[3];
-          cell_of(bj, g.inv_h, cj);
-          if (cj[0] - c[0] < -1 || cj[0] - c[0] > 1 || cj[1] - c[1] < -1 || cj[1] - c[1] > 1 ||
-              cj[2] - c[2] < -1 || cj[2] - c[2] > 1)
-            continue;
-          if (overlap(bi, bj)) emit(j);
-        }
-      }
-  const int nl = *g.nlarge;
-  for (int o = 0; o < nl; ++o) {
-    const int j = g.large[o];
-    if (j > i && overlap(bi, g.box[j])) emit(j);
-  }
-}
-
-__global__ __launch_bounds__(256) void k_grid_count(GridArgs g, int* __restrict__ count) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= g.n) return;
-  int c = 0;
-  for_partners(g, i, [&](int) { ++c; });
-  count[i] = c;
-}
-
-__global__ __launch_bounds__(256) void k_grid_fill(GridArgs g, const long long* __restrict__ start,
-                                                   long long cap, int* __restrict__ pairs) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= g.n) return;
-  const long long s = start[i];
-  long long w = s;
-  for_partners(g, i, [&](int j) {
-    if (w < cap) {
-      pairs[2 * w] = i;
-      pairs[2 * w + 1] = j;
-    }
-    ++w;
+  int* lst = list[w];
+  const int found = grid_partners(g, i, bi, lane, [&](int pos, int j) {
+    if (pos < kBpList) lst[pos] = j;
   });
-  // insertion sort of this particle's partners (short lists)
-  const long long e = w < cap ? w : cap;
-  for (long long a = s + 1; a < e; ++a) {
-    const int v = pairs[2 * a + 1];
-    long long b = a - 1;
-    while (b >= s && pairs[2 * b + 1] > v) {
-      pairs[2 * (b + 1) + 1] = pairs[2 * b + 1];
-      --b;
-    }
-    pairs[2 * (b + 1) + 1] = v;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  const int mm = found < kBpList ? found : kBpList;
+  // partners are distinct: the rank of v is the number of smaller partners
+  for (int e = lane; e < mm; e += 64) {
+    const int v = lst[e];
+    int rank = 0;
+    for (int q = 0; q < mm; ++q) rank += lst[q] < v ? 1 : 0;
+    direct(rank, v);
   }
 }
 
@@ -224,7 +293,8 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
   g.large = g_bp.obst.as<int>();
   g.nlarge = g_bp.err.as<int>();
   hipLaunchKernelGGL(k_grid_insert, grid_for(n, 256), 256, 0, st, g);
-  hipLaunchKernelGGL(k_grid_count, grid_for(n, 256), 256, 0, st, g, g_bp.count.as<int>());
+  hipLaunchKernelGGL(k_grid_count, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
+                     g_bp.count.as<int>());
   DH_HIP_CHECK(hipGetLastError());
   if ((rc = scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st))) return rc;
   long long last_start = 0;
@@ -236,8 +306,8 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
   DH_HIP_CHECK(hipStreamSynchronize(st));
   *npairs = last_start + last_count;
   if (*npairs > cap) return DH_ERR_CAPACITY;
-  hipLaunchKernelGGL(k_grid_fill, grid_for(n, 256), 256, 0, st, g, g_bp.start.as<long long>(), cap,
-                     pairs);
+  hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
+                     g_bp.start.as<long long>(), cap, pairs);
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
 }
@@ -283,10 +353,11 @@ int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double ce
   g.large = g_bp.obst.as<int>();
   g.nlarge = g_bp.err.as<int>();
   hipLaunchKernelGGL(k_grid_insert, grid_for(n, 256), 256, 0, st, g);
-  hipLaunchKernelGGL(k_grid_count, grid_for(n, 256), 256, 0, st, g, g_bp.count.as<int>());
+  hipLaunchKernelGGL(k_grid_count, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
+                     g_bp.count.as<int>());
   if ((rc = scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st))) return rc;
-  hipLaunchKernelGGL(k_grid_fill, grid_for(n, 256), 256, 0, st, g, g_bp.start.as<long long>(), cap,
-                     pairs);
+  hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
+                     g_bp.start.as<long long>(), cap, pairs);
   DH_HIP_CHECK(hipGetLastError());
   *np_dev = g_bp.start.as<long long>() + n;
   return DH_OK;

@@ -3,14 +3,17 @@
 // hybrid{Batch,Triangle}Stat (bf.cpp:28-149, penalty.cpp:50-241,
 // hybrid.cpp:59-627) over a batch of particle pairs.
 //
-// One workgroup (4 waves) owns one particle pair and walks its nA x nB
-// triangle pairs in row-major chunks of 256.  Row decisions of the hybrid
-// drivers (how many penalty solves failed in row iA) are accumulated in LDS
-// in a first pass; the emitting pass then evaluates each pair with the
-// method its row requires.  Contacts are compacted per chunk with wave
-// ballots and one global atomic per chunk, tagged with (pair, rank in pair),
-// and scattered into pair order afterwards, so the output order equals the
-// reference's serial emission order and does not depend on scheduling.
+// A particle pair's rows (triangles of A) are split into `tiles` contiguous
+// ranges, one workgroup (4 waves) per range, which walks its rows' triangle
+// pairs in row-major chunks of 256.  Several workgroups per particle pair
+// keep the CUs busy when there are few pairs (a 64-particle scene has a few
+// hundred).  Row decisions of the hybrid drivers (how many penalty solves
+// failed in row iA) are accumulated in LDS in a first pass; the emitting
+// pass then evaluates each pair with the method its row requires.  Contacts
+// are compacted per chunk with wave ballots and one global atomic per chunk,
+// tagged with (tile, rank in tile), and scattered into pair order
+// afterwards, so the output order equals the reference's serial emission
+// order and does not depend on scheduling or on the tiling.
 #include <cstring>
 #include <vector>
 
@@ -47,10 +50,11 @@ struct DetectArgs {
   const int* id;
   const int* pairs;
   int npairs;
-  dh_contact* temp;  // unordered contacts, pad_ = rank within the pair
+  dh_contact* temp;  // unordered contacts: pair = tile, pad_ = rank within the tile
   long long cap;
   unsigned long long* counter;  // [0] contacts reserved
-  int* pair_count;
+  int tiles;                    // row ranges (workgroups) per particle pair
+  int* tile_count;              // contacts per (pair, tile), pair-major
   unsigned long long* stats;  // ST_COUNT counters
   int* error;                 // DetectError of the first failing pair
   // When set, the number of pairs is read here (at most npairs, which is then
@@ -113,7 +117,8 @@ __device__ __forceinline__ bool row_falls_back(int fails, int nB, double max_err
 
 template <int M>
 __device__ __forceinline__ double final_pair(const DetectArgs& g, const Pair& p, int iA, int iB,
-                                             const int* row_fail, V3<double>& P, V3<double>& Q) {
+                                             const int* row_fail, int r0, V3<double>& P,
+                                             V3<double>& Q) {
   const Tri<double> A = load_tri(g, p.oA + iA);
   const Tri<double> B = load_tri(g, p.oB + iB);
   if (M == DH_BF) {
@@ -123,7 +128,7 @@ __device__ __forceinline__ double final_pair(const DetectArgs& g, const Pair& p,
   } else if (M == DH_PENALTY_STAT) {
     penalty_adaptive(A, B, p.max_err, P, Q);
   } else {
-    if (row_falls_back<M>(row_fail[iA], p.nB, p.max_err))
+    if (row_falls_back<M>(row_fail[iA - r0], p.nB, p.max_err))
       bf_solver(A, B, P, Q);
     else
       penalty_pair(A, B, p.max_err, P, Q);
@@ -162,11 +167,11 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
   __shared__ unsigned long long chunk_base;
   __shared__ unsigned long long local_stats[6];
 
-  const int k = blockIdx.x;
+  const int k = blockIdx.x / g.tiles, tile = blockIdx.x - k * g.tiles;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   if (g.npairs_dev && k >= *g.npairs_dev) {
-    if (tid == 0) g.pair_count[k] = 0;
+    if (tid == 0) g.tile_count[blockIdx.x] = 0;
     return;
   }
   const Pair p = load_pair(g, k);
@@ -177,24 +182,29 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
 
   if (M == DH_HYBRID_TRIANGLE && p.nB > p.nA) {
     if (tid == 0) atomicCAS(g.error, DE_NONE, static_cast<int>(DE_REF_UB));
-    if (tid == 0) g.pair_count[k] = 0;
+    if (tid == 0) g.tile_count[blockIdx.x] = 0;
     return;
   }
   if ((is_hybrid || is_rowstat) && (rows > kMaxRowTriangles || p.nB > kMaxRowTriangles)) {
     if (tid == 0) atomicCAS(g.error, DE_NONE, static_cast<int>(DE_TOO_LONG));
-    if (tid == 0) g.pair_count[k] = 0;
+    if (tid == 0) g.tile_count[blockIdx.x] = 0;
     return;
   }
+  // this workgroup's rows [r0, r1)
+  const int r0 = static_cast<int>(static_cast<long long>(tile) * rows / g.tiles);
+  const int r1 = static_cast<int>(static_cast<long long>(tile + 1) * rows / g.tiles);
+  const int nrows = r1 - r0;
   if (tid < 6) local_stats[tid] = 0;
 
   int* row_fail = reinterpret_cast<int*>(smem);
   int* row_lastnan = row_fail + kMaxRowTriangles;
   unsigned long long* row_min =
       reinterpret_cast<unsigned long long*>(smem + 2 * kMaxRowTriangles * sizeof(int) + 8);
-  const int total = rows * p.nB;
+  const int total = nrows * p.nB;  // triangle pairs of this tile, from row r0
+  const int f0 = r0 * p.nB;
 
   if (is_hybrid) {
-    for (int r = tid; r < rows; r += 256) {
+    for (int r = tid; r < nrows; r += 256) {
       row_fail[r] = 0;
       if (is_rowstat) {
         row_lastnan[r] = -1;
@@ -203,17 +213,17 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     }
     __syncthreads();
     // pass 1: penalty on every pair, failures per row
-    for (int f = tid; f < total; f += 256) {
+    for (int f = f0 + tid; f < f0 + total; f += 256) {
       const int iA = f / p.nB, iB = f - iA * p.nB;
       const Tri<double> A = load_tri(g, p.oA + iA);
       const Tri<double> B = load_tri(g, p.oB + iB);
       V3<double> P, Q;
-      if (penalty_pair(A, B, p.max_err, P, Q)) atomicAdd(&row_fail[iA], 1);
+      if (penalty_pair(A, B, p.max_err, P, Q)) atomicAdd(&row_fail[iA - r0], 1);
     }
     __syncthreads();
     if (tid == 0) {
       unsigned long long fb = 0, fails = 0, bfails = 0;
-      for (int r = 0; r < rows; ++r) {
+      for (int r = 0; r < nrows; ++r) {
         fails += row_fail[r];
         if (row_falls_back<M>(row_fail[r], p.nB, p.max_err)) {
           fb++;
@@ -228,19 +238,19 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
   if (is_rowstat) {
     // pass 2: last NaN distance per row; pass 3: minimum after it
     // (std::min(d, minD) semantics of hybrid.cpp:316-318)
-    for (int f = tid; f < total; f += 256) {
+    for (int f = f0 + tid; f < f0 + total; f += 256) {
       const int iA = f / p.nB, iB = f - iA * p.nB;
       V3<double> P, Q;
-      const double d = final_pair<M>(g, p, iA, iB, row_fail, P, Q);
-      if (d != d) atomicMax(&row_lastnan[iA], iB);
+      const double d = final_pair<M>(g, p, iA, iB, row_fail, r0, P, Q);
+      if (d != d) atomicMax(&row_lastnan[iA - r0], iB);
     }
     __syncthreads();
-    for (int f = tid; f < total; f += 256) {
+    for (int f = f0 + tid; f < f0 + total; f += 256) {
       const int iA = f / p.nB, iB = f - iA * p.nB;
-      if (iB <= row_lastnan[iA]) continue;
+      if (iB <= row_lastnan[iA - r0]) continue;
       V3<double> P, Q;
-      const double d = final_pair<M>(g, p, iA, iB, row_fail, P, Q);
-      atomicMin(&row_min[iA], static_cast<unsigned long long>(__double_as_longlong(d)));
+      const double d = final_pair<M>(g, p, iA, iB, row_fail, r0, P, Q);
+      atomicMin(&row_min[iA - r0], static_cast<unsigned long long>(__double_as_longlong(d)));
     }
     __syncthreads();
   }
@@ -253,8 +263,8 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
     int iA = 0, iB = 0;
     if (f < total) {
-      iA = f / p.nB;
-      iB = f - iA * p.nB;
+      iA = (f0 + f) / p.nB;
+      iB = (f0 + f) - iA * p.nB;
       if (M == DH_PENALTY_STAT) {
         const Tri<double> A = load_tri(g, p.oA + iA);
         const Tri<double> B = load_tri(g, p.oB + iB);
@@ -263,16 +273,16 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
         const double d = pq_dist(P, Q);
         emit = (3 * iB < p.nB) && (d <= p.margin);
       } else {
-        const double d = final_pair<M>(g, p, iA, iB, row_fail, P, Q);
+        const double d = final_pair<M>(g, p, iA, iB, row_fail, r0, P, Q);
         if (M == DH_BF) {
           emit = d <= p.margin;
         } else if (is_rowstat) {
           double mn;
-          if (row_lastnan[iA] == p.nB - 1) {
+          if (row_lastnan[iA - r0] == p.nB - 1) {
             mn = __longlong_as_double(0x7ff8000000000000ll);
           } else {
-            mn = __longlong_as_double(static_cast<long long>(row_min[iA]));
-            if (row_lastnan[iA] < 0) mn = (1E99 < mn) ? 1E99 : mn;
+            mn = __longlong_as_double(static_cast<long long>(row_min[iA - r0]));
+            if (row_lastnan[iA - r0] < 0) mn = (1E99 < mn) ? 1E99 : mn;
           }
           emit = (d <= mn) && (d < p.margin);
         } else {
@@ -296,6 +306,7 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
       if (slot < static_cast<unsigned long long>(g.cap)) {
         dh_contact c;
         make_contact(p, k, iA, iB, P, Q, &c);
+        c.pair = static_cast<int>(blockIdx.x);  // the tile, until the scatter
         c.pad_ = static_cast<int>(running + off + pre);
         g.temp[slot] = c;
       }
@@ -304,20 +315,20 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     __syncthreads();
   }
   if (tid == 0) {
-    g.pair_count[k] = static_cast<int>(running);
+    g.tile_count[blockIdx.x] = static_cast<int>(running);
     // evaluations the reference performs: every pair once, plus bfSolver
-    // again over each fallback row
+    // again over each fallback row (sums over the tiles)
     const unsigned long long evals =
         static_cast<unsigned long long>(total) +
         (is_hybrid ? local_stats[0] * static_cast<unsigned long long>(p.nB) : 0ull);
-    atomicAdd(&g.stats[ST_CHECKS], static_cast<unsigned long long>(p.nA) * p.nB);
+    if (tile == 0) atomicAdd(&g.stats[ST_CHECKS], static_cast<unsigned long long>(p.nA) * p.nB);
     atomicAdd(&g.stats[ST_EVALUATED], evals);
     if (is_hybrid) {
       atomicAdd(&g.stats[ST_FALLBACK_ROWS], local_stats[0]);
       if (is_rowstat) {
         atomicAdd(&g.stats[ST_PENALTY_FAILS], local_stats[1]);
         atomicAdd(&g.stats[ST_BATCH_FAILS], local_stats[2]);
-        atomicAdd(&g.stats[ST_BATCH_SIZE], static_cast<unsigned long long>(rows) * p.nB);
+        atomicAdd(&g.stats[ST_BATCH_SIZE], static_cast<unsigned long long>(nrows) * p.nB);
       }
     }
   }
@@ -346,7 +357,7 @@ __global__ __launch_bounds__(256) void k_gjk(DetectArgs g) {
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= g.npairs) return;
   if (g.npairs_dev && k >= *g.npairs_dev) {
-    if (lane == 0) g.pair_count[k] = 0;
+    if (lane == 0) g.tile_count[k] = 0;
     return;
   }
   const Pair p = load_pair(g, k);
@@ -394,7 +405,7 @@ __global__ __launch_bounds__(256) void k_gjk(DetectArgs g) {
         g.temp[slot] = c;
       }
     }
-    g.pair_count[k] = emit ? 1 : 0;
+    g.tile_count[k] = emit ? 1 : 0;
     atomicAdd(&g.stats[ST_CHECKS], static_cast<unsigned long long>(p.nA) * p.nB);
     atomicAdd(&g.stats[ST_EVALUATED], 1ull);
   }
@@ -440,22 +451,38 @@ __global__ void k_scan_total(const int* __restrict__ in, int n, long long* __res
   if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = (n > 0) ? out[n - 1] + in[n - 1] : 0;
 }
 
+// Per-pair contact counts from the per-tile counts.
+__global__ __launch_bounds__(256) void k_tile_sums(const int* __restrict__ tile_count, int tiles,
+                                                   int npairs, int* __restrict__ counts) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= npairs) return;
+  int c = 0;
+  for (int t = 0; t < tiles; ++t) c += tile_count[static_cast<long long>(k) * tiles + t];
+  counts[k] = c;
+}
+
+// Contacts into pair order: pair offset + the counts of the pair's earlier
+// tiles + the rank within the tile.
 __global__ __launch_bounds__(256) void k_scatter(const dh_contact* __restrict__ temp, long long n,
                                                  const long long* __restrict__ offs,
+                                                 const int* __restrict__ tile_count, int tiles,
                                                  dh_contact* __restrict__ out,
                                                  const unsigned long long* __restrict__ n_dev) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   // n is the capacity (of temp and out) when the count is read from n_dev
   if (i >= n || (n_dev && static_cast<unsigned long long>(i) >= *n_dev)) return;
   dh_contact c = temp[i];
-  const long long dst = offs[c.pair] + c.pad_;
+  const int pair = c.pair / tiles;
+  long long dst = offs[pair] + c.pad_;
+  for (int t = pair * tiles; t < c.pair; ++t) dst += tile_count[t];
   if (dst >= n) return;  // more contacts than the capacity (reported by the caller)
+  c.pair = pair;
   c.pad_ = 0;
   out[dst] = c;
 }
 
 struct DetectScratch {
-  DevBuf temp, counter, stats, error, counts, offs, sums;
+  DevBuf temp, counter, stats, error, counts, offs, sums, tiles;
 };
 static DetectScratch g_scratch;
 
@@ -478,6 +505,14 @@ static void launch_detect(unsigned grid, size_t shm, hipStream_t st, const Detec
   hipLaunchKernelGGL(k_detect<M>, grid, 256, shm, st, a);
 }
 
+// Row ranges per particle pair: enough workgroups to fill the 256 CUs
+// several times over when the pairs are few, one per pair otherwise.
+static int tiles_for(long long npairs, int method) {
+  if (method == DH_GJK || npairs <= 0) return 1;
+  long long t = (4096 + npairs - 1) / npairs;
+  return static_cast<int>(t < 1 ? 1 : (t > 16 ? 16 : t));
+}
+
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
                       dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
                       long long* offs_out, dh_stats* stats, hipStream_t st) {
@@ -493,6 +528,12 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   DH_HIP_CHECK(hipMemsetAsync(g_scratch.stats.ptr, 0, sizeof(unsigned long long) * ST_COUNT, st));
   DH_HIP_CHECK(hipMemsetAsync(g_scratch.error.ptr, 0, sizeof(int), st));
   int* counts = pair_count_user ? pair_count_user : g_scratch.counts.as<int>();
+  const int tiles = tiles_for(npairs, method);
+  int* tile_count = counts;
+  if (tiles > 1) {
+    if ((rc = g_scratch.tiles.grow(sizeof(int) * static_cast<size_t>(npairs) * tiles))) return rc;
+    tile_count = g_scratch.tiles.as<int>();
+  }
   DetectArgs a;
   a.x = geom->x;
   a.y = geom->y;
@@ -506,12 +547,13 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.temp = g_scratch.temp.as<dh_contact>();
   a.cap = cap;
   a.counter = g_scratch.counter.as<unsigned long long>();
-  a.pair_count = counts;
+  a.tiles = tiles;
+  a.tile_count = tile_count;
   a.stats = g_scratch.stats.as<unsigned long long>();
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = nullptr;
   const size_t shm_rows = 2 * kMaxRowTriangles * sizeof(int) + 8 + kMaxRowTriangles * 8;
-  const unsigned grid = static_cast<unsigned>(npairs);
+  const unsigned grid = static_cast<unsigned>(static_cast<long long>(npairs) * tiles);
   switch (method) {
     case DH_BF: launch_detect<DH_BF>(grid, 0, st, a); break;
     case DH_PENALTY: launch_detect<DH_PENALTY>(grid, 0, st, a); break;
@@ -557,17 +599,23 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   }
   if (static_cast<long long>(total) > cap) return DH_ERR_CAPACITY;
   long long* offs = offs_out ? offs_out : g_scratch.offs.as<long long>();
+  if (tiles > 1)
+    hipLaunchKernelGGL(k_tile_sums, grid_for(npairs, 256), 256, 0, st, tile_count, tiles, npairs,
+                       counts);
   if ((rc = scan_counts(counts, npairs, offs, st))) return rc;
   if (total > 0) {
     hipLaunchKernelGGL(k_scatter, grid_for(static_cast<int64_t>(total), 256), 256, 0, st,
-                       g_scratch.temp.as<dh_contact>(), static_cast<long long>(total), offs, out,
-                       nullptr);
+                       g_scratch.temp.as<dh_contact>(), static_cast<long long>(total), offs,
+                       tile_count, tiles, out, nullptr);
     DH_HIP_CHECK(hipGetLastError());
   }
   return DH_OK;
 }
 
 int detect_reserve(int pair_cap, long long contact_cap);
+
+// Tiles of the captured step, sized for half the pair capacity live.
+static int graph_tiles(int pair_cap) { return tiles_for(pair_cap / 2 > 0 ? pair_cap / 2 : 1, DH_BF); }
 
 // Detection for a captured step: launches for `pair_cap` pairs of which
 // *np_dev are live, contacts into `out` (capacity contact_cap), per-pair
@@ -594,11 +642,13 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.temp = g_scratch.temp.as<dh_contact>();
   a.cap = contact_cap;
   a.counter = g_scratch.counter.as<unsigned long long>();
-  a.pair_count = counts;
+  const int tiles = graph_tiles(pair_cap);
+  a.tiles = tiles;
+  a.tile_count = tiles > 1 ? g_scratch.tiles.as<int>() : counts;
   a.stats = g_scratch.stats.as<unsigned long long>();
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = np_dev;
-  const unsigned grid = static_cast<unsigned>(pair_cap);
+  const unsigned grid = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
   switch (method) {
     case DH_BF: launch_detect<DH_BF>(grid, 0, st, a); break;
     case DH_PENALTY: launch_detect<DH_PENALTY>(grid, 0, st, a); break;
@@ -606,9 +656,12 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
       return set_error(DH_ERR_UNSUPPORTED, "captured step: method %d (use DH_BF or DH_PENALTY)",
                        method);
   }
+  if (tiles > 1)
+    hipLaunchKernelGGL(k_tile_sums, grid_for(pair_cap, 256), 256, 0, st, a.tile_count, tiles,
+                       pair_cap, counts);
   if ((rc = scan_counts(counts, pair_cap, offs, st))) return rc;
   hipLaunchKernelGGL(k_scatter, grid_for(contact_cap, 256), 256, 0, st,
-                     g_scratch.temp.as<dh_contact>(), contact_cap, offs, out,
+                     g_scratch.temp.as<dh_contact>(), contact_cap, offs, a.tile_count, tiles, out,
                      g_scratch.counter.as<unsigned long long>());
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
@@ -624,6 +677,10 @@ int detect_reserve(int pair_cap, long long contact_cap) {
   if ((rc = g_scratch.error.grow(sizeof(int)))) return rc;
   const int nb = (pair_cap + 1023) / 1024 > 0 ? (pair_cap + 1023) / 1024 : 1;
   if ((rc = g_scratch.sums.grow(sizeof(long long) * (nb + 1)))) return rc;
+  const int tiles = graph_tiles(pair_cap);
+  if (tiles > 1 &&
+      (rc = g_scratch.tiles.grow(sizeof(int) * static_cast<size_t>(pair_cap) * tiles)))
+    return rc;
   return DH_OK;
 }
 

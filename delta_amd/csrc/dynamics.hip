@@ -133,6 +133,7 @@ __global__ __launch_bounds__(256) void k_particle_sum(int n, const int* __restri
         best = en;
       }
     }
+    if (best < 0) break;  // no larger partner: inconsistent counts, never read outside pf
     last = bestp;
     const double* o = pf + 12 * static_cast<long long>(best >> 1) + ((best & 1) ? 6 : 0);
     for (int q = 0; q < 3; ++q) {

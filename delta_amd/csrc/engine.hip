@@ -206,7 +206,10 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
     destroy_graph(g);
     return rc;
   }
-  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  // a blocking stream: replays on it start after earlier work on the null
+  // stream, whose entry points (dh_step, dh_detect_pairs, ...) share the
+  // library's scratch buffers with the graph
+  hipError_t e = hipStreamCreate(&g->stream);
   if (e == hipSuccess) e = hipMemsetAsync(g->status.ptr, 0, sizeof(long long) * 5, g->stream);
   if (e == hipSuccess) e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) {

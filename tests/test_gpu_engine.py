@@ -115,6 +115,7 @@ def test_step_graph_replays_match_dh_step(method):
     try:
         for s in range(6):
             info = engine.step(ds_eager, p)
+            _lib.check(_lib.lib().dh_synchronize(), "sync")
             g.launch()
             ginfo, overflow = g.status()
             assert overflow == 0
@@ -136,6 +137,7 @@ def test_step_graph_reports_overflow():
     ds = engine.DeviceScene(hs)
     p = engine.step_params(method=_lib.DH_BF, cell=hs.cell_size())
     info = engine.step(engine.DeviceScene(hs), p)
+    _lib.check(_lib.lib().dh_synchronize(), "sync")
     assert info.candidate_pairs > 8 and info.contacts > 16
     g = engine.StepGraph(ds, p, pair_cap=8, contact_cap=4096)
     try:
