@@ -204,3 +204,28 @@ extern "C" void hc_zoltan_hybrid(int it, int nt, const double* abc, const double
       }
   }
 }
+
+#include "ispc_math.hpp"
+
+// ispc_bf / ispc_pen over vertex-major SoA tables pp[v][k][i] (P, Q,
+// distance in/out for the brute force, as in the kernel).
+extern "C" void hc_ispc(int pen, unsigned n, double* const pp[6][3], double* const P[3],
+                        double* const Q[3], double* distance) {
+  for (unsigned i = 0; i < n; ++i) {
+    V3<double> p[6];
+    for (int v = 0; v < 6; ++v) p[v] = {pp[v][0][i], pp[v][1][i], pp[v][2][i]};
+    V3<double> Pi = {P[0][i], P[1][i], P[2][i]}, Qi = {Q[0][i], Q[1][i], Q[2][i]};
+    if (pen) {
+      const Tri<double> A = {{p[0], p[1], p[2]}}, B = {{p[3], p[4], p[5]}};
+      distance[i] = dh::ispc_penalty(A, B, Pi, Qi);
+    } else {
+      dh::ispc_bf(p, Pi, Qi, distance[i]);
+    }
+    P[0][i] = Pi.x;
+    P[1][i] = Pi.y;
+    P[2][i] = Pi.z;
+    Q[0][i] = Qi.x;
+    Q[1][i] = Qi.y;
+    Q[2][i] = Qi.z;
+  }
+}

@@ -260,6 +260,30 @@ int dh_zoltan_detection2(int s1, int e1, int s2, int e2, double* const t[6][3],
                          const int* tid_dev, const int* pid_dev, int npid,
                          dh_zoltan_contact* out_dev, int cap, int* n_out, void* stream);
 
+/* ------------------------------------------------------- ISPC kernels */
+/* ispc_bf (delta/contact/detection/bf.ispc:3-6), REAL = double: n pairs,
+ * ppV[k] = device array of coordinate k of vertex V (pp1..pp3 triangle A,
+ * pp4..pp6 triangle B).  P, Q and distance are in/out, as in the reference:
+ * - a zero normal of B, or the mixed normal test of bf.ispc:110, writes
+ *   distance = -1 and leaves P and Q;
+ * - otherwise a plane pre-pass writes a code (2 segment in plane, -2 no
+ *   crossing, -1 crossing outside, 0 crossing inside) and its crossing point;
+ *   the nine edge-edge solves (SMALL_NUM 1e-7) overwrite P and Q, and when
+ *   the smallest vertex-face distance is below the edge-edge one it replaces
+ *   distance, P and Q; otherwise distance keeps the code;
+ * - both output switches fall through from case 1 into case 2 (edge pair 1
+ *   takes its Q from B's edge 2-0, vertex-face 1 reports B's vertex 2). */
+int dh_ispc_bf(unsigned n, double* const pp1[3], double* const pp2[3], double* const pp3[3],
+               double* const pp4[3], double* const pp5[3], double* const pp6[3],
+               double* const P[3], double* const Q[3], double* distance, void* stream);
+
+/* ispc_pen (penalty.ispc:154-174): five Newton steps of the zoltan penalty
+ * (eps 0.1, r = lambda 10^(ln lambda + 14)), the regularisation multiplied by
+ * r before the fifth step; writes p, q and distance = |p - q|. */
+int dh_ispc_pen(unsigned n, double* const a[3], double* const b[3], double* const c[3],
+                double* const d[3], double* const e[3], double* const f[3],
+                double* const p[3], double* const q[3], double* distance, void* stream);
+
 /* ------------------------------------------------- learned narrow phase */
 /* Feature network (no counterpart in the reference): 18 canonical-frame
  * coordinates -> 64 -> 64 -> 32 outputs (15 closest-feature logits, an

@@ -1463,3 +1463,209 @@ int or_zoltan_detection(int s1, int e1, int s2, int e2, int all, double* const t
   }
   return n;
 }
+
+/* ---------------------------------------------------------------------------
+ * ISPC kernels (delta/contact/detection/bf.ispc:3-937, penalty.ispc:3-174),
+ * REAL = double.  Their algo.h is not in the reference; DOT and SUB are taken
+ * as the Solfec-family definitions the reference carries elsewhere
+ * (zoltan/algo.h:90-96,165-166): DOT(a,b) = (a0*b0 + a1*b1 + a2*b2),
+ * SUB(a,b,c): c = a - b.  Arrays are vertex-major SoA: pp[v][k][i] is
+ * coordinate k of vertex v of pair i.
+ * ------------------------------------------------------------------------- */
+
+/* Segment p1-p2 (origin p2) against p3-p4 (origin p4), SMALL_NUM 1e-7
+ * (bf.ispc:360-440); returns the distance of the clamped parameters. */
+static double ispc_segseg(const double p1[3], const double p2[3], const double p3[3],
+                          const double p4[3], double* sc_out, double* tc_out) {
+  double u[3], v[3], w[3];
+  sub3(p1, p2, u);
+  sub3(p3, p4, v);
+  sub3(p2, p4, w);
+  const double a = OR_DOT(u, u), b = OR_DOT(u, v), c = OR_DOT(v, v), d = OR_DOT(u, w),
+               e = OR_DOT(v, w);
+  const double D = (a * c - b * b);
+  double sD = D, tD = D, sN = 0, tN = 0;
+  const double small = 0.0000001;
+  if (D < small) {
+    sN = 0.0; sD = 1.0; tN = e; tD = c;
+  } else {
+    sN = (b * e - c * d);
+    tN = (a * e - b * d);
+    if (sN < 0.0) { sN = 0.0; tN = e; tD = c; }
+    else if (sN > sD) { sN = sD; tN = e + b; tD = c; }
+  }
+  if (tN < 0.0) {
+    tN = 0.0;
+    if (-d < 0.0) sN = 0.0;
+    else if (-d > a) sN = sD;
+    else { sN = -d; sD = a; }
+  } else if (tN > tD) {
+    tN = tD;
+    if ((-d + b) < 0.0) sN = 0;
+    else if ((-d + b) > a) sN = sD;
+    else { sN = (-d + b); sD = a; }
+  }
+  const double sc = (fabs(sN) < small) ? 0.0 : sN / sD;
+  const double tc = (fabs(tN) < small) ? 0.0 : tN / tD;
+  double dP[3];
+  for (int k = 0; k < 3; ++k) dP[k] = w[k] + (sc * u[k]) - (tc * v[k]);
+  *sc_out = sc;
+  *tc_out = tc;
+  return sqrt(dP[0] * dP[0] + dP[1] * dP[1] + dP[2] * dP[2]);
+}
+
+/* bf.ispc:9-935 for one pair: v[0..2] triangle A (pp1..pp3), v[3..5] B.
+ * P, Q and *distance are in/out: the early returns on a zero normal write
+ * only distance = -1 (bf.ispc:65-68,110-113); when the edge-edge minimum is
+ * not beaten (ptmin < ssmin fails, bf.ispc:874-877) distance keeps the code
+ * of the plane pre-pass (2, -2, -1 or 0, bf.ispc:114-174) while P and Q hold
+ * the edge-edge points.  The pre-pass reads nn[3][2], w0[3][6] and dir[3][6]
+ * through DOT on a row, so it uses nn[0][0], nn[0][1], nn[1][0] (or
+ * nn[1][0], nn[1][1], nn[2][0]) and the x components of columns 0..2 of w0
+ * and dir for every j; u and v are A's edges there.  An edge-edge index
+ * that is never set (every distance NaN or >= 1e30) selects no case. */
+static void ispc_bf_one(const double v[6][3], double P[3], double Q[3], double* distance) {
+  const double *p1 = v[0], *p2 = v[1], *p3 = v[2], *p4 = v[3], *p5 = v[4], *p6 = v[5];
+  double u[3], vv[3], nn[6], w0[18], dir[18];
+  sub3(p5, p4, u);
+  sub3(p6, p5, vv);
+  nn[0] = u[1] * vv[2] - u[2] * vv[1];
+  nn[2] = u[2] * vv[0] - u[0] * vv[2];
+  nn[4] = u[0] * vv[1] - u[1] * vv[0];
+  if (nn[0] == 0 && nn[2] == 0 && nn[4] == 0) { *distance = -1; return; }
+  const double* pa[6] = {p1, p2, p3, p4, p5, p6};
+  /* columns 0..2: A's edges against B's plane; 3..5: B's edges against A's */
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < 3; ++k) {
+      dir[6 * k + j] = pa[(j + 1) % 3][k] - pa[j][k];
+      w0[6 * k + j] = pa[j][k] - p4[k];
+    }
+  sub3(p2, p1, u);
+  sub3(p3, p1, vv);
+  nn[1] = u[1] * vv[2] - u[2] * vv[1];
+  nn[3] = u[2] * vv[0] - u[0] * vv[2];
+  nn[5] = u[0] * vv[1] - u[1] * vv[0];
+  if (nn[0] == 0 && nn[3] == 0 && nn[5] == 0) { *distance = -1; return; }
+  for (int j = 3; j < 6; ++j)
+    for (int k = 0; k < 3; ++k) {
+      dir[6 * k + j] = pa[3 + (j - 3 + 1) % 3][k] - pa[j][k];
+      w0[6 * k + j] = pa[j][k] - p1[k];
+    }
+  for (int j = 0; j < 6; ++j) {
+    const double* nr = (j < 3) ? nn : nn + 2;
+    const double a = -(nr[0] * w0[0] + nr[1] * w0[1] + nr[2] * w0[2]);
+    const double b = (nr[0] * dir[0] + nr[1] * dir[1] + nr[2] * dir[2]);
+    if (fabs(b) < 1E-30 && a == 0) { *distance = 2; continue; }
+    const double r = a / b;
+    for (int k = 0; k < 3; ++k) {
+      P[k] = pa[j][k] + r * dir[6 * k + j];
+      Q[k] = P[k];
+    }
+    if (r < 0.0 || r > 1.0 || r == INFINITY) { *distance = -2; break; }
+    double w[3];
+    sub3(P, j < 3 ? p4 : p1, w);
+    const double uv = OR_DOT(u, vv), uu = OR_DOT(u, u), vvv = OR_DOT(vv, vv);
+    const double D = uv * uv - uu * vvv;
+    const double s = (uv * OR_DOT(w, vv) - vvv * OR_DOT(w, u)) / D;
+    if (s < 0.0 || s > 1.0) { *distance = -1; break; }
+    const double t = (uv * OR_DOT(w, u) - uu * OR_DOT(w, vv)) / D;
+    if (t < 0.0 || (s + t) > 1.0) { *distance = -1; break; }
+    *distance = 0;
+  }
+  /* nine edge pairs: A's edge k (origin A[k+1]) x B's edge m (origin B[m+1]) */
+  double ssmin = 1E+30, ttc = 0, ssc = 0;
+  int ssid = -1;
+  for (int j = 0; j < 9; ++j) {
+    const int ka = j / 3, kb = j % 3;
+    double sc, tc;
+    const double dist = ispc_segseg(pa[ka], pa[(ka + 1) % 3], pa[3 + kb], pa[3 + (kb + 1) % 3],
+                                    &sc, &tc);
+    if (dist < ssmin) { ssmin = dist; ttc = tc; ssc = sc; ssid = j; }
+  }
+  /* case 1 falls through into case 2 (bf.ispc:505-521) */
+  if (ssid >= 0) {
+    const int ka = ssid / 3, kb = (ssid == 1) ? 2 : ssid % 3;
+    double ue[3], ve[3];
+    sub3(pa[ka], pa[(ka + 1) % 3], ue);
+    sub3(pa[3 + kb], pa[3 + (kb + 1) % 3], ve);
+    for (int k = 0; k < 3; ++k) {
+      P[k] = pa[(ka + 1) % 3][k] + (ue[k] * ssc);
+      Q[k] = pa[3 + (kb + 1) % 3][k] + (ve[k] * ttc);
+    }
+  }
+  /* six vertex-face pairs: A's face against p4, p5, p6, then B's against p1..p3 */
+  double ss = 0, tt = 0, ptmin = 1E30;
+  int id = 0;
+  for (int j = 0; j < 6; ++j) {
+    const double* f0 = j < 3 ? p1 : p4;
+    const double* f1 = j < 3 ? p2 : p5;
+    const double* f2 = j < 3 ? p3 : p6;
+    const double* q = pa[j < 3 ? 3 + j : j - 3];
+    double e0[3], e1[3], dd[3], s, t;
+    sub3(f1, f0, e0);
+    sub3(f2, f0, e1);
+    sub3(f0, q, dd);
+    const double sqr = pt_tri_sqr(OR_DOT(e0, e0), OR_DOT(e0, e1), OR_DOT(e1, e1), OR_DOT(e0, dd),
+                                  OR_DOT(e1, dd), OR_DOT(dd, dd), &s, &t);
+    const double dist = sqrt(sqr);
+    if (dist < ptmin) { ptmin = dist; ss = s; tt = t; id = j; }
+  }
+  if (!(ptmin < ssmin)) return;
+  *distance = ptmin;
+  /* case 1 falls through into case 2 (bf.ispc:891-899): Q = p6 */
+  if (id < 3) {
+    for (int k = 0; k < 3; ++k) {
+      P[k] = p1[k] + ((p3[k] - p1[k]) * tt) + ((p2[k] - p1[k]) * ss);
+      Q[k] = pa[3 + (id == 1 ? 2 : id)][k];
+    }
+  } else {
+    for (int k = 0; k < 3; ++k) {
+      P[k] = pa[id - 3][k];
+      Q[k] = p4[k] + ((p6[k] - p4[k]) * tt) + ((p5[k] - p4[k]) * ss);
+    }
+  }
+}
+
+void or_ispc_bf(unsigned n, double* const pp[6][3], double* const P[3], double* const Q[3],
+                double* distance) {
+  for (unsigned i = 0; i < n; ++i) {
+    double v[6][3], p[3], q[3];
+    for (int a = 0; a < 6; ++a)
+      for (int k = 0; k < 3; ++k) v[a][k] = pp[a][k][i];
+    for (int k = 0; k < 3; ++k) { p[k] = P[k][i]; q[k] = Q[k][i]; }
+    ispc_bf_one(v, p, q, &distance[i]);
+    for (int k = 0; k < 3; ++k) { P[k][i] = p[k]; Q[k][i] = q[k]; }
+  }
+}
+
+/* penalty.ispc:3-142: 5 Newton steps, eps 0.1, r = lambda 10^(ln lambda + 14);
+ * the regularisation is multiplied by r before the fifth step only (the
+ * Newton index decides, penalty.ispc:97). */
+void or_ispc_pen(unsigned n, double* const pp[6][3], double* const P[3], double* const Q[3],
+                 double* distance) {
+  for (unsigned i = 0; i < n; ++i) {
+    double xa[3], ya[3], za[3], xb[3], yb[3], zb[3];
+    for (int v = 0; v < 3; ++v) {
+      xa[v] = pp[v][0][i]; ya[v] = pp[v][1][i]; za[v] = pp[v][2][i];
+      xb[v] = pp[3 + v][0][i]; yb[v] = pp[3 + v][1][i]; zb[v] = pp[3 + v][2][i];
+    }
+    pen_ctx cx;
+    pen_setup(xa, ya, za, xb, yb, zb, &cx);
+    const double trace = cx.h[0] + cx.h[5] + cx.h[10] + cx.h[15];
+    double delta = trace * 0.1;
+    const double lambda = sqrt(0.0125 * trace);
+    cx.r = lambda * pow(10, log(lambda) + 14);
+    double x[4] = {0.33, 0.33, 0.33, 0.33};
+    for (int j = 0; j < 5; ++j) {
+      double dx[4];
+      delta = j < 4 ? delta : cx.r * delta;
+      pen_step(&cx, x, delta, dx);
+      for (int k = 0; k < 4; ++k) x[k] = x[k] - dx[k];
+    }
+    double p[3], q[3], d[3];
+    pen_points(&cx, x, p, q);
+    sub3(p, q, d);
+    distance[i] = sqrt(OR_DOT(d, d));
+    for (int k = 0; k < 3; ++k) { P[k][i] = p[k]; Q[k][i] = q[k]; }
+  }
+}

@@ -29,6 +29,17 @@
  *                           of pow/log/log10) + convergence to the brute force.
  *   or_zoltan_detection     composition of or_zoltan_bf with the contact
  *                           formulas; hand-built de-duplication cases.
+ *   or_ispc_bf              no ispc compiler here: the point-triangle solve is
+ *                           or_bf_solver's (golden-pinned); the result equals
+ *                           an independent long-double brute force over the 15
+ *                           features where the reference's selection is sound,
+ *                           closed forms (vertex over face), hand-worked plane
+ *                           pre-pass codes, both case-1 fall-throughs and the
+ *                           early returns on zero normals.
+ *   or_ispc_pen             or_zoltan_penalty's setup, step and output
+ *                           (golden-pinned) with its own schedule, and an
+ *                           independent numpy Gauss-Newton formulation of the
+ *                           5 regularised steps (dense 4x4 solve).
  */
 #ifndef DELTA_ORACLE_H
 #define DELTA_ORACLE_H
@@ -132,6 +143,13 @@ void or_zoltan_hybrid(int it, int nt, const double a[3], const double b[3], cons
  * [s1, e1), else j in [s2, e2).  Returns the number of kept contacts. */
 int or_zoltan_detection(int s1, int e1, int s2, int e2, int all, double* const t[6][3],
                         const int* pid, dh_zoltan_contact* out, int cap);
+
+/* ISPC kernels bf.ispc:3-937 and penalty.ispc:3-174 (REAL = double) over
+ * vertex-major SoA tables pp[v][k][i]; P, Q, distance in/out as in ispc_bf. */
+void or_ispc_bf(unsigned n, double* const pp[6][3], double* const P[3], double* const Q[3],
+                double* distance);
+void or_ispc_pen(unsigned n, double* const pp[6][3], double* const P[3], double* const Q[3],
+                 double* distance);
 
 /* GJK distance between two convex point sets (gjk.cpp:302-460). */
 double or_gjk(const double* a, int na, const double* b, int nb, double p[3], double q[3]);
