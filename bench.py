@@ -39,9 +39,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,"
                                          "penalty_cert,learned_penalty,learned")
-    ap.add_argument("--workload", default="pairs", choices=["pairs", "c1-dem"],
+    ap.add_argument("--workload", default="pairs", choices=["pairs", "c1-dem", "hopper"],
                     help="pairs: 2^24 unique triangle pairs (the headline); c1-dem: the "
-                         "64-icosphere DEM step (particle-steps/s)")
+                         "64-icosphere DEM step (particle-steps/s); hopper: the reference's "
+                         "hopper scene (input/hopper.py, 15,625 spheres as 80-triangle "
+                         "icospheres in 26 wall prisms)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel three times "
                          "(reference for scaling HBM byte counters) and exit")
@@ -243,11 +245,12 @@ def dem_c1(a):
     e0, e1 = C.c_void_p(), C.c_void_p()
     L.dh_event_create(C.byref(e0))
     L.dh_event_create(C.byref(e1))
-    checks = contacts = pairs = 0
+    checks = contacts = pairs = culled = 0
     L.dh_event_record(e0, None)
     for _ in range(a.steps):
         info = engine.step(ds, prm)
         checks += info.triangle_checks
+        culled += info.pairs_culled
         contacts += info.contacts
         pairs += info.candidate_pairs
         max_pairs = max(max_pairs, info.candidate_pairs)
@@ -267,6 +270,7 @@ def dem_c1(a):
                                   "2-triangle floor, dt 1e-6 s, FP64 brute force",
                       "particles": nparticles},
            "triangle_checks_per_s": checks / (ms.value * 1e-3),
+           "culled_fraction": culled / max(checks, 1),
            "mean_candidate_pairs": pairs / a.steps, "mean_contacts": contacts / a.steps}
     # the same steps as one captured graph per step (no host synchronisation
     # between steps); the final state must equal the eager one bit for bit
@@ -307,6 +311,76 @@ def dem_c1(a):
     print(json.dumps(out), flush=True)
 
 
+def dem_hopper(a):
+    """The hopper of input/hopper.py (also input/hopper.mbfcp): 25^3 touching
+    spheres of radius 0.02 as 80-triangle icospheres, 26 wall prisms,
+    density 1000, gravity -10 z, spring 1e6 (the scene's SPRING), damping
+    0.1, friction off, margin 1e-3, dt 1e-6 s, FP64 brute force.  The lattice
+    starts in contact (every neighbour inside the margin); at dt 1e-6 s it
+    stays packed over the timed steps (at 1e-5 s it flies apart within ten
+    steps and the later steps measure an almost empty scene)."""
+    from delta_amd import _lib, engine, scene_io
+    print("[bench] building the hopper scene", file=sys.stderr, flush=True)
+    hs = scene_io.hopper_scene(eps=1e-3, sphere_level=1)
+    ds = engine.DeviceScene(hs)
+    prm = engine.step_params(method=_lib.DH_BF, dt=1e-6, gravity=(0.0, 0.0, -10.0), spring=1e6,
+                             damper=0.1, friction=0.0, cell=hs.cell_size())
+    L = _lib.lib()
+    for _ in range(a.warmup):
+        engine.step(ds, prm)
+    _lib.check(L.dh_synchronize(), "sync")
+    print("[bench] hopper warm", file=sys.stderr, flush=True)
+    checks = contacts = pairs = culled = 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        info = engine.step(ds, prm)
+        checks += info.triangle_checks
+        culled += info.pairs_culled
+        contacts += info.contacts
+        pairs += info.candidate_pairs
+    _lib.check(L.dh_synchronize(), "sync")
+    dt = time.perf_counter() - t0
+    out = {"metric": "particle-steps/s (hopper DEM step, FP64 brute force)",
+           "value": hs.n * a.steps / dt, "unit": "particle-steps/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (input/hopper.py geometry)",
+           "config": {"workload": "hopper: 15,625 spheres r 0.02 as 80-triangle icospheres, "
+                                  "26 wall prisms, dt 1e-6 s, FP64 brute force",
+                      "particles": hs.n, "triangles": int(hs.tri_offset[-1])},
+           "triangle_checks_per_s": checks / dt, "culled_fraction": culled / max(checks, 1),
+           "mean_candidate_pairs": pairs / a.steps, "mean_contacts": contacts / a.steps}
+    if not a.no_cpu:
+        # the oracle's brute-force detection on the CPU over a sample of the
+        # step's candidate pairs (one thread): triangle checks per second
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        rng = np.random.default_rng(0)
+        sample = []
+        off = hs.tri_offset
+        for i in rng.choice(np.arange(26, hs.n), 64, replace=False):
+            j = i + 1 if i + 1 < hs.n else i - 1
+            sample.append((i, j))
+        t0 = time.perf_counter()
+        k = nchk = 0
+        while time.perf_counter() - t0 < a.cpu_seconds and k < len(sample):
+            i, j = sample[k]
+            A = np.stack([hs.x[3 * off[i]:3 * off[i + 1]], hs.y[3 * off[i]:3 * off[i + 1]],
+                          hs.z[3 * off[i]:3 * off[i + 1]]], 1).reshape(-1, 3, 3)
+            B = np.stack([hs.x[3 * off[j]:3 * off[j + 1]], hs.y[3 * off[j]:3 * off[j + 1]],
+                          hs.z[3 * off[j]:3 * off[j + 1]]], 1).reshape(-1, 3, 3)
+            ol.or_detect(0, A, B, 1e-3, 1e-3)
+            nchk += len(A) * len(B)
+            k += 1
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nchk / cdt, "unit": "triangle checks/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle bf over {k} lattice-neighbour particle pairs "
+                                         f"({nchk} triangle pairs), one thread, {cdt:.1f} s; "
+                                         "compare with triangle_checks_per_s"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -318,6 +392,9 @@ def main():
         sys.exit(2)
     if a.workload == "c1-dem":
         dem_c1(a)
+        return
+    if a.workload == "hopper":
+        dem_hopper(a)
         return
     from delta_amd import narrow
     n = 1 << a.pairs_log2

@@ -189,7 +189,9 @@ def reset_all_statistics():
 
 def engine_counters():
     """Counters with no reference getter: nTriA*nTriB per call (the
-    reference's tri-cmp, Engine.cpp:451), evaluations run, fallback rows."""
+    reference's tri-cmp, Engine.cpp:451), the reference's evaluations,
+    fallback rows, and the brute-force pairs the box test skipped."""
     return {"triangle_checks": int(_stats.triangle_checks),
             "pairs_evaluated": int(_stats.pairs_evaluated),
-            "fallback_rows": int(_stats.fallback_rows)}
+            "fallback_rows": int(_stats.fallback_rows),
+            "pairs_culled": int(_stats.pairs_culled)}

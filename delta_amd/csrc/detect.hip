@@ -27,6 +27,11 @@ namespace dh {
 // 10000 entries indexed by 3*iB (bf.cpp:98, penalty.cpp:186, hybrid.cpp:402).
 constexpr int kMaxRowTriangles = 3333;
 
+// LDS list of DH_BF candidates that pass the box test (per workgroup), and
+// the most triangles per particle whose boxes are staged in LDS.
+constexpr int kSurvivors = 1024;
+constexpr int kBoxTriangles = 320;
+
 enum DetectError : int { DE_NONE = 0, DE_REF_UB = 1, DE_TOO_LONG = 2 };
 
 enum StatSlot : int {
@@ -37,7 +42,8 @@ enum StatSlot : int {
   ST_CHECKS = 20,
   ST_EVALUATED = 21,
   ST_FALLBACK_ROWS = 22,
-  ST_COUNT = 23
+  ST_CULLED = 23,  // DH_BF triangle pairs skipped by the box test
+  ST_COUNT = 24
 };
 
 struct DetectArgs {
@@ -68,6 +74,40 @@ __device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) r.v[k] = {g.x[b + k], g.y[b + k], g.z[b + k]};
   return r;
+}
+
+// Mid-phase of DH_BF: true when the triangles' bounding boxes, grown by the
+// contact margin, are apart along some axis by more than a rounding band.
+// The distance then exceeds the margin, so bf (bf.cpp:125-127) emits no
+// contact and the pair need not be solved.  Coordinates that are NaN drop
+// out of the boxes; bfSolver's result never comes from them either (its
+// strict minimum skips NaN features), so the test stays exact.
+__device__ __forceinline__ void tri_box(const Tri<double>& T, double* lo, double* hi) {
+  lo[0] = fmin(fmin(T.v[0].x, T.v[1].x), T.v[2].x);
+  lo[1] = fmin(fmin(T.v[0].y, T.v[1].y), T.v[2].y);
+  lo[2] = fmin(fmin(T.v[0].z, T.v[1].z), T.v[2].z);
+  hi[0] = fmax(fmax(T.v[0].x, T.v[1].x), T.v[2].x);
+  hi[1] = fmax(fmax(T.v[0].y, T.v[1].y), T.v[2].y);
+  hi[2] = fmax(fmax(T.v[0].z, T.v[1].z), T.v[2].z);
+}
+
+__device__ __forceinline__ bool boxes_apart(const double* loA, const double* hiA,
+                                            const double* loB, const double* hiB, double margin) {
+  double gap = -INFINITY, mag = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    gap = fmax(gap, fmax(loB[q] - hiA[q], loA[q] - hiB[q]));
+    mag = fmax(mag, fmax(fmax(fabs(loA[q]), fabs(hiA[q])), fmax(fabs(loB[q]), fabs(hiB[q]))));
+  }
+  return gap > margin * (1.0 + 0x1p-20) + mag * 0x1p-40;
+}
+
+__device__ __forceinline__ bool boxes_apart(const Tri<double>& A, const Tri<double>& B,
+                                            double margin) {
+  double a[6], b[6];
+  tri_box(A, a, a + 3);
+  tri_box(B, b, b + 3);
+  return boxes_apart(a, a + 3, b, b + 3, margin);
 }
 
 // Lanes below this one whose predicate is set (wave64).
@@ -255,41 +295,12 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     __syncthreads();
   }
 
-  // emitting pass, chunk by chunk in row-major order
+  // Emission of one chunk of 256 candidates (in row-major order across
+  // chunks): block prefix over the emitting lanes, one global atomic for the
+  // chunk's slots, rank within the tile in pad_.
   unsigned long long running = 0;
-  for (int base = 0; base < total; base += 256) {
-    const int f = base + tid;
-    bool emit = false;
-    V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
-    int iA = 0, iB = 0;
-    if (f < total) {
-      iA = (f0 + f) / p.nB;
-      iB = (f0 + f) - iA * p.nB;
-      if (M == DH_PENALTY_STAT) {
-        const Tri<double> A = load_tri(g, p.oA + iA);
-        const Tri<double> B = load_tri(g, p.oB + iB);
-        const int its = penalty_adaptive(A, B, p.max_err, P, Q);
-        atomicAdd(&g.stats[ST_HIST + (its < 0 ? 0 : its)], 1ull);
-        const double d = pq_dist(P, Q);
-        emit = (3 * iB < p.nB) && (d <= p.margin);
-      } else {
-        const double d = final_pair<M>(g, p, iA, iB, row_fail, r0, P, Q);
-        if (M == DH_BF) {
-          emit = d <= p.margin;
-        } else if (is_rowstat) {
-          double mn;
-          if (row_lastnan[iA - r0] == p.nB - 1) {
-            mn = __longlong_as_double(0x7ff8000000000000ll);
-          } else {
-            mn = __longlong_as_double(static_cast<long long>(row_min[iA - r0]));
-            if (row_lastnan[iA - r0] < 0) mn = (1E99 < mn) ? 1E99 : mn;
-          }
-          emit = (d <= mn) && (d < p.margin);
-        } else {
-          emit = d < p.margin;
-        }
-      }
-    }
+  unsigned culled = 0;
+  auto emit_chunk = [&](bool emit, int iA, int iB, const V3<double>& P, const V3<double>& Q) {
     const unsigned long long mask = __ballot(emit);
     const int pre = lane_prefix(mask);
     if (lane == 0) wave_cnt[wave] = __popcll(mask);
@@ -313,6 +324,106 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     }
     running += cnt;
     __syncthreads();
+  };
+
+  if (M == DH_BF) {
+    // Box test first, survivors compacted in row-major order into LDS, then
+    // solved 256 at a time, so that no lane idles beside a culled pair.  The
+    // boxes of the tile's rows and of B's triangles are staged in LDS when
+    // they fit, so a test reads 12 doubles from LDS instead of 18 from HBM.
+    __shared__ int surv[kSurvivors];
+    __shared__ double boxes[2 * kBoxTriangles * 6];
+    const bool staged = nrows <= kBoxTriangles && p.nB <= kBoxTriangles;  // block-uniform
+    if (staged) {
+      for (int t = tid; t < nrows + p.nB; t += 256) {
+        const bool rowA = t < nrows;
+        double* b = boxes + 6 * (rowA ? t : kBoxTriangles + (t - nrows));
+        tri_box(load_tri(g, rowA ? p.oA + r0 + t : p.oB + (t - nrows)), b, b + 3);
+      }
+      __syncthreads();
+    }
+    for (int fpos = 0; fpos < total;) {
+      int nl = 0;
+      while (fpos < total && nl <= kSurvivors - 256) {
+        const int f = fpos + tid;
+        bool keep = false;
+        if (f < total) {
+          const int iA = (f0 + f) / p.nB, iB = (f0 + f) - iA * p.nB;
+          if (staged) {
+            const double* a = boxes + 6 * (iA - r0);
+            const double* b = boxes + 6 * (kBoxTriangles + iB);
+            keep = !boxes_apart(a, a + 3, b, b + 3, p.margin);
+          } else {
+            keep = !boxes_apart(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), p.margin);
+          }
+          culled += keep ? 0u : 1u;
+        }
+        const unsigned long long m = __ballot(keep);
+        if (lane == 0) wave_cnt[wave] = __popcll(m);
+        __syncthreads();
+        int off = 0, cnt = 0;
+        for (int w = 0; w < 4; ++w) {
+          off += (w < wave) ? wave_cnt[w] : 0;
+          cnt += wave_cnt[w];
+        }
+        if (keep) surv[nl + off + lane_prefix(m)] = f;
+        nl += cnt;
+        fpos += 256;
+        __syncthreads();
+      }
+      for (int b = 0; b < nl; b += 256) {
+        const int e = b + tid;
+        bool emit = false;
+        V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
+        int iA = 0, iB = 0;
+        if (e < nl) {
+          const int f = surv[e];
+          iA = (f0 + f) / p.nB;
+          iB = (f0 + f) - iA * p.nB;
+          bf_solver(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), P, Q);
+          emit = pq_dist(P, Q) <= p.margin;
+        }
+        emit_chunk(emit, iA, iB, P, Q);
+      }
+    }
+  } else {
+    for (int base = 0; base < total; base += 256) {
+      const int f = base + tid;
+      bool emit = false;
+      V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
+      int iA = 0, iB = 0;
+      if (f < total) {
+        iA = (f0 + f) / p.nB;
+        iB = (f0 + f) - iA * p.nB;
+        if (M == DH_PENALTY_STAT) {
+          const Tri<double> A = load_tri(g, p.oA + iA);
+          const Tri<double> B = load_tri(g, p.oB + iB);
+          const int its = penalty_adaptive(A, B, p.max_err, P, Q);
+          atomicAdd(&g.stats[ST_HIST + (its < 0 ? 0 : its)], 1ull);
+          const double d = pq_dist(P, Q);
+          emit = (3 * iB < p.nB) && (d <= p.margin);
+        } else {
+          const double d = final_pair<M>(g, p, iA, iB, row_fail, r0, P, Q);
+          if (is_rowstat) {
+            double mn;
+            if (row_lastnan[iA - r0] == p.nB - 1) {
+              mn = __longlong_as_double(0x7ff8000000000000ll);
+            } else {
+              mn = __longlong_as_double(static_cast<long long>(row_min[iA - r0]));
+              if (row_lastnan[iA - r0] < 0) mn = (1E99 < mn) ? 1E99 : mn;
+            }
+            emit = (d <= mn) && (d < p.margin);
+          } else {
+            emit = d < p.margin;
+          }
+        }
+      }
+      emit_chunk(emit, iA, iB, P, Q);
+    }
+  }
+  if (M == DH_BF) {
+    for (int o = 32; o > 0; o >>= 1) culled += __shfl_xor(culled, o);
+    if (lane == 0 && culled) atomicAdd(&g.stats[ST_CULLED], static_cast<unsigned long long>(culled));
   }
   if (tid == 0) {
     g.tile_count[blockIdx.x] = static_cast<int>(running);
@@ -596,6 +707,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
     stats->triangle_checks += st_host[ST_CHECKS];
     stats->pairs_evaluated += st_host[ST_EVALUATED];
     stats->fallback_rows += st_host[ST_FALLBACK_ROWS];
+    stats->pairs_culled += st_host[ST_CULLED];
   }
   if (static_cast<long long>(total) > cap) return DH_ERR_CAPACITY;
   long long* offs = offs_out ? offs_out : g_scratch.offs.as<long long>();

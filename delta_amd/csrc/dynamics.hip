@@ -107,6 +107,46 @@ __global__ __launch_bounds__(256) void k_fill_incidence(const int* __restrict__ 
   entries[start[c] + ic] = 2 * p + 1;  // side B
 }
 
+// Particles with more than this many contact-carrying pairs (walls, floors)
+// are summed by a workgroup each (k_particle_sum_wide); the rest by one lane.
+constexpr int kNarrowSum = 64;
+constexpr int kWideSort = 4096;  // partners rank-sorted in LDS
+
+__device__ __forceinline__ int entry_partner(const int* __restrict__ pairs, int en) {
+  const int pr = en >> 1;
+  return (en & 1) ? pairs[2 * pr] : pairs[2 * pr + 1];
+}
+
+__device__ __forceinline__ const double* entry_forces(const double* __restrict__ pf, int en) {
+  return pf + 12 * static_cast<long long>(en >> 1) + ((en & 1) ? 6 : 0);
+}
+
+// Sum of particle i's entries in ascending partner index by repeated
+// selection of the next larger partner (m small).
+__device__ void sum_by_selection(const int* __restrict__ pairs, const int* __restrict__ entries,
+                                 const double* __restrict__ pf, long long s, int m, double f[3],
+                                 double t[3]) {
+  int last = -1;
+  for (int done = 0; done < m; ++done) {
+    int best = -1, bestp = 0x7fffffff;
+    for (int e = 0; e < m; ++e) {
+      const int en = entries[s + e];
+      const int partner = entry_partner(pairs, en);
+      if (partner > last && partner < bestp) {
+        bestp = partner;
+        best = en;
+      }
+    }
+    if (best < 0) break;  // no larger partner: inconsistent counts, never read outside pf
+    last = bestp;
+    const double* o = entry_forces(pf, best);
+    for (int q = 0; q < 3; ++q) {
+      f[q] += o[q];
+      t[q] += o[3 + q];
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_particle_sum(int n, const int* __restrict__ pairs,
                                                       const long long* __restrict__ start,
                                                       const int* __restrict__ count,
@@ -117,35 +157,77 @@ __global__ __launch_bounds__(256) void k_particle_sum(int n, const int* __restri
                                                       int* __restrict__ touched) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const long long s = start[i];
   const int m = count[i];
+  if (m > kNarrowSum) return;  // k_particle_sum_wide
   double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
-  // ascending partner index: repeated selection of the next larger partner
-  int last = -1;
-  for (int done = 0; done < m; ++done) {
-    int best = -1, bestp = 0x7fffffff;
-    for (int e = 0; e < m; ++e) {
-      const int en = entries[s + e];
-      const int pr = en >> 1;
-      const int partner = (en & 1) ? pairs[2 * pr] : pairs[2 * pr + 1];
-      if (partner > last && partner < bestp) {
-        bestp = partner;
-        best = en;
-      }
-    }
-    if (best < 0) break;  // no larger partner: inconsistent counts, never read outside pf
-    last = bestp;
-    const double* o = pf + 12 * static_cast<long long>(best >> 1) + ((best & 1) ? 6 : 0);
-    for (int q = 0; q < 3; ++q) {
-      f[q] += o[q];
-      t[q] += o[3 + q];
-    }
-  }
+  sum_by_selection(pairs, entries, pf, start[i], m, f, t);
   for (int q = 0; q < 3; ++q) {
     force[3 * i + q] = f[q];
     torque[3 * i + q] = t[q];
   }
   if (touched) touched[i] = m > 0 ? 1 : 0;
+}
+
+// One workgroup per particle with more than kNarrowSum entries: partners
+// rank-sorted in LDS (distinct for unique pairs), the
+// entries' force rows staged 256 at a time, and one lane adding them in
+// ascending partner order, the same order and arithmetic as the narrow sum.
+__global__ __launch_bounds__(256) void k_particle_sum_wide(int n, const int* __restrict__ pairs,
+                                                           const long long* __restrict__ start,
+                                                           const int* __restrict__ count,
+                                                           const int* __restrict__ entries,
+                                                           const double* __restrict__ pf,
+                                                           double* __restrict__ force,
+                                                           double* __restrict__ torque,
+                                                           int* __restrict__ touched) {
+  __shared__ int part[kWideSort];
+  __shared__ int ord[kWideSort];
+  __shared__ double stage[256 * 6];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int m = count[i];
+  if (m <= kNarrowSum) return;  // block-uniform
+  const int tid = threadIdx.x;
+  const long long s = start[i];
+  double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+  if (m > kWideSort) {
+    if (tid == 0) sum_by_selection(pairs, entries, pf, s, m, f, t);
+  } else {
+    for (int e = tid; e < m; e += 256) part[e] = entry_partner(pairs, entries[s + e]);
+    __syncthreads();
+    for (int e = tid; e < m; e += 256) {
+      const int v = part[e];
+      int rank = 0;
+      // ties (a repeated pair) by position, so that ord is a permutation
+      for (int q = 0; q < m; ++q) rank += (part[q] < v || (part[q] == v && q < e)) ? 1 : 0;
+      ord[rank] = e;
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < m; c0 += 256) {
+      const int r = c0 + tid;
+      if (r < m) {
+        const double* o = entry_forces(pf, entries[s + ord[r]]);
+        for (int q = 0; q < 6; ++q) stage[6 * tid + q] = o[q];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        const int nr = m - c0 < 256 ? m - c0 : 256;
+        for (int k = 0; k < nr; ++k)
+          for (int q = 0; q < 3; ++q) {
+            f[q] += stage[6 * k + q];
+            t[q] += stage[6 * k + 3 + q];
+          }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    for (int q = 0; q < 3; ++q) {
+      force[3 * i + q] = f[q];
+      torque[3 * i + q] = t[q];
+    }
+    if (touched) touched[i] = 1;
+  }
 }
 
 struct Gravity {
@@ -257,6 +339,9 @@ int contacts_forces_impl(const dh_bodies* bodies, const int* pairs_dev, int npai
   }
   if (n > 0) {
     hipLaunchKernelGGL(k_particle_sum, grid_for(n, 256), 256, 0, st, n, pairs_dev,
+                       g_fs.start.as<long long>(), g_fs.incid.as<int>(), g_fs.entries.as<int>(),
+                       g_fs.pf.as<double>(), force_dev, torque_dev, touched_dev);
+    hipLaunchKernelGGL(k_particle_sum_wide, n, 256, 0, st, n, pairs_dev,
                        g_fs.start.as<long long>(), g_fs.incid.as<int>(), g_fs.entries.as<int>(),
                        g_fs.pf.as<double>(), force_dev, torque_dev, touched_dev);
     DH_HIP_CHECK(hipGetLastError());

@@ -94,6 +94,7 @@ extern "C" int dh_step(dh_scene* s, const dh_step_params* p, dh_step_info* info,
     info->candidate_pairs = np;
     info->contacts = total;
     info->triangle_checks = stats.triangle_checks;
+    info->pairs_culled = stats.pairs_culled;
   }
   return DH_OK;
 }
@@ -118,6 +119,7 @@ __global__ void k_step_status(const long long* __restrict__ np,
     out[2] = e;
     out[3] = static_cast<long long>(stats[20]);  // triangle checks
     out[4] |= (p > pair_cap ? 1 : 0) | (c > contact_cap ? 2 : 0) | (e ? 4 : 0);
+    out[5] = static_cast<long long>(stats[23]);  // pairs culled by the box test
   }
 }
 
@@ -200,7 +202,7 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
       (rc = g->force.grow(sizeof(double) * 3 * (n + 1))) ||
       (rc = g->torque.grow(sizeof(double) * 3 * (n + 1))) ||
       (rc = g->touched.grow(sizeof(int) * (n + 1))) ||
-      (rc = g->status.grow(sizeof(long long) * 5)) || (rc = broadphase_reserve(n)) ||
+      (rc = g->status.grow(sizeof(long long) * 6)) || (rc = broadphase_reserve(n)) ||
       (rc = detect_reserve(static_cast<int>(pair_cap > n ? pair_cap : n), contact_cap)) ||
       (rc = forces_reserve(n, static_cast<int>(pair_cap)))) {
     destroy_graph(g);
@@ -210,7 +212,7 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
   // stream, whose entry points (dh_step, dh_detect_pairs, ...) share the
   // library's scratch buffers with the graph
   hipError_t e = hipStreamCreate(&g->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(g->status.ptr, 0, sizeof(long long) * 5, g->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(g->status.ptr, 0, sizeof(long long) * 6, g->stream);
   if (e == hipSuccess) e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) {
     destroy_graph(g);
@@ -242,7 +244,7 @@ int dh_step_graph_launch(dh_step_graph* g, void* stream) {
 
 int dh_step_graph_status(dh_step_graph* g, void* stream, dh_step_info* info, int* overflow) {
   if (!g || !info || !overflow) return set_error(DH_ERR_ARG, "dh_step_graph_status: bad arguments");
-  long long h[5];
+  long long h[6];
   const hipStream_t st = stream ? as_stream(stream) : g->stream;
   DH_HIP_CHECK(hipMemcpyAsync(h, g->status.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
   // the sticky overflow word restarts at zero (outside the captured graph)
@@ -251,6 +253,7 @@ int dh_step_graph_status(dh_step_graph* g, void* stream, dh_step_info* info, int
   info->candidate_pairs = h[0];
   info->contacts = h[1];
   info->triangle_checks = h[3];
+  info->pairs_culled = h[5];
   *overflow = static_cast<int>(h[4]);
   return DH_OK;
 }

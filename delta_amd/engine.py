@@ -130,7 +130,7 @@ class DhStepParams(C.Structure):
 
 class DhStepInfo(C.Structure):
     _fields_ = [("candidate_pairs", C.c_int64), ("contacts", C.c_int64),
-                ("triangle_checks", C.c_int64)]
+                ("triangle_checks", C.c_int64), ("pairs_culled", C.c_int64)]
 
 
 def step_params(method=_lib.DH_BF, dt=1e-6, gravity=(0.0, -9.81, 0.0), spring=1e5,

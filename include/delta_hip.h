@@ -108,8 +108,11 @@ typedef struct dh_stats {
   int64_t batch_size;           /* _batchSize                                 */
   double batch_error;           /* _batchError                                */
   int64_t triangle_checks;      /* nTriA*nTriB per particle pair (Engine.cpp:451) */
-  int64_t pairs_evaluated;      /* triangle-pair evaluations actually run     */
+  int64_t pairs_evaluated;      /* triangle-pair evaluations of the reference */
   int64_t fallback_rows;        /* rows recomputed with bfSolver              */
+  int64_t pairs_culled;         /* of those, DH_BF pairs not solved: their    */
+                                /* boxes grown by the margin are apart, so no */
+                                /* contact (no counterpart in the reference)  */
 } dh_stats;
 
 /* ------------------------------------------------------------ housekeeping */
@@ -424,6 +427,7 @@ typedef struct dh_step_info {
   int64_t candidate_pairs;  /* broad-phase pairs                              */
   int64_t contacts;         /* contact points emitted                         */
   int64_t triangle_checks;  /* sum of nTriA*nTriB over candidates             */
+  int64_t pairs_culled;     /* of those, skipped by the DH_BF box test        */
 } dh_step_info;
 
 int dh_step(dh_scene* scene, const dh_step_params* params, dh_step_info* info, void* stream);

@@ -31,7 +31,8 @@ class DhStats(C.Structure):
     _fields_ = [("newton_histogram", C.c_int64 * 17), ("penalty_fails", C.c_int64),
                 ("batch_fails", C.c_int64), ("batch_size", C.c_int64),
                 ("batch_error", C.c_double), ("triangle_checks", C.c_int64),
-                ("pairs_evaluated", C.c_int64), ("fallback_rows", C.c_int64)]
+                ("pairs_evaluated", C.c_int64), ("fallback_rows", C.c_int64),
+                ("pairs_culled", C.c_int64)]
 
 
 class DhForceParams(C.Structure):

@@ -169,3 +169,25 @@ def test_step_graph_reports_overflow():
         assert overflow == 0
     finally:
         g.close()
+
+
+def test_wide_particle_sum_matches_oracle():
+    """A floor under 10 x 10 particles inside the margin: the floor has 100
+    contact-carrying pairs, more than one lane sums (k_particle_sum_wide);
+    two steps against the oracle, every state array bit for bit."""
+    r = 0.4e-3
+    meshes = [scenes.particle(0, r, 900 + 10 * i + j, [1.0e-3 * i, r * 0.98, 1.0e-3 * j])
+              for i in range(10) for j in range(10)]
+    meshes.append(scenes.floor(0.0, half=12e-3))
+    n = len(meshes)
+    obst = np.zeros(n, np.int32)
+    obst[-1] = 1
+    hs = engine.HostScene(meshes, 1e-4, 0, np.arange(n), 2500.0, obst)
+    ds = engine.DeviceScene(hs)
+    p = engine.step_params(method=_lib.DH_BF, spring=1e3, cell=hs.cell_size())
+    for s in range(2):
+        info = engine.step(ds, p)
+        nc, checks, npairs = ol.or_step(hs, _lib.DH_BF, 1e3, 0.1, 0.5, 1e-6, (0.0, -9.81, 0.0))
+        assert (info.contacts, info.candidate_pairs) == (nc, npairs)
+        assert info.candidate_pairs >= 100
+        compare(ds, hs, s, exact=not sincos_branch_used(hs, 1e-6))

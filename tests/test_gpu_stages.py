@@ -29,11 +29,11 @@ def host_candidates(hs):
         hi[p] = v.max(0) + m
     out = []
     for i in range(hs.n):
-        for j in range(i + 1, hs.n):
-            if hs.obstacle[i] and hs.obstacle[j]:
-                continue
-            if np.all(lo[i] <= hi[j]) and np.all(lo[j] <= hi[i]):
-                out.append((i, j))
+        j = np.arange(i + 1, hs.n)
+        ok = np.all(lo[i] <= hi[j], axis=1) & np.all(lo[j] <= hi[i], axis=1)
+        if hs.obstacle[i]:
+            ok &= hs.obstacle[j] == 0
+        out += [(i, int(k)) for k in j[ok]]
     return out
 
 
@@ -49,12 +49,25 @@ def gpu_pairs(ds, hs, cell):
     return [tuple(p) for p in out.download()[:2 * n.value].reshape(-1, 2)], out
 
 
-@pytest.mark.parametrize("scene", ["spinning", "c1"])
+def _scene(name):
+    if name == "spinning":
+        return spinning_scene()
+    if name == "c1":
+        return engine.c1_scene()
+    from delta_amd import scene_io
+    return scene_io.hopper_scene(num=int(name.split("-")[1]), sphere_level=1)
+
+
+@pytest.mark.parametrize("scene", ["spinning", "c1", "hopper-6", "hopper-12"])
 def test_broadphase_equals_all_pairs(scene):
-    hs = spinning_scene() if scene == "spinning" else engine.c1_scene()
+    """Exact candidate set; the hopper lattices (touching spheres in wall
+    prisms, obstacles first) have thousands of particles and pairs."""
+    hs = _scene(scene)
     ds = engine.DeviceScene(hs)
     got, _ = gpu_pairs(ds, hs, hs.cell_size())
-    assert got == host_candidates(hs)
+    want = host_candidates(hs)
+    assert len(got) == len(want), (len(got), len(want))
+    assert got == want
     # a cell smaller than the particles still yields the same set (all large)
     got2, _ = gpu_pairs(ds, hs, hs.cell_size() * 0.3)
     assert got2 == got
