@@ -328,12 +328,19 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
 
   if (M == DH_BF) {
     // Box test first, survivors compacted in row-major order into LDS, then
-    // solved 256 at a time, so that no lane idles beside a culled pair.  The
-    // boxes of the tile's rows and of B's triangles are staged in LDS when
-    // they fit, so a test reads 12 doubles from LDS instead of 18 from HBM.
+    // solved 256 at a time, so that no lane idles beside a culled pair.  When
+    // the boxes of the tile's rows and of B's triangles fit in LDS they are
+    // staged there, and only rows whose box meets the union of B's boxes and
+    // columns whose box meets the union of the rows' boxes are paired (the
+    // union test implies the pair test, so nothing else changes).
     __shared__ int surv[kSurvivors];
     __shared__ double boxes[2 * kBoxTriangles * 6];
+    __shared__ int act[2 * kBoxTriangles];  // active rows (tile-relative), then columns
+    __shared__ double uni[2][6];            // union of the rows' boxes, of B's boxes
+    __shared__ double red[4][12];
+    __shared__ int nact[2];
     const bool staged = nrows <= kBoxTriangles && p.nB <= kBoxTriangles;  // block-uniform
+    int nra = nrows, nca = p.nB;
     if (staged) {
       for (int t = tid; t < nrows + p.nB; t += 256) {
         const bool rowA = t < nrows;
@@ -341,14 +348,81 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
         tri_box(load_tri(g, rowA ? p.oA + r0 + t : p.oB + (t - nrows)), b, b + 3);
       }
       __syncthreads();
+      // unions: lanes over strided boxes, then waves, then the block
+      double u[12];
+      for (int q = 0; q < 3; ++q) {
+        u[q] = u[6 + q] = INFINITY;
+        u[3 + q] = u[9 + q] = -INFINITY;
+      }
+      for (int t = tid; t < nrows; t += 256)
+        for (int q = 0; q < 3; ++q) {
+          u[q] = fmin(u[q], boxes[6 * t + q]);
+          u[3 + q] = fmax(u[3 + q], boxes[6 * t + 3 + q]);
+        }
+      for (int t = tid; t < p.nB; t += 256)
+        for (int q = 0; q < 3; ++q) {
+          u[6 + q] = fmin(u[6 + q], boxes[6 * (kBoxTriangles + t) + q]);
+          u[9 + q] = fmax(u[9 + q], boxes[6 * (kBoxTriangles + t) + 3 + q]);
+        }
+      for (int o = 32; o > 0; o >>= 1)
+        for (int q = 0; q < 12; ++q) {
+          const double v = __shfl_xor(u[q], o);
+          u[q] = ((q % 6) < 3) ? fmin(u[q], v) : fmax(u[q], v);
+        }
+      if (lane == 0)
+        for (int q = 0; q < 12; ++q) red[wave][q] = u[q];
+      __syncthreads();
+      if (tid < 12) {
+        double v = red[0][tid];
+        for (int w = 1; w < 4; ++w) v = ((tid % 6) < 3) ? fmin(v, red[w][tid]) : fmax(v, red[w][tid]);
+        uni[tid / 6][tid % 6] = v;
+      }
+      if (tid < 2) nact[tid] = 0;
+      __syncthreads();
+      // ordered lists of active rows and columns
+      for (int side = 0; side < 2; ++side) {
+        const int cnt_all = side == 0 ? nrows : p.nB;
+        const double* other = uni[1 - side];
+        for (int t0 = 0; t0 < cnt_all; t0 += 256) {
+          const int t = t0 + tid;
+          bool on = false;
+          if (t < cnt_all) {
+            const double* b = boxes + 6 * (side == 0 ? t : kBoxTriangles + t);
+            on = !boxes_apart(b, b + 3, other, other + 3, p.margin);
+          }
+          const unsigned long long m = __ballot(on);
+          if (lane == 0) wave_cnt[wave] = __popcll(m);
+          __syncthreads();
+          int off = 0, cnt = 0;
+          for (int w = 0; w < 4; ++w) {
+            off += (w < wave) ? wave_cnt[w] : 0;
+            cnt += wave_cnt[w];
+          }
+          if (on) act[side * kBoxTriangles + nact[side] + off + lane_prefix(m)] = t;
+          __syncthreads();
+          if (tid == 0) nact[side] += cnt;
+          __syncthreads();
+        }
+      }
+      nra = nact[0];
+      nca = nact[1];
     }
-    for (int fpos = 0; fpos < total;) {
+    // candidate fe in [0, nra * nca): row-major over the active rows and columns
+    auto pair_of = [&](int fe, int& iA, int& iB) {
+      const int ra = fe / nca, cb = fe - ra * nca;
+      iA = staged ? r0 + act[ra] : r0 + ra;
+      iB = staged ? act[kBoxTriangles + cb] : cb;
+    };
+    const int tested = nra * nca;
+    culled += tid == 0 ? static_cast<unsigned>(total - tested) : 0u;
+    for (int fpos = 0; fpos < tested;) {
       int nl = 0;
-      while (fpos < total && nl <= kSurvivors - 256) {
-        const int f = fpos + tid;
+      while (fpos < tested && nl <= kSurvivors - 256) {
+        const int fe = fpos + tid;
         bool keep = false;
-        if (f < total) {
-          const int iA = (f0 + f) / p.nB, iB = (f0 + f) - iA * p.nB;
+        if (fe < tested) {
+          int iA, iB;
+          pair_of(fe, iA, iB);
           if (staged) {
             const double* a = boxes + 6 * (iA - r0);
             const double* b = boxes + 6 * (kBoxTriangles + iB);
@@ -366,7 +440,7 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
           off += (w < wave) ? wave_cnt[w] : 0;
           cnt += wave_cnt[w];
         }
-        if (keep) surv[nl + off + lane_prefix(m)] = f;
+        if (keep) surv[nl + off + lane_prefix(m)] = fe;
         nl += cnt;
         fpos += 256;
         __syncthreads();
@@ -377,9 +451,7 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
         V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
         int iA = 0, iB = 0;
         if (e < nl) {
-          const int f = surv[e];
-          iA = (f0 + f) / p.nB;
-          iB = (f0 + f) - iA * p.nB;
+          pair_of(surv[e], iA, iB);
           bf_solver(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), P, Q);
           emit = pq_dist(P, Q) <= p.margin;
         }
