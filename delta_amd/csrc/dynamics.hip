@@ -236,7 +236,8 @@ struct Gravity {
 
 __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __restrict__ force,
                                                   const double* __restrict__ torque,
-                                                  const int* __restrict__ touched, double h,
+                                                  const int* __restrict__ touched, double h_arg,
+                                                  const double* __restrict__ h_dev,
                                                   Gravity grav, double* __restrict__ x,
                                                   double* __restrict__ y, double* __restrict__ z,
                                                   const double* __restrict__ rx,
@@ -245,6 +246,7 @@ __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __r
                                                   const int* __restrict__ tri_offset) {
   const int i = blockIdx.x;
   if (b.obstacle[i]) return;
+  const double h = h_dev ? *h_dev : h_arg;  // the adaptive controller's step
   double v[3], W[3], R[9], pos[3], refpos[3], w[3];
   load3(b.linvel, i, v);
   load3(b.refangvel, i, W);
@@ -349,6 +351,20 @@ int contacts_forces_impl(const dh_bodies* bodies, const int* pairs_dev, int npai
   return DH_OK;
 }
 
+// dt_dev (may be null): the step is read there when the kernel runs.
+int integrate_impl(const dh_bodies* bodies, const double* force_dev, const double* torque_dev,
+                   const int* touched_dev, double dt, const double* dt_dev,
+                   const double gravity[3], double* x_dev, double* y_dev, double* z_dev,
+                   const double* refx_dev, const double* refy_dev, const double* refz_dev,
+                   const int* tri_offset_dev, hipStream_t st) {
+  if (bodies->n == 0) return DH_OK;
+  Gravity g{{gravity[0], gravity[1], gravity[2]}};
+  hipLaunchKernelGGL(k_integrate, bodies->n, 64, 0, st, to_dev(bodies), force_dev, torque_dev,
+                     touched_dev, dt, dt_dev, g, x_dev, y_dev, z_dev, refx_dev, refy_dev, refz_dev,
+                     tri_offset_dev);
+  DH_HIP_CHECK(hipGetLastError());
+  return DH_OK;
+}
 }  // namespace dh
 
 using namespace dh;
@@ -370,6 +386,7 @@ int dh_contacts_forces(const dh_bodies* bodies, const int* pairs_dev, int npairs
                               torque_dev, touched_dev, as_stream(stream));
 }
 
+
 int dh_integrate(dh_bodies* bodies, const double* force_dev, const double* torque_dev,
                  const int* touched_dev, double dt, const double gravity[3], double* x_dev,
                  double* y_dev, double* z_dev, const double* refx_dev, const double* refy_dev,
@@ -378,13 +395,9 @@ int dh_integrate(dh_bodies* bodies, const double* force_dev, const double* torqu
   if (rc) return rc;
   if (!bodies || !force_dev || !torque_dev || !gravity || !tri_offset_dev)
     return set_error(DH_ERR_ARG, "dh_integrate: bad arguments");
-  if (bodies->n == 0) return DH_OK;
-  Gravity g{{gravity[0], gravity[1], gravity[2]}};
-  hipLaunchKernelGGL(k_integrate, bodies->n, 64, 0, as_stream(stream), to_dev(bodies), force_dev,
-                     torque_dev, touched_dev, dt, g, x_dev, y_dev, z_dev, refx_dev, refy_dev,
-                     refz_dev, tri_offset_dev);
-  DH_HIP_CHECK(hipGetLastError());
-  return DH_OK;
+  return integrate_impl(bodies, force_dev, torque_dev, touched_dev, dt, nullptr, gravity, x_dev,
+                        y_dev, z_dev, refx_dev, refy_dev, refz_dev, tri_offset_dev,
+                        as_stream(stream));
 }
 
 int dh_update_angular(double refAngular[3], const double rotation[9], const double inertia[9],

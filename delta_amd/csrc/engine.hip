@@ -29,6 +29,17 @@ int contacts_forces_impl(const dh_bodies* bodies, const int* pairs_dev, int npai
                          const dh_force_params* fp, double* force_dev, double* torque_dev,
                          int* touched_dev, hipStream_t st);
 int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
+int integrate_impl(const dh_bodies* bodies, const double* force_dev, const double* torque_dev,
+                   const int* touched_dev, double dt, const double* dt_dev,
+                   const double gravity[3], double* x_dev, double* y_dev, double* z_dev,
+                   const double* refx_dev, const double* refy_dev, const double* refz_dev,
+                   const int* tri_offset_dev, hipStream_t st);
+int adaptive_begin(dh_adaptive_state* s, hipStream_t st);
+int adaptive_pairs(const int* pairs, int npairs, const long long* np_dev, const double* pos,
+                   const double* linvel, const double* radius, const double* eps,
+                   dh_adaptive_state* s, hipStream_t st);
+int adaptive_finish(const double* linvel, const int* obstacle, int n, dh_adaptive_state* s,
+                    hipStream_t st);
 
 struct StepScratch {
   DevBuf pairs, contacts, counts, offs, force, torque, touched;
@@ -43,7 +54,8 @@ using namespace dh;
 extern "C" int dh_step(dh_scene* s, const dh_step_params* p, dh_step_info* info, void* stream) {
   int rc = ensure_init();
   if (rc) return rc;
-  if (!s || !p) return set_error(DH_ERR_ARG, "dh_step: bad arguments");
+  if (!s || !p || (p->adaptive && !p->radius))
+    return set_error(DH_ERR_ARG, "dh_step: bad arguments");
   const hipStream_t st = as_stream(stream);
   const int n = s->geom.nparticles;
   if (g_step.pair_cap == 0) g_step.pair_cap = 16LL * (n + 16);
@@ -57,6 +69,13 @@ extern "C" int dh_step(dh_scene* s, const dh_step_params* p, dh_step_info* info,
     g_step.pair_cap = 2 * np;
   }
   if (rc) return rc;
+  if (p->adaptive) {
+    // triggerParticleTooClose on the candidates, at the start of the step
+    if ((rc = adaptive_begin(p->adaptive, st))) return rc;
+    if ((rc = adaptive_pairs(g_step.pairs.as<int>(), static_cast<int>(np), nullptr, s->bodies.pos,
+                             s->bodies.linvel, p->radius, s->geom.eps, p->adaptive, st)))
+      return rc;
+  }
   if ((rc = g_step.counts.grow(sizeof(int) * (np + 1)))) return rc;
   if ((rc = g_step.offs.grow(sizeof(long long) * (np + 2)))) return rc;
   long long total = 0;
@@ -86,10 +105,14 @@ extern "C" int dh_step(dh_scene* s, const dh_step_params* p, dh_step_info* info,
   } else {
     DH_HIP_CHECK(hipMemsetAsync(g_step.touched.ptr, 0, sizeof(int) * n, st));
   }
-  rc = dh_integrate(&s->bodies, g_step.force.as<double>(), g_step.torque.as<double>(),
-                    g_step.touched.as<int>(), p->dt, p->gravity, s->x, s->y, s->z, s->refx,
-                    s->refy, s->refz, s->geom.tri_offset, stream);
+  rc = integrate_impl(&s->bodies, g_step.force.as<double>(), g_step.torque.as<double>(),
+                      g_step.touched.as<int>(), p->dt, p->adaptive ? &p->adaptive->dt : nullptr,
+                      p->gravity, s->x, s->y, s->z, s->refx, s->refy, s->refz,
+                      s->geom.tri_offset, st);
   if (rc) return rc;
+  if (p->adaptive &&
+      (rc = adaptive_finish(s->bodies.linvel, s->bodies.obstacle, n, p->adaptive, st)))
+    return rc;
   if (info) {
     info->candidate_pairs = np;
     info->contacts = total;
@@ -142,6 +165,14 @@ static int record_step(dh_step_graph* g, hipStream_t st) {
   if ((rc = broadphase_graph(&s->geom, s->bodies.obstacle, g->params.cell, g->pairs.as<int>(),
                              g->pair_cap, &np_dev, st)))
     return rc;
+  dh_adaptive_state* ad = g->params.adaptive;
+  if (ad) {
+    if ((rc = adaptive_begin(ad, st))) return rc;
+    if ((rc = adaptive_pairs(g->pairs.as<int>(), static_cast<int>(g->pair_cap), np_dev,
+                             s->bodies.pos, s->bodies.linvel, g->params.radius, s->geom.eps, ad,
+                             st)))
+      return rc;
+  }
   if ((rc = detect_pairs_graph(g->params.method, &s->geom, g->pairs.as<int>(),
                                static_cast<int>(g->pair_cap), np_dev,
                                g->contacts.as<dh_contact>(), g->contact_cap, g->counts.as<int>(),
@@ -153,10 +184,12 @@ static int record_step(dh_step_graph* g, hipStream_t st) {
                                  &g->params.forces, g->force.as<double>(),
                                  g->torque.as<double>(), g->touched.as<int>(), st)))
     return rc;
-  if ((rc = dh_integrate(const_cast<dh_bodies*>(&s->bodies), g->force.as<double>(),
-                         g->torque.as<double>(), g->touched.as<int>(), g->params.dt,
-                         g->params.gravity, s->x, s->y, s->z, s->refx, s->refy, s->refz,
-                         s->geom.tri_offset, st)))
+  if ((rc = integrate_impl(&s->bodies, g->force.as<double>(), g->torque.as<double>(),
+                           g->touched.as<int>(), g->params.dt, ad ? &ad->dt : nullptr,
+                           g->params.gravity, s->x, s->y, s->z, s->refx, s->refy, s->refz,
+                           s->geom.tri_offset, st)))
+    return rc;
+  if (ad && (rc = adaptive_finish(s->bodies.linvel, s->bodies.obstacle, s->bodies.n, ad, st)))
     return rc;
   hipLaunchKernelGGL(k_step_status, 1, 64, 0, st, np_dev, detect_counter_dev(), detect_error_dev(),
                      detect_stats_dev(), g->pair_cap, g->contact_cap, g->status.as<long long>());
@@ -182,6 +215,7 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
   int rc = ensure_init();
   if (rc) return rc;
   if (!scene || !params || !out || pair_cap <= 0 || contact_cap <= 0 ||
+      (params->adaptive && !params->radius) ||
       pair_cap > 0x7fffffffLL || scene->geom.nparticles < 1)
     return set_error(DH_ERR_ARG, "dh_step_graph_create: bad arguments");
   if (params->method != DH_BF && params->method != DH_PENALTY)

@@ -53,6 +53,13 @@ class HostScene:
         self.refangvel = self.angvel.copy()
         self.rot = np.tile(np.eye(3).reshape(-1), (n, 1))
         self.material = np.zeros(n, np.int32)
+        self.radius = np.array([self._bounding_radius(i) for i in range(n)])
+
+    def _bounding_radius(self, i):
+        """Largest distance of particle i's vertices from its centre of mass."""
+        a, b = self.tri_offset[i], self.tri_offset[i + 1]
+        p = np.stack([self.x[3 * a:3 * b], self.y[3 * a:3 * b], self.z[3 * a:3 * b]], 1)
+        return float(np.linalg.norm(p - self.pos[i], axis=1).max()) if b > a else 0.0
 
     def cell_size(self):
         """Largest inflated AABB extent any non-obstacle particle can reach
@@ -96,7 +103,7 @@ class DeviceScene:
 
     FIELDS = ["x", "y", "z", "rx", "ry", "rz", "tri_offset", "eps", "friction", "id", "pos",
               "refpos", "linvel", "angvel", "refangvel", "rot", "inertia", "inverse", "mass",
-              "obstacle", "material"]
+              "obstacle", "material", "radius"]
 
     def __init__(self, hs):
         self.n = hs.n
@@ -123,9 +130,35 @@ class DhScene(C.Structure):
                 ("refz", C.c_void_p), ("bodies", _lib.DhBodies)]
 
 
+class DhAdaptiveState(C.Structure):
+    _fields_ = [("dt", C.c_double), ("time", C.c_double), ("min_dt", C.c_double),
+                ("increase", C.c_double), ("mesh_width", C.c_double), ("max_dt", C.c_double),
+                ("close", C.c_double), ("max_vel_approach", C.c_double),
+                ("max_vel_travel", C.c_double), ("step", C.c_int64), ("separate", C.c_int32),
+                ("reserved_", C.c_int32), ("scratch", C.c_uint64 * 4)]
+
+
+class AdaptiveState:
+    """A dh_adaptive_state in device memory (the Peano runner's adaptive
+    step control): start with `dt`, the lower bound `min_dt` (State.cpp's
+    1e-4 by default), the increase factor 1.1 and the mesh width (the
+    broad-phase cell)."""
+
+    def __init__(self, dt, mesh_width, min_dt=1e-4, increase=1.1):
+        st = DhAdaptiveState()
+        st.dt, st.min_dt, st.increase, st.mesh_width = dt, min_dt, increase, mesh_width
+        st.max_dt = float("inf")
+        self.dev = DeviceArray((C.sizeof(DhAdaptiveState),), np.uint8)
+        self.dev.upload(np.frombuffer(bytes(st), np.uint8))
+
+    def read(self):
+        return DhAdaptiveState.from_buffer_copy(self.dev.download().tobytes())
+
+
 class DhStepParams(C.Structure):
     _fields_ = [("method", C.c_int), ("dt", C.c_double), ("gravity", C.c_double * 3),
-                ("forces", _lib.DhForceParams), ("cell", C.c_double)]
+                ("forces", _lib.DhForceParams), ("cell", C.c_double),
+                ("adaptive", C.c_void_p), ("radius", C.c_void_p)]
 
 
 class DhStepInfo(C.Structure):
@@ -134,9 +167,14 @@ class DhStepInfo(C.Structure):
 
 
 def step_params(method=_lib.DH_BF, dt=1e-6, gravity=(0.0, -9.81, 0.0), spring=1e5,
-                damper=0.1, friction=0.5, cell=1.3e-3):
+                damper=0.1, friction=0.5, cell=1.3e-3, adaptive=None, dscene=None):
+    """adaptive: an AdaptiveState (then dscene, for the particles' radii)."""
+    if adaptive is not None and dscene is None:
+        raise ValueError("the adaptive step needs the DeviceScene (particle radii)")
     return DhStepParams(method, dt, (C.c_double * 3)(*gravity),
-                        _lib.DhForceParams(spring, damper, friction, 0), cell)
+                        _lib.DhForceParams(spring, damper, friction, 0), cell,
+                        adaptive.dev.ptr if adaptive is not None else None,
+                        dscene.buf["radius"].ptr if adaptive is not None else None)
 
 
 def step(dscene, params, stream=None):

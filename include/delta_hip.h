@@ -415,13 +415,49 @@ typedef struct dh_scene {
   dh_bodies bodies;
 } dh_scene;
 
+/* Adaptive time-step state of the Peano runner (multiscale/dem/State.cpp,
+ * mappings/Collision.cpp:188-299), kept in device memory so that captured
+ * steps follow it.  The caller fills dt, min_dt, increase, mesh_width,
+ * max_dt (+inf to start), max_vel_travel (0) and step (0); the rest is
+ * written by the step.  Each step integrates with `dt` and leaves the next
+ * step's `dt` there. */
+typedef struct dh_adaptive_state {
+  double dt;               /* step size of the next step                      */
+  double time;             /* simulated time                                  */
+  double min_dt;           /* mindt, State.cpp:156 (1e-4 there)               */
+  double increase;         /* increaseFactor and StepIncrement (1.1)          */
+  double mesh_width;       /* getMinMeshWidth: the broad-phase cell here      */
+  double max_dt;           /* _maxdt of the last update                       */
+  double close;            /* TwoParticlesAreClose of the last step, 0: none  */
+  double max_vel_approach; /* MaxVelocityApproach of the last step            */
+  double max_vel_travel;   /* fastest particle after the last step            */
+  int64_t step;            /* steps taken                                     */
+  int32_t separate;        /* some pair separated in the last step            */
+  int32_t reserved_;
+  uint64_t scratch[4];     /* reductions of the running step (library use)   */
+} dh_adaptive_state;
+
 typedef struct dh_step_params {
   int method;               /* DH_BF, DH_PENALTY, DH_HYBRID_*                 */
-  double dt;
+  double dt;                /* ignored when `adaptive` is set                 */
   double gravity[3];        /* reference: (0, -9.8, 0), Engine.cpp:534        */
   dh_force_params forces;
   double cell;              /* broad-phase cell size                          */
+  dh_adaptive_state* adaptive; /* device; NULL: fixed dt (no counterpart on   */
+                               /* the delta engine path, Peano's controller)  */
+  const double* radius;     /* device, n: bounding radius of each particle    */
+                            /* about its centre (needed with `adaptive`)      */
 } dh_step_params;
+
+/* The adaptive controller on its own (the step calls the same kernels):
+ * phase 0 at the start of a step, on its candidate pairs (clears the
+ * reductions, then triggerParticleTooClose for every pair); phase 1 after
+ * the step's update (fastest particle, then finishedTimeStep /
+ * adaptiveTimeStep / clearAccumulatedData).  No counterpart on the delta
+ * engine path; mirrors multiscale/dem/State.cpp:122-199. */
+int dh_adaptive_step(const int* pairs_dev, int npairs, const dh_bodies* bodies,
+                     const double* radius_dev, const double* eps_dev,
+                     dh_adaptive_state* state_dev, int phase, void* stream);
 
 typedef struct dh_step_info {
   int64_t candidate_pairs;  /* broad-phase pairs                              */

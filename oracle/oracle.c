@@ -1669,3 +1669,83 @@ void or_ispc_pen(unsigned n, double* const pp[6][3], double* const P[3], double*
     for (int k = 0; k < 3; ++k) { P[k][i] = p[k]; Q[k][i] = q[k]; }
   }
 }
+
+/* ---------------------------------------------------------------------------
+ * Adaptive time step of the Peano runner: triggerParticleTooClose
+ * (multiscale/dem/mappings/Collision.cpp:188-299) over the candidate pairs,
+ * then the travel speed (MoveParticles.cpp:84-89), finishedTimeStep,
+ * adaptiveTimeStep and clearAccumulatedData (State.cpp:26-35,137-199).
+ * MaxVelocityApproach is the maximum of -vBA over approaching pairs and the
+ * previous travel speed (see dh_adaptive_state).
+ * ------------------------------------------------------------------------- */
+void or_adaptive_pairs(const int* pairs, int npairs, const double* pos, const double* linvel,
+                       const double* radius, const double* eps, dh_adaptive_state* s) {
+  double close = INFINITY, approach = s->max_vel_travel;
+  int sep = 0;
+  for (int k = 0; k < npairs; ++k) {
+    const int a = pairs[2 * k], b = pairs[2 * k + 1];
+    double pdt = s->dt * s->increase;
+    if (pdt >= s->max_dt) pdt = s->max_dt;
+    double cA[3], cB[3], dAB[3], pdAB[3], vA[3], vB[3];
+    for (int q = 0; q < 3; ++q) {
+      vA[q] = linvel[3 * a + q];
+      vB[q] = linvel[3 * b + q];
+      cA[q] = pos[3 * a + q] + pdt * vA[q];
+      cB[q] = pos[3 * b + q] + pdt * vB[q];
+    }
+    for (int q = 0; q < 3; ++q) {
+      dAB[q] = pos[3 * b + q] - pos[3 * a + q];
+      pdAB[q] = cB[q] - cA[q];
+    }
+    const double d = sqrt((dAB[0] * dAB[0]) + (dAB[1] * dAB[1]) + (dAB[2] * dAB[2]));
+    const double pd = sqrt((pdAB[0] * pdAB[0]) + (pdAB[1] * pdAB[1]) + (pdAB[2] * pdAB[2]));
+    const double vBA = ((vB[0] * dAB[0]) + (vB[1] * dAB[1]) + (vB[2] * dAB[2])) -
+                       ((vA[0] * dAB[0]) + (vA[1] * dAB[1]) + (vA[2] * dAB[2]));
+    const double pvBA = ((vB[0] * pdAB[0]) + (vB[1] * pdAB[1]) + (vB[2] * pdAB[2])) -
+                        ((vA[0] * pdAB[0]) + (vA[1] * pdAB[1]) + (vA[2] * pdAB[2]));
+    if (vBA > 0 && pvBA > 0) { sep = 1; continue; }
+    if (-vBA > approach) approach = -vBA;
+    const double rA = ((2.0 * radius[a] + eps[a] * 2) * 1.1) / 2.0;
+    const double rB = ((2.0 * radius[b] + eps[b] * 2) * 1.1) / 2.0;
+    const double pdistancePerStep = (pd - rA - rB) / pdt;
+    if (-pvBA >= pdistancePerStep) {
+      const double rrA = (2.0 * radius[a]) / 2.0, rrB = (2.0 * radius[b]) / 2.0;
+      double remaining = (d - rrA - rrB - eps[a] - eps[b]);
+      if (remaining < 0.0) remaining = remaining * -1.1;
+      double localMaxdt = remaining * 0.82;
+      if (localMaxdt < 0.0) localMaxdt = -1 * localMaxdt;
+      /* informStateThatTwoParticlesAreClose: the smallest positive proposal */
+      if (localMaxdt > 0.0 && localMaxdt < close) close = localMaxdt;
+    }
+  }
+  s->close = close < INFINITY ? close : 0.0;
+  s->max_vel_approach = approach;
+  s->separate = sep;
+}
+
+void or_adaptive_finish(const double* linvel, const int* obstacle, int n, dh_adaptive_state* s) {
+  double travel = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (obstacle[i]) continue;
+    const double* u = linvel + 3 * i;
+    const double v = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    if (v > travel) travel = v;
+  }
+  s->time = s->time + s->dt;
+  const double inc = s->increase, mindt = s->min_dt;
+  s->max_dt = s->mesh_width / (2.0 * inc * s->max_vel_approach) * 0.80;
+  int clamp = 1;
+  if (s->close > 0.0) {
+    if (s->dt >= mindt) {
+      if (!(s->close > inc * s->dt)) s->dt = s->close;
+      clamp = 0;
+    } else if (s->dt < mindt) {
+      s->dt = mindt;
+    }
+  } else if (s->step > 2) {
+    s->dt = inc * s->dt;
+  }
+  if (clamp && s->dt > s->max_dt) s->dt = s->max_dt;
+  s->max_vel_travel = travel;
+  s->step = s->step + 1;
+}

@@ -36,6 +36,9 @@
  *                           closed forms (vertex over face), hand-worked plane
  *                           pre-pass codes, both case-1 fall-throughs and the
  *                           early returns on zero normals.
+ *   or_adaptive_*           hand-worked head-on cases (approach, separation,
+ *                           the close proposal and each branch of the update)
+ *                           in tests/test_oracle_adaptive.py.
  *   or_ispc_pen             or_zoltan_penalty's setup, step and output
  *                           (golden-pinned) with its own schedule, and an
  *                           independent numpy Gauss-Newton formulation of the
@@ -150,6 +153,13 @@ void or_ispc_bf(unsigned n, double* const pp[6][3], double* const P[3], double* 
                 double* distance);
 void or_ispc_pen(unsigned n, double* const pp[6][3], double* const P[3], double* const Q[3],
                  double* distance);
+
+/* Adaptive time step (multiscale/dem/mappings/Collision.cpp:188-299,
+ * State.cpp:26-35,137-199): the pairs at the start of a step, then the
+ * update after it (see dh_adaptive_state). */
+void or_adaptive_pairs(const int* pairs, int npairs, const double* pos, const double* linvel,
+                       const double* radius, const double* eps, dh_adaptive_state* s);
+void or_adaptive_finish(const double* linvel, const int* obstacle, int n, dh_adaptive_state* s);
 
 /* GJK distance between two convex point sets (gjk.cpp:302-460). */
 double or_gjk(const double* a, int na, const double* b, int nb, double p[3], double q[3]);

@@ -191,3 +191,75 @@ def test_wide_particle_sum_matches_oracle():
         assert (info.contacts, info.candidate_pairs) == (nc, npairs)
         assert info.candidate_pairs >= 100
         compare(ds, hs, s, exact=not sincos_branch_used(hs, 1e-6))
+
+
+def _adaptive_host(dt, mesh, min_dt):
+    from delta_amd.engine import DhAdaptiveState
+    s = DhAdaptiveState()
+    s.dt, s.min_dt, s.increase, s.mesh_width = dt, min_dt, 1.1, mesh
+    s.max_dt = float("inf")
+    return s
+
+
+STATE_FIELDS = ["dt", "time", "max_dt", "close", "max_vel_approach", "max_vel_travel", "step",
+                "separate"]
+
+
+def test_adaptive_step_matches_oracle():
+    """dh_step with the adaptive controller against the oracle's pairs pass,
+    step (with the controller's dt) and update, 5 steps of the spinning
+    scene: the controller state and every state array bit for bit."""
+    import ctypes as C
+    from test_gpu_stages import host_candidates
+    hs = spinning_scene()
+    ds = engine.DeviceScene(hs)
+    cell = hs.cell_size()
+    ad = engine.AdaptiveState(1e-6, cell, min_dt=1e-7)
+    k = 1e1  # soft: rotations stay in expmap's series branch (bit-exact)
+    p = engine.step_params(method=_lib.DH_BF, spring=k, cell=cell, adaptive=ad, dscene=ds)
+    sh = _adaptive_host(1e-6, cell, 1e-7)
+    o = ol.oracle()
+    for s in range(6):
+        pr = np.ascontiguousarray(np.array(host_candidates(hs), np.int32).reshape(-1))
+        o.or_adaptive_pairs(pr.ctypes.data_as(ol.IP), len(pr) // 2, ol.dptr(hs.pos),
+                            ol.dptr(hs.linvel), ol.dptr(hs.radius), ol.dptr(hs.eps), C.byref(sh))
+        engine.step(ds, p)
+        dt = sh.dt
+        ol.or_step(hs, _lib.DH_BF, k, 0.1, 0.5, dt, (0.0, -9.81, 0.0))
+        o.or_adaptive_finish(ol.dptr(hs.linvel), hs.obstacle.ctypes.data_as(ol.IP), hs.n,
+                             C.byref(sh))
+        gs = ad.read()
+        for f in STATE_FIELDS:
+            assert getattr(gs, f) == getattr(sh, f), f"step {s}: {f}"
+        assert not sincos_branch_used(hs, dt)
+        compare(ds, hs, s)
+    # from the fourth update on, contact-free steps grow dt by 1.1
+    assert sh.step == 6 and (sh.close > 0 or sh.dt > 1e-6)
+
+
+def test_adaptive_graph_matches_eager():
+    """The captured step carries the controller: 4 replays against 4 eager
+    adaptive steps of the C1 lattice, controller state and scene bit for
+    bit (the dt of each replay is read from device memory)."""
+    hs = engine.c1_scene()
+    cell = hs.cell_size()
+    de, dg = engine.DeviceScene(hs), engine.DeviceScene(hs)
+    ae, ag = engine.AdaptiveState(1e-6, cell, min_dt=1e-7), engine.AdaptiveState(1e-6, cell,
+                                                                                   min_dt=1e-7)
+    pe = engine.step_params(method=_lib.DH_BF, cell=cell, adaptive=ae, dscene=de)
+    pg = engine.step_params(method=_lib.DH_BF, cell=cell, adaptive=ag, dscene=dg)
+    g = engine.StepGraph(dg, pg, pair_cap=4096, contact_cap=1 << 16)
+    try:
+        for s in range(4):
+            engine.step(de, pe)
+            _lib.check(_lib.lib().dh_synchronize(), "sync")
+            g.launch()
+            _, overflow = g.status()
+            assert overflow == 0
+            a, b = ae.read(), ag.read()
+            for f in STATE_FIELDS:
+                assert getattr(a, f) == getattr(b, f), f"step {s}: {f}"
+            for f in STATE:
+                assert np.array_equal(de.download(f).view(np.uint64), dg.download(f).view(np.uint64))
+    finally:
+        g.close()
