@@ -4,9 +4,10 @@
 * icospheres (level 0: 20 triangles, 1: 80, 2: 320) with radial jitter;
 * uniform random rotations (Shoemake's quaternion method);
 * mass, centre of mass and inertia of a closed triangle mesh by integration
-  over the tetrahedra spanned with the origin (the approach of the
-  reference's simplex integration, delta/geometry/operators/physics.cpp:48-140,
-  written here from the divergence theorem).
+  over the tetrahedra spanned with the origin: `mass_properties` vectorised
+  (used for scenes), `simplex_inertia` in the reference's expressions and
+  order (delta/geometry/operators/physics.cpp:48-140, bit for bit);
+* hull particles of granulate.cpp:53-138 (`granulate_particle`).
 
 Meshes are (ntri, 3, 3) arrays [triangle][vertex][axis]; `to_soa` turns one
 into the reference's flat per-axis arrays x[3t+k]."""
@@ -129,3 +130,84 @@ def box_mesh(lo, hi):
     f = [(0, 2, 1), (0, 3, 2), (4, 5, 6), (4, 6, 7), (0, 1, 5), (0, 5, 4), (2, 3, 7), (2, 7, 6),
          (1, 2, 6), (1, 6, 5), (0, 4, 7), (0, 7, 3)]
     return p[np.array(f)]
+
+
+def _seq_sum(a):
+    """Sum of a 1-D array in index order (the reference's accumulation loop)."""
+    a = np.asarray(a, np.float64)
+    return float(np.cumsum(a)[-1]) if a.size else 0.0
+
+
+def simplex_inertia(mesh, density):
+    """delta/geometry/operators/physics.cpp:48-140 (simplex integration of
+    Solfec, operators/hull/spx.h:35-75): per triangle the tetrahedron with
+    the origin, J = density * det, the moments accumulated in triangle
+    order, then the Euler tensor shifted to the centre and turned into the
+    inertia tensor tr(E) 1 - E.  Same expressions and order as the
+    reference, so the result is the reference's to the last bit.  Returns
+    (mass, centre (3,), inertia (9,) row-major)."""
+    m = np.asarray(mesh, dtype=np.float64)
+    a, b, c = m[:, 0], m[:, 1], m[:, 2]
+    ax, ay, az = a[:, 0], a[:, 1], a[:, 2]
+    bx, by, bz = b[:, 0], b[:, 1], b[:, 2]
+    cx, cy, cz = c[:, 0], c[:, 1], c[:, 2]
+    # spx.cpp simplex_J(zero, a, b, c): q = (ax bx cx ay by cy az bz cz)
+    J = density * (ax * by * cz + ay * bz * cx + az * bx * cy - az * by * cx - ax * bz * cy -
+                   ay * bx * cz)
+    P = (a, b, c)
+
+    def second(u, v):
+        # sum over (i, j) of (1 + [i == j]) p_i[u] p_j[v], i, j over (0, a, b, c);
+        # the origin's terms are zeros and leave the running sum unchanged
+        t = None
+        for i in range(3):
+            for j in range(3):
+                x = (2.0 * P[i][:, u]) * P[j][:, v] if i == j else P[i][:, u] * P[j][:, v]
+                t = x if t is None else t + x
+        return (J / 120.0) * t
+
+    me = _seq_sum(J / 6.0)
+    sx = _seq_sum((J / 24.0) * (ax + bx + cx))
+    sy = _seq_sum((J / 24.0) * (ay + by + cy))
+    sz = _seq_sum((J / 24.0) * (az + bz + cz))
+    e = [0.0] * 9
+    e[0] = _seq_sum(second(0, 0))
+    e[3] = _seq_sum(second(0, 1))
+    e[4] = _seq_sum(second(1, 1))
+    e[6] = _seq_sum(second(0, 2))
+    e[7] = _seq_sum(second(1, 2))
+    e[8] = _seq_sum(second(2, 2))
+    cen = [sx / me, sy / me, sz / me]
+    e[0] -= (2 * sx - cen[0] * me) * cen[0]
+    e[4] -= (2 * sy - cen[1] * me) * cen[1]
+    e[8] -= (2 * sz - cen[2] * me) * cen[2]
+    e[3] -= cen[0] * sy + cen[1] * sx - cen[0] * cen[1] * me
+    e[6] -= cen[0] * sz + cen[2] * sx - cen[0] * cen[2] * me
+    e[7] -= cen[1] * sz + cen[2] * sy - cen[1] * cen[2] * me
+    e[1], e[2], e[5] = e[3], e[6], e[7]
+    tr = e[0] + e[4] + e[8]
+    inertia = [tr - e[0], -e[1], -e[2], -e[3], tr - e[4], -e[5], -e[6], -e[7], tr - e[8]]
+    return me, np.array(cen), np.array(inertia)
+
+
+def granulate_particle(center, h, npoints, rng, roundness=0.25):
+    """delta/geometry/shape/granulate.cpp:53-138: `npoints` points at angles
+    phi, theta uniform in [0, 360) (used as radians, as there) and radius
+    uniform in [(1 - roundness) h/2, h/2], then their convex hull, moved to
+    `center`.  The hull is Qhull's (scipy) instead of the reference's
+    hull.cpp: the same surface, its triangles in another order; each is
+    turned to face outwards.  `rng`: a shapes.Rng (drand48 there)."""
+    from scipy.spatial import ConvexHull
+    u = rng.uniform((npoints, 3))
+    phi = 360.0 * u[:, 0]
+    theta = 360.0 * u[:, 1]
+    r = (u[:, 2] * roundness + 1.0 - roundness) * (h / 2)
+    pts = np.stack([r * np.sin(phi) * np.cos(theta), r * np.sin(phi) * np.sin(theta),
+                    r * np.cos(phi)], 1)
+    hull = ConvexHull(pts)
+    tris = pts[hull.simplices]
+    inner = pts[hull.vertices].mean(0)
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    flip = np.einsum("ij,ij->i", n, tris[:, 0] - inner) < 0
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    return tris + np.asarray(center, float)
