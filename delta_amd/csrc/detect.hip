@@ -34,6 +34,12 @@ constexpr int kBoxTriangles = 320;
 
 enum DetectError : int { DE_NONE = 0, DE_REF_UB = 1, DE_TOO_LONG = 2 };
 
+// Per-tile counters (tile_stats) and the stats slot each one is summed into.
+enum TileStat : int {
+  TS_CHECKS = 0, TS_EVALUATED, TS_FALLBACK_ROWS, TS_PENALTY_FAILS, TS_BATCH_FAILS,
+  TS_BATCH_SIZE, TS_CULLED, kTileStats
+};
+
 enum StatSlot : int {
   ST_HIST = 0,            // 17 slots
   ST_PENALTY_FAILS = 17,
@@ -61,6 +67,9 @@ struct DetectArgs {
   unsigned long long* counter;  // [0] contacts reserved
   int tiles;                    // row ranges (workgroups) per particle pair
   int* tile_count;              // contacts per (pair, tile), pair-major
+  // kTileStats counters per tile (TS_*), summed by k_tile_stats afterwards:
+  // one global atomic per workgroup and counter would serialise the launch
+  unsigned long long* tile_stats;
   unsigned long long* stats;  // ST_COUNT counters
   int* error;                 // DetectError of the first failing pair
   // When set, the number of pairs is read here (at most npairs, which is then
@@ -210,8 +219,10 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
   const int k = blockIdx.x / g.tiles, tile = blockIdx.x - k * g.tiles;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  unsigned long long* ts = g.tile_stats + static_cast<long long>(blockIdx.x) * kTileStats;
   if (g.npairs_dev && k >= *g.npairs_dev) {
     if (tid == 0) g.tile_count[blockIdx.x] = 0;
+    if (tid < kTileStats) ts[tid] = 0;
     return;
   }
   const Pair p = load_pair(g, k);
@@ -223,11 +234,13 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
   if (M == DH_HYBRID_TRIANGLE && p.nB > p.nA) {
     if (tid == 0) atomicCAS(g.error, DE_NONE, static_cast<int>(DE_REF_UB));
     if (tid == 0) g.tile_count[blockIdx.x] = 0;
+    if (tid < kTileStats) ts[tid] = 0;
     return;
   }
   if ((is_hybrid || is_rowstat) && (rows > kMaxRowTriangles || p.nB > kMaxRowTriangles)) {
     if (tid == 0) atomicCAS(g.error, DE_NONE, static_cast<int>(DE_TOO_LONG));
     if (tid == 0) g.tile_count[blockIdx.x] = 0;
+    if (tid < kTileStats) ts[tid] = 0;
     return;
   }
   // this workgroup's rows [r0, r1)
@@ -494,8 +507,10 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     }
   }
   if (M == DH_BF) {
+    // slot 3 of local_stats (zeroed at the start, unused by BF otherwise)
     for (int o = 32; o > 0; o >>= 1) culled += __shfl_xor(culled, o);
-    if (lane == 0 && culled) atomicAdd(&g.stats[ST_CULLED], static_cast<unsigned long long>(culled));
+    if (lane == 0) atomicAdd(&local_stats[3], static_cast<unsigned long long>(culled));
+    __syncthreads();
   }
   if (tid == 0) {
     g.tile_count[blockIdx.x] = static_cast<int>(running);
@@ -504,16 +519,38 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     const unsigned long long evals =
         static_cast<unsigned long long>(total) +
         (is_hybrid ? local_stats[0] * static_cast<unsigned long long>(p.nB) : 0ull);
-    if (tile == 0) atomicAdd(&g.stats[ST_CHECKS], static_cast<unsigned long long>(p.nA) * p.nB);
-    atomicAdd(&g.stats[ST_EVALUATED], evals);
-    if (is_hybrid) {
-      atomicAdd(&g.stats[ST_FALLBACK_ROWS], local_stats[0]);
-      if (is_rowstat) {
-        atomicAdd(&g.stats[ST_PENALTY_FAILS], local_stats[1]);
-        atomicAdd(&g.stats[ST_BATCH_FAILS], local_stats[2]);
-        atomicAdd(&g.stats[ST_BATCH_SIZE], static_cast<unsigned long long>(nrows) * p.nB);
-      }
-    }
+    ts[TS_CHECKS] = tile == 0 ? static_cast<unsigned long long>(p.nA) * p.nB : 0ull;
+    ts[TS_EVALUATED] = evals;
+    ts[TS_FALLBACK_ROWS] = is_hybrid ? local_stats[0] : 0ull;
+    ts[TS_PENALTY_FAILS] = is_rowstat ? local_stats[1] : 0ull;
+    ts[TS_BATCH_FAILS] = is_rowstat ? local_stats[2] : 0ull;
+    ts[TS_BATCH_SIZE] = is_rowstat ? static_cast<unsigned long long>(nrows) * p.nB : 0ull;
+    ts[TS_CULLED] = M == DH_BF ? local_stats[3] : 0ull;
+  }
+}
+
+// Sums of the per-tile counters into the stats slots (one atomic per
+// workgroup and counter).
+__global__ __launch_bounds__(256) void k_tile_stats(const unsigned long long* __restrict__ ts,
+                                                    long long ntiles,
+                                                    unsigned long long* __restrict__ stats) {
+  __shared__ unsigned long long red[kTileStats];
+  if (threadIdx.x < kTileStats) red[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long acc[kTileStats] = {};
+  for (long long t = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; t < ntiles;
+       t += static_cast<long long>(gridDim.x) * 256)
+    for (int q = 0; q < kTileStats; ++q) acc[q] += ts[t * kTileStats + q];
+  for (int q = 0; q < kTileStats; ++q) {
+    unsigned long long v = acc[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&red[q], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < kTileStats && red[threadIdx.x]) {
+    const int slot[kTileStats] = {ST_CHECKS, ST_EVALUATED, ST_FALLBACK_ROWS, ST_PENALTY_FAILS,
+                                  ST_BATCH_FAILS, ST_BATCH_SIZE, ST_CULLED};
+    atomicAdd(&stats[slot[threadIdx.x]], red[threadIdx.x]);
   }
 }
 
@@ -539,8 +576,10 @@ __global__ __launch_bounds__(256) void k_gjk(DetectArgs g) {
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= g.npairs) return;
+  unsigned long long* ts = g.tile_stats + static_cast<long long>(k) * kTileStats;
   if (g.npairs_dev && k >= *g.npairs_dev) {
     if (lane == 0) g.tile_count[k] = 0;
+    if (lane < kTileStats) ts[lane] = 0;
     return;
   }
   const Pair p = load_pair(g, k);
@@ -589,8 +628,10 @@ __global__ __launch_bounds__(256) void k_gjk(DetectArgs g) {
       }
     }
     g.tile_count[k] = emit ? 1 : 0;
-    atomicAdd(&g.stats[ST_CHECKS], static_cast<unsigned long long>(p.nA) * p.nB);
-    atomicAdd(&g.stats[ST_EVALUATED], 1ull);
+  }
+  if (lane < kTileStats) {
+    ts[lane] = lane == TS_CHECKS ? static_cast<unsigned long long>(p.nA) * p.nB
+             : lane == TS_EVALUATED ? 1ull : 0ull;
   }
 }
 
@@ -665,7 +706,7 @@ __global__ __launch_bounds__(256) void k_scatter(const dh_contact* __restrict__ 
 }
 
 struct DetectScratch {
-  DevBuf temp, counter, stats, error, counts, offs, sums, tiles;
+  DevBuf temp, counter, stats, error, counts, offs, sums, tiles, tstats;
 };
 static DetectScratch g_scratch;
 
@@ -686,6 +727,14 @@ int scan_counts(const int* counts, int n, long long* offs, hipStream_t st) {
 template <int M>
 static void launch_detect(unsigned grid, size_t shm, hipStream_t st, const DetectArgs& a) {
   hipLaunchKernelGGL(k_detect<M>, grid, 256, shm, st, a);
+}
+
+static void launch_tile_stats(const unsigned long long* ts, long long ntiles,
+                              unsigned long long* stats, hipStream_t st) {
+  if (ntiles <= 0) return;
+  long long grid = (ntiles + 255) / 256;
+  if (grid > 256) grid = 256;
+  hipLaunchKernelGGL(k_tile_stats, static_cast<unsigned>(grid), 256, 0, st, ts, ntiles, stats);
 }
 
 // Row ranges per particle pair: enough workgroups to fill the 256 CUs
@@ -732,6 +781,9 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.counter = g_scratch.counter.as<unsigned long long>();
   a.tiles = tiles;
   a.tile_count = tile_count;
+  const long long ntiles = static_cast<long long>(npairs) * tiles;
+  if ((rc = g_scratch.tstats.grow(sizeof(unsigned long long) * kTileStats * ntiles))) return rc;
+  a.tile_stats = g_scratch.tstats.as<unsigned long long>();
   a.stats = g_scratch.stats.as<unsigned long long>();
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = nullptr;
@@ -754,6 +806,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
       return set_error(DH_ERR_UNSUPPORTED, "dh_detect_pairs: method %d", method);
   }
   DH_HIP_CHECK(hipGetLastError());
+  launch_tile_stats(a.tile_stats, ntiles, a.stats, st);
   unsigned long long total = 0;
   int err = 0;
   unsigned long long st_host[ST_COUNT];
@@ -829,6 +882,8 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   const int tiles = graph_tiles(pair_cap);
   a.tiles = tiles;
   a.tile_count = tiles > 1 ? g_scratch.tiles.as<int>() : counts;
+  const long long ntiles = static_cast<long long>(pair_cap) * tiles;
+  a.tile_stats = g_scratch.tstats.as<unsigned long long>();
   a.stats = g_scratch.stats.as<unsigned long long>();
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = np_dev;
@@ -840,6 +895,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
       return set_error(DH_ERR_UNSUPPORTED, "captured step: method %d (use DH_BF or DH_PENALTY)",
                        method);
   }
+  launch_tile_stats(a.tile_stats, ntiles, a.stats, st);
   if (tiles > 1)
     hipLaunchKernelGGL(k_tile_sums, grid_for(pair_cap, 256), 256, 0, st, a.tile_count, tiles,
                        pair_cap, counts);
@@ -864,6 +920,9 @@ int detect_reserve(int pair_cap, long long contact_cap) {
   const int tiles = graph_tiles(pair_cap);
   if (tiles > 1 &&
       (rc = g_scratch.tiles.grow(sizeof(int) * static_cast<size_t>(pair_cap) * tiles)))
+    return rc;
+  if ((rc = g_scratch.tstats.grow(sizeof(unsigned long long) * kTileStats *
+                                  static_cast<size_t>(pair_cap) * tiles)))
     return rc;
   return DH_OK;
 }
