@@ -47,8 +47,8 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel three times "
                          "(reference for scaling HBM byte counters) and exit")
-    ap.add_argument("--train-pairs-log2", type=int, default=23)
-    ap.add_argument("--train-epochs", type=int, default=10)
+    ap.add_argument("--train-pairs-log2", type=int, default=24)
+    ap.add_argument("--train-epochs", type=int, default=20)
     return ap.parse_args()
 
 
@@ -92,6 +92,22 @@ def train_network(a):
     return tr, ds, info
 
 
+def contact_errors(P0, Q0, d0, P, Q, d, margin=2e-4):
+    """Contact-record errors (contactpoint.cpp:116-155) over the pairs that
+    are contacts in both: point (P+Q)/2, normal (P-Q)/|P-Q|, depth."""
+    contact = (d0 <= margin) & (d <= margin)
+    if not contact.any():
+        return {"contacts": 0, "max_contact_point_error_m": 0.0, "max_normal_error": 0.0,
+                "max_depth_error_m": 0.0}
+    x0, x1 = (P0 + Q0) / 2, (P + Q) / 2
+    n0 = (P0 - Q0) / np.where(d0 > 0, d0, 1.0)
+    n1 = (P - Q) / np.where(d > 0, d, 1.0)
+    return {"contacts": int(contact.sum()),
+            "max_contact_point_error_m": float(np.abs(x1 - x0)[:, contact].max()),
+            "max_normal_error": float(np.abs(n1 - n0)[:, contact].max()),
+            "max_depth_error_m": float(np.abs(d - d0)[contact].max())}
+
+
 def learned_accuracy(batch):
     """Learned results against the FP64 brute force over every pair."""
     import ctypes as C
@@ -106,23 +122,16 @@ def learned_accuracy(batch):
     margin = 2e-4
     band = np.abs(d0 - margin) <= 1e-9
     mism = int(np.count_nonzero((d <= margin)[~band] != (d0 <= margin)[~band]))
-    contact = (d0 <= margin) & (d <= margin)
-    # contact-record errors (contactpoint.cpp:116-155) on contacts
-    x0, x1 = (P0 + Q0) / 2, (P + Q) / 2
-    n0 = (P0 - Q0) / np.where(d0 > 0, d0, 1.0)
-    n1 = (P - Q) / np.where(d > 0, d, 1.0)
     cert = (fl & 8) != 0
-    return {"pairs": int(batch.n), "certified_fraction": float(cert.mean()),
-            "brute_force_fraction": float(1 - cert.mean()),
-            "features_evaluated_per_pair": cnt.features_fp64 / batch.n,
-            "max_rel_distance_error": float(rel.max()),
-            "contact_set_mismatches_outside_1e-9_band": mism,
-            "pairs_in_band": int(band.sum()),
-            "contacts": int(contact.sum()),
-            "max_contact_point_error_m": float(np.abs(x1 - x0)[:, contact].max()) if contact.any() else 0.0,
-            "max_normal_error": float(np.abs(n1 - n0)[:, contact].max()) if contact.any() else 0.0,
-            "max_depth_error_m": float(np.abs(d - d0)[contact].max()) if contact.any() else 0.0,
-            "bitwise_equal_fraction": float(np.mean(d.view(np.uint64) == d0.view(np.uint64)))}
+    out = {"pairs": int(batch.n), "certified_fraction": float(cert.mean()),
+           "brute_force_fraction": float(1 - cert.mean()),
+           "features_evaluated_per_pair": cnt.features_fp64 / batch.n,
+           "max_rel_distance_error": float(rel.max()),
+           "contact_set_mismatches_outside_1e-9_band": mism,
+           "pairs_in_band": int(band.sum())}
+    out.update(contact_errors(P0, Q0, d0, P, Q, d, margin))
+    out["bitwise_equal_fraction"] = float(np.mean(d.view(np.uint64) == d0.view(np.uint64)))
+    return out
 
 
 def warm_start_report(batch):
@@ -144,19 +153,21 @@ def mixed_accuracy(batch):
     """FP32 screen + FP64 recheck against the FP64 brute force."""
     from delta_amd import _lib
     batch.run("bf64")
-    _, _, d0, _ = batch.results()
+    P0, Q0, d0, _ = batch.results()
     batch.run("mixed")
-    _, _, d, fl = batch.results()
+    P, Q, d, fl = batch.results()
     re = (fl & 2) != 0
     rel = np.abs(d - d0) / np.where(d0 > 0, d0, 1.0)
     margin = 2e-4
     band = np.abs(d0 - margin) <= 1e-9
-    return {"rechecked_fraction": float(re.mean()),
-            "max_rel_distance_error": float(rel.max()),
-            "max_rel_distance_error_fp32_pairs": float(rel[~re].max()) if (~re).any() else 0.0,
-            "pairs_above_1e-6_rel_error": int(np.count_nonzero(rel > 1e-6)),
-            "contact_set_mismatches_outside_1e-9_band":
-                int(np.count_nonzero((d <= margin)[~band] != (d0 <= margin)[~band]))}
+    out = {"rechecked_fraction": float(re.mean()),
+           "max_rel_distance_error": float(rel.max()),
+           "max_rel_distance_error_fp32_pairs": float(rel[~re].max()) if (~re).any() else 0.0,
+           "pairs_above_1e-6_rel_error": int(np.count_nonzero(rel > 1e-6)),
+           "contact_set_mismatches_outside_1e-9_band":
+               int(np.count_nonzero((d <= margin)[~band] != (d0 <= margin)[~band]))}
+    out.update(contact_errors(P0, Q0, d0, P, Q, d, margin))
+    return out
 
 
 def cpu_baseline(batch, target_s):
@@ -311,6 +322,61 @@ def dem_c1(a):
     print(json.dumps(out), flush=True)
 
 
+def hopper_hbm(hs, ds, prm, L, reps=20):
+    """HBM rates of binning (dh_broadphase: particle boxes from every vertex
+    replica, then the grid) and of the fused update (dh_integrate on every
+    particle, zero forces), each timed with HIP events over `reps` calls.
+    Algorithmic bytes: binning 24 B per vertex replica read plus 64 B per
+    particle (offsets, margin, box); update 48 B per vertex replica
+    (referential read, spatial write) plus 560 B of particle state."""
+    from delta_amd import _lib
+    from delta_amd.device import DeviceArray
+    nv = 3 * int(hs.tri_offset[-1])
+    n = hs.n
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.dh_event_create(C.byref(e0))
+    L.dh_event_create(C.byref(e1))
+    ms = C.c_float()
+    cap = 16 * (n + 16)
+    pairs = DeviceArray((2 * cap,), np.int32)
+    npairs = C.c_int(0)
+    L.dh_broadphase.argtypes = [C.POINTER(_lib.DhGeometry), C.c_void_p, C.c_double, C.c_void_p,
+                                C.c_int, C.POINTER(C.c_int), C.c_void_p]
+    geom = C.byref(ds.scene.geom)
+    L.dh_broadphase(geom, ds.buf["obstacle"].ptr, prm.cell, pairs.ptr, cap, C.byref(npairs), None)
+    L.dh_event_record(e0, None)
+    for _ in range(reps):
+        L.dh_broadphase(geom, ds.buf["obstacle"].ptr, prm.cell, pairs.ptr, cap, C.byref(npairs),
+                        None)
+    L.dh_event_record(e1, None)
+    _lib.check(L.dh_synchronize(), "sync")
+    L.dh_event_elapsed_ms(C.byref(ms), e0, e1)
+    t_bin = ms.value * 1e-3 / reps
+    zero = DeviceArray.from_host(np.zeros(3 * n))
+    g = (C.c_double * 3)(0.0, 0.0, -10.0)
+    L.dh_integrate.argtypes = [C.POINTER(_lib.DhBodies), C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_double,
+                               C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    sc = ds.scene
+    args = (C.byref(sc.bodies), zero.ptr, zero.ptr, None, 1e-9, g, sc.x, sc.y, sc.z, sc.refx,
+            sc.refy, sc.refz, sc.geom.tri_offset, None)
+    L.dh_integrate(*args)
+    L.dh_event_record(e0, None)
+    for _ in range(reps):
+        L.dh_integrate(*args)
+    L.dh_event_record(e1, None)
+    _lib.check(L.dh_synchronize(), "sync")
+    L.dh_event_elapsed_ms(C.byref(ms), e0, e1)
+    t_int = ms.value * 1e-3 / reps
+    b_bin = 24.0 * nv + 64.0 * n
+    b_int = 48.0 * nv + 560.0 * n
+    return {"binning_GBps": b_bin / t_bin / 1e9, "binning_ms": t_bin * 1e3,
+            "binning_bytes": b_bin, "update_GBps": b_int / t_int / 1e9, "update_ms": t_int * 1e3,
+            "update_bytes": b_int, "peak_GBps": 8000.0,
+            "note": "binning time includes the grid passes and the pair-count read-back"}
+
+
 def dem_hopper(a):
     """The hopper of input/hopper.py (also input/hopper.mbfcp): 25^3 touching
     spheres of radius 0.02 as 80-triangle icospheres, 26 wall prisms,
@@ -340,6 +406,7 @@ def dem_hopper(a):
         pairs += info.candidate_pairs
     _lib.check(L.dh_synchronize(), "sync")
     dt = time.perf_counter() - t0
+    hbm = hopper_hbm(hs, ds, prm, L)
     out = {"metric": "particle-steps/s (hopper DEM step, FP64 brute force)",
            "value": hs.n * a.steps / dt, "unit": "particle-steps/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps,
@@ -349,7 +416,8 @@ def dem_hopper(a):
                                   "26 wall prisms, dt 1e-6 s, FP64 brute force",
                       "particles": hs.n, "triangles": int(hs.tri_offset[-1])},
            "triangle_checks_per_s": checks / dt, "culled_fraction": culled / max(checks, 1),
-           "mean_candidate_pairs": pairs / a.steps, "mean_contacts": contacts / a.steps}
+           "mean_candidate_pairs": pairs / a.steps, "mean_contacts": contacts / a.steps,
+           "hbm": hbm}
     if not a.no_cpu:
         # the oracle's brute-force detection on the CPU over a sample of the
         # step's candidate pairs (one thread): triangle checks per second
