@@ -73,15 +73,22 @@ __device__ __forceinline__ unsigned hash_cell(long long cx, long long cy, long l
   return static_cast<unsigned>(h) & mask;
 }
 
+// The grid is a cell list sorted by hash bucket: bucket b holds the
+// particles sorted[cstart[b] .. cstart[b+1]) (a counting sort: counts,
+// exclusive scan, scatter), so a bucket is one contiguous range and its
+// members load without pointer chasing.
 struct GridArgs {
   const Aabb* box;
   const int* obstacle;  // may be null
   int n;
   double h, inv_h;
-  int* head;  // table of `mask+1` buckets
-  int* next;
-  unsigned mask;
-  int* large;        // indices of large particles (any order)
+  unsigned mask;             // mask + 1 buckets
+  int* ccount;               // particles per bucket
+  int* cfill;                // scatter cursor per bucket
+  const long long* cstart;   // bucket starts (mask + 2 entries)
+  int* sorted;               // particle ids by bucket
+  unsigned* cellh;           // bucket of each particle (large: unused)
+  int* large;                // indices of large particles (any order)
   int* nlarge;
 };
 
@@ -92,7 +99,7 @@ __device__ __forceinline__ bool is_large(const GridArgs& g, int i, const Aabb& b
   return false;
 }
 
-__global__ __launch_bounds__(256) void k_grid_insert(GridArgs g) {
+__global__ __launch_bounds__(256) void k_grid_bin(GridArgs g) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= g.n) return;
   const Aabb b = g.box[i];
@@ -103,15 +110,23 @@ __global__ __launch_bounds__(256) void k_grid_insert(GridArgs g) {
   long long c[3];
   cell_of(b, g.inv_h, c);
   const unsigned hsh = hash_cell(c[0], c[1], c[2], g.mask);
-  g.next[i] = atomicExch(&g.head[hsh], i);
+  g.cellh[i] = hsh;
+  atomicAdd(&g.ccount[hsh], 1);
 }
 
-// One wave searches the partners j > i of one particle i.  A search through
-// the grid is latency-bound (bucket head, chain links and boxes are
-// dependent loads), so the 27 neighbour buckets are walked by 27 lanes at
-// once, one chain link per lane per round, and the large-particle list and
-// the all-particle scan of a large i are strided over the wave.  Finds are
-// compacted with ballots, so every lane sees the running count.
+__global__ __launch_bounds__(256) void k_grid_scatter(GridArgs g) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.n) return;
+  if (is_large(g, i, g.box[i])) return;
+  const unsigned hsh = g.cellh[i];
+  g.sorted[g.cstart[hsh] + atomicAdd(&g.cfill[hsh], 1)] = i;
+}
+
+// One wave searches the partners j > i of one particle i: the 27 neighbour
+// buckets are walked by 27 lanes at once, one member per lane per round, and
+// the large-particle list and the all-particle scan of a large i are strided
+// over the wave.  Finds are compacted with ballots, so every lane sees the
+// running count.
 constexpr int kBpWaves = 4;    // particles per workgroup
 constexpr int kBpList = 512;   // partners sorted in LDS per particle (else a serial sort)
 
@@ -135,20 +150,21 @@ __device__ int grid_partners(const GridArgs& g, int i, const Aabb& bi, int lane,
     const unsigned o = __shfl(hsh, l);
     if (l < lane && o == hsh) walk = false;
   }
-  int j = walk ? g.head[hsh] : -1;
+  long long pos = walk ? g.cstart[hsh] : 0, end = walk ? g.cstart[hsh + 1] : 0;
   int found = 0;
-  while (__ballot(j >= 0)) {
+  while (__ballot(pos < end)) {
     bool e = false;
-    const int cand = j;
-    if (j >= 0) {
-      if (j > i) {
-        const Aabb bj = g.box[j];
+    int cand = -1;
+    if (pos < end) {
+      cand = g.sorted[pos];
+      if (cand > i) {
+        const Aabb bj = g.box[cand];
         long long cj[3];
         cell_of(bj, g.inv_h, cj);
         e = cj[0] - c[0] >= -1 && cj[0] - c[0] <= 1 && cj[1] - c[1] >= -1 && cj[1] - c[1] <= 1 &&
             cj[2] - c[2] >= -1 && cj[2] - c[2] <= 1 && overlap(bi, bj);
       }
-      j = g.next[j];
+      ++pos;
     }
     const unsigned long long mask = __ballot(e);
     if (e) put(found + bp_lane_prefix(mask), cand);
@@ -176,13 +192,20 @@ template <typename F>
 __device__ int large_partners(const GridArgs& g, int i, const Aabb& bi, int lane, F put) {
   const bool obst_i = g.obstacle && g.obstacle[i];
   int found = 0;
-  for (int j0 = i + 1; j0 < g.n; j0 += 64) {
-    const int j = j0 + lane;
-    bool e = false;
-    if (j < g.n) e = !(obst_i && g.obstacle && g.obstacle[j]) && overlap(bi, g.box[j]);
-    const unsigned long long mask = __ballot(e);
-    if (e) put(found + bp_lane_prefix(mask), j);
-    found += __popcll(mask);
+  // four strips of 64 per round: their loads are issued together
+  for (int j0 = i + 1; j0 < g.n; j0 += 256) {
+    bool e[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = j0 + 64 * q + lane;
+      e[q] = j < g.n && !(obst_i && g.obstacle && g.obstacle[j]) && overlap(bi, g.box[j]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const unsigned long long mask = __ballot(e[q]);
+      if (e[q]) put(found + bp_lane_prefix(mask), j0 + 64 * q + lane);
+      found += __popcll(mask);
+    }
   }
   return found;
 }
@@ -258,72 +281,88 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_grid_fill(GridArgs g,
 }
 
 int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
+int scan_reserve(int n);
 
 struct BpScratch {
-  DevBuf box, head, next, count, start, err, obst;
+  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst;
 };
 static BpScratch g_bp;
 
-int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
-                    long long cap, long long* npairs, hipStream_t st) {
-  const int n = geom->nparticles;
-  int rc;
-  if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
+static unsigned table_size(int n) {
   unsigned tbl = 1;
   while (tbl < 2u * static_cast<unsigned>(n)) tbl <<= 1;
-  if ((rc = g_bp.head.grow(sizeof(int) * tbl))) return rc;
-  if ((rc = g_bp.next.grow(sizeof(int) * (n + 1)))) return rc;
+  return tbl;
+}
+
+// Buffers of the broad phase for n particles (call before a capture).
+int broadphase_reserve(int n) {
+  int rc;
+  const unsigned tbl = table_size(n);
+  if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
+  if ((rc = g_bp.ccount.grow(sizeof(int) * (tbl + 1)))) return rc;
+  if ((rc = g_bp.cfill.grow(sizeof(int) * tbl))) return rc;
+  if ((rc = g_bp.cstart.grow(sizeof(long long) * (tbl + 1)))) return rc;
+  if ((rc = g_bp.sorted.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_bp.cellh.grow(sizeof(unsigned) * (n + 1)))) return rc;
   if ((rc = g_bp.count.grow(sizeof(int) * (n + 1)))) return rc;
   if ((rc = g_bp.start.grow(sizeof(long long) * (n + 1)))) return rc;
   if ((rc = g_bp.err.grow(sizeof(int)))) return rc;
   if ((rc = g_bp.obst.grow(sizeof(int) * (n + 1)))) return rc;
-  DH_HIP_CHECK(hipMemsetAsync(g_bp.err.ptr, 0, sizeof(int), st));
-  DH_HIP_CHECK(hipMemsetAsync(g_bp.head.ptr, 0xFF, sizeof(int) * tbl, st));
+  return scan_reserve(static_cast<int>(tbl) > n ? static_cast<int>(tbl) : n);
+}
+
+// Boxes, the sorted cell list and the partner counts, on cleared counters.
+static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double cell, GridArgs& g,
+                      hipStream_t st) {
+  const int n = geom->nparticles;
+  const unsigned tbl = table_size(n);
   hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
                      geom->eps, g_bp.box.as<Aabb>());
-  GridArgs g;
   g.box = g_bp.box.as<Aabb>();
   g.obstacle = obstacle_dev;
   g.n = n;
   g.h = cell;
   g.inv_h = 1.0 / cell;
-  g.head = g_bp.head.as<int>();
-  g.next = g_bp.next.as<int>();
   g.mask = tbl - 1;
+  g.ccount = g_bp.ccount.as<int>();
+  g.cfill = g_bp.cfill.as<int>();
+  g.cstart = g_bp.cstart.as<long long>();
+  g.sorted = g_bp.sorted.as<int>();
+  g.cellh = g_bp.cellh.as<unsigned>();
   g.large = g_bp.obst.as<int>();
   g.nlarge = g_bp.err.as<int>();
-  hipLaunchKernelGGL(k_grid_insert, grid_for(n, 256), 256, 0, st, g);
+  hipLaunchKernelGGL(k_grid_bin, grid_for(n, 256), 256, 0, st, g);
+  int rc;
+  if ((rc = scan_counts(g_bp.ccount.as<int>(), static_cast<int>(tbl), g_bp.cstart.as<long long>(),
+                        st)))
+    return rc;
+  hipLaunchKernelGGL(k_grid_scatter, grid_for(n, 256), 256, 0, st, g);
   hipLaunchKernelGGL(k_grid_count, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
                      g_bp.count.as<int>());
   DH_HIP_CHECK(hipGetLastError());
-  if ((rc = scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st))) return rc;
-  long long last_start = 0;
-  int last_count = 0;
-  DH_HIP_CHECK(hipMemcpyAsync(&last_start, g_bp.start.as<long long>() + (n - 1), sizeof(long long),
-                              hipMemcpyDeviceToHost, st));
-  DH_HIP_CHECK(hipMemcpyAsync(&last_count, g_bp.count.as<int>() + (n - 1), sizeof(int),
+  return scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st);
+}
+
+int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
+                    long long cap, long long* npairs, hipStream_t st) {
+  const int n = geom->nparticles;
+  int rc;
+  if ((rc = broadphase_reserve(n))) return rc;
+  const unsigned tbl = table_size(n);
+  DH_HIP_CHECK(hipMemsetAsync(g_bp.err.ptr, 0, sizeof(int), st));
+  DH_HIP_CHECK(hipMemsetAsync(g_bp.ccount.ptr, 0, sizeof(int) * tbl, st));
+  DH_HIP_CHECK(hipMemsetAsync(g_bp.cfill.ptr, 0, sizeof(int) * tbl, st));
+  GridArgs g;
+  if ((rc = grid_build(geom, obstacle_dev, cell, g, st))) return rc;
+  long long total = 0;
+  DH_HIP_CHECK(hipMemcpyAsync(&total, g_bp.start.as<long long>() + n, sizeof(long long),
                               hipMemcpyDeviceToHost, st));
   DH_HIP_CHECK(hipStreamSynchronize(st));
-  *npairs = last_start + last_count;
+  *npairs = total;
   if (*npairs > cap) return DH_ERR_CAPACITY;
   hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
                      g_bp.start.as<long long>(), cap, pairs);
   DH_HIP_CHECK(hipGetLastError());
-  return DH_OK;
-}
-
-// Buffers of broadphase_graph (before a capture).
-int broadphase_reserve(int n) {
-  int rc;
-  unsigned tbl = 1;
-  while (tbl < 2u * static_cast<unsigned>(n)) tbl <<= 1;
-  if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
-  if ((rc = g_bp.head.grow(sizeof(int) * tbl))) return rc;
-  if ((rc = g_bp.next.grow(sizeof(int) * (n + 1)))) return rc;
-  if ((rc = g_bp.count.grow(sizeof(int) * (n + 1)))) return rc;
-  if ((rc = g_bp.start.grow(sizeof(long long) * (n + 1)))) return rc;
-  if ((rc = g_bp.err.grow(sizeof(int)))) return rc;
-  if ((rc = g_bp.obst.grow(sizeof(int) * (n + 1)))) return rc;
   return DH_OK;
 }
 
@@ -335,27 +374,12 @@ int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double ce
   const int n = geom->nparticles;
   int rc;
   if ((rc = broadphase_reserve(n))) return rc;
-  unsigned tbl = 1;
-  while (tbl < 2u * static_cast<unsigned>(n)) tbl <<= 1;
+  const unsigned tbl = table_size(n);
   fill_async(g_bp.err.as<int>(), 0, 1, st);
-  fill_async(g_bp.head.as<int>(), -1, static_cast<long long>(tbl), st);
-  hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
-                     geom->eps, g_bp.box.as<Aabb>());
+  fill_async(g_bp.ccount.as<int>(), 0, static_cast<long long>(tbl), st);
+  fill_async(g_bp.cfill.as<int>(), 0, static_cast<long long>(tbl), st);
   GridArgs g;
-  g.box = g_bp.box.as<Aabb>();
-  g.obstacle = obstacle_dev;
-  g.n = n;
-  g.h = cell;
-  g.inv_h = 1.0 / cell;
-  g.head = g_bp.head.as<int>();
-  g.next = g_bp.next.as<int>();
-  g.mask = tbl - 1;
-  g.large = g_bp.obst.as<int>();
-  g.nlarge = g_bp.err.as<int>();
-  hipLaunchKernelGGL(k_grid_insert, grid_for(n, 256), 256, 0, st, g);
-  hipLaunchKernelGGL(k_grid_count, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
-                     g_bp.count.as<int>());
-  if ((rc = scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st))) return rc;
+  if ((rc = grid_build(geom, obstacle_dev, cell, g, st))) return rc;
   hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
                      g_bp.start.as<long long>(), cap, pairs);
   DH_HIP_CHECK(hipGetLastError());

@@ -710,6 +710,12 @@ struct DetectScratch {
 };
 static DetectScratch g_scratch;
 
+// Scratch of scan_counts for up to n entries (before a capture).
+int scan_reserve(int n) {
+  const int nb = (n + 1023) / 1024 > 0 ? (n + 1023) / 1024 : 1;
+  return g_scratch.sums.grow(sizeof(long long) * (nb + 1));
+}
+
 // offs[0..n]: exclusive prefix sums of counts, offs[n] = total
 int scan_counts(const int* counts, int n, long long* offs, hipStream_t st) {
   const int nb = (n + 1023) / 1024 > 0 ? (n + 1023) / 1024 : 1;
