@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch parameters")
     ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,"
                                          "penalty_cert,learned_penalty,learned")
     ap.add_argument("--workload", default="pairs", choices=["pairs", "c1-dem", "hopper"],
@@ -90,6 +92,25 @@ def train_network(a):
             "top1": float(top1), "top2": float(top2), "seconds": time.perf_counter() - t0}
     print(f"[bench] trained network: {info}", file=sys.stderr, flush=True)
     return tr, ds, info
+
+
+def autotune(batch, methods):
+    """Launch parameters for this device and workload, chosen before the
+    timed region by the library's autotuner on the batch itself (one knob per
+    path: the FP64 block size from bf64, the persistent grids from mixed and
+    learned); every candidate gives identical results."""
+    from delta_amd import tuning
+    trials = {}
+    for m in ("bf64", "mixed", "learned"):
+        if m not in methods:
+            continue
+        try:
+            trials[m] = [(t["value"], round(t["ms"], 4)) for t in tuning.autotune_pairs(batch, m)]
+        except Exception as ex:
+            print(f"[bench] autotune {m}: {ex}", file=sys.stderr, flush=True)
+    table = tuning.get()
+    print(f"[bench] tuning {table}", file=sys.stderr, flush=True)
+    return {"table": table, "trials_value_ms": trials}
 
 
 def contact_errors(P0, Q0, d0, P, Q, d, margin=2e-4):
@@ -474,6 +495,7 @@ def main():
         trainer, _, train_info = train_network(a)
     if "mixed" in a.methods.split(","):
         batch.make_f32()  # the binary32 bulk copy is an input format, made before timing
+    tune_info = None if a.no_autotune else autotune(batch, a.methods.split(","))
     for method in a.methods.split(","):
         try:
             ms, wall = time_method(batch, method, a.steps, a.warmup)
@@ -521,6 +543,8 @@ def main():
                     for k, v in results.items()},
     }
     out.update(extra)
+    if tune_info:
+        out["tuning"] = tune_info
     if train_info:
         out["network"] = train_info
         out["learned_vs_bf64"] = learned_accuracy(batch)

@@ -12,7 +12,7 @@ HOSTCHECK = os.path.join(HERE, "libdh_hostcheck.so")
 
 HIP_SOURCES = ["runtime.hip", "tri_pairs.hip", "mixed.hip", "detect.hip", "dynamics.hip",
                "broadphase.hip", "engine.hip", "learned.hip", "zoltan.hip",
-               "ispc.hip", "adaptive.hip"]
+               "ispc.hip", "adaptive.hip", "tuning.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

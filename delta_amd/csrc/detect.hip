@@ -744,11 +744,13 @@ static void launch_tile_stats(const unsigned long long* ts, long long ntiles,
 }
 
 // Row ranges per particle pair: enough workgroups to fill the 256 CUs
-// several times over when the pairs are few, one per pair otherwise.
+// several times over when the pairs are few (detect_tile_target workgroups
+// in all, at most detect_max_tiles per pair), one per pair otherwise.
 static int tiles_for(long long npairs, int method) {
   if (method == DH_GJK || npairs <= 0) return 1;
-  long long t = (4096 + npairs - 1) / npairs;
-  return static_cast<int>(t < 1 ? 1 : (t > 16 ? 16 : t));
+  const long long target = tuning().detect_tile_target, cap = tuning().detect_max_tiles;
+  long long t = (target + npairs - 1) / npairs;
+  return static_cast<int>(t < 1 ? 1 : (t > cap ? cap : t));
 }
 
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,

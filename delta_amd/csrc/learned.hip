@@ -692,7 +692,9 @@ int learned_pairs(int mode, const double* soa, int64_t n, double* P, double* Q, 
     return set_error(DH_ERR_ARG, "learned narrow phase: no network set (dh_learned_set_model)");
   const int64_t chunks = (n + 63) / 64;
   int64_t grid = (chunks + kInferWaves - 1) / kInferWaves;
-  if (grid > 256 * 8) grid = 256 * 8;  // persistent: 8 workgroups per CU at most
+  // persistent: at most learned_wg_per_cu workgroups per CU
+  const int64_t max_grid = static_cast<int64_t>(device_cus()) * tuning().learned_wg_per_cu;
+  if (grid > max_grid) grid = max_grid;
   if (grid < 1) grid = 1;
   // every wave of a workgroup takes at most `per_wave` chunks
   const int64_t per_wave = (chunks + grid * kInferWaves - 1) / (grid * kInferWaves);

@@ -138,7 +138,8 @@ static int tri_pairs_mixed(const double* soa, const float* soa32, int64_t n, dou
                                  "dh_convert_f32)");
   const int64_t chunks = (n + 63) / 64;
   int64_t grid = (chunks + kMixedWaves - 1) / kMixedWaves;
-  if (grid > 256 * 8) grid = 256 * 8;
+  const int64_t max_grid = static_cast<int64_t>(device_cus()) * tuning().mixed_wg_per_cu;
+  if (grid > max_grid) grid = max_grid;
   if (grid < 1) grid = 1;
   const int64_t per_wave = (chunks + grid * kMixedWaves - 1) / (grid * kMixedWaves);
   const int64_t region = per_wave * kMixedWaves * 64;

@@ -39,6 +39,13 @@ struct DevBuf {
 // Step graphs alive (+1 on creation, -1 on destruction); see DevBuf::grow.
 void graphs_alive_add(int delta);
 
+// Launch parameters of the hot kernels (process-wide table of tuning.hip,
+// set by dh_tuning_set and the autotuner).
+const dh_tuning& tuning();
+
+// Compute units of the current device (256 on MI355X).
+int device_cus();
+
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 inline unsigned grid_for(int64_t n, int block) {

@@ -243,26 +243,27 @@ int dh_tri_pairs(int method, const double* soa, const float* soa32, int64_t n, d
   if (n == 0) return DH_OK;
   const hipStream_t st = as_stream(stream);
   const double max_err = eps_total / 16.0;
-  const unsigned grid = grid_for(n, 256);
+  const int block = tuning().exact_block;
+  const unsigned grid = grid_for(n, block);
   switch (method) {
     case DH_PAIR_BF64:
-      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_BF64>, grid, 256, 0, st, soa, n, max_err, P,
+      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_BF64>, grid, block, 0, st, soa, n, max_err, P,
                          Q, d, flags);
       break;
     case DH_PAIR_PENALTY64:
-      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_PENALTY64>, grid, 256, 0, st, soa, n,
+      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_PENALTY64>, grid, block, 0, st, soa, n,
                          max_err, P, Q, d, flags);
       break;
     case DH_PAIR_HYBRID64:
-      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_HYBRID64>, grid, 256, 0, st, soa, n, max_err,
+      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_HYBRID64>, grid, block, 0, st, soa, n, max_err,
                          P, Q, d, flags);
       break;
     case DH_PAIR_PENALTY_ADAPT64:
-      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_PENALTY_ADAPT64>, grid, 256, 0, st, soa, n,
+      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_PENALTY_ADAPT64>, grid, block, 0, st, soa, n,
                          max_err, P, Q, d, flags);
       break;
     case DH_PAIR_BF64_EXACT:
-      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_BF64_EXACT>, grid, 256, 0, st, soa, n, max_err,
+      hipLaunchKernelGGL(k_tri_pairs_exact<DH_PAIR_BF64_EXACT>, grid, block, 0, st, soa, n, max_err,
                          P, Q, d, flags);
       break;
     case DH_PAIR_MIXED:

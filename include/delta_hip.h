@@ -321,6 +321,57 @@ typedef struct dh_pair_counters {
 } dh_pair_counters;
 int dh_tri_pairs_counters(dh_pair_counters* out);
 
+/* ------------------------------------------------------------- autotuning */
+/* No counterpart in the reference (its kernels have fixed ISPC task and
+ * OpenMP schedules).  Launch parameters of the hot kernels, process-wide:
+ * they change how the work is spread over the GPU, never the results (every
+ * setting gives bit-identical outputs).  Read at each launch; a captured
+ * step graph keeps the values of its creation. */
+typedef struct dh_tuning {
+  int32_t learned_wg_per_cu;  /* learned passes: persistent workgroups per CU
+                                 (1..32); also sizes their failure queues   */
+  int32_t mixed_wg_per_cu;    /* DH_PAIR_MIXED: the same (1..32)             */
+  int32_t exact_block;        /* FP64 one-pair-per-lane kernels of
+                                 dh_tri_pairs: 64, 128, 192 or 256           */
+  int32_t detect_tile_target; /* particle-pair detection: workgroups per pair
+                                 = clamp(ceil(target / npairs), 1, max)      */
+  int32_t detect_max_tiles;   /* that max (1..64)                            */
+  int32_t reserved[3];
+} dh_tuning;
+int dh_tuning_default(dh_tuning* out);
+int dh_tuning_get(dh_tuning* out);
+int dh_tuning_set(const dh_tuning* t);  /* DH_ERR_ARG if a field is out of range */
+
+/* One timed candidate: knob, value, mean milliseconds per call. */
+enum {
+  DH_KNOB_LEARNED_WG_PER_CU = 0,
+  DH_KNOB_MIXED_WG_PER_CU = 1,
+  DH_KNOB_EXACT_BLOCK = 2,
+  DH_KNOB_DETECT_TILE_TARGET = 3,
+  DH_KNOB_DETECT_MAX_TILES = 4
+};
+typedef struct dh_tune_trial {
+  int32_t knob;
+  int32_t value;
+  float ms;
+} dh_tune_trial;
+
+/* Times dh_tri_pairs(method, ...) on the caller's pairs for every candidate
+ * of the knob that governs `method` (exact methods: exact_block; MIXED:
+ * mixed_wg_per_cu; learned methods: learned_wg_per_cu), one untimed run and
+ * `reps` timed runs each (HIP events on `stream`), and keeps the fastest in
+ * the table.  trials[0..trial_cap) receive the candidates, *ntrials their
+ * number.  On return the outputs hold the chosen configuration's results. */
+int dh_autotune_tri_pairs(int method, const double* soa_dev, const float* soa32_dev, int64_t n,
+                          double eps_total, double* P_dev, double* Q_dev, double* d_dev,
+                          uint8_t* flags_dev, int reps, dh_tune_trial* trials, int trial_cap,
+                          int* ntrials, void* stream);
+/* The same for dh_detect_pairs: sweeps detect_tile_target (with the cap at
+ * 64), then detect_max_tiles.  `cap` must hold every contact. */
+int dh_autotune_detect(int method, const dh_geometry* geom, const int* pairs_dev, int npairs,
+                       dh_contact* out_dev, int cap, int reps, dh_tune_trial* trials,
+                       int trial_cap, int* ntrials, void* stream);
+
 /* ------------------------------------------------------- forces / dynamics */
 /* Rigid-body state, one record per particle (device, AoS of small arrays). */
 typedef struct dh_bodies {

@@ -32,3 +32,26 @@ def test_library_exports_every_declared_symbol():
 def test_package_import_does_not_touch_gpu():
     import delta_amd
     assert hasattr(delta_amd, "lib")
+
+
+def test_tuning_table_validates_without_gpu():
+    """dh_tuning_* touch no device: defaults, a round trip and rejection of
+    out-of-range values."""
+    from delta_amd import _lib, tuning
+    d = tuning.default()
+    assert d == {"learned_wg_per_cu": 8, "mixed_wg_per_cu": 8, "exact_block": 256,
+                 "detect_tile_target": 4096, "detect_max_tiles": 16}
+    try:
+        assert tuning.set(exact_block=128, learned_wg_per_cu=3)["exact_block"] == 128
+        assert tuning.get()["learned_wg_per_cu"] == 3
+        for bad in ({"exact_block": 100}, {"exact_block": 320}, {"mixed_wg_per_cu": 0},
+                    {"learned_wg_per_cu": 33}, {"detect_max_tiles": 65},
+                    {"detect_tile_target": 0}):
+            with pytest.raises(_lib.DhError):
+                tuning.set(**bad)
+        assert tuning.get()["exact_block"] == 128  # a rejected set changes nothing
+        with pytest.raises(KeyError):
+            tuning.set(block=64)
+    finally:
+        tuning.set(**d)
+    assert tuning.get() == d
