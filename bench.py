@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch parameters")
+    ap.add_argument("--tuning", default="",
+                    help="fixed launch parameters instead of autotuning, e.g. "
+                         "learned_wg_per_cu=16,exact_block=256 (profiling runs)")
     ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,"
                                          "penalty_cert,learned_penalty,learned")
     ap.add_argument("--workload", default="pairs", choices=["pairs", "c1-dem", "hopper"],
@@ -495,7 +498,13 @@ def main():
         trainer, _, train_info = train_network(a)
     if "mixed" in a.methods.split(","):
         batch.make_f32()  # the binary32 bulk copy is an input format, made before timing
-    tune_info = None if a.no_autotune else autotune(batch, a.methods.split(","))
+    if a.tuning:
+        from delta_amd import tuning
+        fixed = tuning.set(**{k: int(v) for k, v in
+                              (kv.split("=") for kv in a.tuning.split(","))})
+        tune_info = {"table": fixed, "fixed": True}
+    else:
+        tune_info = None if a.no_autotune else autotune(batch, a.methods.split(","))
     for method in a.methods.split(","):
         try:
             ms, wall = time_method(batch, method, a.steps, a.warmup)

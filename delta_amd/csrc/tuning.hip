@@ -11,6 +11,7 @@
 // fastest in the table, where later launches read it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -55,57 +56,66 @@ static int check_tuning(const dh_tuning& t) {
   return DH_OK;
 }
 
-// Mean milliseconds of `reps` runs of `run` after one untimed run; a
-// failing run returns its code.
+// Milliseconds of one call of `run` between two events on `st`.
 template <typename F>
-static int time_runs(F run, int reps, hipStream_t st, float* ms) {
-  int rc = run();
-  if (rc) return rc;
-  hipEvent_t e0, e1;
-  DH_HIP_CHECK(hipEventCreate(&e0));
-  DH_HIP_CHECK(hipEventCreate(&e1));
+static int time_one(F& run, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms) {
   DH_HIP_CHECK(hipEventRecord(e0, st));
-  for (int r = 0; r < reps && rc == DH_OK; ++r) rc = run();
-  hipError_t e = hipEventRecord(e1, st);
-  if (e == hipSuccess) e = hipEventSynchronize(e1);
-  float total = 0.f;
-  if (e == hipSuccess) e = hipEventElapsedTime(&total, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  const int rc = run();
   if (rc) return rc;
-  if (e != hipSuccess)
-    return set_error(DH_ERR_HIP, "autotune timing: %s", hipGetErrorString(e));
-  *ms = total / static_cast<float>(reps);
+  DH_HIP_CHECK(hipEventRecord(e1, st));
+  DH_HIP_CHECK(hipEventSynchronize(e1));
+  DH_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
   return DH_OK;
 }
 
-// Sweeps `values` of the knob `field` of the table, timing `run` for each,
-// records the trials and leaves the fastest value in the table (the table
-// is restored on failure).
+// Sweeps `values` of the knob `field` of the table and leaves the fastest
+// in it (the table is restored on failure).  One untimed round over all
+// candidates, then `reps` timed rounds, each starting one candidate further
+// on, so that a drift of the clocks over the sweep (the GPU ramps up under
+// sustained load) is spread over every candidate instead of favouring the
+// last ones; a candidate's time is the median of its rounds.
 template <typename F>
 static int sweep(int knob, int dh_tuning::*field, const std::vector<int>& values, F run, int reps,
                  hipStream_t st, dh_tune_trial* trials, int trial_cap, int* ntrials) {
   const dh_tuning saved = g_tuning;
+  const int nv = static_cast<int>(values.size());
+  std::vector<std::vector<float>> t(nv);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = DH_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = set_error(DH_ERR_HIP, "autotune: hipEventCreate failed");
+  for (int k = 0; k < nv && rc == DH_OK; ++k) {
+    g_tuning.*field = values[k];
+    rc = run();
+  }
+  for (int r = 0; r < reps && rc == DH_OK; ++r)
+    for (int c = 0; c < nv && rc == DH_OK; ++c) {
+      const int k = (c + r) % nv;
+      g_tuning.*field = values[k];
+      float ms = 0.f;
+      rc = time_one(run, st, e0, e1, &ms);
+      t[k].push_back(ms);
+    }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) {
+    g_tuning = saved;
+    return rc;
+  }
   int best = -1;
   float best_ms = 0.f;
-  int count = 0;
-  for (int v : values) {
-    g_tuning.*field = v;
-    float ms = 0.f;
-    const int rc = time_runs(run, reps, st, &ms);
-    if (rc) {
-      g_tuning = saved;
-      return rc;
-    }
-    if (trials && count < trial_cap) trials[count] = dh_tune_trial{knob, v, ms};
-    ++count;
+  for (int k = 0; k < nv; ++k) {
+    std::sort(t[k].begin(), t[k].end());
+    const float ms = t[k][t[k].size() / 2];
+    if (trials && k < trial_cap) trials[k] = dh_tune_trial{knob, values[k], ms};
     if (best < 0 || ms < best_ms) {
-      best = v;
+      best = values[k];
       best_ms = ms;
     }
   }
+  g_tuning = saved;
   g_tuning.*field = best;
-  if (ntrials) *ntrials = count;
+  if (ntrials) *ntrials = nv;
   return DH_OK;
 }
 

@@ -342,7 +342,7 @@ int dh_tuning_default(dh_tuning* out);
 int dh_tuning_get(dh_tuning* out);
 int dh_tuning_set(const dh_tuning* t);  /* DH_ERR_ARG if a field is out of range */
 
-/* One timed candidate: knob, value, mean milliseconds per call. */
+/* One timed candidate: knob, value, median milliseconds per call. */
 enum {
   DH_KNOB_LEARNED_WG_PER_CU = 0,
   DH_KNOB_MIXED_WG_PER_CU = 1,
@@ -358,9 +358,10 @@ typedef struct dh_tune_trial {
 
 /* Times dh_tri_pairs(method, ...) on the caller's pairs for every candidate
  * of the knob that governs `method` (exact methods: exact_block; MIXED:
- * mixed_wg_per_cu; learned methods: learned_wg_per_cu), one untimed run and
- * `reps` timed runs each (HIP events on `stream`), and keeps the fastest in
- * the table.  trials[0..trial_cap) receive the candidates, *ntrials their
+ * mixed_wg_per_cu; learned methods: learned_wg_per_cu): one untimed round
+ * over the candidates, then `reps` timed rounds in rotated order (HIP events
+ * on `stream`, one call each), and keeps the candidate of least median time
+ * in the table.  trials[0..trial_cap) receive the candidates, *ntrials their
  * number.  On return the outputs hold the chosen configuration's results. */
 int dh_autotune_tri_pairs(int method, const double* soa_dev, const float* soa32_dev, int64_t n,
                           double eps_total, double* P_dev, double* Q_dev, double* d_dev,
