@@ -44,11 +44,14 @@ def parse():
                          "learned_wg_per_cu=16,exact_block=256 (profiling runs)")
     ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,"
                                          "penalty_cert,learned_penalty,learned")
-    ap.add_argument("--workload", default="pairs", choices=["pairs", "c1-dem", "hopper"],
+    ap.add_argument("--workload", default="pairs",
+                    choices=["pairs", "c1-dem", "hopper", "funnel", "drum"],
                     help="pairs: 2^24 unique triangle pairs (the headline); c1-dem: the "
                          "64-icosphere DEM step (particle-steps/s); hopper: the reference's "
                          "hopper scene (input/hopper.py, 15,625 spheres as 80-triangle "
-                         "icospheres in 26 wall prisms)")
+                         "icospheres in 26 wall prisms); funnel: 10^4 uneven particles "
+                         "through a 2,048-triangle cone; drum: 2*10^5 particles in a "
+                         "rotating box (both as captured steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel three times "
                          "(reference for scaling HBM byte counters) and exit")
@@ -473,6 +476,140 @@ def dem_hopper(a):
     print(json.dumps(out), flush=True)
 
 
+LARGE = {
+    "funnel": dict(metric="particle-steps/s (uneven funnel fill, FP64 brute force, captured step)",
+                   workload="10,000 particles of 20/80/320-triangle jittered icospheres and "
+                            "60-triangle hulls (radius 0.3-0.8 mm) in a 2,048-triangle cone "
+                            "(half-angle 30 deg, 4 mm outlet), dt 1e-6 s, FP64 brute force",
+                   n=10000, dt=1e-6),
+    "drum": dict(metric="particle-steps/s (rotating box, FP64 brute force, captured step)",
+                 workload="200,000 particles: 80-triangle jittered icospheres and 128-triangle "
+                          "3:1 ellipsoids (0.5-1.5 mm) in a 12-triangle box turning at 2 rad/s, "
+                          "dt 5e-7 s, FP64 brute force",
+                 n=200000, dt=5e-7),
+}
+
+
+def dem_large(a, kind):
+    """The uneven funnel fill (10^4 particles) or the rotating box (2*10^5
+    particles): eager warm-up steps size the capacities, the detection
+    tiling is autotuned on the live candidate pairs, then the whole step is
+    captured once and replayed `--steps` times (host clock around the
+    replays, synchronised at both ends).  Spring 1e5 N/m, damping 0.1,
+    friction off, margin 1e-4 m, gravity -9.81 y.  The particles start on a
+    lattice whose pitch exceeds the largest particle: a lattice below the
+    particle sizes puts tens of triangle-pair contacts on each overlap, whose
+    summed springs fling the particles apart within a few steps at these
+    time steps (and the timed steps then measure an emptying scene)."""
+    from delta_amd import _lib, engine, tuning, workloads
+    from delta_amd.device import DeviceArray
+    cfg = LARGE[kind]
+    L = _lib.lib()
+    _lib.init()
+    free0, total = C.c_size_t(), C.c_size_t()
+    _lib.check(L.dh_mem_info(C.byref(free0), C.byref(total)), "dh_mem_info")
+    print(f"[bench] building the {kind} scene", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    hs = getattr(workloads, kind + "_scene")(n=cfg["n"])
+    ds = engine.DeviceScene(hs)
+    build_s = time.perf_counter() - t0
+    prm = engine.step_params(method=_lib.DH_BF, dt=cfg["dt"], spring=1e5, damper=0.1,
+                             friction=0.0, cell=hs.cell_size())
+    max_pairs = max_contacts = 0
+    for _ in range(max(a.warmup, 1)):
+        info = engine.step(ds, prm)
+        max_pairs = max(max_pairs, info.candidate_pairs)
+        max_contacts = max(max_contacts, info.contacts)
+    _lib.check(L.dh_synchronize(), "sync")
+    print(f"[bench] {kind} warm: {max_pairs} pairs, {max_contacts} contacts", file=sys.stderr,
+          flush=True)
+    # eager steps, timed (each reads its counts back to size the next launches)
+    checks = contacts = pairs = culled = 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        info = engine.step(ds, prm)
+        checks += info.triangle_checks
+        culled += info.pairs_culled
+        contacts += info.contacts
+        pairs += info.candidate_pairs
+        max_pairs = max(max_pairs, info.candidate_pairs)
+        max_contacts = max(max_contacts, info.contacts)
+    _lib.check(L.dh_synchronize(), "sync")
+    eager_s = time.perf_counter() - t0
+    # detection tiling for these pairs
+    cap = max(4 * max_pairs, 1024)
+    d_pairs = DeviceArray((2 * cap,), np.int32)
+    npairs = C.c_int(0)
+    L.dh_broadphase.argtypes = [C.POINTER(_lib.DhGeometry), C.c_void_p, C.c_double, C.c_void_p,
+                                C.c_int, C.POINTER(C.c_int), C.c_void_p]
+    _lib.check(L.dh_broadphase(C.byref(ds.scene.geom), ds.buf["obstacle"].ptr, prm.cell,
+                               d_pairs.ptr, cap, C.byref(npairs), None), "dh_broadphase")
+    ccap = 4 * max_contacts + 4096
+    out_c = DeviceArray((ccap * 144,), np.uint8)
+    trials = tuning.autotune_detect(ds.scene.geom, d_pairs.ptr, npairs.value, out_c.ptr, ccap,
+                                    reps=2)
+    del out_c, d_pairs
+    table = tuning.get()
+    print(f"[bench] tuning {table}", file=sys.stderr, flush=True)
+    g = engine.StepGraph(ds, prm, pair_cap=2 * max_pairs + 1024,
+                         contact_cap=2 * max_contacts + 4096)
+    try:
+        g.launch()
+        _, overflow = g.status()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            g.launch()
+        ginfo, ov = g.status()
+        gdt = time.perf_counter() - t0
+        overflow |= ov
+    finally:
+        g.close()
+    free1 = C.c_size_t()
+    _lib.check(L.dh_mem_info(C.byref(free1), C.byref(total)), "dh_mem_info")
+    hbm = hopper_hbm(hs, ds, prm, L)
+    n = hs.n
+    out = {"metric": cfg["metric"], "value": n * a.steps / gdt, "unit": "particle-steps/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": gdt * 1e3 / a.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": cfg["workload"], "particles": n,
+                      "triangles": int(hs.tri_offset[-1])},
+           "overflow": overflow,
+           "eager": {"ms_per_step": eager_s * 1e3 / a.steps,
+                     "triangle_checks_per_s": checks / eager_s,
+                     "culled_fraction": culled / max(checks, 1),
+                     "mean_candidate_pairs": pairs / a.steps,
+                     "mean_contacts": contacts / a.steps},
+           "tuning": {"table": table,
+                      "trials_value_ms": [(t["knob"], t["value"], round(t["ms"], 4))
+                                          for t in trials]},
+           "hbm_footprint_bytes": int(free0.value - free1.value),
+           "scene_build_s": build_s,
+           "hbm": hbm}
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        rng = np.random.default_rng(0)
+        off = hs.tri_offset
+        mesh = lambda i: np.stack([hs.x[3 * off[i]:3 * off[i + 1]], hs.y[3 * off[i]:3 * off[i + 1]],
+                                   hs.z[3 * off[i]:3 * off[i + 1]]], 1).reshape(-1, 3, 3)
+        t0 = time.perf_counter()
+        k = nchk = 0
+        while time.perf_counter() - t0 < a.cpu_seconds and k < 4096:
+            i = int(rng.integers(1, n - 1))
+            A, B = mesh(i), mesh(i + 1)
+            ol.or_detect(0, A, B, 1e-4, 1e-4)
+            nchk += len(A) * len(B)
+            k += 1
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nchk / cdt, "unit": "triangle checks/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle bf over {k} neighbouring particle pairs "
+                                         f"({nchk} triangle pairs), one thread, {cdt:.1f} s; "
+                                         "compare with eager.triangle_checks_per_s"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -487,6 +624,9 @@ def main():
         return
     if a.workload == "hopper":
         dem_hopper(a)
+        return
+    if a.workload in LARGE:
+        dem_large(a, a.workload)
         return
     from delta_amd import narrow
     n = 1 << a.pairs_log2

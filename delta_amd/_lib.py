@@ -81,6 +81,7 @@ _SIGS = {
     "dh_memcpy_d2h": (C.c_int, [VP, VP, C.c_size_t]),
     "dh_memset": (C.c_int, [VP, C.c_int, C.c_size_t]),
     "dh_synchronize": (C.c_int, []),
+    "dh_mem_info": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "dh_event_create": (C.c_int, [C.POINTER(VP)]),
     "dh_event_destroy": (C.c_int, [VP]),
     "dh_event_record": (C.c_int, [VP, VP]),

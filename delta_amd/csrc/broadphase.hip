@@ -90,6 +90,12 @@ struct GridArgs {
   unsigned* cellh;           // bucket of each particle (large: unused)
   int* large;                // indices of large particles (any order)
   int* nlarge;
+  // partners of a large particle are searched in chunks of `chunk`
+  // candidates j (nchunks per particle, chunk c covers [c*chunk, (c+1)*chunk)),
+  // one wave per (large particle, chunk); lcount[slot * nchunks + c] holds
+  // each chunk's partner count
+  int chunk, nchunks;
+  int* lcount;
 };
 
 __device__ __forceinline__ bool is_large(const GridArgs& g, int i, const Aabb& b) {
@@ -124,8 +130,8 @@ __global__ __launch_bounds__(256) void k_grid_scatter(GridArgs g) {
 
 // One wave searches the partners j > i of one particle i: the 27 neighbour
 // buckets are walked by 27 lanes at once, one member per lane per round, and
-// the large-particle list and the all-particle scan of a large i are strided
-// over the wave.  Finds are compacted with ballots, so every lane sees the
+// the large-particle list is strided over the wave.  A large particle's scan
+// over all j is split into chunks, one wave each.  Finds are compacted with ballots, so every lane sees the
 // running count.
 constexpr int kBpWaves = 4;    // particles per workgroup
 constexpr int kBpList = 512;   // partners sorted in LDS per particle (else a serial sort)
@@ -186,19 +192,21 @@ __device__ int grid_partners(const GridArgs& g, int i, const Aabb& bi, int lane,
   return found;
 }
 
-// Partners of a large particle i: every j > i (not both obstacles), in
-// ascending order; put(position, j) as above.
+// Partners j of a large particle i in chunk c: every j > i of the chunk
+// (not both obstacles), in ascending order; put(position, j) as above.
 template <typename F>
-__device__ int large_partners(const GridArgs& g, int i, const Aabb& bi, int lane, F put) {
+__device__ int large_chunk(const GridArgs& g, int i, const Aabb& bi, int c, int lane, F put) {
   const bool obst_i = g.obstacle && g.obstacle[i];
+  const int lo = c * g.chunk > i + 1 ? c * g.chunk : i + 1;
+  const int hi = (c + 1) * g.chunk < g.n ? (c + 1) * g.chunk : g.n;
   int found = 0;
   // four strips of 64 per round: their loads are issued together
-  for (int j0 = i + 1; j0 < g.n; j0 += 256) {
+  for (int j0 = lo; j0 < hi; j0 += 256) {
     bool e[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int j = j0 + 64 * q + lane;
-      e[q] = j < g.n && !(obst_i && g.obstacle && g.obstacle[j]) && overlap(bi, g.box[j]);
+      e[q] = j < hi && !(obst_i && g.obstacle && g.obstacle[j]) && overlap(bi, g.box[j]);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -210,6 +218,62 @@ __device__ int large_partners(const GridArgs& g, int i, const Aabb& bi, int lane
   return found;
 }
 
+// Large particles, one wave per (slot, chunk) item, strided over a
+// persistent grid (the number of large particles is known on the device
+// only): chunk partner counts.
+__global__ __launch_bounds__(64 * kBpWaves) void k_large_count(GridArgs g) {
+  const int lane = threadIdx.x & 63;
+  const long long nw = static_cast<long long>(gridDim.x) * kBpWaves;
+  const long long items = static_cast<long long>(*g.nlarge) * g.nchunks;
+  auto none = [](int, int) {};
+  for (long long t = static_cast<long long>(blockIdx.x) * kBpWaves + (threadIdx.x >> 6);
+       t < items; t += nw) {
+    const int o = static_cast<int>(t / g.nchunks), c = static_cast<int>(t - o * static_cast<long long>(g.nchunks));
+    const int i = g.large[o];
+    const int m = large_chunk(g, i, g.box[i], c, lane, none);
+    if (lane == 0) g.lcount[t] = m;
+  }
+}
+
+// Partner count of each large particle: the sum of its chunk counts.
+__global__ __launch_bounds__(64 * kBpWaves) void k_large_sum(GridArgs g, int* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * kBpWaves;
+  const int nl = *g.nlarge;
+  for (int o = blockIdx.x * kBpWaves + (threadIdx.x >> 6); o < nl; o += nw) {
+    int m = 0;
+    for (int c = lane; c < g.nchunks; c += 64) m += g.lcount[static_cast<long long>(o) * g.nchunks + c];
+    for (int k = 32; k > 0; k >>= 1) m += __shfl_xor(m, k);
+    if (lane == 0) count[g.large[o]] = m;
+  }
+}
+
+// Writes each chunk's partners after those of the particle's earlier chunks.
+__global__ __launch_bounds__(64 * kBpWaves) void k_large_fill(GridArgs g,
+                                                               const long long* __restrict__ start,
+                                                               long long cap,
+                                                               int* __restrict__ pairs) {
+  const int lane = threadIdx.x & 63;
+  const long long nw = static_cast<long long>(gridDim.x) * kBpWaves;
+  const long long items = static_cast<long long>(*g.nlarge) * g.nchunks;
+  for (long long t = static_cast<long long>(blockIdx.x) * kBpWaves + (threadIdx.x >> 6);
+       t < items; t += nw) {
+    const int o = static_cast<int>(t / g.nchunks), c = static_cast<int>(t - o * static_cast<long long>(g.nchunks));
+    const int i = g.large[o];
+    long long before = 0;
+    for (int q = lane; q < c; q += 64) before += g.lcount[t - c + q];
+    for (int k = 32; k > 0; k >>= 1) before += __shfl_xor(before, k);
+    const long long s = start[i] + before;
+    large_chunk(g, i, g.box[i], c, lane, [&](int pos, int j) {
+      const long long q = s + pos;
+      if (q < cap) {
+        pairs[2 * q] = i;
+        pairs[2 * q + 1] = j;
+      }
+    });
+  }
+}
+
 __global__ __launch_bounds__(64 * kBpWaves) void k_grid_count(GridArgs g,
                                                                int* __restrict__ count) {
   const int lane = threadIdx.x & 63;
@@ -217,8 +281,8 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_grid_count(GridArgs g,
   if (i >= g.n) return;  // wave-uniform
   const Aabb bi = g.box[i];
   auto none = [](int, int) {};
-  const int m = is_large(g, i, bi) ? large_partners(g, i, bi, lane, none)
-                                   : grid_partners(g, i, bi, lane, none);
+  // large particles: k_large_count / k_large_sum, launched after this
+  const int m = is_large(g, i, bi) ? 0 : grid_partners(g, i, bi, lane, none);
   if (lane == 0) count[i] = m;
 }
 
@@ -242,10 +306,7 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_grid_fill(GridArgs g,
       pairs[2 * q + 1] = j;
     }
   };
-  if (is_large(g, i, bi)) {
-    large_partners(g, i, bi, lane, direct);  // already ascending
-    return;
-  }
+  if (is_large(g, i, bi)) return;  // k_large_fill
   if (m > kBpList) {
     // more partners than the LDS list holds: unsorted, then a serial sort
     grid_partners(g, i, bi, lane, direct);
@@ -284,15 +345,39 @@ int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
 int scan_reserve(int n);
 
 struct BpScratch {
-  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst;
+  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst, lcount;
 };
 static BpScratch g_bp;
+
+// Candidates per large-particle chunk: 1024 (four rounds of 256 loads per
+// wave), more for very large n so that a particle has at most 256 chunks.
+static int large_chunk_len(int n) {
+  int c = 1024;
+  while (static_cast<long long>(c) * 256 < n) c *= 2;
+  return c;
+}
+
+static int large_chunks(int n) {
+  const int c = large_chunk_len(n);
+  return (n + c - 1) / c > 0 ? (n + c - 1) / c : 1;
+}
+
+// Persistent grid of the large-particle kernels.
+static unsigned large_grid(long long items) {
+  const long long g = (items + kBpWaves - 1) / kBpWaves;
+  return static_cast<unsigned>(g < 1 ? 1 : (g > 2048 ? 2048 : g));
+}
 
 static unsigned table_size(int n) {
   unsigned tbl = 1;
   while (tbl < 2u * static_cast<unsigned>(n)) tbl <<= 1;
   return tbl;
 }
+
+// The particle boxes of the last broad phase (lo xyz, hi xyz per particle:
+// the vertex box inflated by the margin and 0.5e-9 m), valid until the next
+// call on more particles.
+const double* broadphase_boxes() { return g_bp.box.as<double>(); }
 
 // Buffers of the broad phase for n particles (call before a capture).
 int broadphase_reserve(int n) {
@@ -308,6 +393,9 @@ int broadphase_reserve(int n) {
   if ((rc = g_bp.start.grow(sizeof(long long) * (n + 1)))) return rc;
   if ((rc = g_bp.err.grow(sizeof(int)))) return rc;
   if ((rc = g_bp.obst.grow(sizeof(int) * (n + 1)))) return rc;
+  // every particle may be large (a cell smaller than the particles)
+  if ((rc = g_bp.lcount.grow(sizeof(int) * static_cast<size_t>(n + 1) * large_chunks(n))))
+    return rc;
   return scan_reserve(static_cast<int>(tbl) > n ? static_cast<int>(tbl) : n);
 }
 
@@ -331,6 +419,9 @@ static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double c
   g.cellh = g_bp.cellh.as<unsigned>();
   g.large = g_bp.obst.as<int>();
   g.nlarge = g_bp.err.as<int>();
+  g.chunk = large_chunk_len(n);
+  g.nchunks = large_chunks(n);
+  g.lcount = g_bp.lcount.as<int>();
   hipLaunchKernelGGL(k_grid_bin, grid_for(n, 256), 256, 0, st, g);
   int rc;
   if ((rc = scan_counts(g_bp.ccount.as<int>(), static_cast<int>(tbl), g_bp.cstart.as<long long>(),
@@ -339,6 +430,9 @@ static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double c
   hipLaunchKernelGGL(k_grid_scatter, grid_for(n, 256), 256, 0, st, g);
   hipLaunchKernelGGL(k_grid_count, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
                      g_bp.count.as<int>());
+  const long long items = static_cast<long long>(n) * g.nchunks;  // bound on the device count
+  hipLaunchKernelGGL(k_large_count, large_grid(items), 64 * kBpWaves, 0, st, g);
+  hipLaunchKernelGGL(k_large_sum, large_grid(n), 64 * kBpWaves, 0, st, g, g_bp.count.as<int>());
   DH_HIP_CHECK(hipGetLastError());
   return scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st);
 }
@@ -362,6 +456,8 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
   if (*npairs > cap) return DH_ERR_CAPACITY;
   hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
                      g_bp.start.as<long long>(), cap, pairs);
+  hipLaunchKernelGGL(k_large_fill, large_grid(static_cast<long long>(n) * g.nchunks),
+                     64 * kBpWaves, 0, st, g, g_bp.start.as<long long>(), cap, pairs);
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
 }
@@ -382,6 +478,8 @@ int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double ce
   if ((rc = grid_build(geom, obstacle_dev, cell, g, st))) return rc;
   hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
                      g_bp.start.as<long long>(), cap, pairs);
+  hipLaunchKernelGGL(k_large_fill, large_grid(static_cast<long long>(n) * g.nchunks),
+                     64 * kBpWaves, 0, st, g, g_bp.start.as<long long>(), cap, pairs);
   DH_HIP_CHECK(hipGetLastError());
   *np_dev = g_bp.start.as<long long>() + n;
   return DH_OK;

@@ -75,6 +75,11 @@ struct DetectArgs {
   // When set, the number of pairs is read here (at most npairs, which is then
   // the launch capacity); pairs beyond it record no contacts.
   const long long* npairs_dev;
+  // When set, a box per particle (lo xyz, hi xyz) containing its vertices:
+  // the broad phase's inflated boxes.  Rows apart from B's particle box are
+  // apart from every box of B's triangles, so a block of rows that is all
+  // apart is skipped before B's triangles are loaded.
+  const double* pbox;
 };
 
 __device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
@@ -342,68 +347,133 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
   if (M == DH_BF) {
     // Box test first, survivors compacted in row-major order into LDS, then
     // solved 256 at a time, so that no lane idles beside a culled pair.  When
-    // the boxes of the tile's rows and of B's triangles fit in LDS they are
-    // staged there, and only rows whose box meets the union of B's boxes and
-    // columns whose box meets the union of the rows' boxes are paired (the
-    // union test implies the pair test, so nothing else changes).
+    // B's triangle boxes fit in LDS they are staged there once, the tile's
+    // rows are staged in blocks of up to kBoxTriangles (a large A such as a
+    // funnel or a wall takes several blocks, in row order), and only rows
+    // whose box meets the union of B's boxes and columns whose box meets the
+    // union of the block's rows are paired (the union test implies the pair
+    // test, so nothing else changes).  A larger B takes one unstaged pass.
     __shared__ int surv[kSurvivors];
     __shared__ double boxes[2 * kBoxTriangles * 6];
-    __shared__ int act[2 * kBoxTriangles];  // active rows (tile-relative), then columns
-    __shared__ double uni[2][6];            // union of the rows' boxes, of B's boxes
+    __shared__ int act[2 * kBoxTriangles];  // active rows (block-relative), then columns
+    __shared__ double uni[2][6];            // union of the block's row boxes, of B's boxes
     __shared__ double red[4][12];
     __shared__ int nact[2];
-    const bool staged = nrows <= kBoxTriangles && p.nB <= kBoxTriangles;  // block-uniform
-    int nra = nrows, nca = p.nB;
-    if (staged) {
-      for (int t = tid; t < nrows + p.nB; t += 256) {
-        const bool rowA = t < nrows;
-        double* b = boxes + 6 * (rowA ? t : kBoxTriangles + (t - nrows));
-        tri_box(load_tri(g, rowA ? p.oA + r0 + t : p.oB + (t - nrows)), b, b + 3);
-      }
-      __syncthreads();
-      // unions: lanes over strided boxes, then waves, then the block
-      double u[12];
-      for (int q = 0; q < 3; ++q) {
-        u[q] = u[6 + q] = INFINITY;
-        u[3 + q] = u[9 + q] = -INFINITY;
-      }
-      for (int t = tid; t < nrows; t += 256)
-        for (int q = 0; q < 3; ++q) {
-          u[q] = fmin(u[q], boxes[6 * t + q]);
-          u[3 + q] = fmax(u[3 + q], boxes[6 * t + 3 + q]);
-        }
-      for (int t = tid; t < p.nB; t += 256)
-        for (int q = 0; q < 3; ++q) {
-          u[6 + q] = fmin(u[6 + q], boxes[6 * (kBoxTriangles + t) + q]);
-          u[9 + q] = fmax(u[9 + q], boxes[6 * (kBoxTriangles + t) + 3 + q]);
-        }
-      for (int o = 32; o > 0; o >>= 1)
-        for (int q = 0; q < 12; ++q) {
-          const double v = __shfl_xor(u[q], o);
-          u[q] = ((q % 6) < 3) ? fmin(u[q], v) : fmax(u[q], v);
-        }
-      if (lane == 0)
-        for (int q = 0; q < 12; ++q) red[wave][q] = u[q];
-      __syncthreads();
-      if (tid < 12) {
-        double v = red[0][tid];
-        for (int w = 1; w < 4; ++w) v = ((tid % 6) < 3) ? fmin(v, red[w][tid]) : fmax(v, red[w][tid]);
-        uni[tid / 6][tid % 6] = v;
-      }
-      if (tid < 2) nact[tid] = 0;
-      __syncthreads();
-      // ordered lists of active rows and columns
-      for (int side = 0; side < 2; ++side) {
-        const int cnt_all = side == 0 ? nrows : p.nB;
-        const double* other = uni[1 - side];
-        for (int t0 = 0; t0 < cnt_all; t0 += 256) {
-          const int t = t0 + tid;
-          bool on = false;
-          if (t < cnt_all) {
-            const double* b = boxes + 6 * (side == 0 ? t : kBoxTriangles + t);
-            on = !boxes_apart(b, b + 3, other, other + 3, p.margin);
+    const bool staged = p.nB <= kBoxTriangles;  // block-uniform
+    const int blk = staged ? kBoxTriangles : (nrows > 0 ? nrows : 1);
+    bool b_staged = false;  // B's boxes are loaded on the first block that needs them
+    for (int rb = 0; rb < nrows; rb += blk) {
+      const int rs = r0 + rb;  // first row of the block
+      const int nrb = nrows - rb < blk ? nrows - rb : blk;
+      int nra = nrb, nca = p.nB;
+      if (staged) {
+        bool near = false;
+        for (int t = tid; t < nrb; t += 256) {
+          double* b = boxes + 6 * t;
+          tri_box(load_tri(g, p.oA + rs + t), b, b + 3);
+          if (g.pbox) {
+            const double* pb = g.pbox + 6 * static_cast<long long>(p.pb);
+            near = near || !boxes_apart(b, b + 3, pb, pb + 3, p.margin);
           }
-          const unsigned long long m = __ballot(on);
+        }
+        if (g.pbox && !__syncthreads_or(near)) {
+          // every row of the block misses B's particle box
+          culled += tid == 0 ? static_cast<unsigned>(nrb * p.nB) : 0u;
+          continue;
+        }
+        if (!b_staged) {
+          for (int t = tid; t < p.nB; t += 256) {
+            double* b = boxes + 6 * (kBoxTriangles + t);
+            tri_box(load_tri(g, p.oB + t), b, b + 3);
+          }
+          b_staged = true;
+        }
+        __syncthreads();
+        // unions: lanes over strided boxes, then waves, then the block
+        double u[12];
+        for (int q = 0; q < 3; ++q) {
+          u[q] = u[6 + q] = INFINITY;
+          u[3 + q] = u[9 + q] = -INFINITY;
+        }
+        for (int t = tid; t < nrb; t += 256)
+          for (int q = 0; q < 3; ++q) {
+            u[q] = fmin(u[q], boxes[6 * t + q]);
+            u[3 + q] = fmax(u[3 + q], boxes[6 * t + 3 + q]);
+          }
+        for (int t = tid; t < p.nB; t += 256)
+          for (int q = 0; q < 3; ++q) {
+            u[6 + q] = fmin(u[6 + q], boxes[6 * (kBoxTriangles + t) + q]);
+            u[9 + q] = fmax(u[9 + q], boxes[6 * (kBoxTriangles + t) + 3 + q]);
+          }
+        for (int o = 32; o > 0; o >>= 1)
+          for (int q = 0; q < 12; ++q) {
+            const double v = __shfl_xor(u[q], o);
+            u[q] = ((q % 6) < 3) ? fmin(u[q], v) : fmax(u[q], v);
+          }
+        if (lane == 0)
+          for (int q = 0; q < 12; ++q) red[wave][q] = u[q];
+        __syncthreads();
+        if (tid < 12) {
+          double v = red[0][tid];
+          for (int w = 1; w < 4; ++w) v = ((tid % 6) < 3) ? fmin(v, red[w][tid]) : fmax(v, red[w][tid]);
+          uni[tid / 6][tid % 6] = v;
+        }
+        if (tid < 2) nact[tid] = 0;
+        __syncthreads();
+        // ordered lists of active rows and columns
+        for (int side = 0; side < 2; ++side) {
+          const int cnt_all = side == 0 ? nrb : p.nB;
+          const double* other = uni[1 - side];
+          for (int t0 = 0; t0 < cnt_all; t0 += 256) {
+            const int t = t0 + tid;
+            bool on = false;
+            if (t < cnt_all) {
+              const double* b = boxes + 6 * (side == 0 ? t : kBoxTriangles + t);
+              on = !boxes_apart(b, b + 3, other, other + 3, p.margin);
+            }
+            const unsigned long long m = __ballot(on);
+            if (lane == 0) wave_cnt[wave] = __popcll(m);
+            __syncthreads();
+            int off = 0, cnt = 0;
+            for (int w = 0; w < 4; ++w) {
+              off += (w < wave) ? wave_cnt[w] : 0;
+              cnt += wave_cnt[w];
+            }
+            if (on) act[side * kBoxTriangles + nact[side] + off + lane_prefix(m)] = t;
+            __syncthreads();
+            if (tid == 0) nact[side] += cnt;
+            __syncthreads();
+          }
+        }
+        nra = nact[0];
+        nca = nact[1];
+      }
+      // candidate fe in [0, nra * nca): row-major over the active rows and columns
+      auto pair_of = [&](int fe, int& iA, int& iB) {
+        const int ra = fe / nca, cb = fe - ra * nca;
+        iA = staged ? rs + act[ra] : rs + ra;
+        iB = staged ? act[kBoxTriangles + cb] : cb;
+      };
+      const int tested = nra * nca;
+      culled += tid == 0 ? static_cast<unsigned>(nrb * p.nB - tested) : 0u;
+      for (int fpos = 0; fpos < tested;) {
+        int nl = 0;
+        while (fpos < tested && nl <= kSurvivors - 256) {
+          const int fe = fpos + tid;
+          bool keep = false;
+          if (fe < tested) {
+            int iA, iB;
+            pair_of(fe, iA, iB);
+            if (staged) {
+              const double* a = boxes + 6 * (iA - rs);
+              const double* b = boxes + 6 * (kBoxTriangles + iB);
+              keep = !boxes_apart(a, a + 3, b, b + 3, p.margin);
+            } else {
+              keep = !boxes_apart(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), p.margin);
+            }
+            culled += keep ? 0u : 1u;
+          }
+          const unsigned long long m = __ballot(keep);
           if (lane == 0) wave_cnt[wave] = __popcll(m);
           __syncthreads();
           int off = 0, cnt = 0;
@@ -411,65 +481,26 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
             off += (w < wave) ? wave_cnt[w] : 0;
             cnt += wave_cnt[w];
           }
-          if (on) act[side * kBoxTriangles + nact[side] + off + lane_prefix(m)] = t;
-          __syncthreads();
-          if (tid == 0) nact[side] += cnt;
+          if (keep) surv[nl + off + lane_prefix(m)] = fe;
+          nl += cnt;
+          fpos += 256;
           __syncthreads();
         }
-      }
-      nra = nact[0];
-      nca = nact[1];
-    }
-    // candidate fe in [0, nra * nca): row-major over the active rows and columns
-    auto pair_of = [&](int fe, int& iA, int& iB) {
-      const int ra = fe / nca, cb = fe - ra * nca;
-      iA = staged ? r0 + act[ra] : r0 + ra;
-      iB = staged ? act[kBoxTriangles + cb] : cb;
-    };
-    const int tested = nra * nca;
-    culled += tid == 0 ? static_cast<unsigned>(total - tested) : 0u;
-    for (int fpos = 0; fpos < tested;) {
-      int nl = 0;
-      while (fpos < tested && nl <= kSurvivors - 256) {
-        const int fe = fpos + tid;
-        bool keep = false;
-        if (fe < tested) {
-          int iA, iB;
-          pair_of(fe, iA, iB);
-          if (staged) {
-            const double* a = boxes + 6 * (iA - r0);
-            const double* b = boxes + 6 * (kBoxTriangles + iB);
-            keep = !boxes_apart(a, a + 3, b, b + 3, p.margin);
-          } else {
-            keep = !boxes_apart(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), p.margin);
+        for (int b = 0; b < nl; b += 256) {
+          const int e = b + tid;
+          bool emit = false;
+          V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
+          int iA = 0, iB = 0;
+          if (e < nl) {
+            pair_of(surv[e], iA, iB);
+            bf_solver(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), P, Q);
+            emit = pq_dist(P, Q) <= p.margin;
           }
-          culled += keep ? 0u : 1u;
+          emit_chunk(emit, iA, iB, P, Q);
         }
-        const unsigned long long m = __ballot(keep);
-        if (lane == 0) wave_cnt[wave] = __popcll(m);
-        __syncthreads();
-        int off = 0, cnt = 0;
-        for (int w = 0; w < 4; ++w) {
-          off += (w < wave) ? wave_cnt[w] : 0;
-          cnt += wave_cnt[w];
-        }
-        if (keep) surv[nl + off + lane_prefix(m)] = fe;
-        nl += cnt;
-        fpos += 256;
-        __syncthreads();
       }
-      for (int b = 0; b < nl; b += 256) {
-        const int e = b + tid;
-        bool emit = false;
-        V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
-        int iA = 0, iB = 0;
-        if (e < nl) {
-          pair_of(surv[e], iA, iB);
-          bf_solver(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), P, Q);
-          emit = pq_dist(P, Q) <= p.margin;
-        }
-        emit_chunk(emit, iA, iB, P, Q);
-      }
+      // the next block overwrites the row boxes and the active lists
+      __syncthreads();
     }
   } else {
     for (int base = 0; base < total; base += 256) {
@@ -755,7 +786,8 @@ static int tiles_for(long long npairs, int method) {
 
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
                       dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
-                      long long* offs_out, dh_stats* stats, hipStream_t st) {
+                      long long* offs_out, dh_stats* stats, hipStream_t st,
+                      const double* pbox) {
   int rc;
   const long long temp_cap = cap > 0 ? cap : 1;
   if ((rc = g_scratch.temp.grow(sizeof(dh_contact) * temp_cap))) return rc;
@@ -795,6 +827,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.stats = g_scratch.stats.as<unsigned long long>();
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = nullptr;
+  a.pbox = pbox;
   const size_t shm_rows = 2 * kMaxRowTriangles * sizeof(int) + 8 + kMaxRowTriangles * 8;
   const unsigned grid = static_cast<unsigned>(static_cast<long long>(npairs) * tiles);
   switch (method) {
@@ -868,7 +901,7 @@ static int graph_tiles(int pair_cap) { return tiles_for(pair_cap / 2 > 0 ? pair_
 // dev_counters()[0] = contacts found (may exceed contact_cap), error flag.
 int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, int pair_cap,
                        const long long* np_dev, dh_contact* out, long long contact_cap,
-                       int* counts, long long* offs, hipStream_t st) {
+                       int* counts, long long* offs, hipStream_t st, const double* pbox) {
   int rc;
   if ((rc = detect_reserve(pair_cap, contact_cap))) return rc;
   fill_async(g_scratch.counter.as<unsigned long long>(), 0ull, 1, st);
@@ -895,6 +928,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.stats = g_scratch.stats.as<unsigned long long>();
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = np_dev;
+  a.pbox = pbox;
   const unsigned grid = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
   switch (method) {
     case DH_BF: launch_detect<DH_BF>(grid, 0, st, a); break;
@@ -973,7 +1007,7 @@ int dh_detect_pairs(int method, const dh_geometry* geom, const int* pairs_dev, i
   if (npairs == 0) return DH_OK;
   long long total = 0;
   rc = detect_pairs_impl(method, geom, pairs_dev, npairs, out_dev, cap, &total, pair_count_dev,
-                         nullptr, stats, as_stream(stream));
+                         nullptr, stats, as_stream(stream), nullptr);
   *n_out = static_cast<int>(total);
   if (stats && (method == DH_HYBRID_BATCH_STAT || method == DH_HYBRID_TRIANGLE_STAT) &&
       (rc == DH_OK || rc == DH_ERR_CAPACITY)) {
@@ -1060,7 +1094,7 @@ int dh_detect_pair(int method, const double* xA, const double* yA, const double*
   if ((rc = outbuf.grow(sizeof(dh_contact) * dcap))) return rc;
   long long total = 0;
   rc = detect_pairs_impl(method, &g, dm->pairs, 1, outbuf.as<dh_contact>(), cap, &total, nullptr,
-                         nullptr, stats, nullptr);
+                         nullptr, stats, nullptr, nullptr);
   *n_out = static_cast<int>(total);
   if (rc != DH_OK) return rc;
   if (stats && (method == DH_HYBRID_BATCH_STAT || method == DH_HYBRID_TRIANGLE_STAT))

@@ -245,15 +245,21 @@ __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __r
                                                   const double* __restrict__ rz,
                                                   const int* __restrict__ tri_offset) {
   const int i = blockIdx.x;
-  if (b.obstacle[i]) return;
   const double h = h_dev ? *h_dev : h_arg;  // the adaptive controller's step
   double v[3], W[3], R[9], pos[3], refpos[3], w[3];
   load3(b.linvel, i, v);
   load3(b.refangvel, i, W);
+  // Obstacles at rest stay as built (the reference skips them,
+  // Engine.cpp:532).  An obstacle given a velocity (a rotating drum or box)
+  // moves with it kinematically: no force, no gravity.
+  const bool kinematic = b.obstacle[i] != 0;
+  if (kinematic && v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0 && W[0] == 0.0 && W[1] == 0.0 &&
+      W[2] == 0.0)
+    return;
   load3(b.pos, i, pos);
   load3(b.refpos, i, refpos);
   for (int q = 0; q < 9; ++q) R[q] = b.rot[9 * i + q];
-  if (!touched || touched[i]) {
+  if (!kinematic && (!touched || touched[i])) {
     const double m = b.mass[i];
     double F[3], T[3], J[9], Ji[9];
     load3(force, i, F);
@@ -266,7 +272,7 @@ __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __r
     update_angular(W, R, J, Ji, T, h);
   }
   for (int q = 0; q < 3; ++q)
-    if (grav.g[q] != 0.0) v[q] += h * grav.g[q];
+    if (grav.g[q] != 0.0 && !kinematic) v[q] += h * grav.g[q];
   for (int q = 0; q < 3; ++q) pos[q] += h * v[q];
   update_rotation(w, W, R, h);
   if (threadIdx.x == 0) {

@@ -121,6 +121,14 @@ int dh_memset(void* p, int value, size_t bytes) {
   return DH_OK;
 }
 
+int dh_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!free_bytes || !total_bytes) return set_error(DH_ERR_ARG, "dh_mem_info: null");
+  DH_HIP_CHECK(hipMemGetInfo(free_bytes, total_bytes));
+  return DH_OK;
+}
+
 int dh_synchronize(void) {
   int rc = ensure_init();
   if (rc) return rc;

@@ -211,3 +211,95 @@ def granulate_particle(center, h, npoints, rng, roundness=0.25):
     flip = np.einsum("ij,ij->i", n, tris[:, 0] - inner) < 0
     tris[flip] = tris[flip][:, [0, 2, 1]]
     return tris + np.asarray(center, float)
+
+
+def uv_ellipsoid(n_seg=8, n_ring=9, axes=(3.0, 1.0, 1.0)):
+    """Closed, outward ellipsoid mesh from a latitude-longitude sphere:
+    n_seg segments around, n_ring bands from pole to pole, 2 n_seg (n_ring-1)
+    triangles (8, 9: 128), then the axes scaled.  Returns (ntri, 3, 3)."""
+    th = np.pi * np.arange(1, n_ring) / n_ring
+    ph = 2 * np.pi * np.arange(n_seg) / n_seg
+    ring = np.stack([np.outer(np.sin(th), np.cos(ph)), np.repeat(np.cos(th)[:, None], n_seg, 1),
+                     np.outer(np.sin(th), np.sin(ph))], -1)  # (n_ring-1, n_seg, 3), y axis polar
+    v = np.concatenate([[[0.0, 1.0, 0.0]], ring.reshape(-1, 3), [[0.0, -1.0, 0.0]]])
+    top, bot = 0, len(v) - 1
+    idx = lambda r, s: 1 + r * n_seg + (s % n_seg)
+    f = []
+    for s in range(n_seg):
+        f.append((top, idx(0, s + 1), idx(0, s)))
+        f.append((bot, idx(n_ring - 2, s), idx(n_ring - 2, s + 1)))
+        for r in range(n_ring - 2):
+            a, b, c, d = idx(r, s), idx(r, s + 1), idx(r + 1, s), idx(r + 1, s + 1)
+            f += [(a, b, d), (a, d, c)]
+    m = v[np.array(f)] * np.asarray(axes, float)
+    n = np.cross(m[:, 1] - m[:, 0], m[:, 2] - m[:, 0])
+    flip = np.einsum("ij,ij->i", n, m.mean(1)) < 0
+    m[flip] = m[flip][:, [0, 2, 1]]
+    return m
+
+
+def sphere_hull(npoints, radius, rng):
+    """Convex hull of `npoints` points uniform on a sphere (2 npoints - 4
+    triangles: 60 for 32 points), outward.  Hull by Qhull (scipy)."""
+    from scipy.spatial import ConvexHull
+    u = rng.uniform((npoints, 2))
+    z = 2.0 * u[:, 0] - 1.0
+    phi = 2.0 * np.pi * u[:, 1]
+    s = np.sqrt(1.0 - z * z)
+    pts = radius * np.stack([s * np.cos(phi), s * np.sin(phi), z], 1)
+    hull = ConvexHull(pts)
+    tris = pts[hull.simplices]
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    flip = np.einsum("ij,ij->i", n, tris.mean(1)) < 0
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    return tris
+
+
+def cone_funnel(outlet_radius, top_radius, half_angle_deg, n_theta=64, n_h=16, y0=0.0):
+    """Conical funnel surface about the y axis: radius outlet_radius at y0,
+    growing at tan(half_angle) per unit height up to top_radius; n_theta x n_h
+    quads, 2 n_theta n_h triangles (64 x 16: 2,048), open at both ends."""
+    t = np.tan(np.radians(half_angle_deg))
+    height = (top_radius - outlet_radius) / t
+    ys = y0 + height * np.arange(n_h + 1) / n_h
+    rs = outlet_radius + (ys - y0) * t
+    ph = 2 * np.pi * np.arange(n_theta + 1) / n_theta
+    P = np.stack([rs[:, None] * np.cos(ph)[None], np.repeat(ys[:, None], n_theta + 1, 1),
+                  rs[:, None] * np.sin(ph)[None]], -1)
+    a, b = P[:-1, :-1], P[:-1, 1:]
+    c, d = P[1:, :-1], P[1:, 1:]
+    t1 = np.stack([a, d, b], 2).reshape(-1, 3, 3)
+    t2 = np.stack([a, c, d], 2).reshape(-1, 3, 3)
+    return np.concatenate([t1, t2])
+
+
+def random_rotations(rng, n):
+    """n uniform rotation matrices (Shoemake), (n, 3, 3)."""
+    u = rng.uniform((n, 3))
+    u1, u2, u3 = u[:, 0], u[:, 1], u[:, 2]
+    x = np.sqrt(1 - u1) * np.sin(2 * np.pi * u2)
+    y = np.sqrt(1 - u1) * np.cos(2 * np.pi * u2)
+    z = np.sqrt(u1) * np.sin(2 * np.pi * u3)
+    w = np.sqrt(u1) * np.cos(2 * np.pi * u3)
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+                     np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+                     np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)],
+                    1)
+
+
+def mass_properties_batch(meshes, density):
+    """mass_properties over a batch of meshes with equal triangle counts
+    (k, ntri, 3, 3): masses (k,), centres (k, 3), inertia tensors (k, 3, 3)."""
+    m = np.asarray(meshes, dtype=np.float64)
+    a, b, c = m[:, :, 0], m[:, :, 1], m[:, :, 2]
+    det = np.einsum("kij,kij->ki", a, np.cross(b, c))
+    vol = det.sum(1) / 6.0
+    com = ((a + b + c) * det[..., None]).sum(1) / (24.0 * vol)[:, None]
+    s = a + b + c
+    outer = lambda p: np.einsum("kni,knj->knij", p, p)
+    cov = (det[..., None, None] / 120.0) * (outer(s) + outer(a) + outer(b) + outer(c))
+    C = cov.sum(1) - vol[:, None, None] * np.einsum("ki,kj->kij", com, com)
+    tr = np.trace(C, axis1=1, axis2=2)
+    J = tr[:, None, None] * np.eye(3)[None] - C
+    rho = np.broadcast_to(np.asarray(density, float), vol.shape)
+    return rho * vol, com, rho[:, None, None] * J

@@ -125,6 +125,8 @@ int dh_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
 int dh_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
 int dh_memset(void* dev_ptr, int value, size_t bytes);
 int dh_synchronize(void);
+/* Free and total HBM of the device (hipMemGetInfo), for footprint reports. */
+int dh_mem_info(size_t* free_bytes, size_t* total_bytes);
 /* Timing on the library stream (or a caller stream when `stream` != NULL). */
 int dh_event_create(void** ev);
 int dh_event_destroy(void* ev);
@@ -385,7 +387,8 @@ typedef struct dh_bodies {
   double* inertia;    /* 9n  referential inertia                              */
   double* inverse;    /* 9n  its inverse                                      */
   double* mass;       /* n   (may be +inf for obstacles)                      */
-  int* obstacle;      /* n   1: never moved                                   */
+  int* obstacle;      /* n   1: no force moves it; at rest it stays as built,
+                              with a velocity it moves kinematically          */
   int* material;      /* n                                                    */
   int n;
 } dh_bodies;
@@ -417,6 +420,9 @@ int dh_contacts_forces(const dh_bodies* bodies, const int* pairs_dev, int npairs
  * for touched particles v += dt F/m and updateAngular; then for every
  * non-obstacle particle v += dt*g (non-zero components), x += dt v,
  * updateRotationMatrix, and every vertex replica x = c + R (X - C_ref).
+ * An obstacle with a non-zero linear or referential angular velocity (no
+ * counterpart in the reference, whose obstacles never move) takes the same
+ * x, R and vertex update without force or gravity: a rotating box or drum.
  * touched_dev may be NULL (all particles touched). */
 int dh_integrate(dh_bodies* bodies, const double* force_dev, const double* torque_dev,
                  const int* touched_dev, double dt, const double gravity[3],

@@ -1059,9 +1059,17 @@ int64_t or_step(or_scene* s, int method, const dh_force_params* fp, double dt,
                       b->inverse + 9 * p, torque + 3 * p, dt);
   }
   for (int p = 0; p < n; ++p) {
-    if (b->obstacle[p]) continue;
+    /* obstacles at rest are skipped (Engine.cpp:532); one with a velocity
+     * moves with it kinematically (no force, no gravity) */
+    const int kinematic = b->obstacle[p] != 0;
+    if (kinematic) {
+      const double* u = b->linvel + 3 * p;
+      const double* W = b->refangvel + 3 * p;
+      if (u[0] == 0.0 && u[1] == 0.0 && u[2] == 0.0 && W[0] == 0.0 && W[1] == 0.0 && W[2] == 0.0)
+        continue;
+    }
     for (int q = 0; q < 3; ++q)
-      if (gravity[q] != 0.0) b->linvel[3 * p + q] += dt * gravity[q];
+      if (gravity[q] != 0.0 && !kinematic) b->linvel[3 * p + q] += dt * gravity[q];
     for (int q = 0; q < 3; ++q) b->pos[3 * p + q] += dt * b->linvel[3 * p + q];
     or_update_rotation_matrix(b->angvel + 3 * p, b->refangvel + 3 * p, b->rot + 9 * p, dt);
     for (int v = 3 * s->tri_offset[p]; v < 3 * s->tri_offset[p + 1]; ++v)

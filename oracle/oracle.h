@@ -117,7 +117,9 @@ void or_update_vertex(double* x, double* y, double* z, double rx, double ry, dou
 /* One explicit DEM step over a scene (Engine.cpp:155-163 order: contact
  * detection on the current geometry, forces, velocity/angular update, gravity,
  * position/rotation/vertex update).  Candidate pairs are all (i<j) whose
- * AABBs inflated by eps overlap and that are not both obstacles.
+ * AABBs inflated by eps overlap and that are not both obstacles.  Obstacles
+ * with a velocity move kinematically (the engine's extension for rotating
+ * boxes; pinned by a rigid-rotation closed form in the tests).
  * Returns the number of contacts. */
 typedef struct or_scene {
   int n;                 /* particles                                 */

@@ -119,3 +119,37 @@ def test_batched_pairs_match_per_pair_oracle(method):
     for f in ("penalty_fails", "batch_fails", "batch_size", "triangle_checks",
               "pairs_evaluated", "fallback_rows"):
         assert getattr(st, f) == getattr(ost_total, f), f
+
+
+def _wall_touching_particle():
+    """The 2,048-triangle funnel and an 80-triangle sphere pressed 0.05 mm
+    into its wall at mid height."""
+    from delta_amd import workloads
+    funnel = shapes.cone_funnel(2e-3, 20e-3, 30.0)
+    t = np.tan(np.radians(30.0))
+    y = 15e-3
+    inward = -np.array([1.0, -t, 0.0]) / np.hypot(1.0, t)
+    c = np.array([2e-3 + y * t, y, 0.0]) + inward * 0.45e-3
+    return funnel, scenes.particle(1, 0.5e-3, 61, c)
+
+
+@pytest.mark.parametrize("max_tiles", [1, 16])
+@pytest.mark.parametrize("swap", [False, True])
+def test_bf_large_particle_matches_oracle(max_tiles, swap):
+    """A particle with more rows than one LDS block holds (the funnel as A:
+    2,048 rows, staged in blocks of 320 when one workgroup takes them all)
+    and as B (2,048 columns: the unstaged pass): contacts bit-exact and in
+    the reference's order."""
+    from delta_amd import tuning
+    A, B = _wall_touching_particle()
+    if swap:
+        A, B = B, A
+    tuning.set(detect_max_tiles=max_tiles)
+    try:
+        got = contact.bf(*shapes.to_soa(A), len(A), EPS, True, 7, *shapes.to_soa(B), len(B),
+                         EPS, True, 9)
+    finally:
+        tuning.set(**tuning.default())
+    rc, want, _ = ol.or_detect(_lib.DH_BF, A, B, EPS, EPS, 1, 1, 7, 9)
+    assert rc == 0 and len(want) > 0
+    same_contacts(got, want)

@@ -54,14 +54,22 @@ def _scene(name):
         return spinning_scene()
     if name == "c1":
         return engine.c1_scene()
+    if name.startswith("drum-") or name.startswith("funnel-"):
+        from delta_amd import workloads
+        kind, n = name.split("-")
+        return getattr(workloads, kind + "_scene")(n=int(n))
     from delta_amd import scene_io
     return scene_io.hopper_scene(num=int(name.split("-")[1]), sphere_level=1)
 
 
-@pytest.mark.parametrize("scene", ["spinning", "c1", "hopper-6", "hopper-12"])
+@pytest.mark.parametrize("scene", ["spinning", "c1", "hopper-6", "hopper-12", "drum-3000",
+                                   "funnel-1500"])
 def test_broadphase_equals_all_pairs(scene):
     """Exact candidate set; the hopper lattices (touching spheres in wall
-    prisms, obstacles first) have thousands of particles and pairs."""
+    prisms, obstacles first) have thousands of particles and pairs; the
+    box of drum-3000 and the funnel of funnel-1500 enclose every particle,
+    so their partner lists (3,000 and 1,500 pairs) span several of the
+    large-particle search chunks."""
     hs = _scene(scene)
     ds = engine.DeviceScene(hs)
     got, _ = gpu_pairs(ds, hs, hs.cell_size())
