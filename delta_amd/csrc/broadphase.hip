@@ -379,6 +379,20 @@ static unsigned table_size(int n) {
 // call on more particles.
 const double* broadphase_boxes() { return g_bp.box.as<double>(); }
 
+// The same boxes for the current geometry, without the grid (for callers
+// of the detection that did not run the broad phase).
+int particle_boxes(const dh_geometry* geom, const double** out, hipStream_t st) {
+  const int n = geom->nparticles;
+  int rc;
+  if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
+  if (n > 0)
+    hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
+                       geom->tri_offset, geom->eps, g_bp.box.as<Aabb>());
+  DH_HIP_CHECK(hipGetLastError());
+  *out = g_bp.box.as<double>();
+  return DH_OK;
+}
+
 // Buffers of the broad phase for n particles (call before a capture).
 int broadphase_reserve(int n) {
   int rc;

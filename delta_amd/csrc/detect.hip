@@ -214,19 +214,22 @@ __device__ __forceinline__ void make_contact(const Pair& p, int k, int iA, int i
   c->pair = k;
 }
 
+// One (particle pair, row range) item of the detection: `item` = pair *
+// tiles + tile.  Returns when the item is done; the caller synchronises the
+// workgroup before its next item.
 template <int M>
-__global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
+__device__ __forceinline__ void detect_item(const DetectArgs& g, const int item) {
   extern __shared__ unsigned char smem[];
   __shared__ int wave_cnt[4];
   __shared__ unsigned long long chunk_base;
   __shared__ unsigned long long local_stats[6];
 
-  const int k = blockIdx.x / g.tiles, tile = blockIdx.x - k * g.tiles;
+  const int k = item / g.tiles, tile = item - k * g.tiles;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  unsigned long long* ts = g.tile_stats + static_cast<long long>(blockIdx.x) * kTileStats;
+  unsigned long long* ts = g.tile_stats + static_cast<long long>(item) * kTileStats;
   if (g.npairs_dev && k >= *g.npairs_dev) {
-    if (tid == 0) g.tile_count[blockIdx.x] = 0;
+    if (tid == 0) g.tile_count[item] = 0;
     if (tid < kTileStats) ts[tid] = 0;
     return;
   }
@@ -238,13 +241,13 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
 
   if (M == DH_HYBRID_TRIANGLE && p.nB > p.nA) {
     if (tid == 0) atomicCAS(g.error, DE_NONE, static_cast<int>(DE_REF_UB));
-    if (tid == 0) g.tile_count[blockIdx.x] = 0;
+    if (tid == 0) g.tile_count[item] = 0;
     if (tid < kTileStats) ts[tid] = 0;
     return;
   }
   if ((is_hybrid || is_rowstat) && (rows > kMaxRowTriangles || p.nB > kMaxRowTriangles)) {
     if (tid == 0) atomicCAS(g.error, DE_NONE, static_cast<int>(DE_TOO_LONG));
-    if (tid == 0) g.tile_count[blockIdx.x] = 0;
+    if (tid == 0) g.tile_count[item] = 0;
     if (tid < kTileStats) ts[tid] = 0;
     return;
   }
@@ -335,7 +338,7 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
       if (slot < static_cast<unsigned long long>(g.cap)) {
         dh_contact c;
         make_contact(p, k, iA, iB, P, Q, &c);
-        c.pair = static_cast<int>(blockIdx.x);  // the tile, until the scatter
+        c.pair = item;  // the tile, until the scatter
         c.pad_ = static_cast<int>(running + off + pre);
         g.temp[slot] = c;
       }
@@ -544,7 +547,7 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     __syncthreads();
   }
   if (tid == 0) {
-    g.tile_count[blockIdx.x] = static_cast<int>(running);
+    g.tile_count[item] = static_cast<int>(running);
     // evaluations the reference performs: every pair once, plus bfSolver
     // again over each fallback row (sums over the tiles)
     const unsigned long long evals =
@@ -557,6 +560,18 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g) {
     ts[TS_BATCH_FAILS] = is_rowstat ? local_stats[2] : 0ull;
     ts[TS_BATCH_SIZE] = is_rowstat ? static_cast<unsigned long long>(nrows) * p.nB : 0ull;
     ts[TS_CULLED] = M == DH_BF ? local_stats[3] : 0ull;
+  }
+}
+
+// Persistent over the items: workgroup b takes items b, b + gridDim.x, ...
+// (with as many workgroups as items, one each).  Pairs of very uneven cost
+// (a 2,048-triangle funnel against 20-triangle particles) share the
+// workgroups of one launch.
+template <int M>
+__global__ __launch_bounds__(256) void k_detect(DetectArgs g, int items) {
+  for (int item = blockIdx.x; item < items; item += gridDim.x) {
+    detect_item<M>(g, item);
+    __syncthreads();  // the next item reuses the workgroup's LDS
   }
 }
 
@@ -761,9 +776,19 @@ int scan_counts(const int* counts, int n, long long* offs, hipStream_t st) {
   return DH_OK;
 }
 
+// Workgroups of a detection launch over `items`: one per item, or the
+// persistent grid of detect_wg_per_cu workgroups per CU.
+static unsigned detect_grid(long long items) {
+  const int per_cu = tuning().detect_wg_per_cu;
+  long long g = items;
+  if (per_cu > 0 && g > static_cast<long long>(device_cus()) * per_cu)
+    g = static_cast<long long>(device_cus()) * per_cu;
+  return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
 template <int M>
-static void launch_detect(unsigned grid, size_t shm, hipStream_t st, const DetectArgs& a) {
-  hipLaunchKernelGGL(k_detect<M>, grid, 256, shm, st, a);
+static void launch_detect(unsigned items, size_t shm, hipStream_t st, const DetectArgs& a) {
+  hipLaunchKernelGGL(k_detect<M>, detect_grid(items), 256, shm, st, a, static_cast<int>(items));
 }
 
 static void launch_tile_stats(const unsigned long long* ts, long long ntiles,
@@ -829,19 +854,19 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.npairs_dev = nullptr;
   a.pbox = pbox;
   const size_t shm_rows = 2 * kMaxRowTriangles * sizeof(int) + 8 + kMaxRowTriangles * 8;
-  const unsigned grid = static_cast<unsigned>(static_cast<long long>(npairs) * tiles);
+  const unsigned items = static_cast<unsigned>(static_cast<long long>(npairs) * tiles);
   switch (method) {
-    case DH_BF: launch_detect<DH_BF>(grid, 0, st, a); break;
-    case DH_PENALTY: launch_detect<DH_PENALTY>(grid, 0, st, a); break;
-    case DH_PENALTY_STAT: launch_detect<DH_PENALTY_STAT>(grid, 0, st, a); break;
-    case DH_HYBRID_BATCH: launch_detect<DH_HYBRID_BATCH>(grid, shm_rows, st, a); break;
-    case DH_HYBRID_TRIANGLE: launch_detect<DH_HYBRID_TRIANGLE>(grid, shm_rows, st, a); break;
-    case DH_HYBRID_BATCH_STAT: launch_detect<DH_HYBRID_BATCH_STAT>(grid, shm_rows, st, a); break;
+    case DH_BF: launch_detect<DH_BF>(items, 0, st, a); break;
+    case DH_PENALTY: launch_detect<DH_PENALTY>(items, 0, st, a); break;
+    case DH_PENALTY_STAT: launch_detect<DH_PENALTY_STAT>(items, 0, st, a); break;
+    case DH_HYBRID_BATCH: launch_detect<DH_HYBRID_BATCH>(items, shm_rows, st, a); break;
+    case DH_HYBRID_TRIANGLE: launch_detect<DH_HYBRID_TRIANGLE>(items, shm_rows, st, a); break;
+    case DH_HYBRID_BATCH_STAT: launch_detect<DH_HYBRID_BATCH_STAT>(items, shm_rows, st, a); break;
     case DH_HYBRID_TRIANGLE_STAT:
-      launch_detect<DH_HYBRID_TRIANGLE_STAT>(grid, shm_rows, st, a);
+      launch_detect<DH_HYBRID_TRIANGLE_STAT>(items, shm_rows, st, a);
       break;
     case DH_GJK:
-      hipLaunchKernelGGL(k_gjk, (grid + 3) / 4, 256, 0, st, a);
+      hipLaunchKernelGGL(k_gjk, (items + 3) / 4, 256, 0, st, a);
       break;
     default:
       return set_error(DH_ERR_UNSUPPORTED, "dh_detect_pairs: method %d", method);
@@ -929,10 +954,10 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = np_dev;
   a.pbox = pbox;
-  const unsigned grid = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
+  const unsigned items = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
   switch (method) {
-    case DH_BF: launch_detect<DH_BF>(grid, 0, st, a); break;
-    case DH_PENALTY: launch_detect<DH_PENALTY>(grid, 0, st, a); break;
+    case DH_BF: launch_detect<DH_BF>(items, 0, st, a); break;
+    case DH_PENALTY: launch_detect<DH_PENALTY>(items, 0, st, a); break;
     default:
       return set_error(DH_ERR_UNSUPPORTED, "captured step: method %d (use DH_BF or DH_PENALTY)",
                        method);

@@ -19,7 +19,7 @@
 
 namespace dh {
 
-static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, {0, 0, 0}};
+static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, 0, {0, 0}};
 static dh_tuning g_tuning = kDefaults;
 static int g_cus = 0;
 
@@ -53,6 +53,9 @@ static int check_tuning(const dh_tuning& t) {
   if (t.detect_max_tiles < 1 || t.detect_max_tiles > 64)
     return set_error(DH_ERR_ARG, "dh_tuning: detect_max_tiles %d outside 1..64",
                      t.detect_max_tiles);
+  if (t.detect_wg_per_cu < 0 || t.detect_wg_per_cu > 64)
+    return set_error(DH_ERR_ARG, "dh_tuning: detect_wg_per_cu %d outside 0..64",
+                     t.detect_wg_per_cu);
   return DH_OK;
 }
 
@@ -118,6 +121,11 @@ static int sweep(int knob, int dh_tuning::*field, const std::vector<int>& values
   if (ntrials) *ntrials = nv;
   return DH_OK;
 }
+
+int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
+                      dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
+                      long long* offs_out, dh_stats* stats, hipStream_t st, const double* pbox);
+int particle_boxes(const dh_geometry* geom, const double** out, hipStream_t st);
 
 static bool is_exact(int method) {
   return method == DH_PAIR_BF64 || method == DH_PAIR_PENALTY64 || method == DH_PAIR_HYBRID64 ||
@@ -194,27 +202,38 @@ int dh_autotune_detect(int method, const dh_geometry* geom, const int* pairs_dev
   if (!geom || npairs <= 0 || !pairs_dev || reps < 1 || trial_cap < 0)
     return set_error(DH_ERR_ARG, "dh_autotune_detect: bad arguments");
   const hipStream_t st = as_stream(stream);
-  int n_out = 0;
+  // timed as in a step: with the particle boxes of the broad phase, which
+  // let the detection skip rows apart from the partner particle
+  const double* pbox = nullptr;
+  if ((rc = particle_boxes(geom, &pbox, st))) return rc;
+  long long n_out = 0;
   auto run = [&] {
-    return dh_detect_pairs(method, geom, pairs_dev, npairs, out_dev, cap, &n_out, nullptr,
-                           nullptr, stream);
+    const int r = detect_pairs_impl(method, geom, pairs_dev, npairs, out_dev, cap, &n_out,
+                                    nullptr, nullptr, nullptr, st, pbox);
+    return r == DH_ERR_CAPACITY
+               ? set_error(DH_ERR_CAPACITY, "dh_autotune_detect: %lld contacts, cap %d", n_out, cap)
+               : r;
   };
   // tiles per pair = clamp(ceil(target / npairs), 1, max): the target is
   // swept with the cap at 64, then the cap with the chosen target
   const dh_tuning saved = g_tuning;
   g_tuning.detect_max_tiles = 64;
-  int n1 = 0, n2 = 0;
+  int n1 = 0, n2 = 0, n3 = 0;
   rc = sweep(DH_KNOB_DETECT_TILE_TARGET, &dh_tuning::detect_tile_target,
              {1024, 2048, 4096, 8192, 16384, 32768}, run, reps, st, trials, trial_cap, &n1);
+  auto rest = [&](int used) { return trials ? trials + (used < trial_cap ? used : trial_cap) : nullptr; };
+  auto left = [&](int used) { return trial_cap > used ? trial_cap - used : 0; };
   if (rc == DH_OK)
     rc = sweep(DH_KNOB_DETECT_MAX_TILES, &dh_tuning::detect_max_tiles, {4, 8, 16, 32, 64}, run,
-               reps, st, trials ? trials + (n1 < trial_cap ? n1 : trial_cap) : nullptr,
-               trial_cap > n1 ? trial_cap - n1 : 0, &n2);
+               reps, st, rest(n1), left(n1), &n2);
+  if (rc == DH_OK)
+    rc = sweep(DH_KNOB_DETECT_WG_PER_CU, &dh_tuning::detect_wg_per_cu, {0, 2, 4, 8, 16, 32},
+               run, reps, st, rest(n1 + n2), left(n1 + n2), &n3);
   if (rc) {
     g_tuning = saved;
     return rc;
   }
-  if (ntrials) *ntrials = n1 + n2;
+  if (ntrials) *ntrials = n1 + n2 + n3;
   return run();
 }
 

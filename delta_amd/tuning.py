@@ -13,13 +13,14 @@ import json
 from . import _lib
 
 KNOBS = {0: "learned_wg_per_cu", 1: "mixed_wg_per_cu", 2: "exact_block",
-         3: "detect_tile_target", 4: "detect_max_tiles"}
+         3: "detect_tile_target", 4: "detect_max_tiles", 5: "detect_wg_per_cu"}
 
 
 class DhTuning(C.Structure):
     _fields_ = [("learned_wg_per_cu", C.c_int32), ("mixed_wg_per_cu", C.c_int32),
                 ("exact_block", C.c_int32), ("detect_tile_target", C.c_int32),
-                ("detect_max_tiles", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("detect_max_tiles", C.c_int32), ("detect_wg_per_cu", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class DhTuneTrial(C.Structure):

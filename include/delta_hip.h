@@ -338,7 +338,10 @@ typedef struct dh_tuning {
   int32_t detect_tile_target; /* particle-pair detection: workgroups per pair
                                  = clamp(ceil(target / npairs), 1, max)      */
   int32_t detect_max_tiles;   /* that max (1..64)                            */
-  int32_t reserved[3];
+  int32_t detect_wg_per_cu;   /* particle-pair detection: 0 = one workgroup
+                                 per (pair, rows) item; else a persistent grid
+                                 of this many workgroups per CU (1..64)      */
+  int32_t reserved[2];
 } dh_tuning;
 int dh_tuning_default(dh_tuning* out);
 int dh_tuning_get(dh_tuning* out);
@@ -350,7 +353,8 @@ enum {
   DH_KNOB_MIXED_WG_PER_CU = 1,
   DH_KNOB_EXACT_BLOCK = 2,
   DH_KNOB_DETECT_TILE_TARGET = 3,
-  DH_KNOB_DETECT_MAX_TILES = 4
+  DH_KNOB_DETECT_MAX_TILES = 4,
+  DH_KNOB_DETECT_WG_PER_CU = 5
 };
 typedef struct dh_tune_trial {
   int32_t knob;
@@ -370,7 +374,9 @@ int dh_autotune_tri_pairs(int method, const double* soa_dev, const float* soa32_
                           uint8_t* flags_dev, int reps, dh_tune_trial* trials, int trial_cap,
                           int* ntrials, void* stream);
 /* The same for dh_detect_pairs: sweeps detect_tile_target (with the cap at
- * 64), then detect_max_tiles.  `cap` must hold every contact. */
+ * 64), then detect_max_tiles, then detect_wg_per_cu, timing the detection
+ * as dh_step runs it (with the particle boxes of the broad phase).  `cap`
+ * must hold every contact. */
 int dh_autotune_detect(int method, const dh_geometry* geom, const int* pairs_dev, int npairs,
                        dh_contact* out_dev, int cap, int reps, dh_tune_trial* trials,
                        int trial_cap, int* ntrials, void* stream);

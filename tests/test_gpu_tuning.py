@@ -103,21 +103,22 @@ def test_detect_tilings_identical_and_autotune(scene):
     assert n > 0
     want = _detect(ds, d_pairs, n)
     assert len(want[0]) > 0
-    for target, cap in ((1, 1), (1 << 20, 64), (2048, 7), (n * 3, 3)):
-        tuning.set(detect_tile_target=target, detect_max_tiles=cap)
+    for target, cap, per_cu in ((1, 1, 0), (1 << 20, 64, 0), (2048, 7, 0), (n * 3, 3, 0),
+                                (1, 1, 1), (n * 3, 3, 2), (1 << 20, 64, 64)):
+        tuning.set(detect_tile_target=target, detect_max_tiles=cap, detect_wg_per_cu=per_cu)
         got = _detect(ds, d_pairs, n)
-        assert got[0] == want[0], (scene, target, cap)
+        assert got[0] == want[0], (scene, target, cap, per_cu)
         assert np.array_equal(got[1], want[1])
         assert got[2] == want[2]
     tuning.set(**tuning.default())
     cap = 1 << 17
     out = DeviceArray((cap * 144,), np.uint8)
     trials = tuning.autotune_detect(ds.scene.geom, d_pairs.ptr, n, out.ptr, cap, reps=2)
-    assert [t["knob"] for t in trials] == ["detect_tile_target"] * 6 + ["detect_max_tiles"] * 5
+    assert [t["knob"] for t in trials] == (["detect_tile_target"] * 6 + ["detect_max_tiles"] * 5
+                                           + ["detect_wg_per_cu"] * 6)
     t = tuning.get()
-    first = min(trials[:6], key=lambda r: r["ms"])["value"]
-    second = min(trials[6:], key=lambda r: r["ms"])["value"]
-    assert (t["detect_tile_target"], t["detect_max_tiles"]) == (first, second)
+    best = [min(trials[a:b], key=lambda r: r["ms"])["value"] for a, b in ((0, 6), (6, 11), (11, 17))]
+    assert [t["detect_tile_target"], t["detect_max_tiles"], t["detect_wg_per_cu"]] == best
     assert out.download().tobytes()[:len(want[0])] == want[0]
     assert _detect(ds, d_pairs, n)[0] == want[0]
 
