@@ -96,6 +96,9 @@ struct GridArgs {
   // each chunk's partner count
   int chunk, nchunks;
   int* lcount;
+  // the first kBpKeep partners of each particle as the count pass found
+  // them (discovery order), so that the fill pass only sorts them
+  int* keep;
 };
 
 __device__ __forceinline__ bool is_large(const GridArgs& g, int i, const Aabb& b) {
@@ -135,6 +138,7 @@ __global__ __launch_bounds__(256) void k_grid_scatter(GridArgs g) {
 // running count.
 constexpr int kBpWaves = 4;    // particles per workgroup
 constexpr int kBpList = 512;   // partners sorted in LDS per particle (else a serial sort)
+constexpr int kBpKeep = 64;    // partners kept by the count pass per particle (one per lane)
 
 __device__ __forceinline__ int bp_lane_prefix(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
@@ -280,9 +284,12 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_grid_count(GridArgs g,
   const int i = blockIdx.x * kBpWaves + (threadIdx.x >> 6);
   if (i >= g.n) return;  // wave-uniform
   const Aabb bi = g.box[i];
-  auto none = [](int, int) {};
+  int* kept = g.keep + static_cast<long long>(i) * kBpKeep;
+  auto keep = [&](int pos, int j) {
+    if (pos < kBpKeep) kept[pos] = j;
+  };
   // large particles: k_large_count / k_large_sum, launched after this
-  const int m = is_large(g, i, bi) ? 0 : grid_partners(g, i, bi, lane, none);
+  const int m = is_large(g, i, bi) ? 0 : grid_partners(g, i, bi, lane, keep);
   if (lane == 0) count[i] = m;
 }
 
@@ -307,6 +314,18 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_grid_fill(GridArgs g,
     }
   };
   if (is_large(g, i, bi)) return;  // k_large_fill
+  if (m <= kBpKeep) {
+    // the count pass kept them all: lane e ranks entry e among the m
+    // (distinct) partners by shuffles and writes it in place
+    const int v = lane < m ? g.keep[static_cast<long long>(i) * kBpKeep + lane] : 0;
+    int rank = 0;
+    for (int q = 0; q < m; ++q) {
+      const int u = __shfl(v, q);
+      rank += u < v ? 1 : 0;
+    }
+    if (lane < m) direct(rank, v);
+    return;
+  }
   if (m > kBpList) {
     // more partners than the LDS list holds: unsorted, then a serial sort
     grid_partners(g, i, bi, lane, direct);
@@ -345,7 +364,7 @@ int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
 int scan_reserve(int n);
 
 struct BpScratch {
-  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst, lcount;
+  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst, lcount, keep;
 };
 static BpScratch g_bp;
 
@@ -410,6 +429,7 @@ int broadphase_reserve(int n) {
   // every particle may be large (a cell smaller than the particles)
   if ((rc = g_bp.lcount.grow(sizeof(int) * static_cast<size_t>(n + 1) * large_chunks(n))))
     return rc;
+  if ((rc = g_bp.keep.grow(sizeof(int) * static_cast<size_t>(n + 1) * kBpKeep))) return rc;
   return scan_reserve(static_cast<int>(tbl) > n ? static_cast<int>(tbl) : n);
 }
 
@@ -436,6 +456,7 @@ static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double c
   g.chunk = large_chunk_len(n);
   g.nchunks = large_chunks(n);
   g.lcount = g_bp.lcount.as<int>();
+  g.keep = g_bp.keep.as<int>();
   hipLaunchKernelGGL(k_grid_bin, grid_for(n, 256), 256, 0, st, g);
   int rc;
   if ((rc = scan_counts(g_bp.ccount.as<int>(), static_cast<int>(tbl), g_bp.cstart.as<long long>(),
