@@ -700,25 +700,45 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const int* __restrict__ in
   if (threadIdx.x == 1023) block_sums[blockIdx.x] = buf[1023];
 }
 
-__global__ void k_scan_sums(long long* __restrict__ sums, int nb) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    long long acc = 0;
-    for (int b = 0; b < nb; ++b) {
-      const long long v = sums[b];
-      sums[b] = acc;
-      acc += v;
+// Exclusive scan of the block sums in place, by one workgroup: 1,024 at a
+// time with a running carry.
+__global__ __launch_bounds__(1024) void k_scan_sums(long long* __restrict__ sums, int nb) {
+  __shared__ long long buf[1024];
+  __shared__ long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    const int i = base + threadIdx.x;
+    const long long v = i < nb ? sums[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < 1024; s <<= 1) {
+      const long long t = (threadIdx.x >= s) ? buf[threadIdx.x - s] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
     }
+    const long long c = carry;
+    if (i < nb) sums[i] = c + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = c + buf[1023];
+    __syncthreads();
   }
 }
 
+// Adds the block offsets; the last element's thread also writes the total
+// out[n] (out[0] = 0 when n = 0).
 __global__ __launch_bounds__(1024) void k_scan_add(long long* __restrict__ out, int n,
-                                                   const long long* __restrict__ sums) {
+                                                   const long long* __restrict__ sums,
+                                                   const int* __restrict__ in) {
   const int i = blockIdx.x * 1024 + threadIdx.x;
-  if (i < n) out[i] += sums[blockIdx.x];
-}
-
-__global__ void k_scan_total(const int* __restrict__ in, int n, long long* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = (n > 0) ? out[n - 1] + in[n - 1] : 0;
+  if (i < n) {
+    const long long o = out[i] + sums[blockIdx.x];
+    out[i] = o;
+    if (i == n - 1) out[n] = o + in[i];
+  } else if (n == 0 && i == 0) {
+    out[0] = 0;
+  }
 }
 
 // Per-pair contact counts from the per-tile counts.
@@ -769,9 +789,9 @@ int scan_counts(const int* counts, int n, long long* offs, hipStream_t st) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_scan_blocks, nb, 1024, 0, st, counts, n, offs,
                      g_scratch.sums.as<long long>());
-  hipLaunchKernelGGL(k_scan_sums, 1, 64, 0, st, g_scratch.sums.as<long long>(), nb);
-  hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, g_scratch.sums.as<long long>());
-  hipLaunchKernelGGL(k_scan_total, 1, 64, 0, st, counts, n, offs);
+  hipLaunchKernelGGL(k_scan_sums, 1, 1024, 0, st, g_scratch.sums.as<long long>(), nb);
+  hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, g_scratch.sums.as<long long>(),
+                     counts);
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
 }

@@ -148,3 +148,19 @@ def test_forces_and_update_match_oracle():
                 assert np.array_equal(bits(got), bits(want)), (name, p)
             else:  # expmap's sin/cos branch: device vs host libm, <= 1 ulp each
                 assert np.allclose(got, want, rtol=1e-14, atol=1e-300), (name, p)
+
+
+@pytest.mark.parametrize("n", [0, 1, 1023, 1024, 1025, 5000, 1 << 20, 3_000_001])
+def test_exclusive_scan(n):
+    """dh_exclusive_scan against numpy: offs[k] = sum of counts[:k] and
+    offs[n] the total, for sizes around the 1,024-entry blocks and past
+    1,024 blocks (the block sums are scanned 1,024 at a time)."""
+    L = _lib.lib()
+    L.dh_exclusive_scan.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(n)
+    counts = rng.integers(0, 1000, max(n, 1)).astype(np.int32)[:n]
+    d_counts = DeviceArray.from_host(counts if n else np.zeros(1, np.int32))
+    offs = DeviceArray((n + 1,), np.int64)
+    _lib.check(L.dh_exclusive_scan(d_counts.ptr, n, offs.ptr, None), "scan")
+    want = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
+    assert np.array_equal(offs.download(), want)
