@@ -80,6 +80,21 @@ struct DetectArgs {
   // apart from every box of B's triangles, so a block of rows that is all
   // apart is skipped before B's triangles are loaded.
   const double* pbox;
+  // Survivor queue (null: every item solves its own survivors).  A staged
+  // single-block DH_BF item whose survivors all fit in its LDS list appends
+  // them {item, iA | iB << 16} as one contiguous range and records its start
+  // in qbase[item]; when the range does not fit in qcap entries, the item
+  // fills the part inside with {-1, 0} and solves its survivors itself.
+  int2* queue;
+  long long qcap;
+  unsigned long long* qcounter;
+  long long* qbase;
+  // Mid-phase-only launches (no solver in the kernel) append the items they
+  // cannot queue here; a second launch runs those items in full, taking its
+  // items from item_list.
+  int* deferred;
+  unsigned* ndeferred;
+  const int* item_list;
 };
 
 __device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
@@ -217,7 +232,7 @@ __device__ __forceinline__ void make_contact(const Pair& p, int k, int iA, int i
 // One (particle pair, row range) item of the detection: `item` = pair *
 // tiles + tile.  Returns when the item is done; the caller synchronises the
 // workgroup before its next item.
-template <int M>
+template <int M, bool QONLY>
 __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item) {
   extern __shared__ unsigned char smem[];
   __shared__ int wave_cnt[4];
@@ -365,6 +380,15 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
     const bool staged = p.nB <= kBoxTriangles;  // block-uniform
     const int blk = staged ? kBoxTriangles : (nrows > 0 ? nrows : 1);
     bool b_staged = false;  // B's boxes are loaded on the first block that needs them
+    // mid-phase-only launch: an item that cannot queue its survivors is
+    // handed to the full launch (which also writes its counters)
+    auto defer = [&]() {
+      if (tid == 0) g.deferred[atomicAdd(g.ndeferred, 1u)] = item;
+    };
+    if (QONLY && !(staged && nrows <= blk && p.nA < 65536 && p.nB < 65536)) {
+      defer();
+      return;
+    }
     for (int rb = 0; rb < nrows; rb += blk) {
       const int rs = r0 + rb;  // first row of the block
       const int nrb = nrows - rb < blk ? nrows - rb : blk;
@@ -461,6 +485,7 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
       culled += tid == 0 ? static_cast<unsigned>(nrb * p.nB - tested) : 0u;
       for (int fpos = 0; fpos < tested;) {
         int nl = 0;
+        const bool first = fpos == 0;
         while (fpos < tested && nl <= kSurvivors - 256) {
           const int fe = fpos + tid;
           bool keep = false;
@@ -489,7 +514,38 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
           fpos += 256;
           __syncthreads();
         }
-        for (int b = 0; b < nl; b += 256) {
+        if (QONLY && fpos < tested) {  // more survivors than the LDS list holds
+          defer();
+          return;
+        }
+        if (g.queue && first && fpos >= tested && staged && nrows <= blk && p.nA < 65536 &&
+            p.nB < 65536) {
+          // every survivor of the item is in LDS: append them, in row-major
+          // order, as one range of the survivor queue (solved there with
+          // every lane busy); the item solves them itself if they do not fit
+          if (tid == 0 && nl > 0)
+            chunk_base = atomicAdd(g.qcounter, static_cast<unsigned long long>(nl));
+          __syncthreads();
+          const long long qb = nl > 0 ? static_cast<long long>(chunk_base) : 0;
+          __syncthreads();
+          if (qb + nl <= g.qcap) {
+            if (tid == 0 && nl > 0) g.qbase[item] = qb;
+            for (int e = tid; e < nl; e += 256) {
+              int iA, iB;
+              pair_of(surv[e], iA, iB);
+              g.queue[qb + e] = make_int2(
+                  item, static_cast<int>(static_cast<unsigned>(iA) | (static_cast<unsigned>(iB) << 16)));
+            }
+            continue;  // fpos >= tested: the item is done
+          }
+          for (long long q = qb + tid; q < qb + nl && q < g.qcap; q += 256)
+            g.queue[q] = make_int2(-1, 0);
+          if (QONLY) {
+            defer();
+            return;
+          }
+        }
+        for (int b = 0; b < (QONLY ? 0 : nl); b += 256) {
           const int e = b + tid;
           bool emit = false;
           V3<double> P = {0, 0, 0}, Q = {0, 0, 0};
@@ -567,10 +623,10 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
 // (with as many workgroups as items, one each).  Pairs of very uneven cost
 // (a 2,048-triangle funnel against 20-triangle particles) share the
 // workgroups of one launch.
-template <int M>
+template <int M, bool QONLY = false>
 __global__ __launch_bounds__(256) void k_detect(DetectArgs g, int items) {
-  for (int item = blockIdx.x; item < items; item += gridDim.x) {
-    detect_item<M>(g, item);
+  for (int t = blockIdx.x; t < items; t += gridDim.x) {
+    detect_item<M, QONLY>(g, g.item_list ? g.item_list[t] : t);
     __syncthreads();  // the next item reuses the workgroup's LDS
   }
 }
@@ -771,8 +827,73 @@ __global__ __launch_bounds__(256) void k_scatter(const dh_contact* __restrict__ 
   out[dst] = c;
 }
 
+// Survivor queue, pass 2: one lane per queued triangle pair (bfSolver, the
+// contact test of bf.cpp:125); flags[e] = 1 and P, Q kept when it emits.
+__global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long live,
+                                                     int* __restrict__ flags,
+                                                     double* __restrict__ pq) {
+  const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= live) return;
+  const int2 en = g.queue[e];
+  if (en.x < 0) {
+    flags[e] = 0;
+    return;
+  }
+  const int k = en.x / g.tiles;
+  const int pa = g.pairs[2 * k], pb = g.pairs[2 * k + 1];
+  const int iA = static_cast<int>(static_cast<unsigned>(en.y) & 0xffffu);
+  const int iB = static_cast<int>(static_cast<unsigned>(en.y) >> 16);
+  V3<double> P, Q;
+  bf_solver(load_tri(g, g.tri_offset[pa] + iA), load_tri(g, g.tri_offset[pb] + iB), P, Q);
+  const bool emit = pq_dist(P, Q) <= g.eps[pa] + g.eps[pb];
+  flags[e] = emit ? 1 : 0;
+  if (emit) {
+    double* o = pq + 6 * e;
+    o[0] = P.x; o[1] = P.y; o[2] = P.z;
+    o[3] = Q.x; o[4] = Q.y; o[5] = Q.z;
+  }
+}
+
+// Pass 3: the queue's contacts take one range of the contact slots.
+__global__ void k_queue_reserve(DetectArgs g, const long long* __restrict__ pos, long long live,
+                                unsigned long long* __restrict__ base) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    *base = atomicAdd(g.counter, static_cast<unsigned long long>(pos[live]));
+}
+
+// Pass 4: contacts of the emitting entries, tagged (item, rank within the
+// item): the item's entries are contiguous and in row-major order, so the
+// rank is the number of emitting entries before it in the item's range.
+__global__ __launch_bounds__(256) void k_queue_emit(DetectArgs g, long long live,
+                                                    const int* __restrict__ flags,
+                                                    const long long* __restrict__ pos,
+                                                    const double* __restrict__ pq,
+                                                    const unsigned long long* __restrict__ base) {
+  const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= live || !flags[e]) return;
+  const int2 en = g.queue[e];
+  const int item = en.x;
+  const int k = item / g.tiles;
+  const Pair p = load_pair(g, k);
+  const int iA = static_cast<int>(static_cast<unsigned>(en.y) & 0xffffu);
+  const int iB = static_cast<int>(static_cast<unsigned>(en.y) >> 16);
+  const double* o = pq + 6 * e;
+  const V3<double> P = {o[0], o[1], o[2]}, Q = {o[3], o[4], o[5]};
+  const unsigned long long slot = *base + static_cast<unsigned long long>(pos[e]);
+  if (slot < static_cast<unsigned long long>(g.cap)) {
+    dh_contact c;
+    make_contact(p, k, iA, iB, P, Q, &c);
+    c.pair = item;
+    c.pad_ = static_cast<int>(pos[e] - pos[g.qbase[item]]);
+    g.temp[slot] = c;
+  }
+  atomicAdd(&g.tile_count[item], 1);
+}
+
 struct DetectScratch {
   DevBuf temp, counter, stats, error, counts, offs, sums, tiles, tstats;
+  DevBuf queue, qcounter, qbase, qflags, qpos, qpq, qres, deferred, ndeferred;
+  long long qcap = 0;  // queue entries of the next eager DH_BF call
 };
 static DetectScratch g_scratch;
 
@@ -808,7 +929,12 @@ static unsigned detect_grid(long long items) {
 
 template <int M>
 static void launch_detect(unsigned items, size_t shm, hipStream_t st, const DetectArgs& a) {
-  hipLaunchKernelGGL(k_detect<M>, detect_grid(items), 256, shm, st, a, static_cast<int>(items));
+  if (M == DH_BF && a.queue)
+    hipLaunchKernelGGL((k_detect<DH_BF, true>), detect_grid(items), 256, shm, st, a,
+                       static_cast<int>(items));
+  else
+    hipLaunchKernelGGL((k_detect<M, false>), detect_grid(items), 256, shm, st, a,
+                       static_cast<int>(items));
 }
 
 static void launch_tile_stats(const unsigned long long* ts, long long ntiles,
@@ -827,6 +953,49 @@ static int tiles_for(long long npairs, int method) {
   const long long target = tuning().detect_tile_target, cap = tuning().detect_max_tiles;
   long long t = (target + npairs - 1) / npairs;
   return static_cast<int>(t < 1 ? 1 : (t > cap ? cap : t));
+}
+
+// Passes 2-4 of the survivor queue after the detection launch: the queued
+// triangle pairs solved one per lane, an exclusive scan of the emitting
+// flags, one reservation of contact slots, the contacts.  Reads the queue
+// length back (one synchronisation); the next call's queue holds this
+// call's need.
+static int solve_queue(const DetectArgs& a, hipStream_t st) {
+  unsigned long long need = 0;
+  unsigned ndef = 0;
+  DH_HIP_CHECK(hipMemcpyAsync(&need, a.qcounter, sizeof(need), hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipMemcpyAsync(&ndef, a.ndeferred, sizeof(ndef), hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipStreamSynchronize(st));
+  if (ndef > 0) {
+    // the items the mid-phase launch could not queue, in full
+    DetectArgs f = a;
+    f.queue = nullptr;
+    f.item_list = a.deferred;
+    hipLaunchKernelGGL((k_detect<DH_BF, false>), detect_grid(ndef), 256, 0, st, f,
+                       static_cast<int>(ndef));
+  }
+  if (static_cast<long long>(need) > a.qcap)
+    g_scratch.qcap = static_cast<long long>(need) + static_cast<long long>(need) / 4;
+  const long long live = static_cast<long long>(need) < a.qcap ? static_cast<long long>(need) : a.qcap;
+  if (live == 0) return DH_OK;
+  int rc;
+  if ((rc = g_scratch.qflags.grow(sizeof(int) * (live + 1)))) return rc;
+  if ((rc = g_scratch.qpos.grow(sizeof(long long) * (live + 1)))) return rc;
+  if ((rc = g_scratch.qpq.grow(sizeof(double) * 6 * live))) return rc;
+  if ((rc = g_scratch.qres.grow(sizeof(unsigned long long)))) return rc;
+  const unsigned grid = grid_for(live, 256);
+  hipLaunchKernelGGL(k_queue_solve, grid, 256, 0, st, a, live, g_scratch.qflags.as<int>(),
+                     g_scratch.qpq.as<double>());
+  if ((rc = scan_counts(g_scratch.qflags.as<int>(), static_cast<int>(live),
+                        g_scratch.qpos.as<long long>(), st)))
+    return rc;
+  hipLaunchKernelGGL(k_queue_reserve, 1, 64, 0, st, a, g_scratch.qpos.as<long long>(), live,
+                     g_scratch.qres.as<unsigned long long>());
+  hipLaunchKernelGGL(k_queue_emit, grid, 256, 0, st, a, live, g_scratch.qflags.as<int>(),
+                     g_scratch.qpos.as<long long>(), g_scratch.qpq.as<double>(),
+                     g_scratch.qres.as<unsigned long long>());
+  DH_HIP_CHECK(hipGetLastError());
+  return DH_OK;
 }
 
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
@@ -873,6 +1042,34 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = nullptr;
   a.pbox = pbox;
+  a.queue = nullptr;
+  a.qcap = 0;
+  a.qcounter = nullptr;
+  a.qbase = nullptr;
+  a.deferred = nullptr;
+  a.ndeferred = nullptr;
+  a.item_list = nullptr;
+  const long long nitems = static_cast<long long>(npairs) * tiles;
+  // the queue costs a read-back and four launches: for many items only
+  if (method == DH_BF && nitems >= tuning().detect_queue_min_items) {
+    // survivor queue: sized from the last call's need, at least 16 entries
+    // per item (an overflowing item solves its own survivors)
+    long long qcap = g_scratch.qcap > 16 * nitems + 65536 ? g_scratch.qcap : 16 * nitems + 65536;
+    if (qcap > (1ll << 30)) qcap = 1ll << 30;
+    if ((rc = g_scratch.queue.grow(sizeof(int2) * qcap))) return rc;
+    if ((rc = g_scratch.qcounter.grow(sizeof(unsigned long long)))) return rc;
+    if ((rc = g_scratch.qbase.grow(sizeof(long long) * (nitems + 1)))) return rc;
+    if ((rc = g_scratch.deferred.grow(sizeof(int) * (nitems + 1)))) return rc;
+    if ((rc = g_scratch.ndeferred.grow(sizeof(unsigned)))) return rc;
+    DH_HIP_CHECK(hipMemsetAsync(g_scratch.qcounter.ptr, 0, sizeof(unsigned long long), st));
+    DH_HIP_CHECK(hipMemsetAsync(g_scratch.ndeferred.ptr, 0, sizeof(unsigned), st));
+    a.deferred = g_scratch.deferred.as<int>();
+    a.ndeferred = g_scratch.ndeferred.as<unsigned>();
+    a.queue = g_scratch.queue.as<int2>();
+    a.qcap = qcap;
+    a.qcounter = g_scratch.qcounter.as<unsigned long long>();
+    a.qbase = g_scratch.qbase.as<long long>();
+  }
   const size_t shm_rows = 2 * kMaxRowTriangles * sizeof(int) + 8 + kMaxRowTriangles * 8;
   const unsigned items = static_cast<unsigned>(static_cast<long long>(npairs) * tiles);
   switch (method) {
@@ -892,6 +1089,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
       return set_error(DH_ERR_UNSUPPORTED, "dh_detect_pairs: method %d", method);
   }
   DH_HIP_CHECK(hipGetLastError());
+  if (a.queue && (rc = solve_queue(a, st))) return rc;
   launch_tile_stats(a.tile_stats, ntiles, a.stats, st);
   unsigned long long total = 0;
   int err = 0;
@@ -974,6 +1172,13 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = np_dev;
   a.pbox = pbox;
+  a.queue = nullptr;  // the queue needs the survivor count on the host
+  a.qcap = 0;
+  a.qcounter = nullptr;
+  a.qbase = nullptr;
+  a.deferred = nullptr;
+  a.ndeferred = nullptr;
+  a.item_list = nullptr;
   const unsigned items = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
   switch (method) {
     case DH_BF: launch_detect<DH_BF>(items, 0, st, a); break;

@@ -19,7 +19,7 @@
 
 namespace dh {
 
-static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, 0, {0, 0}};
+static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, 0, 8192, {0}};
 static dh_tuning g_tuning = kDefaults;
 static int g_cus = 0;
 
@@ -53,6 +53,9 @@ static int check_tuning(const dh_tuning& t) {
   if (t.detect_max_tiles < 1 || t.detect_max_tiles > 64)
     return set_error(DH_ERR_ARG, "dh_tuning: detect_max_tiles %d outside 1..64",
                      t.detect_max_tiles);
+  if (t.detect_queue_min_items < 0 || t.detect_queue_min_items > (1 << 30))
+    return set_error(DH_ERR_ARG, "dh_tuning: detect_queue_min_items %d outside 0..2^30",
+                     t.detect_queue_min_items);
   if (t.detect_wg_per_cu < 0 || t.detect_wg_per_cu > 64)
     return set_error(DH_ERR_ARG, "dh_tuning: detect_wg_per_cu %d outside 0..64",
                      t.detect_wg_per_cu);

@@ -20,7 +20,7 @@ class DhTuning(C.Structure):
     _fields_ = [("learned_wg_per_cu", C.c_int32), ("mixed_wg_per_cu", C.c_int32),
                 ("exact_block", C.c_int32), ("detect_tile_target", C.c_int32),
                 ("detect_max_tiles", C.c_int32), ("detect_wg_per_cu", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("detect_queue_min_items", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class DhTuneTrial(C.Structure):

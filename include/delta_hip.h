@@ -341,7 +341,11 @@ typedef struct dh_tuning {
   int32_t detect_wg_per_cu;   /* particle-pair detection: 0 = one workgroup
                                  per (pair, rows) item; else a persistent grid
                                  of this many workgroups per CU (1..64)      */
-  int32_t reserved[2];
+  int32_t detect_queue_min_items; /* DH_BF detection of at least this many
+                                 items (pairs x tiles) solves the survivors of
+                                 the box test from one queue, one per lane
+                                 (0..2^30; not in captured steps)            */
+  int32_t reserved[1];
 } dh_tuning;
 int dh_tuning_default(dh_tuning* out);
 int dh_tuning_get(dh_tuning* out);

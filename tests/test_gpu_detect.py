@@ -72,6 +72,22 @@ def test_hybrid_triangle_rejects_reference_out_of_bounds():
 def test_batched_pairs_match_per_pair_oracle(method):
     """dh_detect_pairs over several particle pairs: contacts grouped by pair
     in pair order, each group in reference order."""
+    _batched_pairs(method)
+
+
+@pytest.mark.parametrize("max_tiles", [1, 16])
+def test_batched_bf_through_the_survivor_queue(max_tiles):
+    """The same with the survivor queue forced on (every item's survivors
+    solved from one queue, one per lane): bit-exact and in order."""
+    from delta_amd import tuning
+    tuning.set(detect_queue_min_items=0, detect_max_tiles=max_tiles)
+    try:
+        _batched_pairs("bf")
+    finally:
+        tuning.set(**tuning.default())
+
+
+def _batched_pairs(method):
     parts = [scenes.particle(1 + (i % 2), 0.5e-3, 50 + i, [0.98e-3 * i, 0.05e-3 * (i % 3), 0])
              for i in range(6)]
     parts.append(scenes.floor(-0.45e-3))

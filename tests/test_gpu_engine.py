@@ -293,3 +293,26 @@ def test_large_scene_generators_step_bitexact(kind):
     assert total > 0
     if kind == "drum":
         assert hs.rot[0][1] != 0.0  # the box turned
+
+
+def test_steps_through_the_survivor_queue():
+    """dh_step with the survivor queue forced on for every detection: the
+    spinning scene's state bit for bit against the oracle over 4 steps."""
+    from delta_amd import tuning
+    hs = spinning_scene()
+    ds = engine.DeviceScene(hs)
+    k = 1e1
+    p = engine.step_params(method=_lib.DH_BF, dt=1e-6, spring=k, damper=0.1, friction=0.5,
+                           cell=hs.cell_size())
+    tuning.set(detect_queue_min_items=0)
+    try:
+        total = 0
+        for s in range(4):
+            info = engine.step(ds, p)
+            nc, checks, npairs = ol.or_step(hs, _lib.DH_BF, k, 0.1, 0.5, 1e-6, (0.0, -9.81, 0.0))
+            assert (info.contacts, info.candidate_pairs, info.triangle_checks) == (nc, npairs, checks)
+            compare(ds, hs, s)
+            total += nc
+        assert total > 0
+    finally:
+        tuning.set(**tuning.default())

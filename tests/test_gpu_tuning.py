@@ -103,11 +103,15 @@ def test_detect_tilings_identical_and_autotune(scene):
     assert n > 0
     want = _detect(ds, d_pairs, n)
     assert len(want[0]) > 0
-    for target, cap, per_cu in ((1, 1, 0), (1 << 20, 64, 0), (2048, 7, 0), (n * 3, 3, 0),
-                                (1, 1, 1), (n * 3, 3, 2), (1 << 20, 64, 64)):
-        tuning.set(detect_tile_target=target, detect_max_tiles=cap, detect_wg_per_cu=per_cu)
+    for target, cap, per_cu, qmin in ((1, 1, 0, 1 << 30), (1 << 20, 64, 0, 1 << 30),
+                                      (2048, 7, 0, 1 << 30), (n * 3, 3, 0, 1 << 30),
+                                      (1, 1, 1, 1 << 30), (n * 3, 3, 2, 1 << 30),
+                                      (1 << 20, 64, 64, 1 << 30), (1, 1, 0, 0), (2048, 7, 4, 0),
+                                      (1 << 20, 64, 0, 0)):
+        tuning.set(detect_tile_target=target, detect_max_tiles=cap, detect_wg_per_cu=per_cu,
+                   detect_queue_min_items=qmin)
         got = _detect(ds, d_pairs, n)
-        assert got[0] == want[0], (scene, target, cap, per_cu)
+        assert got[0] == want[0], (scene, target, cap, per_cu, qmin)
         assert np.array_equal(got[1], want[1])
         assert got[2] == want[2]
     tuning.set(**tuning.default())
