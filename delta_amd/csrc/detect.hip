@@ -95,6 +95,7 @@ struct DetectArgs {
   int* deferred;
   unsigned* ndeferred;
   const int* item_list;
+  const unsigned* nitems_dev;  // with item_list: its length (else the launch's)
 };
 
 __device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
@@ -625,6 +626,8 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
 // workgroups of one launch.
 template <int M, bool QONLY = false>
 __global__ __launch_bounds__(256) void k_detect(DetectArgs g, int items) {
+  if (g.item_list && g.nitems_dev && static_cast<int>(*g.nitems_dev) < items)
+    items = static_cast<int>(*g.nitems_dev);
   for (int t = blockIdx.x; t < items; t += gridDim.x) {
     detect_item<M, QONLY>(g, g.item_list ? g.item_list[t] : t);
     __syncthreads();  // the next item reuses the workgroup's LDS
@@ -829,12 +832,17 @@ __global__ __launch_bounds__(256) void k_scatter(const dh_contact* __restrict__ 
 
 // Survivor queue, pass 2: one lane per queued triangle pair (bfSolver, the
 // contact test of bf.cpp:125); flags[e] = 1 and P, Q kept when it emits.
-__global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long live,
+// `n` lanes; with need_dev (a captured step) the live entries are
+// min(*need_dev, n) and the lanes past them clear their flags.
+__global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
+                                                     const unsigned long long* __restrict__ need_dev,
                                                      int* __restrict__ flags,
                                                      double* __restrict__ pq) {
   const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (e >= live) return;
-  const int2 en = g.queue[e];
+  if (e >= n) return;
+  const long long live =
+      need_dev && static_cast<long long>(*need_dev) < n ? static_cast<long long>(*need_dev) : n;
+  const int2 en = e < live ? g.queue[e] : make_int2(-1, 0);
   if (en.x < 0) {
     flags[e] = 0;
     return;
@@ -955,6 +963,36 @@ static int tiles_for(long long npairs, int method) {
   return static_cast<int>(t < 1 ? 1 : (t > cap ? cap : t));
 }
 
+// Buffers of the queue passes for n entries.
+static int queue_reserve(long long n) {
+  int rc;
+  if ((rc = g_scratch.qflags.grow(sizeof(int) * (n + 1)))) return rc;
+  if ((rc = g_scratch.qpos.grow(sizeof(long long) * (n + 1)))) return rc;
+  if ((rc = g_scratch.qpq.grow(sizeof(double) * 6 * n))) return rc;
+  if ((rc = g_scratch.qres.grow(sizeof(unsigned long long)))) return rc;
+  return scan_reserve(static_cast<int>(n));
+}
+
+// Solve, scan, reserve and emit over n queue entries (live: all of them, or
+// min(*need_dev, n) in a captured step).
+static int queue_passes(const DetectArgs& a, long long n, const unsigned long long* need_dev,
+                        hipStream_t st) {
+  const unsigned grid = grid_for(n, 256);
+  hipLaunchKernelGGL(k_queue_solve, grid, 256, 0, st, a, n, need_dev, g_scratch.qflags.as<int>(),
+                     g_scratch.qpq.as<double>());
+  int rc;
+  if ((rc = scan_counts(g_scratch.qflags.as<int>(), static_cast<int>(n),
+                        g_scratch.qpos.as<long long>(), st)))
+    return rc;
+  hipLaunchKernelGGL(k_queue_reserve, 1, 64, 0, st, a, g_scratch.qpos.as<long long>(), n,
+                     g_scratch.qres.as<unsigned long long>());
+  hipLaunchKernelGGL(k_queue_emit, grid, 256, 0, st, a, n, g_scratch.qflags.as<int>(),
+                     g_scratch.qpos.as<long long>(), g_scratch.qpq.as<double>(),
+                     g_scratch.qres.as<unsigned long long>());
+  DH_HIP_CHECK(hipGetLastError());
+  return DH_OK;
+}
+
 // Passes 2-4 of the survivor queue after the detection launch: the queued
 // triangle pairs solved one per lane, an exclusive scan of the emitting
 // flags, one reservation of contact slots, the contacts.  Reads the queue
@@ -971,6 +1009,7 @@ static int solve_queue(const DetectArgs& a, hipStream_t st) {
     DetectArgs f = a;
     f.queue = nullptr;
     f.item_list = a.deferred;
+    f.nitems_dev = nullptr;
     hipLaunchKernelGGL((k_detect<DH_BF, false>), detect_grid(ndef), 256, 0, st, f,
                        static_cast<int>(ndef));
   }
@@ -979,23 +1018,8 @@ static int solve_queue(const DetectArgs& a, hipStream_t st) {
   const long long live = static_cast<long long>(need) < a.qcap ? static_cast<long long>(need) : a.qcap;
   if (live == 0) return DH_OK;
   int rc;
-  if ((rc = g_scratch.qflags.grow(sizeof(int) * (live + 1)))) return rc;
-  if ((rc = g_scratch.qpos.grow(sizeof(long long) * (live + 1)))) return rc;
-  if ((rc = g_scratch.qpq.grow(sizeof(double) * 6 * live))) return rc;
-  if ((rc = g_scratch.qres.grow(sizeof(unsigned long long)))) return rc;
-  const unsigned grid = grid_for(live, 256);
-  hipLaunchKernelGGL(k_queue_solve, grid, 256, 0, st, a, live, g_scratch.qflags.as<int>(),
-                     g_scratch.qpq.as<double>());
-  if ((rc = scan_counts(g_scratch.qflags.as<int>(), static_cast<int>(live),
-                        g_scratch.qpos.as<long long>(), st)))
-    return rc;
-  hipLaunchKernelGGL(k_queue_reserve, 1, 64, 0, st, a, g_scratch.qpos.as<long long>(), live,
-                     g_scratch.qres.as<unsigned long long>());
-  hipLaunchKernelGGL(k_queue_emit, grid, 256, 0, st, a, live, g_scratch.qflags.as<int>(),
-                     g_scratch.qpos.as<long long>(), g_scratch.qpq.as<double>(),
-                     g_scratch.qres.as<unsigned long long>());
-  DH_HIP_CHECK(hipGetLastError());
-  return DH_OK;
+  if ((rc = queue_reserve(live))) return rc;
+  return queue_passes(a, live, nullptr, st);
 }
 
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
@@ -1049,6 +1073,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.deferred = nullptr;
   a.ndeferred = nullptr;
   a.item_list = nullptr;
+  a.nitems_dev = nullptr;
   const long long nitems = static_cast<long long>(npairs) * tiles;
   // the queue costs a read-back and four launches: for many items only
   if (method == DH_BF && nitems >= tuning().detect_queue_min_items) {
@@ -1133,10 +1158,18 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   return DH_OK;
 }
 
-int detect_reserve(int pair_cap, long long contact_cap);
+int detect_reserve(int pair_cap, long long contact_cap, int method);
 
 // Tiles of the captured step, sized for half the pair capacity live.
 static int graph_tiles(int pair_cap) { return tiles_for(pair_cap / 2 > 0 ? pair_cap / 2 : 1, DH_BF); }
+
+// Survivor-queue entries of a captured DH_BF step (0: no queue).
+static long long graph_qcap(int method, int pair_cap) {
+  const long long items = static_cast<long long>(pair_cap) * graph_tiles(pair_cap);
+  if (method != DH_BF || items < tuning().detect_queue_min_items) return 0;
+  const long long q = 16 * items + 65536;
+  return q < (1ll << 30) ? q : (1ll << 30);
+}
 
 // Detection for a captured step: launches for `pair_cap` pairs of which
 // *np_dev are live, contacts into `out` (capacity contact_cap), per-pair
@@ -1146,7 +1179,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
                        const long long* np_dev, dh_contact* out, long long contact_cap,
                        int* counts, long long* offs, hipStream_t st, const double* pbox) {
   int rc;
-  if ((rc = detect_reserve(pair_cap, contact_cap))) return rc;
+  if ((rc = detect_reserve(pair_cap, contact_cap, method))) return rc;
   fill_async(g_scratch.counter.as<unsigned long long>(), 0ull, 1, st);
   fill_async(g_scratch.stats.as<unsigned long long>(), 0ull, ST_COUNT, st);
   fill_async(g_scratch.error.as<int>(), 0, 1, st);
@@ -1179,13 +1212,38 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.deferred = nullptr;
   a.ndeferred = nullptr;
   a.item_list = nullptr;
+  a.nitems_dev = nullptr;
   const unsigned items = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
+  const long long qcap = graph_qcap(method, pair_cap);
+  if (qcap > 0) {
+    // the survivor queue with its counts kept on the device: every pass is
+    // launched for the capacity and reads the live length
+    fill_async(g_scratch.qcounter.as<unsigned long long>(), 0ull, 1, st);
+    fill_async(g_scratch.ndeferred.as<unsigned>(), 0u, 1, st);
+    a.queue = g_scratch.queue.as<int2>();
+    a.qcap = qcap;
+    a.qcounter = g_scratch.qcounter.as<unsigned long long>();
+    a.qbase = g_scratch.qbase.as<long long>();
+    a.deferred = g_scratch.deferred.as<int>();
+    a.ndeferred = g_scratch.ndeferred.as<unsigned>();
+  }
   switch (method) {
     case DH_BF: launch_detect<DH_BF>(items, 0, st, a); break;
     case DH_PENALTY: launch_detect<DH_PENALTY>(items, 0, st, a); break;
     default:
       return set_error(DH_ERR_UNSUPPORTED, "captured step: method %d (use DH_BF or DH_PENALTY)",
                        method);
+  }
+  if (a.queue) {
+    // the items the mid-phase launch could not queue, in full (as many as
+    // *ndeferred of the capacity), then the queue's passes
+    DetectArgs f = a;
+    f.queue = nullptr;
+    f.item_list = a.deferred;
+    f.nitems_dev = a.ndeferred;
+    hipLaunchKernelGGL((k_detect<DH_BF, false>), detect_grid(items), 256, 0, st, f,
+                       static_cast<int>(items));
+    if ((rc = queue_passes(a, a.qcap, a.qcounter, st))) return rc;
   }
   launch_tile_stats(a.tile_stats, ntiles, a.stats, st);
   if (tiles > 1)
@@ -1201,8 +1259,18 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
 
 // Buffers of detect_pairs_graph (call before a capture: no allocation may
 // happen inside one).
-int detect_reserve(int pair_cap, long long contact_cap) {
+int detect_reserve(int pair_cap, long long contact_cap, int method) {
   int rc;
+  const long long qcap = graph_qcap(method, pair_cap);
+  if (qcap > 0) {
+    const long long items = static_cast<long long>(pair_cap) * graph_tiles(pair_cap);
+    if ((rc = g_scratch.queue.grow(sizeof(int2) * qcap))) return rc;
+    if ((rc = g_scratch.qcounter.grow(sizeof(unsigned long long)))) return rc;
+    if ((rc = g_scratch.qbase.grow(sizeof(long long) * (items + 1)))) return rc;
+    if ((rc = g_scratch.deferred.grow(sizeof(int) * (items + 1)))) return rc;
+    if ((rc = g_scratch.ndeferred.grow(sizeof(unsigned)))) return rc;
+    if ((rc = queue_reserve(qcap))) return rc;
+  }
   if ((rc = g_scratch.temp.grow(sizeof(dh_contact) * (contact_cap > 0 ? contact_cap : 1)))) return rc;
   if ((rc = g_scratch.counter.grow(sizeof(unsigned long long)))) return rc;
   if ((rc = g_scratch.stats.grow(sizeof(unsigned long long) * ST_COUNT))) return rc;

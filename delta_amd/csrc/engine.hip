@@ -19,7 +19,7 @@ int broadphase_reserve(int n);
 int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, int pair_cap,
                        const long long* np_dev, dh_contact* out, long long contact_cap,
                        int* counts, long long* offs, hipStream_t st, const double* pbox);
-int detect_reserve(int pair_cap, long long contact_cap);
+int detect_reserve(int pair_cap, long long contact_cap, int method);
 const unsigned long long* detect_counter_dev();
 const int* detect_error_dev();
 const unsigned long long* detect_stats_dev();
@@ -239,7 +239,9 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
       (rc = g->torque.grow(sizeof(double) * 3 * (n + 1))) ||
       (rc = g->touched.grow(sizeof(int) * (n + 1))) ||
       (rc = g->status.grow(sizeof(long long) * 6)) || (rc = broadphase_reserve(n)) ||
-      (rc = detect_reserve(static_cast<int>(pair_cap > n ? pair_cap : n), contact_cap)) ||
+      (rc = detect_reserve(static_cast<int>(pair_cap > n ? pair_cap : n), contact_cap,
+                           params->method)) ||
+      (rc = detect_reserve(static_cast<int>(pair_cap), contact_cap, params->method)) ||
       (rc = forces_reserve(n, static_cast<int>(pair_cap)))) {
     destroy_graph(g);
     return rc;

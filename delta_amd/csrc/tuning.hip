@@ -19,7 +19,7 @@
 
 namespace dh {
 
-static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, 0, 8192, {0}};
+static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, 0, 32768, {0}};
 static dh_tuning g_tuning = kDefaults;
 static int g_cus = 0;
 
@@ -232,11 +232,15 @@ int dh_autotune_detect(int method, const dh_geometry* geom, const int* pairs_dev
   if (rc == DH_OK)
     rc = sweep(DH_KNOB_DETECT_WG_PER_CU, &dh_tuning::detect_wg_per_cu, {0, 2, 4, 8, 16, 32},
                run, reps, st, rest(n1 + n2), left(n1 + n2), &n3);
+  int n4 = 0;
+  if (rc == DH_OK)  // survivor queue always / never
+    rc = sweep(DH_KNOB_DETECT_QUEUE_MIN_ITEMS, &dh_tuning::detect_queue_min_items, {0, 1 << 30},
+               run, reps, st, rest(n1 + n2 + n3), left(n1 + n2 + n3), &n4);
   if (rc) {
     g_tuning = saved;
     return rc;
   }
-  if (ntrials) *ntrials = n1 + n2 + n3;
+  if (ntrials) *ntrials = n1 + n2 + n3 + n4;
   return run();
 }
 

@@ -13,7 +13,8 @@ import json
 from . import _lib
 
 KNOBS = {0: "learned_wg_per_cu", 1: "mixed_wg_per_cu", 2: "exact_block",
-         3: "detect_tile_target", 4: "detect_max_tiles", 5: "detect_wg_per_cu"}
+         3: "detect_tile_target", 4: "detect_max_tiles", 5: "detect_wg_per_cu",
+         6: "detect_queue_min_items"}
 
 
 class DhTuning(C.Structure):

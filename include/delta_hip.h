@@ -358,7 +358,8 @@ enum {
   DH_KNOB_EXACT_BLOCK = 2,
   DH_KNOB_DETECT_TILE_TARGET = 3,
   DH_KNOB_DETECT_MAX_TILES = 4,
-  DH_KNOB_DETECT_WG_PER_CU = 5
+  DH_KNOB_DETECT_WG_PER_CU = 5,
+  DH_KNOB_DETECT_QUEUE_MIN_ITEMS = 6
 };
 typedef struct dh_tune_trial {
   int32_t knob;
@@ -378,7 +379,8 @@ int dh_autotune_tri_pairs(int method, const double* soa_dev, const float* soa32_
                           uint8_t* flags_dev, int reps, dh_tune_trial* trials, int trial_cap,
                           int* ntrials, void* stream);
 /* The same for dh_detect_pairs: sweeps detect_tile_target (with the cap at
- * 64), then detect_max_tiles, then detect_wg_per_cu, timing the detection
+ * 64), then detect_max_tiles, then detect_wg_per_cu, then the survivor
+ * queue on (0) or off (2^30) as detect_queue_min_items, timing the detection
  * as dh_step runs it (with the particle boxes of the broad phase).  `cap`
  * must hold every contact. */
 int dh_autotune_detect(int method, const dh_geometry* geom, const int* pairs_dev, int npairs,

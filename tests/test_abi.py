@@ -41,7 +41,7 @@ def test_tuning_table_validates_without_gpu():
     d = tuning.default()
     assert d == {"learned_wg_per_cu": 8, "mixed_wg_per_cu": 8, "exact_block": 256,
                  "detect_tile_target": 4096, "detect_max_tiles": 16, "detect_wg_per_cu": 0,
-                 "detect_queue_min_items": 8192}
+                 "detect_queue_min_items": 32768}
     try:
         assert tuning.set(exact_block=128, learned_wg_per_cu=3)["exact_block"] == 128
         assert tuning.get()["learned_wg_per_cu"] == 3

@@ -106,7 +106,22 @@ def test_c1_lattice_steps():
 def test_step_graph_replays_match_dh_step(method):
     """The captured step (dh_step_graph) replayed 4 times against 4 dh_step
     calls on a second copy of the C1 lattice: every state array bit for bit
-    and the same counts after each step, no overflow."""
+    and the same counts after each step, no overflow.  (With 4,096 pairs of
+    capacity the BF graph runs the survivor queue, the eager steps not.)"""
+    _graph_vs_eager(method)
+
+
+def test_step_graph_and_dh_step_through_the_survivor_queue():
+    """The same with the survivor queue forced on in both."""
+    from delta_amd import tuning
+    tuning.set(detect_queue_min_items=0)
+    try:
+        _graph_vs_eager(_lib.DH_BF)
+    finally:
+        tuning.set(**tuning.default())
+
+
+def _graph_vs_eager(method):
     hs = engine.c1_scene()
     ds_eager = engine.DeviceScene(hs)
     ds_graph = engine.DeviceScene(hs)
