@@ -331,3 +331,39 @@ def test_steps_through_the_survivor_queue():
         assert total > 0
     finally:
         tuning.set(**tuning.default())
+
+
+def test_step_graph_matches_dh_step_on_a_hopper():
+    """A 12^3 hopper lattice in its wall prisms (walls with over 64
+    contact-carrying pairs take the per-particle workgroup sum): 3 replays
+    of the captured step against 3 dh_step calls, every state array bit for
+    bit, with the survivor queue on in both."""
+    from delta_amd import scene_io, tuning
+    hs = scene_io.hopper_scene(eps=1e-3, num=12, sphere_level=1)
+    ds_eager = engine.DeviceScene(hs)
+    ds_graph = engine.DeviceScene(hs)
+    p = engine.step_params(method=_lib.DH_BF, dt=1e-6, gravity=(0.0, 0.0, -10.0), spring=1e6,
+                           damper=0.1, friction=0.0, cell=hs.cell_size())
+    tuning.set(detect_queue_min_items=0)
+    try:
+        info = engine.step(engine.DeviceScene(hs), p)
+        _lib.check(_lib.lib().dh_synchronize(), "sync")
+        g = engine.StepGraph(ds_graph, p, pair_cap=2 * info.candidate_pairs + 1024,
+                             contact_cap=2 * info.contacts + 4096)
+        try:
+            for s in range(3):
+                ie = engine.step(ds_eager, p)
+                _lib.check(_lib.lib().dh_synchronize(), "sync")
+                g.launch()
+                ig, overflow = g.status()
+                assert overflow == 0
+                assert (ig.candidate_pairs, ig.contacts) == (ie.candidate_pairs, ie.contacts)
+                for f in STATE:
+                    a = ds_eager.download(f).reshape(-1).view(np.uint64)
+                    b = ds_graph.download(f).reshape(-1).view(np.uint64)
+                    assert np.array_equal(a, b), f"step {s}: {f}"
+        finally:
+            g.close()
+        assert ie.contacts > 1000
+    finally:
+        tuning.set(**tuning.default())
