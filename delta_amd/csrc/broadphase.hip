@@ -27,28 +27,46 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
                                                       const double* __restrict__ z,
                                                       const int* __restrict__ tri_offset,
                                                       const double* __restrict__ eps,
-                                                      Aabb* __restrict__ box) {
+                                                      Aabb* __restrict__ box,
+                                                      Aabb* __restrict__ cbox) {
   const int i = blockIdx.x;
-  const int v0 = 3 * tri_offset[i], v1 = 3 * tri_offset[i + 1];
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int v = v0 + threadIdx.x; v < v1; v += 64) {
-    const double c[3] = {x[v], y[v], z[v]};
+  const int t0 = tri_offset[i], t1 = tri_offset[i + 1];
+  const bool large = t1 - t0 > kRowBlock;  // keeps a box per block of rows
+  double plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b0 = t0; b0 < t1; b0 += kRowBlock) {
+    const int b1 = b0 + kRowBlock < t1 ? b0 + kRowBlock : t1;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int v = 3 * b0 + threadIdx.x; v < 3 * b1; v += 64) {
+      const double c[3] = {x[v], y[v], z[v]};
+      for (int q = 0; q < 3; ++q) {
+        lo[q] = fmin(lo[q], c[q]);
+        hi[q] = fmax(hi[q], c[q]);
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+      for (int q = 0; q < 3; ++q) {
+        lo[q] = fmin(lo[q], __shfl_xor(lo[q], o));
+        hi[q] = fmax(hi[q], __shfl_xor(hi[q], o));
+      }
+    if (large && cbox && threadIdx.x == 0) {
+      Aabb b;
+      for (int q = 0; q < 3; ++q) {
+        b.lo[q] = lo[q];
+        b.hi[q] = hi[q];
+      }
+      cbox[b0 / 64] = b;
+    }
     for (int q = 0; q < 3; ++q) {
-      lo[q] = fmin(lo[q], c[q]);
-      hi[q] = fmax(hi[q], c[q]);
+      plo[q] = fmin(plo[q], lo[q]);
+      phi[q] = fmax(phi[q], hi[q]);
     }
   }
-  for (int o = 32; o > 0; o >>= 1)
-    for (int q = 0; q < 3; ++q) {
-      lo[q] = fmin(lo[q], __shfl_xor(lo[q], o));
-      hi[q] = fmax(hi[q], __shfl_xor(hi[q], o));
-    }
   if (threadIdx.x == 0) {
     const double m = eps[i] + 0.5e-9;
     Aabb b;
     for (int q = 0; q < 3; ++q) {
-      b.lo[q] = lo[q] - m;
-      b.hi[q] = hi[q] + m;
+      b.lo[q] = plo[q] - m;
+      b.hi[q] = phi[q] + m;
     }
     box[i] = b;
   }
@@ -364,7 +382,7 @@ int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
 int scan_reserve(int n);
 
 struct BpScratch {
-  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst, lcount, keep;
+  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst, lcount, keep, cbox;
 };
 static BpScratch g_bp;
 
@@ -397,6 +415,7 @@ static unsigned table_size(int n) {
 // the vertex box inflated by the margin and 0.5e-9 m), valid until the next
 // call on more particles.
 const double* broadphase_boxes() { return g_bp.box.as<double>(); }
+const double* broadphase_block_boxes() { return g_bp.cbox.as<double>(); }
 
 // The same boxes for the current geometry, without the grid (for callers
 // of the detection that did not run the broad phase).
@@ -404,19 +423,21 @@ int particle_boxes(const dh_geometry* geom, const double** out, hipStream_t st) 
   const int n = geom->nparticles;
   int rc;
   if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
+  if ((rc = g_bp.cbox.grow(sizeof(Aabb) * ((geom->ntri_total + 63) / 64 + 1)))) return rc;
   if (n > 0)
     hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
-                       geom->tri_offset, geom->eps, g_bp.box.as<Aabb>());
+                       geom->tri_offset, geom->eps, g_bp.box.as<Aabb>(), g_bp.cbox.as<Aabb>());
   DH_HIP_CHECK(hipGetLastError());
   *out = g_bp.box.as<double>();
   return DH_OK;
 }
 
 // Buffers of the broad phase for n particles (call before a capture).
-int broadphase_reserve(int n) {
+int broadphase_reserve(int n, int ntri) {
   int rc;
   const unsigned tbl = table_size(n);
   if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
+  if ((rc = g_bp.cbox.grow(sizeof(Aabb) * ((ntri + 63) / 64 + 1)))) return rc;
   if ((rc = g_bp.ccount.grow(sizeof(int) * (tbl + 1)))) return rc;
   if ((rc = g_bp.cfill.grow(sizeof(int) * tbl))) return rc;
   if ((rc = g_bp.cstart.grow(sizeof(long long) * (tbl + 1)))) return rc;
@@ -439,7 +460,7 @@ static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double c
   const int n = geom->nparticles;
   const unsigned tbl = table_size(n);
   hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
-                     geom->eps, g_bp.box.as<Aabb>());
+                     geom->eps, g_bp.box.as<Aabb>(), g_bp.cbox.as<Aabb>());
   g.box = g_bp.box.as<Aabb>();
   g.obstacle = obstacle_dev;
   g.n = n;
@@ -476,7 +497,7 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
                     long long cap, long long* npairs, hipStream_t st) {
   const int n = geom->nparticles;
   int rc;
-  if ((rc = broadphase_reserve(n))) return rc;
+  if ((rc = broadphase_reserve(n, geom->ntri_total))) return rc;
   const unsigned tbl = table_size(n);
   DH_HIP_CHECK(hipMemsetAsync(g_bp.err.ptr, 0, sizeof(int), st));
   DH_HIP_CHECK(hipMemsetAsync(g_bp.ccount.ptr, 0, sizeof(int) * tbl, st));
@@ -504,7 +525,7 @@ int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double ce
                      long long cap, const long long** np_dev, hipStream_t st) {
   const int n = geom->nparticles;
   int rc;
-  if ((rc = broadphase_reserve(n))) return rc;
+  if ((rc = broadphase_reserve(n, geom->ntri_total))) return rc;
   const unsigned tbl = table_size(n);
   fill_async(g_bp.err.as<int>(), 0, 1, st);
   fill_async(g_bp.ccount.as<int>(), 0, static_cast<long long>(tbl), st);

@@ -30,7 +30,7 @@ constexpr int kMaxRowTriangles = 3333;
 // LDS list of DH_BF candidates that pass the box test (per workgroup), and
 // the most triangles per particle whose boxes are staged in LDS.
 constexpr int kSurvivors = 1024;
-constexpr int kBoxTriangles = 320;
+constexpr int kBoxTriangles = kRowBlock;
 
 enum DetectError : int { DE_NONE = 0, DE_REF_UB = 1, DE_TOO_LONG = 2 };
 
@@ -80,6 +80,10 @@ struct DetectArgs {
   // apart from every box of B's triangles, so a block of rows that is all
   // apart is skipped before B's triangles are loaded.
   const double* pbox;
+  // With pbox: the broad phase's boxes of the kRowBlock-row blocks of the
+  // particles with more rows (at bbox[6 * (first triangle / 64)]).  A block
+  // of rows apart from B's particle box is skipped before its rows load.
+  const double* bbox;
   // Survivor queue (null: every item solves its own survivors).  A staged
   // single-block DH_BF item whose survivors all fit in its LDS list appends
   // them {item, iA | iB << 16} as one contiguous range and records its start
@@ -394,6 +398,15 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
       const int rs = r0 + rb;  // first row of the block
       const int nrb = nrows - rb < blk ? nrows - rb : blk;
       int nra = nrb, nca = p.nB;
+      if (staged && g.bbox && g.pbox && p.nA > kRowBlock && rs % kRowBlock == 0) {
+        // the box of A's rows [rs, rs + kRowBlock), which holds this block
+        const double* bb = g.bbox + 6 * ((static_cast<long long>(p.oA) + rs) / 64);
+        const double* pb = g.pbox + 6 * static_cast<long long>(p.pb);
+        if (boxes_apart(bb, bb + 3, pb, pb + 3, p.margin)) {  // block-uniform
+          culled += tid == 0 ? static_cast<unsigned>(nrb * p.nB) : 0u;
+          continue;
+        }
+      }
       if (staged) {
         bool near = false;
         for (int t = tid; t < nrb; t += 256) {
@@ -1025,7 +1038,7 @@ static int solve_queue(const DetectArgs& a, hipStream_t st) {
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
                       dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
                       long long* offs_out, dh_stats* stats, hipStream_t st,
-                      const double* pbox) {
+                      const double* pbox, const double* bbox) {
   int rc;
   const long long temp_cap = cap > 0 ? cap : 1;
   if ((rc = g_scratch.temp.grow(sizeof(dh_contact) * temp_cap))) return rc;
@@ -1066,6 +1079,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = nullptr;
   a.pbox = pbox;
+  a.bbox = bbox;
   a.queue = nullptr;
   a.qcap = 0;
   a.qcounter = nullptr;
@@ -1177,7 +1191,8 @@ static long long graph_qcap(int method, int pair_cap) {
 // dev_counters()[0] = contacts found (may exceed contact_cap), error flag.
 int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, int pair_cap,
                        const long long* np_dev, dh_contact* out, long long contact_cap,
-                       int* counts, long long* offs, hipStream_t st, const double* pbox) {
+                       int* counts, long long* offs, hipStream_t st, const double* pbox,
+                       const double* bbox) {
   int rc;
   if ((rc = detect_reserve(pair_cap, contact_cap, method))) return rc;
   fill_async(g_scratch.counter.as<unsigned long long>(), 0ull, 1, st);
@@ -1205,6 +1220,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = np_dev;
   a.pbox = pbox;
+  a.bbox = bbox;
   a.queue = nullptr;  // the queue needs the survivor count on the host
   a.qcap = 0;
   a.qcounter = nullptr;
@@ -1325,7 +1341,7 @@ int dh_detect_pairs(int method, const dh_geometry* geom, const int* pairs_dev, i
   if (npairs == 0) return DH_OK;
   long long total = 0;
   rc = detect_pairs_impl(method, geom, pairs_dev, npairs, out_dev, cap, &total, pair_count_dev,
-                         nullptr, stats, as_stream(stream), nullptr);
+                         nullptr, stats, as_stream(stream), nullptr, nullptr);
   *n_out = static_cast<int>(total);
   if (stats && (method == DH_HYBRID_BATCH_STAT || method == DH_HYBRID_TRIANGLE_STAT) &&
       (rc == DH_OK || rc == DH_ERR_CAPACITY)) {
@@ -1412,7 +1428,7 @@ int dh_detect_pair(int method, const double* xA, const double* yA, const double*
   if ((rc = outbuf.grow(sizeof(dh_contact) * dcap))) return rc;
   long long total = 0;
   rc = detect_pairs_impl(method, &g, dm->pairs, 1, outbuf.as<dh_contact>(), cap, &total, nullptr,
-                         nullptr, stats, nullptr, nullptr);
+                         nullptr, stats, nullptr, nullptr, nullptr);
   *n_out = static_cast<int>(total);
   if (rc != DH_OK) return rc;
   if (stats && (method == DH_HYBRID_BATCH_STAT || method == DH_HYBRID_TRIANGLE_STAT))

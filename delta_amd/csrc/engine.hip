@@ -10,15 +10,18 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
                     long long cap, long long* npairs, hipStream_t st);
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
                       dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
-                      long long* offs_out, dh_stats* stats, hipStream_t st, const double* pbox);
+                      long long* offs_out, dh_stats* stats, hipStream_t st, const double* pbox,
+                      const double* bbox);
 const double* broadphase_boxes();
+const double* broadphase_block_boxes();
 
 int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
                      long long cap, const long long** np_dev, hipStream_t st);
-int broadphase_reserve(int n);
+int broadphase_reserve(int n, int ntri);
 int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, int pair_cap,
                        const long long* np_dev, dh_contact* out, long long contact_cap,
-                       int* counts, long long* offs, hipStream_t st, const double* pbox);
+                       int* counts, long long* offs, hipStream_t st, const double* pbox,
+                       const double* bbox);
 int detect_reserve(int pair_cap, long long contact_cap, int method);
 const unsigned long long* detect_counter_dev();
 const int* detect_error_dev();
@@ -88,7 +91,7 @@ extern "C" int dh_step(dh_scene* s, const dh_step_params* p, dh_step_info* info,
       rc = detect_pairs_impl(p->method, &s->geom, g_step.pairs.as<int>(), static_cast<int>(np),
                              g_step.contacts.as<dh_contact>(), g_step.contact_cap, &total,
                              g_step.counts.as<int>(), g_step.offs.as<long long>(), &stats, st,
-                             broadphase_boxes());
+                             broadphase_boxes(), broadphase_block_boxes());
       if (rc != DH_ERR_CAPACITY) break;
       g_step.contact_cap = 2 * total;
     }
@@ -178,7 +181,8 @@ static int record_step(dh_step_graph* g, hipStream_t st) {
   if ((rc = detect_pairs_graph(g->params.method, &s->geom, g->pairs.as<int>(),
                                static_cast<int>(g->pair_cap), np_dev,
                                g->contacts.as<dh_contact>(), g->contact_cap, g->counts.as<int>(),
-                               g->offs.as<long long>(), st, broadphase_boxes())))
+                               g->offs.as<long long>(), st, broadphase_boxes(),
+                               broadphase_block_boxes())))
     return rc;
   if ((rc = contacts_forces_impl(&s->bodies, g->pairs.as<int>(), static_cast<int>(g->pair_cap),
                                  np_dev, g->contacts.as<dh_contact>(), g->contact_cap,
@@ -238,7 +242,7 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
       (rc = g->force.grow(sizeof(double) * 3 * (n + 1))) ||
       (rc = g->torque.grow(sizeof(double) * 3 * (n + 1))) ||
       (rc = g->touched.grow(sizeof(int) * (n + 1))) ||
-      (rc = g->status.grow(sizeof(long long) * 6)) || (rc = broadphase_reserve(n)) ||
+      (rc = g->status.grow(sizeof(long long) * 6)) || (rc = broadphase_reserve(n, scene->geom.ntri_total)) ||
       (rc = detect_reserve(static_cast<int>(pair_cap > n ? pair_cap : n), contact_cap,
                            params->method)) ||
       (rc = detect_reserve(static_cast<int>(pair_cap), contact_cap, params->method)) ||

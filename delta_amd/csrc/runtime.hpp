@@ -46,6 +46,11 @@ const dh_tuning& tuning();
 // Compute units of the current device (256 on MI355X).
 int device_cus();
 
+// Rows (triangles of A) per block that detection stages in LDS.  The
+// broad phase's vertex pass keeps a box per block of a particle with more
+// triangles than this, at cbox[(first triangle of the block) / 64].
+constexpr int kRowBlock = 320;
+
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 inline unsigned grid_for(int64_t n, int block) {
