@@ -31,10 +31,14 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
                                                       Aabb* __restrict__ cbox) {
   const int i = blockIdx.x;
   const int t0 = tri_offset[i], t1 = tri_offset[i + 1];
-  const bool large = t1 - t0 > kRowBlock;  // keeps a box per block of rows
+  // a particle with more triangles than one row block keeps the box of each
+  // chunk of 64 of its triangles, chunk c at cbox[t0 / 64 + i + c] (an index
+  // that grows with i and c, so no two chunks share it)
+  const bool chunked = cbox && t1 - t0 > kRowBlock;
+  const int step = chunked ? 64 : (t1 - t0 > 0 ? t1 - t0 : 1);
   double plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b0 = t0; b0 < t1; b0 += kRowBlock) {
-    const int b1 = b0 + kRowBlock < t1 ? b0 + kRowBlock : t1;
+  for (int b0 = t0; b0 < t1; b0 += step) {
+    const int b1 = b0 + step < t1 ? b0 + step : t1;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int v = 3 * b0 + threadIdx.x; v < 3 * b1; v += 64) {
       const double c[3] = {x[v], y[v], z[v]};
@@ -48,13 +52,13 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
         lo[q] = fmin(lo[q], __shfl_xor(lo[q], o));
         hi[q] = fmax(hi[q], __shfl_xor(hi[q], o));
       }
-    if (large && cbox && threadIdx.x == 0) {
+    if (chunked && threadIdx.x == 0) {
       Aabb b;
       for (int q = 0; q < 3; ++q) {
         b.lo[q] = lo[q];
         b.hi[q] = hi[q];
       }
-      cbox[b0 / 64] = b;
+      cbox[t0 / 64 + i + (b0 - t0) / 64] = b;
     }
     for (int q = 0; q < 3; ++q) {
       plo[q] = fmin(plo[q], lo[q]);
@@ -423,7 +427,7 @@ int particle_boxes(const dh_geometry* geom, const double** out, hipStream_t st) 
   const int n = geom->nparticles;
   int rc;
   if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
-  if ((rc = g_bp.cbox.grow(sizeof(Aabb) * ((geom->ntri_total + 63) / 64 + 1)))) return rc;
+  if ((rc = g_bp.cbox.grow(sizeof(Aabb) * ((geom->ntri_total + 63) / 64 + n + 1)))) return rc;
   if (n > 0)
     hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
                        geom->tri_offset, geom->eps, g_bp.box.as<Aabb>(), g_bp.cbox.as<Aabb>());
@@ -437,7 +441,7 @@ int broadphase_reserve(int n, int ntri) {
   int rc;
   const unsigned tbl = table_size(n);
   if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
-  if ((rc = g_bp.cbox.grow(sizeof(Aabb) * ((ntri + 63) / 64 + 1)))) return rc;
+  if ((rc = g_bp.cbox.grow(sizeof(Aabb) * ((ntri + 63) / 64 + n + 1)))) return rc;
   if ((rc = g_bp.ccount.grow(sizeof(int) * (tbl + 1)))) return rc;
   if ((rc = g_bp.cfill.grow(sizeof(int) * tbl))) return rc;
   if ((rc = g_bp.cstart.grow(sizeof(long long) * (tbl + 1)))) return rc;

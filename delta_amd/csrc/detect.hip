@@ -80,9 +80,11 @@ struct DetectArgs {
   // apart from every box of B's triangles, so a block of rows that is all
   // apart is skipped before B's triangles are loaded.
   const double* pbox;
-  // With pbox: the broad phase's boxes of the kRowBlock-row blocks of the
-  // particles with more rows (at bbox[6 * (first triangle / 64)]).  A block
-  // of rows apart from B's particle box is skipped before its rows load.
+  // With pbox: the broad phase's boxes of the 64-triangle chunks of the
+  // particles with more than kRowBlock triangles (chunk c of particle i at
+  // bbox[6 * (tri_offset[i] / 64 + i + c)]).  Rows of a chunk apart from
+  // B's particle box are not loaded; a block without a near chunk is
+  // skipped.
   const double* bbox;
   // Survivor queue (null: every item solves its own survivors).  A staged
   // single-block DH_BF item whose survivors all fit in its LDS list appends
@@ -382,6 +384,7 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
     __shared__ double uni[2][6];            // union of the block's row boxes, of B's boxes
     __shared__ double red[4][12];
     __shared__ int nact[2];
+    __shared__ unsigned char row_far[kBoxTriangles];  // row in a chunk apart from B
     const bool staged = p.nB <= kBoxTriangles;  // block-uniform
     const int blk = staged ? kBoxTriangles : (nrows > 0 ? nrows : 1);
     bool b_staged = false;  // B's boxes are loaded on the first block that needs them
@@ -398,11 +401,18 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
       const int rs = r0 + rb;  // first row of the block
       const int nrb = nrows - rb < blk ? nrows - rb : blk;
       int nra = nrb, nca = p.nB;
-      if (staged && g.bbox && g.pbox && p.nA > kRowBlock && rs % kRowBlock == 0) {
-        // the box of A's rows [rs, rs + kRowBlock), which holds this block
-        const double* bb = g.bbox + 6 * ((static_cast<long long>(p.oA) + rs) / 64);
-        const double* pb = g.pbox + 6 * static_cast<long long>(p.pb);
-        if (boxes_apart(bb, bb + 3, pb, pb + 3, p.margin)) {  // block-uniform
+      const bool chunked = staged && g.bbox && g.pbox && p.nA > kRowBlock;
+      const double* pbB = g.pbox ? g.pbox + 6 * static_cast<long long>(p.pb) : nullptr;
+      // row r of A (particle-relative) lies in a chunk apart from B's box
+      auto chunk_far = [&](int r) {
+        const double* cb = g.bbox + 6 * (static_cast<long long>(p.oA / 64) + p.pa + r / 64);
+        return boxes_apart(cb, cb + 3, pbB, pbB + 3, p.margin);
+      };
+      if (chunked) {
+        bool any_near = false;
+        for (int c = rs / 64 + tid; c <= (rs + nrb - 1) / 64; c += 256)
+          any_near = any_near || !chunk_far(64 * c);
+        if (!__syncthreads_or(any_near)) {  // every chunk of the block is apart
           culled += tid == 0 ? static_cast<unsigned>(nrb * p.nB) : 0u;
           continue;
         }
@@ -410,12 +420,12 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
       if (staged) {
         bool near = false;
         for (int t = tid; t < nrb; t += 256) {
+          const bool far = chunked && chunk_far(rs + t);
+          row_far[t] = far ? 1 : 0;
+          if (far) continue;  // its box is never read
           double* b = boxes + 6 * t;
           tri_box(load_tri(g, p.oA + rs + t), b, b + 3);
-          if (g.pbox) {
-            const double* pb = g.pbox + 6 * static_cast<long long>(p.pb);
-            near = near || !boxes_apart(b, b + 3, pb, pb + 3, p.margin);
-          }
+          if (g.pbox) near = near || !boxes_apart(b, b + 3, pbB, pbB + 3, p.margin);
         }
         if (g.pbox && !__syncthreads_or(near)) {
           // every row of the block misses B's particle box
@@ -437,10 +447,11 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
           u[3 + q] = u[9 + q] = -INFINITY;
         }
         for (int t = tid; t < nrb; t += 256)
-          for (int q = 0; q < 3; ++q) {
-            u[q] = fmin(u[q], boxes[6 * t + q]);
-            u[3 + q] = fmax(u[3 + q], boxes[6 * t + 3 + q]);
-          }
+          if (!row_far[t])
+            for (int q = 0; q < 3; ++q) {
+              u[q] = fmin(u[q], boxes[6 * t + q]);
+              u[3 + q] = fmax(u[3 + q], boxes[6 * t + 3 + q]);
+            }
         for (int t = tid; t < p.nB; t += 256)
           for (int q = 0; q < 3; ++q) {
             u[6 + q] = fmin(u[6 + q], boxes[6 * (kBoxTriangles + t) + q]);
@@ -468,7 +479,7 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
           for (int t0 = 0; t0 < cnt_all; t0 += 256) {
             const int t = t0 + tid;
             bool on = false;
-            if (t < cnt_all) {
+            if (t < cnt_all && !(side == 0 && row_far[t])) {
               const double* b = boxes + 6 * (side == 0 ? t : kBoxTriangles + t);
               on = !boxes_apart(b, b + 3, other, other + 3, p.margin);
             }

@@ -47,8 +47,9 @@ const dh_tuning& tuning();
 int device_cus();
 
 // Rows (triangles of A) per block that detection stages in LDS.  The
-// broad phase's vertex pass keeps a box per block of a particle with more
-// triangles than this, at cbox[(first triangle of the block) / 64].
+// broad phase's vertex pass keeps, for a particle i with more triangles than
+// this, the box of each chunk c of 64 of its triangles at
+// cbox[tri_offset[i] / 64 + i + c].
 constexpr int kRowBlock = 320;
 
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
