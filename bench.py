@@ -404,6 +404,26 @@ def hopper_hbm(hs, ds, prm, L, reps=20):
             "note": "binning time includes the grid passes and the pair-count read-back"}
 
 
+def tune_detection(ds, hs, prm, L, contact_cap):
+    """Detection launch parameters autotuned on the scene's current
+    candidate pairs (the broad phase run once)."""
+    from delta_amd import _lib, tuning
+    from delta_amd.device import DeviceArray
+    cap = 16 * (hs.n + 16)
+    d_pairs = DeviceArray((2 * cap,), np.int32)
+    npairs = C.c_int(0)
+    L.dh_broadphase.argtypes = [C.POINTER(_lib.DhGeometry), C.c_void_p, C.c_double, C.c_void_p,
+                                C.c_int, C.POINTER(C.c_int), C.c_void_p]
+    _lib.check(L.dh_broadphase(C.byref(ds.scene.geom), ds.buf["obstacle"].ptr, prm.cell,
+                               d_pairs.ptr, cap, C.byref(npairs), None), "dh_broadphase")
+    out_c = DeviceArray((contact_cap * 144,), np.uint8)
+    trials = tuning.autotune_detect(ds.scene.geom, d_pairs.ptr, npairs.value, out_c.ptr,
+                                    contact_cap, reps=2)
+    table = tuning.get()
+    print(f"[bench] tuning {table}", file=sys.stderr, flush=True)
+    return table, [(t["knob"], t["value"], round(t["ms"], 4)) for t in trials]
+
+
 def dem_hopper(a):
     """The hopper of input/hopper.py (also input/hopper.mbfcp): 25^3 touching
     spheres of radius 0.02 as 80-triangle icospheres, 26 wall prisms,
@@ -423,6 +443,7 @@ def dem_hopper(a):
         engine.step(ds, prm)
     _lib.check(L.dh_synchronize(), "sync")
     print("[bench] hopper warm", file=sys.stderr, flush=True)
+    table, trials = tune_detection(ds, hs, prm, L, 4_000_000)
     checks = contacts = pairs = culled = 0
     max_pairs = max_contacts = 0
     t0 = time.perf_counter()
@@ -463,6 +484,7 @@ def dem_hopper(a):
                       "particles": hs.n, "triangles": int(hs.tri_offset[-1])},
            "triangle_checks_per_s": checks / gdt, "culled_fraction": culled / max(checks, 1),
            "mean_candidate_pairs": pairs / a.steps, "mean_contacts": contacts / a.steps,
+           "tuning": {"table": table, "trials_value_ms": trials},
            "hbm": hbm}
     if not a.no_cpu:
         # the oracle's brute-force detection on the CPU over a sample of the

@@ -102,6 +102,7 @@ struct DetectArgs {
   unsigned* ndeferred;
   const int* item_list;
   const unsigned* nitems_dev;  // with item_list: its length (else the launch's)
+  int fp32_screen;             // k_queue_solve screens in binary32 first
 };
 
 __device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
@@ -875,9 +876,33 @@ __global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
   const int pa = g.pairs[2 * k], pb = g.pairs[2 * k + 1];
   const int iA = static_cast<int>(static_cast<unsigned>(en.y) & 0xffffu);
   const int iB = static_cast<int>(static_cast<unsigned>(en.y) >> 16);
+  const Tri<double> A = load_tri(g, g.tri_offset[pa] + iA);
+  const Tri<double> B = load_tri(g, g.tri_offset[pb] + iB);
+  const double margin = g.eps[pa] + g.eps[pb];
+  if (g.fp32_screen) {
+    // binary32 brute force relative to A's first vertex: a pair at twice
+    // the margin or more there is no contact (binary32's error at these
+    // scales is far below the margin); the rest are decided in binary64
+    const V3<double> o = A.v[0];
+    Tri<float> a32, b32;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      a32.v[k] = {static_cast<float>(A.v[k].x - o.x), static_cast<float>(A.v[k].y - o.y),
+                  static_cast<float>(A.v[k].z - o.z)};
+      b32.v[k] = {static_cast<float>(B.v[k].x - o.x), static_cast<float>(B.v[k].y - o.y),
+                  static_cast<float>(B.v[k].z - o.z)};
+    }
+    V3<float> p32, q32;
+    bf_solver(a32, b32, p32, q32);
+    const float d32 = pq_dist(p32, q32);
+    if (d32 >= static_cast<float>(2.0 * margin) && d32 <= 3.0e38f) {
+      flags[e] = 0;
+      return;
+    }
+  }
   V3<double> P, Q;
-  bf_solver(load_tri(g, g.tri_offset[pa] + iA), load_tri(g, g.tri_offset[pb] + iB), P, Q);
-  const bool emit = pq_dist(P, Q) <= g.eps[pa] + g.eps[pb];
+  bf_solver(A, B, P, Q);
+  const bool emit = pq_dist(P, Q) <= margin;
   flags[e] = emit ? 1 : 0;
   if (emit) {
     double* o = pq + 6 * e;
@@ -1090,6 +1115,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = nullptr;
   a.pbox = pbox;
+  a.fp32_screen = tuning().detect_fp32_screen;
   a.bbox = bbox;
   a.queue = nullptr;
   a.qcap = 0;
@@ -1231,6 +1257,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = np_dev;
   a.pbox = pbox;
+  a.fp32_screen = tuning().detect_fp32_screen;
   a.bbox = bbox;
   a.queue = nullptr;  // the queue needs the survivor count on the host
   a.qcap = 0;

@@ -19,7 +19,7 @@
 
 namespace dh {
 
-static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, 0, 32768, {0}};
+static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, 0, 32768, 0};
 static dh_tuning g_tuning = kDefaults;
 static int g_cus = 0;
 
@@ -56,6 +56,9 @@ static int check_tuning(const dh_tuning& t) {
   if (t.detect_queue_min_items < 0 || t.detect_queue_min_items > (1 << 30))
     return set_error(DH_ERR_ARG, "dh_tuning: detect_queue_min_items %d outside 0..2^30",
                      t.detect_queue_min_items);
+  if (t.detect_fp32_screen != 0 && t.detect_fp32_screen != 1)
+    return set_error(DH_ERR_ARG, "dh_tuning: detect_fp32_screen %d is not 0 or 1",
+                     t.detect_fp32_screen);
   if (t.detect_wg_per_cu < 0 || t.detect_wg_per_cu > 64)
     return set_error(DH_ERR_ARG, "dh_tuning: detect_wg_per_cu %d outside 0..64",
                      t.detect_wg_per_cu);
@@ -162,8 +165,7 @@ int dh_tuning_get(dh_tuning* out) {
 
 int dh_tuning_set(const dh_tuning* t) {
   if (!t) return set_error(DH_ERR_ARG, "dh_tuning_set: null");
-  dh_tuning v = *t;
-  std::memset(v.reserved, 0, sizeof(v.reserved));
+  const dh_tuning v = *t;
   const int rc = check_tuning(v);
   if (rc) return rc;
   g_tuning = v;
@@ -234,15 +236,18 @@ int dh_autotune_detect(int method, const dh_geometry* geom, const int* pairs_dev
   if (rc == DH_OK)
     rc = sweep(DH_KNOB_DETECT_WG_PER_CU, &dh_tuning::detect_wg_per_cu, {0, 2, 4, 8, 16, 32},
                run, reps, st, rest(n1 + n2), left(n1 + n2), &n3);
-  int n4 = 0;
+  int n4 = 0, n5 = 0;
   if (rc == DH_OK)  // survivor queue always / never
     rc = sweep(DH_KNOB_DETECT_QUEUE_MIN_ITEMS, &dh_tuning::detect_queue_min_items, {0, 1 << 30},
                run, reps, st, rest(n1 + n2 + n3), left(n1 + n2 + n3), &n4);
+  if (rc == DH_OK)  // the queue's binary32 screen off / on
+    rc = sweep(DH_KNOB_DETECT_FP32_SCREEN, &dh_tuning::detect_fp32_screen, {0, 1}, run, reps, st,
+               rest(n1 + n2 + n3 + n4), left(n1 + n2 + n3 + n4), &n5);
   if (rc) {
     g_tuning = saved;
     return rc;
   }
-  if (ntrials) *ntrials = n1 + n2 + n3 + n4;
+  if (ntrials) *ntrials = n1 + n2 + n3 + n4 + n5;
   return run();
 }
 

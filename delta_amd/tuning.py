@@ -14,14 +14,14 @@ from . import _lib
 
 KNOBS = {0: "learned_wg_per_cu", 1: "mixed_wg_per_cu", 2: "exact_block",
          3: "detect_tile_target", 4: "detect_max_tiles", 5: "detect_wg_per_cu",
-         6: "detect_queue_min_items"}
+         6: "detect_queue_min_items", 7: "detect_fp32_screen"}
 
 
 class DhTuning(C.Structure):
     _fields_ = [("learned_wg_per_cu", C.c_int32), ("mixed_wg_per_cu", C.c_int32),
                 ("exact_block", C.c_int32), ("detect_tile_target", C.c_int32),
                 ("detect_max_tiles", C.c_int32), ("detect_wg_per_cu", C.c_int32),
-                ("detect_queue_min_items", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("detect_queue_min_items", C.c_int32), ("detect_fp32_screen", C.c_int32)]
 
 
 class DhTuneTrial(C.Structure):
@@ -54,7 +54,7 @@ def _L():
 
 
 def _as_dict(t):
-    return {name: int(getattr(t, name)) for name, _ in DhTuning._fields_ if name != "reserved"}
+    return {name: int(getattr(t, name)) for name, _ in DhTuning._fields_}
 
 
 def default():

@@ -344,8 +344,13 @@ typedef struct dh_tuning {
   int32_t detect_queue_min_items; /* DH_BF detection of at least this many
                                  items (pairs x tiles) solves the survivors of
                                  the box test from one queue, one per lane
-                                 (0..2^30; not in captured steps)            */
-  int32_t reserved[1];
+                                 (0..2^30)                                   */
+  int32_t detect_fp32_screen; /* 1: queued survivors are first solved in
+                                 binary32 (coordinates relative to A's first
+                                 vertex); those at twice the margin or more
+                                 are dropped, the rest solved in binary64, so
+                                 every contact is still decided and computed
+                                 in binary64 (0 or 1)                        */
 } dh_tuning;
 int dh_tuning_default(dh_tuning* out);
 int dh_tuning_get(dh_tuning* out);
@@ -359,7 +364,8 @@ enum {
   DH_KNOB_DETECT_TILE_TARGET = 3,
   DH_KNOB_DETECT_MAX_TILES = 4,
   DH_KNOB_DETECT_WG_PER_CU = 5,
-  DH_KNOB_DETECT_QUEUE_MIN_ITEMS = 6
+  DH_KNOB_DETECT_QUEUE_MIN_ITEMS = 6,
+  DH_KNOB_DETECT_FP32_SCREEN = 7
 };
 typedef struct dh_tune_trial {
   int32_t knob;
@@ -380,7 +386,8 @@ int dh_autotune_tri_pairs(int method, const double* soa_dev, const float* soa32_
                           int* ntrials, void* stream);
 /* The same for dh_detect_pairs: sweeps detect_tile_target (with the cap at
  * 64), then detect_max_tiles, then detect_wg_per_cu, then the survivor
- * queue on (0) or off (2^30) as detect_queue_min_items, timing the detection
+ * queue on (0) or off (2^30) as detect_queue_min_items, then its binary32
+ * screen (detect_fp32_screen 0, 1), timing the detection
  * as dh_step runs it (with the particle boxes of the broad phase).  `cap`
  * must hold every contact. */
 int dh_autotune_detect(int method, const dh_geometry* geom, const int* pairs_dev, int npairs,

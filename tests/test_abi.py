@@ -41,14 +41,15 @@ def test_tuning_table_validates_without_gpu():
     d = tuning.default()
     assert d == {"learned_wg_per_cu": 8, "mixed_wg_per_cu": 8, "exact_block": 256,
                  "detect_tile_target": 4096, "detect_max_tiles": 16, "detect_wg_per_cu": 0,
-                 "detect_queue_min_items": 32768}
+                 "detect_queue_min_items": 32768, "detect_fp32_screen": 0}
     try:
         assert tuning.set(exact_block=128, learned_wg_per_cu=3)["exact_block"] == 128
         assert tuning.get()["learned_wg_per_cu"] == 3
         for bad in ({"exact_block": 100}, {"exact_block": 320}, {"mixed_wg_per_cu": 0},
                     {"learned_wg_per_cu": 33}, {"detect_max_tiles": 65},
                     {"detect_tile_target": 0}, {"detect_wg_per_cu": -1},
-                    {"detect_wg_per_cu": 65}, {"detect_queue_min_items": -1}):
+                    {"detect_wg_per_cu": 65}, {"detect_queue_min_items": -1},
+                    {"detect_fp32_screen": 2}):
             with pytest.raises(_lib.DhError):
                 tuning.set(**bad)
         assert tuning.get()["exact_block"] == 128  # a rejected set changes nothing

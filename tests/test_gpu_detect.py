@@ -75,12 +75,14 @@ def test_batched_pairs_match_per_pair_oracle(method):
     _batched_pairs(method)
 
 
+@pytest.mark.parametrize("screen", [0, 1])
 @pytest.mark.parametrize("max_tiles", [1, 16])
-def test_batched_bf_through_the_survivor_queue(max_tiles):
+def test_batched_bf_through_the_survivor_queue(max_tiles, screen):
     """The same with the survivor queue forced on (every item's survivors
-    solved from one queue, one per lane): bit-exact and in order."""
+    solved from one queue, one per lane), with and without the binary32
+    screen: bit-exact and in order."""
     from delta_amd import tuning
-    tuning.set(detect_queue_min_items=0, detect_max_tiles=max_tiles)
+    tuning.set(detect_queue_min_items=0, detect_max_tiles=max_tiles, detect_fp32_screen=screen)
     try:
         _batched_pairs("bf")
     finally:

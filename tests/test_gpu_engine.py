@@ -310,16 +310,18 @@ def test_large_scene_generators_step_bitexact(kind):
         assert hs.rot[0][1] != 0.0  # the box turned
 
 
-def test_steps_through_the_survivor_queue():
-    """dh_step with the survivor queue forced on for every detection: the
-    spinning scene's state bit for bit against the oracle over 4 steps."""
+@pytest.mark.parametrize("screen", [0, 1])
+def test_steps_through_the_survivor_queue(screen):
+    """dh_step with the survivor queue forced on for every detection, with
+    and without its binary32 screen: the spinning scene's state bit for bit
+    against the oracle over 4 steps."""
     from delta_amd import tuning
     hs = spinning_scene()
     ds = engine.DeviceScene(hs)
     k = 1e1
     p = engine.step_params(method=_lib.DH_BF, dt=1e-6, spring=k, damper=0.1, friction=0.5,
                            cell=hs.cell_size())
-    tuning.set(detect_queue_min_items=0)
+    tuning.set(detect_queue_min_items=0, detect_fp32_screen=screen)
     try:
         total = 0
         for s in range(4):
@@ -337,14 +339,14 @@ def test_step_graph_matches_dh_step_on_a_hopper():
     """A 12^3 hopper lattice in its wall prisms (walls with over 64
     contact-carrying pairs take the per-particle workgroup sum): 3 replays
     of the captured step against 3 dh_step calls, every state array bit for
-    bit, with the survivor queue on in both."""
+    bit, with the survivor queue and its binary32 screen on in both."""
     from delta_amd import scene_io, tuning
     hs = scene_io.hopper_scene(eps=1e-3, num=12, sphere_level=1)
     ds_eager = engine.DeviceScene(hs)
     ds_graph = engine.DeviceScene(hs)
     p = engine.step_params(method=_lib.DH_BF, dt=1e-6, gravity=(0.0, 0.0, -10.0), spring=1e6,
                            damper=0.1, friction=0.0, cell=hs.cell_size())
-    tuning.set(detect_queue_min_items=0)
+    tuning.set(detect_queue_min_items=0, detect_fp32_screen=1)
     try:
         info = engine.step(engine.DeviceScene(hs), p)
         _lib.check(_lib.lib().dh_synchronize(), "sync")

@@ -120,12 +120,13 @@ def test_detect_tilings_identical_and_autotune(scene):
     trials = tuning.autotune_detect(ds.scene.geom, d_pairs.ptr, n, out.ptr, cap, reps=2)
     assert [t["knob"] for t in trials] == (["detect_tile_target"] * 6 + ["detect_max_tiles"] * 5
                                            + ["detect_wg_per_cu"] * 6
-                                           + ["detect_queue_min_items"] * 2)
+                                           + ["detect_queue_min_items"] * 2
+                                           + ["detect_fp32_screen"] * 2)
     t = tuning.get()
     best = [min(trials[a:b], key=lambda r: r["ms"])["value"]
-            for a, b in ((0, 6), (6, 11), (11, 17), (17, 19))]
+            for a, b in ((0, 6), (6, 11), (11, 17), (17, 19), (19, 21))]
     assert [t["detect_tile_target"], t["detect_max_tiles"], t["detect_wg_per_cu"],
-            t["detect_queue_min_items"]] == best
+            t["detect_queue_min_items"], t["detect_fp32_screen"]] == best
     assert out.download().tobytes()[:len(want[0])] == want[0]
     assert _detect(ds, d_pairs, n)[0] == want[0]
 
