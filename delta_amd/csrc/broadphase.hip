@@ -300,6 +300,77 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_large_fill(GridArgs g,
   }
 }
 
+// grid_partners for two particles per wave: half h = lane / 32 searches
+// particle i (each half's i, bi and `found` are its own); the same partner
+// set as grid_partners, in another discovery order.
+template <typename F>
+__device__ int grid_partners_half(const GridArgs& g, int i, const Aabb& bi, bool active, int lane,
+                                  F put) {
+  const int h = lane >> 5, sl = lane & 31;
+  long long c[3];
+  cell_of(bi, g.inv_h, c);
+  const int dx = sl / 9 - 1, dy = (sl / 3) % 3 - 1, dz = sl % 3 - 1;
+  const unsigned hsh = (active && sl < 27) ? hash_cell(c[0] + dx, c[1] + dy, c[2] + dz, g.mask) : 0u;
+  bool walk = active && sl < 27;
+  for (int l = 0; l < 26; ++l) {
+    const unsigned o = __shfl(hsh, (h << 5) + l);
+    if (l < sl && o == hsh) walk = false;
+  }
+  long long pos = walk ? g.cstart[hsh] : 0, end = walk ? g.cstart[hsh + 1] : 0;
+  const unsigned below = (1u << sl) - 1u;
+  int found = 0;
+  while (__ballot(pos < end)) {
+    bool e = false;
+    int cand = -1;
+    if (pos < end) {
+      cand = g.sorted[pos];
+      if (cand > i) {
+        const Aabb bj = g.box[cand];
+        long long cj[3];
+        cell_of(bj, g.inv_h, cj);
+        e = cj[0] - c[0] >= -1 && cj[0] - c[0] <= 1 && cj[1] - c[1] >= -1 && cj[1] - c[1] <= 1 &&
+            cj[2] - c[2] >= -1 && cj[2] - c[2] <= 1 && overlap(bi, bj);
+      }
+      ++pos;
+    }
+    const unsigned hm = static_cast<unsigned>(__ballot(e) >> (32 * h));
+    if (e) put(found + __popc(hm & below), cand);
+    found += __popc(hm);
+  }
+  const int nl = *g.nlarge;
+  for (int o0 = 0; o0 < nl; o0 += 32) {
+    const int o = o0 + sl;
+    bool e = false;
+    int cand = -1;
+    if (active && o < nl) {
+      cand = g.large[o];
+      e = cand > i && overlap(bi, g.box[cand]);
+    }
+    const unsigned hm = static_cast<unsigned>(__ballot(e) >> (32 * h));
+    if (e) put(found + __popc(hm & below), cand);
+    found += __popc(hm);
+  }
+  return found;
+}
+
+// Partner counts, two particles per wave (the walk is latency bound: two
+// searches share each round of dependent loads).
+__global__ __launch_bounds__(64 * kBpWaves) void k_grid_count2(GridArgs g,
+                                                                int* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  const int i = (blockIdx.x * kBpWaves + (threadIdx.x >> 6)) * 2 + (lane >> 5);
+  const bool valid = i < g.n;
+  const Aabb bi = g.box[valid ? i : 0];
+  const bool active = valid && !is_large(g, i, bi);
+  int* kept = g.keep + static_cast<long long>(valid ? i : 0) * kBpKeep;
+  auto keep = [&](int pos, int j) {
+    if (pos < kBpKeep) kept[pos] = j;
+  };
+  const int m = grid_partners_half(g, i, bi, active, lane, keep);
+  // large particles: k_large_count / k_large_sum, launched after this
+  if (valid && (lane & 31) == 0) count[i] = active ? m : 0;
+}
+
 __global__ __launch_bounds__(64 * kBpWaves) void k_grid_count(GridArgs g,
                                                                int* __restrict__ count) {
   const int lane = threadIdx.x & 63;
@@ -488,7 +559,7 @@ static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double c
                         st)))
     return rc;
   hipLaunchKernelGGL(k_grid_scatter, grid_for(n, 256), 256, 0, st, g);
-  hipLaunchKernelGGL(k_grid_count, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
+  hipLaunchKernelGGL(k_grid_count2, grid_for(n, 2 * kBpWaves), 64 * kBpWaves, 0, st, g,
                      g_bp.count.as<int>());
   const long long items = static_cast<long long>(n) * g.nchunks;  // bound on the device count
   hipLaunchKernelGGL(k_large_count, large_grid(items), 64 * kBpWaves, 0, st, g);
