@@ -103,7 +103,15 @@ struct DetectArgs {
   const int* item_list;
   const unsigned* nitems_dev;  // with item_list: its length (else the launch's)
   int fp32_screen;             // k_queue_solve screens in binary32 first
+  int wave_items;              // small items go to k_detect_wave (mid-phase only)
 };
+
+// Items of a mid-phase-only launch small enough for one wave each.
+constexpr int kWaveRows = 96;       // rows and columns staged per wave
+constexpr int kWaveSurvivors = 256; // survivors listed per wave
+__device__ __forceinline__ bool wave_item(int nrows, int nA, int nB) {
+  return nrows <= kWaveRows && nB <= kWaveRows && nA < 65536;
+}
 
 __device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
   Tri<double> r;
@@ -394,6 +402,7 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
     auto defer = [&]() {
       if (tid == 0) g.deferred[atomicAdd(g.ndeferred, 1u)] = item;
     };
+    if (QONLY && g.wave_items && wave_item(nrows, p.nA, p.nB)) return;  // k_detect_wave's
     if (QONLY && !(staged && nrows <= blk && p.nA < 65536 && p.nB < 65536)) {
       defer();
       return;
@@ -656,6 +665,162 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g, int items) {
   for (int t = blockIdx.x; t < items; t += gridDim.x) {
     detect_item<M, QONLY>(g, g.item_list ? g.item_list[t] : t);
     __syncthreads();  // the next item reuses the workgroup's LDS
+  }
+}
+
+// The mid-phase of DH_BF for small items, one wave per item and no
+// workgroup barrier: A's rows and B's columns (at most kWaveRows each)
+// staged as boxes in the wave's LDS, rows apart from B's particle box
+// dropped, rows and columns prefiltered against the other side's union,
+// the surviving pairs listed in row-major order and appended to the
+// survivor queue as one range.  Items whose survivors overflow the list or
+// the queue are deferred to the full kernel.  Same tests, order and
+// counters as detect_item's queue path.
+__global__ __launch_bounds__(128) void k_detect_wave(DetectArgs g, int items) {
+  constexpr int kW = 2;  // waves per workgroup
+  __shared__ double boxes[kW][2 * kWaveRows * 6];
+  __shared__ unsigned char act[kW][2 * kWaveRows];
+  __shared__ int surv[kW][kWaveSurvivors];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double* bx = boxes[w];
+  for (int item = blockIdx.x * kW + w; item < items; item += gridDim.x * kW) {
+    const int k = item / g.tiles, tile = item - k * g.tiles;
+    unsigned long long* ts = g.tile_stats + static_cast<long long>(item) * kTileStats;
+    if (g.npairs_dev && k >= *g.npairs_dev) {
+      if (lane == 0) g.tile_count[item] = 0;
+      if (lane < kTileStats) ts[lane] = 0;
+      continue;
+    }
+    const Pair p = load_pair(g, k);
+    const int r0 = static_cast<int>(static_cast<long long>(tile) * p.nA / g.tiles);
+    const int r1 = static_cast<int>(static_cast<long long>(tile + 1) * p.nA / g.tiles);
+    const int nrows = r1 - r0;
+    if (!wave_item(nrows, p.nA, p.nB)) continue;  // the workgroup kernel's
+    const int total = nrows * p.nB;
+    unsigned culled = 0;  // lane 0's count is the item's
+    const double* pb = g.pbox ? g.pbox + 6 * static_cast<long long>(p.pb) : nullptr;
+    // rows: boxes, and the test against B's particle box
+    bool near = false;
+    for (int t = lane; t < nrows; t += 64) {
+      double* b = bx + 6 * t;
+      tri_box(load_tri(g, p.oA + r0 + t), b, b + 3);
+      if (pb) near = near || !boxes_apart(b, b + 3, pb, pb + 3, p.margin);
+    }
+    int S = 0;
+    bool deferred = false;
+    if (pb && !__ballot(near)) {
+      culled = total;
+    } else {
+      for (int t = lane; t < p.nB; t += 64) {
+        double* b = bx + 6 * (kWaveRows + t);
+        tri_box(load_tri(g, p.oB + t), b, b + 3);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // unions of the rows' and of B's boxes
+      double u[12];
+      for (int q = 0; q < 3; ++q) {
+        u[q] = u[6 + q] = INFINITY;
+        u[3 + q] = u[9 + q] = -INFINITY;
+      }
+      for (int t = lane; t < nrows; t += 64)
+        for (int q = 0; q < 3; ++q) {
+          u[q] = fmin(u[q], bx[6 * t + q]);
+          u[3 + q] = fmax(u[3 + q], bx[6 * t + 3 + q]);
+        }
+      for (int t = lane; t < p.nB; t += 64)
+        for (int q = 0; q < 3; ++q) {
+          u[6 + q] = fmin(u[6 + q], bx[6 * (kWaveRows + t) + q]);
+          u[9 + q] = fmax(u[9 + q], bx[6 * (kWaveRows + t) + 3 + q]);
+        }
+      for (int o = 32; o > 0; o >>= 1)
+        for (int q = 0; q < 12; ++q) {
+          const double v = __shfl_xor(u[q], o);
+          u[q] = ((q % 6) < 3) ? fmin(u[q], v) : fmax(u[q], v);
+        }
+      // ordered active rows and columns
+      int na = 0, nc = 0;
+      for (int side = 0; side < 2; ++side) {
+        const int cnt_all = side == 0 ? nrows : p.nB;
+        const double* other = u + 6 * (1 - side);
+        int n_on = 0;
+        for (int t0 = 0; t0 < cnt_all; t0 += 64) {
+          const int t = t0 + lane;
+          bool on = false;
+          if (t < cnt_all) {
+            const double* b = bx + 6 * (side == 0 ? t : kWaveRows + t);
+            on = !boxes_apart(b, b + 3, other, other + 3, p.margin);
+          }
+          const unsigned long long m = __ballot(on);
+          if (on) act[w][side * kWaveRows + n_on + lane_prefix(m)] = static_cast<unsigned char>(t);
+          n_on += __popcll(m);
+        }
+        if (side == 0) na = n_on; else nc = n_on;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int tested = na * nc;
+      culled = static_cast<unsigned>(total - tested);
+      // survivors in row-major order over the active rows and columns
+      for (int f0 = 0; f0 < tested; f0 += 64) {
+        const int fe = f0 + lane;
+        bool keep = false;
+        if (fe < tested) {
+          const int ra = fe / nc, cb = fe - ra * nc;
+          const double* a = bx + 6 * act[w][ra];
+          const double* b = bx + 6 * (kWaveRows + act[w][kWaveRows + cb]);
+          keep = !boxes_apart(a, a + 3, b, b + 3, p.margin);
+        }
+        const unsigned long long m = __ballot(keep);
+        if (keep && S + lane_prefix(m) < kWaveSurvivors) surv[w][S + lane_prefix(m)] = fe;
+        S += __popcll(m);
+        culled += static_cast<unsigned>(__popcll(__ballot(fe < tested && !keep)));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (S > kWaveSurvivors) {
+        deferred = true;
+      } else if (S > 0) {
+        unsigned long long qb0 = 0;
+        if (lane == 0) qb0 = atomicAdd(g.qcounter, static_cast<unsigned long long>(S));
+        const long long qb = static_cast<long long>(__shfl(qb0, 0));
+        if (qb + S <= g.qcap) {
+          if (lane == 0) g.qbase[item] = qb;
+          for (int e = lane; e < S; e += 64) {
+            const int fe = surv[w][e];
+            const int ra = fe / nc, cb = fe - ra * nc;
+            const int iA = r0 + act[w][ra], iB = act[w][kWaveRows + cb];
+            g.queue[qb + e] = make_int2(
+                item, static_cast<int>(static_cast<unsigned>(iA) | (static_cast<unsigned>(iB) << 16)));
+          }
+        } else {
+          for (long long q = qb + lane; q < qb + S && q < g.qcap; q += 64)
+            g.queue[q] = make_int2(-1, 0);
+          deferred = true;
+        }
+      }
+      // the next item overwrites the wave's LDS
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (deferred) {
+      if (lane == 0) g.deferred[atomicAdd(g.ndeferred, 1u)] = item;
+      continue;
+    }
+    if (lane == 0) {
+      g.tile_count[item] = 0;
+      ts[TS_CHECKS] = tile == 0 ? static_cast<unsigned long long>(p.nA) * p.nB : 0ull;
+      ts[TS_EVALUATED] = static_cast<unsigned long long>(total);
+      ts[TS_FALLBACK_ROWS] = 0;
+      ts[TS_PENALTY_FAILS] = 0;
+      ts[TS_BATCH_FAILS] = 0;
+      ts[TS_BATCH_SIZE] = 0;
+      ts[TS_CULLED] = culled;
+    }
   }
 }
 
@@ -986,6 +1151,12 @@ static unsigned detect_grid(long long items) {
 
 template <int M>
 static void launch_detect(unsigned items, size_t shm, hipStream_t st, const DetectArgs& a) {
+  if (M == DH_BF && a.queue && a.wave_items) {
+    const long long wg = (static_cast<long long>(items) + 1) / 2;
+    const long long cap = static_cast<long long>(device_cus()) * 16;
+    hipLaunchKernelGGL(k_detect_wave, static_cast<unsigned>(wg < cap ? (wg > 0 ? wg : 1) : cap),
+                       128, 0, st, a, static_cast<int>(items));
+  }
   if (M == DH_BF && a.queue)
     hipLaunchKernelGGL((k_detect<DH_BF, true>), detect_grid(items), 256, shm, st, a,
                        static_cast<int>(items));
@@ -1116,6 +1287,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.npairs_dev = nullptr;
   a.pbox = pbox;
   a.fp32_screen = tuning().detect_fp32_screen;
+  a.wave_items = 0;
   a.bbox = bbox;
   a.queue = nullptr;
   a.qcap = 0;
@@ -1141,6 +1313,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
     DH_HIP_CHECK(hipMemsetAsync(g_scratch.ndeferred.ptr, 0, sizeof(unsigned), st));
     a.deferred = g_scratch.deferred.as<int>();
     a.ndeferred = g_scratch.ndeferred.as<unsigned>();
+    a.wave_items = 1;
     a.queue = g_scratch.queue.as<int2>();
     a.qcap = qcap;
     a.qcounter = g_scratch.qcounter.as<unsigned long long>();
@@ -1258,6 +1431,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.npairs_dev = np_dev;
   a.pbox = pbox;
   a.fp32_screen = tuning().detect_fp32_screen;
+  a.wave_items = 0;
   a.bbox = bbox;
   a.queue = nullptr;  // the queue needs the survivor count on the host
   a.qcap = 0;
@@ -1280,6 +1454,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
     a.qbase = g_scratch.qbase.as<long long>();
     a.deferred = g_scratch.deferred.as<int>();
     a.ndeferred = g_scratch.ndeferred.as<unsigned>();
+    a.wave_items = 1;
   }
   switch (method) {
     case DH_BF: launch_detect<DH_BF>(items, 0, st, a); break;
