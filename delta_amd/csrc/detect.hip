@@ -668,6 +668,41 @@ __global__ __launch_bounds__(256) void k_detect(DetectArgs g, int items) {
   }
 }
 
+// The workgroup mid-phase for the items k_detect_wave leaves (larger ones):
+// the first wave classifies 64 items at a time, one per lane, and the
+// workgroup runs the larger live ones, in order.
+__global__ __launch_bounds__(256) void k_detect_big(DetectArgs g, int items) {
+  __shared__ int list[64];
+  __shared__ int nlist;
+  const int tid = threadIdx.x;
+  for (int base = blockIdx.x * 64; base < items; base += gridDim.x * 64) {
+    if (tid < 64) {
+      const int item = base + tid;
+      bool big = false;
+      if (item < items) {
+        const int k = item / g.tiles, tile = item - k * g.tiles;
+        if (!(g.npairs_dev && k >= *g.npairs_dev)) {
+          const int pa = g.pairs[2 * k], pb = g.pairs[2 * k + 1];
+          const int nA = g.tri_offset[pa + 1] - g.tri_offset[pa];
+          const int nB = g.tri_offset[pb + 1] - g.tri_offset[pb];
+          const int r0 = static_cast<int>(static_cast<long long>(tile) * nA / g.tiles);
+          const int r1 = static_cast<int>(static_cast<long long>(tile + 1) * nA / g.tiles);
+          big = !wave_item(r1 - r0, nA, nB);
+        }
+      }
+      const unsigned long long m = __ballot(big);
+      if (big) list[lane_prefix(m)] = item;
+      if (tid == 0) nlist = __popcll(m);
+    }
+    __syncthreads();
+    const int n = nlist;
+    for (int j = 0; j < n; ++j) {
+      detect_item<DH_BF, true>(g, list[j]);
+      __syncthreads();  // the next item reuses the workgroup's LDS
+    }
+  }
+}
+
 // The mid-phase of DH_BF for small items, one wave per item and no
 // workgroup barrier: A's rows and B's columns (at most kWaveRows each)
 // staged as boxes in the wave's LDS, rows apart from B's particle box
@@ -1157,7 +1192,12 @@ static void launch_detect(unsigned items, size_t shm, hipStream_t st, const Dete
     hipLaunchKernelGGL(k_detect_wave, static_cast<unsigned>(wg < cap ? (wg > 0 ? wg : 1) : cap),
                        128, 0, st, a, static_cast<int>(items));
   }
-  if (M == DH_BF && a.queue)
+  if (M == DH_BF && a.queue && a.wave_items) {
+    const long long wg = (static_cast<long long>(items) + 63) / 64;
+    const long long cap = static_cast<long long>(device_cus()) * 32;
+    hipLaunchKernelGGL(k_detect_big, static_cast<unsigned>(wg < cap ? (wg > 0 ? wg : 1) : cap),
+                       256, shm, st, a, static_cast<int>(items));
+  } else if (M == DH_BF && a.queue)
     hipLaunchKernelGGL((k_detect<DH_BF, true>), detect_grid(items), 256, shm, st, a,
                        static_cast<int>(items));
   else
