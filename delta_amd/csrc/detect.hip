@@ -107,7 +107,7 @@ struct DetectArgs {
 };
 
 // Items of a mid-phase-only launch small enough for one wave each.
-constexpr int kWaveRows = 96;       // rows and columns staged per wave
+constexpr int kWaveRows = 80;       // rows and columns staged per wave
 constexpr int kWaveSurvivors = 256; // survivors listed per wave
 __device__ __forceinline__ bool wave_item(int nrows, int nA, int nB) {
   return nrows <= kWaveRows && nB <= kWaveRows && nA < 65536;
