@@ -1059,6 +1059,7 @@ __global__ __launch_bounds__(256) void k_scatter(const dh_contact* __restrict__ 
 // contact test of bf.cpp:125); flags[e] = 1 and P, Q kept when it emits.
 // `n` lanes; with need_dev (a captured step) the live entries are
 // min(*need_dev, n) and the lanes past them clear their flags.
+template <bool SCREEN>
 __global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
                                                      const unsigned long long* __restrict__ need_dev,
                                                      int* __restrict__ flags,
@@ -1079,7 +1080,7 @@ __global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
   const Tri<double> A = load_tri(g, g.tri_offset[pa] + iA);
   const Tri<double> B = load_tri(g, g.tri_offset[pb] + iB);
   const double margin = g.eps[pa] + g.eps[pb];
-  if (g.fp32_screen) {
+  if (SCREEN) {
     // binary32 brute force relative to A's first vertex: a pair at twice
     // the margin or more there is no contact (binary32's error at these
     // scales is far below the margin); the rest are decided in binary64
@@ -1238,8 +1239,12 @@ static int queue_reserve(long long n) {
 static int queue_passes(const DetectArgs& a, long long n, const unsigned long long* need_dev,
                         hipStream_t st) {
   const unsigned grid = grid_for(n, 256);
-  hipLaunchKernelGGL(k_queue_solve, grid, 256, 0, st, a, n, need_dev, g_scratch.qflags.as<int>(),
-                     g_scratch.qpq.as<double>());
+  if (a.fp32_screen)
+    hipLaunchKernelGGL(k_queue_solve<true>, grid, 256, 0, st, a, n, need_dev,
+                       g_scratch.qflags.as<int>(), g_scratch.qpq.as<double>());
+  else
+    hipLaunchKernelGGL(k_queue_solve<false>, grid, 256, 0, st, a, n, need_dev,
+                       g_scratch.qflags.as<int>(), g_scratch.qpq.as<double>());
   int rc;
   if ((rc = scan_counts(g_scratch.qflags.as<int>(), static_cast<int>(n),
                         g_scratch.qpos.as<long long>(), st)))
