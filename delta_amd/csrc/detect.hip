@@ -397,13 +397,15 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
     const bool staged = p.nB <= kBoxTriangles;  // block-uniform
     const int blk = staged ? kBoxTriangles : (nrows > 0 ? nrows : 1);
     bool b_staged = false;  // B's boxes are loaded on the first block that needs them
+    long long q_first = 0;  // the item's queue range, when a block appended one
+    int q_len = 0;
     // mid-phase-only launch: an item that cannot queue its survivors is
     // handed to the full launch (which also writes its counters)
     auto defer = [&]() {
       if (tid == 0) g.deferred[atomicAdd(g.ndeferred, 1u)] = item;
     };
     if (QONLY && g.wave_items && wave_item(nrows, p.nA, p.nB)) return;  // k_detect_wave's
-    if (QONLY && !(staged && nrows <= blk && p.nA < 65536 && p.nB < 65536)) {
+    if (QONLY && !(staged && p.nA < 65536 && p.nB < 65536)) {
       defer();
       return;
     }
@@ -553,17 +555,22 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
           defer();
           return;
         }
-        if (g.queue && first && fpos >= tested && staged && nrows <= blk && p.nA < 65536 &&
-            p.nB < 65536) {
-          // every survivor of the item is in LDS: append them, in row-major
-          // order, as one range of the survivor queue (solved there with
-          // every lane busy); the item solves them itself if they do not fit
+        if (g.queue && first && fpos >= tested && staged && p.nA < 65536 && p.nB < 65536 &&
+            !(nl > 0 && q_len > 0)) {
+          // every survivor of the block is in LDS: append them, in row-major
+          // order, as the item's one range of the survivor queue (solved
+          // there with every lane busy); the item solves them itself if they
+          // do not fit
           if (tid == 0 && nl > 0)
             chunk_base = atomicAdd(g.qcounter, static_cast<unsigned long long>(nl));
           __syncthreads();
           const long long qb = nl > 0 ? static_cast<long long>(chunk_base) : 0;
           __syncthreads();
           if (qb + nl <= g.qcap) {
+            if (nl > 0) {
+              q_first = qb;
+              q_len = nl;
+            }
             if (tid == 0 && nl > 0) g.qbase[item] = qb;
             for (int e = tid; e < nl; e += 256) {
               int iA, iB;
@@ -579,6 +586,14 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
             defer();
             return;
           }
+        }
+        if (QONLY && nl > 0 && q_len > 0) {
+          // a second block with survivors: the item's survivors would not
+          // be one range; void the first and run the item in full
+          for (long long q = q_first + tid; q < q_first + q_len; q += 256)
+            g.queue[q] = make_int2(-1, 0);
+          defer();
+          return;
         }
         for (int b = 0; b < (QONLY ? 0 : nl); b += 256) {
           const int e = b + tid;

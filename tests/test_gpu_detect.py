@@ -151,9 +151,10 @@ def _wall_touching_particle():
     return funnel, scenes.particle(1, 0.5e-3, 61, c)
 
 
+@pytest.mark.parametrize("queue", [0, 1])
 @pytest.mark.parametrize("max_tiles", [1, 16])
 @pytest.mark.parametrize("swap", [False, True])
-def test_bf_large_particle_matches_oracle(max_tiles, swap):
+def test_bf_large_particle_matches_oracle(max_tiles, swap, queue):
     """A particle with more rows than one LDS block holds (the funnel as A:
     2,048 rows, staged in blocks of 320 when one workgroup takes them all)
     and as B (2,048 columns: the unstaged pass): contacts bit-exact and in
@@ -162,7 +163,7 @@ def test_bf_large_particle_matches_oracle(max_tiles, swap):
     A, B = _wall_touching_particle()
     if swap:
         A, B = B, A
-    tuning.set(detect_max_tiles=max_tiles)
+    tuning.set(detect_max_tiles=max_tiles, detect_queue_min_items=0 if queue else 1 << 30)
     try:
         got = contact.bf(*shapes.to_soa(A), len(A), EPS, True, 7, *shapes.to_soa(B), len(B),
                          EPS, True, 9)

@@ -280,17 +280,19 @@ def test_adaptive_graph_matches_eager():
         g.close()
 
 
-@pytest.mark.parametrize("kind", ["drum", "funnel"])
+@pytest.mark.parametrize("kind", ["drum", "funnel", "funnel-queue"])
 def test_large_scene_generators_step_bitexact(kind):
     """The rotating box (a kinematic obstacle turning at 2 rad/s) and the
     funnel, with particles packed closer than their sizes so that they
     touch: a few steps bit for bit against the oracle, soft springs keeping
     expmap in its series branch."""
-    from delta_amd import workloads
+    from delta_amd import tuning, workloads
     # the oracle solves every triangle pair: the funnel's 2,048 triangles
     # against each particle make its step slow, hence fewer particles and steps
     hs = (workloads.drum_scene(n=216, pitch=1.2e-3) if kind == "drum"
           else workloads.funnel_scene(n=40, pitch=1.3e-3))
+    # funnel-queue: the cone pairs' survivors (several row blocks) through the queue
+    tuning.set(detect_queue_min_items=0 if kind == "funnel-queue" else 1 << 30)
     ds = engine.DeviceScene(hs)
     k = 1e1
     p = engine.step_params(method=_lib.DH_BF, dt=5e-7, spring=k, damper=0.1, friction=0.5,
@@ -305,6 +307,7 @@ def test_large_scene_generators_step_bitexact(kind):
             assert np.isfinite(np.asarray(getattr(hs, f))).all(), (s, f)
         compare(ds, hs, s)
         total += nc
+    tuning.set(**tuning.default())
     assert total > 0
     if kind == "drum":
         assert hs.rot[0][1] != 0.0  # the box turned
