@@ -999,6 +999,25 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const int* __restrict__ in
   if (threadIdx.x == 1023) block_sums[blockIdx.x] = buf[1023];
 }
 
+// The whole scan in one workgroup when n <= 1,024 (the small scenes' pair
+// and particle counts): offs[0..n) exclusive, offs[n] the total.
+__global__ __launch_bounds__(1024) void k_scan_single(const int* __restrict__ in, int n,
+                                                      long long* __restrict__ out) {
+  __shared__ long long buf[1024];
+  const int i = threadIdx.x;
+  const long long v = i < n ? in[i] : 0;
+  buf[i] = v;
+  __syncthreads();
+  for (int s = 1; s < 1024; s <<= 1) {
+    const long long t = i >= s ? buf[i - s] : 0;
+    __syncthreads();
+    buf[i] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = buf[i] - v;
+  if (i == 1023) out[n] = buf[1023];
+}
+
 // Exclusive scan of the block sums in place, by one workgroup: 1,024 at a
 // time with a running carry.
 __global__ __launch_bounds__(1024) void k_scan_sums(long long* __restrict__ sums, int nb) {
@@ -1178,6 +1197,11 @@ int scan_reserve(int n) {
 
 // offs[0..n]: exclusive prefix sums of counts, offs[n] = total
 int scan_counts(const int* counts, int n, long long* offs, hipStream_t st) {
+  if (n <= 1024) {
+    hipLaunchKernelGGL(k_scan_single, 1, 1024, 0, st, counts, n, offs);
+    DH_HIP_CHECK(hipGetLastError());
+    return DH_OK;
+  }
   const int nb = (n + 1023) / 1024 > 0 ? (n + 1023) / 1024 : 1;
   int rc = g_scratch.sums.grow(sizeof(long long) * (nb + 1));
   if (rc) return rc;
