@@ -602,9 +602,10 @@ int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double ce
   int rc;
   if ((rc = broadphase_reserve(n, geom->ntri_total))) return rc;
   const unsigned tbl = table_size(n);
-  fill_async(g_bp.err.as<int>(), 0, 1, st);
-  fill_async(g_bp.ccount.as<int>(), 0, static_cast<long long>(tbl), st);
-  fill_async(g_bp.cfill.as<int>(), 0, static_cast<long long>(tbl), st);
+  zero_async(ZeroRegions{{g_bp.err.as<unsigned>(), g_bp.ccount.as<unsigned>(),
+                          g_bp.cfill.as<unsigned>(), nullptr},
+                         {1, static_cast<long long>(tbl), static_cast<long long>(tbl), 0}},
+             st);
   GridArgs g;
   if ((rc = grid_build(geom, obstacle_dev, cell, g, st))) return rc;
   hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,

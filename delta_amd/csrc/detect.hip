@@ -1489,9 +1489,12 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
                        const double* bbox) {
   int rc;
   if ((rc = detect_reserve(pair_cap, contact_cap, method))) return rc;
-  fill_async(g_scratch.counter.as<unsigned long long>(), 0ull, 1, st);
-  fill_async(g_scratch.stats.as<unsigned long long>(), 0ull, ST_COUNT, st);
-  fill_async(g_scratch.error.as<int>(), 0, 1, st);
+  zero_async(ZeroRegions{{g_scratch.counter.as<unsigned>(), g_scratch.stats.as<unsigned>(),
+                          g_scratch.error.as<unsigned>(),
+                          graph_qcap(method, pair_cap) > 0 ? g_scratch.qcounter.as<unsigned>()
+                                                           : nullptr},
+                         {2, 2 * ST_COUNT, 1, 2}},
+             st);
   DetectArgs a;
   a.x = geom->x;
   a.y = geom->y;
@@ -1530,7 +1533,6 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   if (qcap > 0) {
     // the survivor queue with its counts kept on the device: every pass is
     // launched for the capacity and reads the live length
-    fill_async(g_scratch.qcounter.as<unsigned long long>(), 0ull, 1, st);
     fill_async(g_scratch.ndeferred.as<unsigned>(), 0u, 1, st);
     a.queue = g_scratch.queue.as<int2>();
     a.qcap = qcap;

@@ -328,8 +328,9 @@ int contacts_forces_impl(const dh_bodies* bodies, const int* pairs_dev, int npai
   const int n = bodies->n;
   int rc;
   if ((rc = forces_reserve(n, npairs))) return rc;
-  fill_async(g_fs.incid.as<int>(), 0, n + 1, st);
-  fill_async(g_fs.fill.as<int>(), 0, n + 1, st);
+  zero_async(ZeroRegions{{g_fs.incid.as<unsigned>(), g_fs.fill.as<unsigned>(), nullptr, nullptr},
+                         {n + 1, n + 1, 0, 0}},
+             st);
   const BodiesDev b = to_dev(bodies);
   const ForceConsts k{fp->spring, fp->damper, fp->friction};
   if (npairs > 0) {

@@ -71,4 +71,26 @@ inline void fill_async(T* p, T v, long long n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_fill<T>, grid_for(n, 256), 256, 0, st, p, v, n);
 }
 
+// Up to four regions cleared to zero by one launch (region y of the grid);
+// sizes in 32-bit words.  Used instead of several fills in captured work.
+struct ZeroRegions {
+  unsigned* p[4];
+  long long n[4];
+};
+
+__global__ __launch_bounds__(256) inline void k_zero_regions(ZeroRegions z) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  const int r = blockIdx.y;
+  if (i < z.n[r]) z.p[r][i] = 0u;
+}
+
+inline void zero_async(ZeroRegions z, hipStream_t st) {
+  long long m = 0;
+  for (int r = 0; r < 4; ++r) {
+    if (!z.p[r]) z.n[r] = 0;
+    m = z.n[r] > m ? z.n[r] : m;
+  }
+  if (m > 0) hipLaunchKernelGGL(k_zero_regions, dim3(grid_for(m, 256), 4), 256, 0, st, z);
+}
+
 }  // namespace dh
