@@ -267,7 +267,9 @@ def copy_calibration():
 def dem_c1(a):
     """64 jittered 80-triangle icospheres on a 4x4x4 lattice (1.05 mm) above a
     2-triangle floor, FP64 brute force, spring-dashpot k=1e5 N/m, damping 0.1,
-    margin 1e-4 m, dt = 1e-6 s; `--steps` steps timed after `--warmup`."""
+    margin 1e-4 m, dt = 1e-6 s; `--steps` steps timed after `--warmup` and,
+    unless --no-autotune, after the detection tiling is autotuned on the
+    scene's candidate pairs."""
     from delta_amd import _lib, engine
     hs = engine.c1_scene()
     ds = engine.DeviceScene(hs)
@@ -280,6 +282,7 @@ def dem_c1(a):
         max_pairs = max(max_pairs, info.candidate_pairs)
         max_contacts = max(max_contacts, info.contacts)
     _lib.check(L.dh_synchronize(), "sync")
+    tuned = None if a.no_autotune or a.tuning else tune_detection(ds, hs, prm, L, 1_000_000)
     e0, e1 = C.c_void_p(), C.c_void_p()
     L.dh_event_create(C.byref(e0))
     L.dh_event_create(C.byref(e1))
@@ -310,6 +313,8 @@ def dem_c1(a):
            "triangle_checks_per_s": checks / (ms.value * 1e-3),
            "culled_fraction": culled / max(checks, 1),
            "mean_candidate_pairs": pairs / a.steps, "mean_contacts": contacts / a.steps}
+    if tuned:
+        out["tuning"] = {"table": tuned[0], "trials_value_ms": tuned[1]}
     # the same steps as one captured graph per step (no host synchronisation
     # between steps); the final state must equal the eager one bit for bit
     dg = engine.DeviceScene(hs)
@@ -660,6 +665,11 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a "
                                    "design note only (bench.py --gpus 1)"}))
         sys.exit(2)
+    fixed = None
+    if a.tuning:
+        from delta_amd import tuning
+        fixed = tuning.set(**{k: int(v) for k, v in
+                              (kv.split("=") for kv in a.tuning.split(","))})
     if a.workload == "c1-dem":
         dem_c1(a)
         return
@@ -679,10 +689,7 @@ def main():
         trainer, _, train_info = train_network(a)
     if "mixed" in a.methods.split(","):
         batch.make_f32()  # the binary32 bulk copy is an input format, made before timing
-    if a.tuning:
-        from delta_amd import tuning
-        fixed = tuning.set(**{k: int(v) for k, v in
-                              (kv.split("=") for kv in a.tuning.split(","))})
+    if fixed:
         tune_info = {"table": fixed, "fixed": True}
     else:
         tune_info = None if a.no_autotune else autotune(batch, a.methods.split(","))
