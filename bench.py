@@ -38,10 +38,10 @@ def parse():
                     help="target wall time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the library's default launch parameters")
+                    help="keep the library's default launch parameters (every workload)")
     ap.add_argument("--tuning", default="",
-                    help="fixed launch parameters instead of autotuning, e.g. "
-                         "learned_wg_per_cu=16,exact_block=256 (profiling runs)")
+                    help="fixed launch parameters instead of autotuning (every workload), "
+                         "e.g. learned_wg_per_cu=16,exact_block=256 (profiling runs)")
     ap.add_argument("--methods", default="bf64,bf64_exact,penalty64,hybrid64,mixed,"
                                          "penalty_cert,learned_penalty,learned")
     ap.add_argument("--workload", default="pairs",
@@ -421,6 +421,10 @@ def tune_detection(ds, hs, prm, L, contact_cap):
                                 C.c_int, C.POINTER(C.c_int), C.c_void_p]
     _lib.check(L.dh_broadphase(C.byref(ds.scene.geom), ds.buf["obstacle"].ptr, prm.cell,
                                d_pairs.ptr, cap, C.byref(npairs), None), "dh_broadphase")
+    if npairs.value <= 0:
+        print("[bench] dh_broadphase reported no pairs; detection tiling left untuned",
+              file=sys.stderr, flush=True)
+        return tuning.get(), []
     out_c = DeviceArray((contact_cap * 144,), np.uint8)
     trials = tuning.autotune_detect(ds.scene.geom, d_pairs.ptr, npairs.value, out_c.ptr,
                                     contact_cap, reps=2)
@@ -448,7 +452,11 @@ def dem_hopper(a):
         engine.step(ds, prm)
     _lib.check(L.dh_synchronize(), "sync")
     print("[bench] hopper warm", file=sys.stderr, flush=True)
-    table, trials = tune_detection(ds, hs, prm, L, 4_000_000)
+    if a.no_autotune or a.tuning:
+        from delta_amd import tuning
+        table, trials = tuning.get(), []
+    else:
+        table, trials = tune_detection(ds, hs, prm, L, 4_000_000)
     checks = contacts = pairs = culled = 0
     max_pairs = max_contacts = 0
     t0 = time.perf_counter()
@@ -592,8 +600,8 @@ def dem_large(a, kind):
                                d_pairs.ptr, cap, C.byref(npairs), None), "dh_broadphase")
     ccap = 4 * max_contacts + 4096
     out_c = DeviceArray((ccap * 144,), np.uint8)
-    trials = tuning.autotune_detect(ds.scene.geom, d_pairs.ptr, npairs.value, out_c.ptr, ccap,
-                                    reps=2)
+    trials = [] if a.no_autotune or a.tuning else tuning.autotune_detect(
+        ds.scene.geom, d_pairs.ptr, npairs.value, out_c.ptr, ccap, reps=2)
     del out_c, d_pairs
     table = tuning.get()
     print(f"[bench] tuning {table}", file=sys.stderr, flush=True)
