@@ -229,3 +229,86 @@ extern "C" void hc_ispc(int pen, unsigned n, double* const pp[6][3], double* con
     Q[2][i] = Qi.z;
   }
 }
+
+#include "gen_pairs.hpp"
+
+// Pairs i0 .. i0+m-1 of the unique-pair workload (gen_pairs.hpp) into an
+// 18 x m SoA block; returns the number of column indices that fell outside
+// [0, 18 m) (always 0: the check the device kernel relies on).
+extern "C" long hc_gen_tri_pairs(unsigned long long seed, long i0, long m, double box,
+                                 double sep_min, double sep_max, double frac_parallel,
+                                 double frac_intersect, double max_angle, int bisection_steps,
+                                 double* soa, double* target, unsigned char* kind) {
+  dh::GenParams gp{box, sep_min, sep_max, frac_parallel, frac_intersect, max_angle,
+                   bisection_steps};
+  long bad = 0;
+  for (long j = 0; j < m; ++j) {
+    Tri<double> A, B;
+    double s;
+    uint8_t kd;
+    dh::gen_pair(seed, i0 + j, gp, A, B, s, kd);
+    for (int k = 0; k < 3; ++k) {
+      const double v[6] = {A.v[k].x, A.v[k].y, A.v[k].z, B.v[k].x, B.v[k].y, B.v[k].z};
+      for (int q = 0; q < 6; ++q) {
+        const int64_t at = dh::soa_index(3 * q + k, m, j);
+        if (at < 0 || at >= 18 * m) {
+          ++bad;
+          continue;
+        }
+        soa[at] = v[q];
+      }
+    }
+    if (target) target[j] = s;
+    if (kind) kind[j] = kd;
+  }
+  return bad;
+}
+
+// bfSolver's value of every feature (15 per pair): the numbers its loops
+// compare (edge-edge |w + sc u - tc v|, vertex-face sqrt of the quadratic form).
+extern "C" void hc_feature_values(const double* soa, long n, double* vals) {
+  for (long i = 0; i < n; ++i) {
+    Tri<double> A = load(soa, n, i, 0), B = load(soa, n, i, 9);
+    V3<double> P, Q;
+    for (int f = 0; f < 15; ++f)
+      vals[15 * i + f] = f < 9 ? dh::eval_ee_value(A, B, f, P, Q) : dh::eval_vf_value(A, B, f, P, Q);
+  }
+}
+
+// The learned path's decision for given candidates (the network's most likely
+// edge-edge feature ce[i] and vertex-face feature cv[i]), as k_learned makes
+// it: both evaluated, bfSolver's rule, exact_choice(), ties folded.  status:
+// 0 sent to the brute force (P, Q untouched), 1 accepted as is, 2 + k
+// accepted after folding k tied features; P, Q (component-major) the result.
+// on: on-plane vertex counts at the result (A in bits 0-1, B in bits 2-3).
+extern "C" void hc_learned_decide(const double* soa, long n, const int* ce, const int* cv,
+                                  double* Pout, double* Qout, int* status) {
+  for (long i = 0; i < n; ++i) {
+    Tri<double> A = load(soa, n, i, 0), B = load(soa, n, i, 9);
+    int fe = ce[i], fv = cv[i];
+    V3<double> Pe, Qe, Pv, Qv;
+    double Ve = dh::eval_ee_value(A, B, fe, Pe, Qe);
+    double Vv = dh::eval_vf_value(A, B, fv, Pv, Qv);
+    const bool take_e = Vv > Ve;
+    const int win0 = take_e ? fe : fv;
+    V3<double> P = take_e ? Pe : Pv, Q = take_e ? Qe : Qv;
+    const unsigned open = dh::exact_choice(A, B, P, Q, win0, take_e ? fv : fe, take_e ? Ve : Vv);
+    if (open == dh::kUnresolvedAll) {
+      status[i] = 0;
+      continue;
+    }
+    const unsigned rest = open & ~((1u << fe) | (1u << fv));
+    int k = 0;
+    for (int g = 0; g < 15; ++g)
+      if ((rest >> g) & 1u) {
+        dh::fold_feature(A, B, g, fe, Ve, Pe, Qe, fv, Vv, Pv, Qv);
+        ++k;
+      }
+    const bool te = Vv > Ve;
+    P = te ? Pe : Pv;
+    Q = te ? Qe : Qv;
+    status[i] = rest ? 2 + k : 1;
+    Pout[i] = P.x; Pout[n + i] = P.y; Pout[2 * n + i] = P.z;
+    Qout[i] = Q.x; Qout[n + i] = Q.y; Qout[2 * n + i] = Q.z;
+  }
+}

@@ -3,11 +3,15 @@
 //
 // Inference per pair: canonical-frame inputs -> network (bf16 MFMA) -> the
 // most likely edge-edge and the most likely vertex-face feature -> both
-// evaluated in binary64 with bfSolver's arithmetic, the nearer checked by the
-// optimality certificate (pair_features.hpp).  Certified pairs are finished;
-// the others (and those the network marks for the exact path) are compacted
-// into per-workgroup queues that the FP64 brute-force kernel drains.  The network therefore decides only how much work a pair costs;
-// every distance is either certified within 8e-7 relative or bfSolver's own.
+// evaluated in binary64 with bfSolver's arithmetic and compared by bfSolver's
+// rule -> exact_choice() (pair_features.hpp) proves that the winner is the
+// feature bfSolver itself selects, or names the few tied features of a
+// degenerate closest pair, which are then evaluated and folded in bfSolver's
+// order.  The output of every accepted pair is bfSolver's (P, Q, d) bit for
+// bit.  The rest (intersecting pairs, wrong predictions, pairs the network
+// marks for the exact path) are compacted into per-workgroup queues that the
+// FP64 brute force drains.  The network therefore decides only how much work
+// a pair costs, never its result.
 #include "exact_math.hpp"
 #include "mlp.hpp"
 #include "pair_features.hpp"
@@ -45,7 +49,9 @@ __global__ __launch_bounds__(256) void k_labels(const double* __restrict__ soa, 
   uint16_t* x = X + i * kIn;
   for (int k = 0; k < kIn; ++k) x[k] = (k < kInUsed) ? bf16_bits(in[k]) : 0;
   V3<double> P, Q;
-  const int code = bf_feature_code(A, B, P, Q);
+  int ssid = 0, vid = 0;
+  const bool edge = bf_solver_ids(A, B, P, Q, ssid, vid);
+  const int code = edge ? ssid : 9 + vid;
   feat[i] = static_cast<uint8_t>(code);
   double al, be, ga, de;
   barycentric(A, P, al, be);
@@ -55,7 +61,13 @@ __global__ __launch_bounds__(256) void k_labels(const double* __restrict__ soa, 
   bary[4 * i + 1] = clampf(be);
   bary[4 * i + 2] = clampf(ga);
   bary[4 * i + 3] = clampf(de);
-  flag[i] = (frame && certify(A, B, P, Q)) ? 0.0f : 1.0f;
+  // exact path: bfSolver's own two minima as the candidates do not pass the
+  // exact-choice test (an intersecting pair, or no closest pair certified)
+  V3<double> Pe, Qe, Pv, Qv;
+  const double Ve = eval_ee_value(A, B, ssid, Pe, Qe);
+  const double Vv = eval_vf_value(A, B, 9 + vid, Pv, Qv);
+  const unsigned open = exact_choice(A, B, P, Q, code, edge ? 9 + vid : ssid, edge ? Ve : Vv);
+  flag[i] = (frame && open != kUnresolvedAll) ? 0.0f : 1.0f;
 }
 
 // ------------------------------------------------------------- forward core
@@ -429,7 +441,7 @@ __device__ inline v8s stage_tile(const float* W, int t, int lane) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_learned(LearnedArgs a) {
+__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(3))) void k_learned(LearnedArgs a) {
   __shared__ v8s w_s[kTiles * 64];
   __shared__ float b_s[2 * kHid + kOut];
   __shared__ unsigned wg_tally[3];  // certified, features evaluated, queued
@@ -581,20 +593,60 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
       const bool exact_path =
           !frame || (__uint_as_float(io_s[wave][kFlagOut * kCol + lane]) > 0.f);
       V3<double> Pv, Qv;
-      eval_ee(A, B, fe, P, Q);
-      const V3<double> ne = sub(Q, P);
-      const double de = dot(ne, ne);
+      double Ve = eval_ee_value(A, B, fe, P, Q);
       // one feature at a time: an empty asm makes the vertex-face code depend
       // on the edge-edge result, so the two evaluations are not interleaved
-      // (which would need ~150 registers instead of 128)
-      asm volatile("" : "+v"(fv) : "v"(de));
-      eval_vf(A, B, fv, Pv, Qv);
-      const V3<double> nv = sub(Qv, Pv);
-      const bool take_v = dot(nv, nv) < de;
-      P = sel3(take_v, Pv, P);
-      Q = sel3(take_v, Qv, Q);
-      done = valid && !exact_path && certify(A, B, P, Q);
+      asm volatile("" : "+v"(fv) : "v"(Ve));
+      double Vv = eval_vf_value(A, B, fv, Pv, Qv);
+      // bfSolver's final rule between its two minima (bf.cpp:1102)
+      const bool take_e = Vv > Ve;
+      P = sel3(take_e, P, Pv);
+      Q = sel3(take_e, Q, Qv);
+      const int win0 = take_e ? fe : fv;
+      const unsigned open =
+          (valid && !exact_path)
+              ? exact_choice(A, B, P, Q, win0, take_e ? fv : fe, take_e ? Ve : Vv)
+              : kUnresolvedAll;
       evals += (valid && !exact_path) ? 2 : 0;
+      // Ties of a degenerate closest pair (a vertex against an edge or a
+      // vertex): the few features that reach the same points are evaluated
+      // and folded into the two minima in bfSolver's order.  The wave runs
+      // its lanes' edge-edge ones together, then its vertex-face ones.
+      unsigned rest = (open == kUnresolvedAll) ? 0u : open & ~((1u << fe) | (1u << fv));
+      unsigned rest_e = rest & 0x1FFu, rest_v = rest & 0x7E00u;
+      while (__ballot(rest_e != 0u)) {
+        if (rest_e) {
+          const int g = __builtin_ctz(rest_e);
+          rest_e &= rest_e - 1u;
+          V3<double> Pg, Qg;
+          const double val = eval_ee_value(A, B, g, Pg, Qg);
+          const bool better = val < Ve || (val == Ve && g < fe);
+          fe = better ? g : fe;
+          Ve = better ? val : Ve;
+          ++evals;
+        }
+      }
+      while (__ballot(rest_v != 0u)) {
+        if (rest_v) {
+          const int g = __builtin_ctz(rest_v);
+          rest_v &= rest_v - 1u;
+          V3<double> Pg, Qg;
+          const double val = eval_vf_value(A, B, g, Pg, Qg);
+          const bool better = val < Vv || (val == Vv && g < fv);
+          fv = better ? g : fv;
+          Vv = better ? val : Vv;
+          ++evals;
+        }
+      }
+      // the final pick; its points again when it is not the first winner
+      const int win = Vv > Ve ? fe : fv;
+      if (__ballot(open != kUnresolvedAll && win != win0)) {
+        if (open != kUnresolvedAll && win != win0) {
+          eval_feature(A, B, win, P, Q);
+          ++evals;
+        }
+      }
+      done = valid && open != kUnresolvedAll;
     } else {
       // penalty Newton from the network's barycentric guess (or 0.33), the
       // result clamped onto the triangles and certified like a feature

@@ -72,6 +72,294 @@ DH_HD void eval_feature(const Tri<double>& A, const Tri<double>& B, int code, V3
     eval_vf(A, B, code, P, Q);
 }
 
+// eval_ee, also returning the value bfSolver's edge-edge loop compares for
+// this feature: |w + sc u - tc v| (edge_edge, bf.cpp:586-713).
+DH_HD double eval_ee_value(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+                           V3<double>& Q) {
+  const int k = code / 3, m = code - 3 * (code / 3);
+  const V3<double> ha = pick3(A, k), oa = pick3(A, k == 2 ? 0 : k + 1);
+  const V3<double> hb = pick3(B, m), ob = pick3(B, m == 2 ? 0 : m + 1);
+  const V3<double> u = sub(ha, oa), v = sub(hb, ob), w = sub(oa, ob);
+  const double a = dot(u, u), b = dot(u, v), c = dot(v, v), d = dot(u, w), e = dot(v, w);
+  double sc, tc;
+  segseg(a, b, c, d, e, sc, tc);
+  P = {oa.x + (u.x * sc), oa.y + (u.y * sc), oa.z + (u.z * sc)};
+  Q = {ob.x + (v.x * tc), ob.y + (v.y * tc), ob.z + (v.z * tc)};
+  const V3<double> dP = {w.x + (sc * u.x) - (tc * v.x), w.y + (sc * u.y) - (tc * v.y),
+                         w.z + (sc * u.z) - (tc * v.z)};
+  return dsqrt(dP.x * dP.x + dP.y * dP.y + dP.z * dP.z);
+}
+
+// eval_vf, also returning the value bfSolver's vertex-face loop compares:
+// the square root of the clamped quadratic form (vertex_face, bf.cpp:875-1100).
+DH_HD double eval_vf_value(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+                           V3<double>& Q) {
+  const int j = code - 9;
+  const bool onA = j < 3;
+  const V3<double> f0 = sel3(onA, A.v[0], B.v[0]);
+  const V3<double> f1 = sel3(onA, A.v[1], B.v[1]);
+  const V3<double> f2 = sel3(onA, A.v[2], B.v[2]);
+  const V3<double> pt = sel3(onA, pick3(B, j), pick3(A, j - 3));
+  const V3<double> e0 = sub(f1, f0), e1 = sub(f2, f0), dd = sub(f0, pt);
+  double s, t;
+  const double sqr = pt_tri(dot(e0, e0), dot(e0, e1), dot(e1, e1), dot(e0, dd), dot(e1, dd),
+                            dot(dd, dd), s, t);
+  const V3<double> onf = {f0.x + ((f2.x - f0.x) * t) + ((f1.x - f0.x) * s),
+                          f0.y + ((f2.y - f0.y) * t) + ((f1.y - f0.y) * s),
+                          f0.z + ((f2.z - f0.z) * t) + ((f1.z - f0.z) * s)};
+  P = sel3(onA, onf, pt);
+  Q = sel3(onA, pt, onf);
+  return dsqrt(sqr);
+}
+
+// ---------------------------------------------------------- exact choice
+//
+// exact_choice() decides, without evaluating the other features, whether the
+// feature `win` is the one bfSolver selects.  bfSolver keeps the least value
+// among its 9 edge-edge features (first index on ties), the least among its 6
+// vertex-face features, and the edge-edge one only when the vertex-face
+// minimum is larger (bf.cpp:1102).  `win` and `other` are the two features
+// the caller evaluated with bfSolver's own arithmetic (values Vw <= the rule
+// against Vo already applied by the caller); P, Q are win's points.  The test
+// proves that each of the remaining 13 features has a value above Vw: then
+// bfSolver's minima and its final choice are `win`, and its P, Q are win's
+// bit for bit.  Returns the set of features (bit g) whose bound does not
+// clear Vw: 0 accepts `win` as is; kUnresolvedAll (bit 15 set) means the
+// certificate failed (P, Q not a closest pair up to rounding, or P == Q).
+// The features in a partial set are the ties of a degenerate closest pair
+// (a vertex against an edge or a vertex: several features reach the same
+// points, and bfSolver's pick among them follows the last bits of their
+// values); resolve_ties() evaluates them.
+//
+// Lower bounds.  With n = Q - P, d = |n| and the unit normal m = n/d, every
+// x in A and y in B satisfy
+//     |y - x|^2 = (m.(y - x))^2 + |pi(y - x)|^2,
+//     m.(y - x) = d + sA(x) + sB(y),   sA(x) = m.(P - x),  sB(y) = m.(y - Q),
+// where pi projects onto the plane orthogonal to m (pi(P) = pi(Q) = 0 after
+// translating A by -P and B by -Q).  sA and sB are affine, so over a feature
+// (a vertex, an edge or a face: the convex hull of some vertices) they are
+// bounded by their vertex values.  The certificate requires sA, sB >= -tol at
+// every vertex (then (P, Q) is a closest pair up to rounding).  A vertex is
+// "on the plane" when its slack is at most 4 tol, otherwise its slack s > 0
+// lifts every point that uses it.  For a feature pair (SA, SB):
+//   * if all of SA's vertices are off the plane, sA >= min s over them (same
+//     for SB), and |y - x| >= d + those slacks;
+//   * otherwise write x = (1-ta) x0 + ta x1 with x0 in the hull of SA's
+//     on-plane vertices and x1 in the hull of its off-plane ones: sA(x) >=
+//     ta sa (sa = least off-plane slack) and |pi(x) - pi(x0)| <= ta pa (pa =
+//     largest in-plane distance from an off-plane to an on-plane vertex).
+//     With rho = the in-plane distance between the on-plane hulls of SA and
+//     SB, moving the in-plane gap down by u costs at least kappa u of slack,
+//     kappa = min(sa/pa, sb/pb), so
+//        |y - x|^2 >= min over 0 <= u <= min(rho, U) of
+//                     (d + kappa u)^2 + (rho - u)^2,      U = pa + pb,
+//     a one-dimensional convex minimum in closed form.
+// rho is the distance between points or a point and a segment; a feature
+// pair whose on-plane hulls are both segments, or where one is a triangle,
+// gets rho = 0 (no in-plane gain claimed), which keeps the bound valid.
+//
+// Rounding.  The slacks are computed in binary64 and carry an error below
+// tol = 2^-48 (L + M + M L / d), L the sum of the largest vertex distances
+// from P and from Q, M the largest coordinate magnitude: the vertex
+// differences, n = Q - P and the dot products each round at most a few ulp of
+// M or of L.  The in-plane geometry (basis, projections, rho, pa) is
+// computed in binary32 and widened by 2^-16 L.  bfSolver's computed value of
+// a feature is at least its exact distance less 12 eps (L + d) (edge-edge:
+// the difference of two points on the segments, the clamped parameters lie
+// in [0, 1]) and, for vertex-face, its square at least the exact square less
+// 200 eps (L + d)^2 (the quadratic form's terms are at most 20 (L + d)^2).
+// A feature is excluded when its bound satisfies
+//     LB^2 > (Vw (1 + 2^-40) + 2^-45 (L + Vw))^2 + 2^-42 (L + Vw)^2,
+// which dominates both error terms.  NaN inputs fail every comparison and
+// are never accepted.
+constexpr unsigned kUnresolvedAll = 0xFFFFu;
+
+struct SideGeom {
+  double slack[3];   // m.(P - a) for A (m.(b - Q) for B), binary64
+  float px[3], py[3];  // in-plane coordinates relative to P (Q), binary32
+  unsigned on;       // bit k: vertex k on the plane
+};
+
+// v[i] for a run-time i in 0..2, as selects (no dynamically indexed arrays,
+// which would live in scratch memory on the GPU)
+DH_HD float pickf(const float v[3], int i) { return i == 2 ? v[2] : (i == 1 ? v[1] : v[0]); }
+
+// In-plane distance between the on-plane hulls of vertex sets ma (of A) and
+// mb (of B), each one vertex or an edge; 0 when neither is a single vertex.
+DH_HD float inplane_gap(const SideGeom& sa, unsigned ma, const SideGeom& sb, unsigned mb) {
+  const int na = __builtin_popcount(ma), nb = __builtin_popcount(mb);
+  if ((na != 1 && nb != 1) || na > 2 || nb > 2) return 0.f;
+  // p: the single vertex; (q0, q1): the other hull (q0 == q1 for a vertex)
+  const bool a_single = na == 1;
+  const unsigned mp = a_single ? ma : mb, mq = a_single ? mb : ma;
+  const int ip = __builtin_ctz(mp);
+  const int iq0 = __builtin_ctz(mq);
+  const int iq1 = 31 - __builtin_clz(mq);
+  const float px = a_single ? pickf(sa.px, ip) : pickf(sb.px, ip);
+  const float py = a_single ? pickf(sa.py, ip) : pickf(sb.py, ip);
+  const float ax = a_single ? pickf(sb.px, iq0) : pickf(sa.px, iq0);
+  const float ay = a_single ? pickf(sb.py, iq0) : pickf(sa.py, iq0);
+  const float bx = a_single ? pickf(sb.px, iq1) : pickf(sa.px, iq1);
+  const float by = a_single ? pickf(sb.py, iq1) : pickf(sa.py, iq1);
+  const float ex = bx - ax, ey = by - ay;
+  const float wx = px - ax, wy = py - ay;
+  const float ee = ex * ex + ey * ey;
+  float t = ee > 0.f ? (wx * ex + wy * ey) / ee : 0.f;
+  t = t < 0.f ? 0.f : (t > 1.f ? 1.f : t);
+  const float dx = wx - t * ex, dy = wy - t * ey;
+  return std::sqrt(dx * dx + dy * dy);
+}
+
+// Vertex sets of feature `code`: bit k of *ma is vertex k of A (same for B).
+DH_HD void feature_vertices(int code, unsigned& ma, unsigned& mb) {
+  if (code < 9) {
+    const int k = code / 3, m = code - 3 * (code / 3);
+    ma = (1u << k) | (1u << (k == 2 ? 0 : k + 1));
+    mb = (1u << m) | (1u << (m == 2 ? 0 : m + 1));
+  } else if (code < 12) {  // vertex code-9 of B against face A
+    ma = 7u;
+    mb = 1u << (code - 9);
+  } else {  // vertex code-12 of A against face B
+    ma = 1u << (code - 12);
+    mb = 7u;
+  }
+}
+
+DH_HD unsigned exact_choice(const Tri<double>& A, const Tri<double>& B, const V3<double>& P,
+                            const V3<double>& Q, int win, int other, double Vw) {
+  using std::fabs;
+  using std::fmax;
+  using std::fmin;
+  using std::sqrt;
+  const V3<double> n = sub(Q, P);
+  const double d2 = dot(n, n);
+  if (!(d2 > 0.0)) return kUnresolvedAll;
+  const double d = sqrt(d2);
+  const double inv_d = 1.0 / d;
+  const V3<double> m = {n.x * inv_d, n.y * inv_d, n.z * inv_d};
+  // in-plane basis (binary32): e1 orthogonal to m (m crossed with the axis
+  // least aligned with it), e2 = m x e1
+  const float mx = static_cast<float>(m.x), my = static_cast<float>(m.y),
+              mz = static_cast<float>(m.z);
+  const float ax = fabsf(mx), ay = fabsf(my), az = fabsf(mz);
+  const bool xmin = ax <= ay && ax <= az, ymin = !xmin && ay <= az;
+  float e1x = xmin ? 0.f : (ymin ? -mz : my);
+  float e1y = xmin ? mz : (ymin ? 0.f : -mx);
+  float e1z = xmin ? -my : (ymin ? mx : 0.f);
+  const float k1 = 1.0f / std::sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+  e1x *= k1;
+  e1y *= k1;
+  e1z *= k1;
+  const float e2x = my * e1z - mz * e1y, e2y = mz * e1x - mx * e1z, e2z = mx * e1y - my * e1x;
+  // per vertex: slack (binary64), in-plane coordinates (binary32), sizes
+  double La = 0.0, Lb = 0.0;
+  double M = fmax(fmax(fmax(fabs(P.x), fabs(P.y)), fmax(fabs(P.z), fabs(Q.x))),
+                  fmax(fabs(Q.y), fabs(Q.z)));
+  SideGeom ga, gb;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const V3<double> r = sub(A.v[k], P);
+    ga.slack[k] = -dot(m, r);
+    La = fmax(La, fabs(r.x) + fabs(r.y) + fabs(r.z));
+    M = fmax(M, fmax(fmax(fabs(A.v[k].x), fabs(A.v[k].y)), fabs(A.v[k].z)));
+    const float x = static_cast<float>(r.x), y = static_cast<float>(r.y),
+                z = static_cast<float>(r.z);
+    ga.px[k] = x * e1x + y * e1y + z * e1z;
+    ga.py[k] = x * e2x + y * e2y + z * e2z;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const V3<double> r = sub(B.v[k], Q);
+    gb.slack[k] = dot(m, r);
+    Lb = fmax(Lb, fabs(r.x) + fabs(r.y) + fabs(r.z));
+    M = fmax(M, fmax(fmax(fabs(B.v[k].x), fabs(B.v[k].y)), fabs(B.v[k].z)));
+    const float x = static_cast<float>(r.x), y = static_cast<float>(r.y),
+                z = static_cast<float>(r.z);
+    gb.px[k] = x * e1x + y * e1y + z * e1z;
+    gb.py[k] = x * e2x + y * e2y + z * e2z;
+  }
+  const double L = La + Lb;  // the 1-norms bound the 2-norms
+  const double tol = 0x1.0p-48 * (L + M + M * L * inv_d);
+  ga.on = gb.on = 0u;
+  bool cert = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    cert = cert && (ga.slack[k] >= -tol) && (gb.slack[k] >= -tol);
+    ga.on |= (ga.slack[k] <= 4.0 * tol) ? (1u << k) : 0u;
+    gb.on |= (gb.slack[k] <= 4.0 * tol) ? (1u << k) : 0u;
+  }
+  if (!cert) return kUnresolvedAll;
+  // in-plane distances between the vertices of each side (01, 02, 12)
+  float da[3], db[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
+    const float ux = ga.px[j] - ga.px[i], uy = ga.py[j] - ga.py[i];
+    const float vx = gb.px[j] - gb.px[i], vy = gb.py[j] - gb.py[i];
+    da[k] = std::sqrt(ux * ux + uy * uy);
+    db[k] = std::sqrt(vx * vx + vy * vy);
+  }
+  // m.(y - x) >= d - 4 tol when x, y use on-plane vertices (their slacks are
+  // >= -tol each, computed within tol), and n's own rounding
+  const double dlo = fmax(d * (1.0 - 0x1.0p-50) - 4.0 * tol, 0.0);
+  const double widen = 0x1.0p-16 * L;
+  const double lv = L + Vw;
+  const double thr_lin = Vw * (1.0 + 0x1.0p-40) + 0x1.0p-45 * lv;
+  const double thr = thr_lin * thr_lin + 0x1.0p-42 * lv * lv;
+  unsigned open = 0u;
+#pragma unroll 1
+  for (int g = 0; g < 15; ++g) {
+    unsigned ma, mb;
+    feature_vertices(g, ma, mb);
+    const unsigned a0 = ma & ga.on, a1 = ma & ~ga.on, b0 = mb & gb.on, b1 = mb & ~gb.on;
+    // least off-plane slack of each side (less the slacks' rounding), and
+    // the largest in-plane step from an on-plane to an off-plane vertex
+    double sa = 1e300, sb = 1e300;
+    float pa = 0.f, pb = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const bool ak = (a1 >> k) & 1u, bk = (b1 >> k) & 1u;
+      sa = ak ? fmin(sa, fmax(0.0, ga.slack[k] - tol)) : sa;
+      sb = bk ? fmin(sb, fmax(0.0, gb.slack[k] - tol)) : sb;
+      // pair k joins vertices (i, j): an on-plane one and an off-plane one
+      const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
+      const bool pa_k = (((a0 >> i) & 1u) && ((a1 >> j) & 1u)) || (((a0 >> j) & 1u) && ((a1 >> i) & 1u));
+      const bool pb_k = (((b0 >> i) & 1u) && ((b1 >> j) & 1u)) || (((b0 >> j) & 1u) && ((b1 >> i) & 1u));
+      pa = pa_k ? fmaxf(pa, da[k]) : pa;
+      pb = pb_k ? fmaxf(pb, db[k]) : pb;
+    }
+    double lb2;
+    if (!a0 || !b0) {
+      // a side entirely off the plane: the slab bound
+      const double lb = dlo + (a0 ? 0.0 : sa) + (b0 ? 0.0 : sb);
+      lb2 = lb * lb;
+    } else {
+      const double rho = fmax(0.0, static_cast<double>(inplane_gap(ga, a0, gb, b0)) - widen);
+      const double PA = a1 ? static_cast<double>(pa) + widen : 0.0;
+      const double PB = b1 ? static_cast<double>(pb) + widen : 0.0;
+      // slack per unit of in-plane step (the cheaper side)
+      const double ka = PA > 0.0 ? sa / PA : 1e300, kb = PB > 0.0 ? sb / PB : 1e300;
+      const double kappa = fmin(ka, kb);
+      const double U = PA + PB;
+      // min over u >= 0 of (dlo + kappa u)^2 + (rho - u)^2 is
+      // (dlo + kappa rho)^2 / (1 + kappa^2) at u* = (rho - kappa dlo) /
+      // (1 + kappa^2) when 0 < u* <= U; at u = 0 when u* <= 0; at u = U when
+      // u* exceeds the largest step U
+      const bool capped = U < rho && kappa < 1e150 &&
+                          (rho - kappa * dlo) > U * (1.0 + kappa * kappa) * (1.0 + 0x1.0p-20);
+      const double hU = dlo + kappa * U, rU = rho - U;
+      const double free_min = kappa < 1e150 ? (dlo + kappa * rho) * (dlo + kappa * rho) / (1.0 + kappa * kappa)
+                                            : dlo * dlo + rho * rho;
+      // u* <= 0: the minimum over u >= 0 is at u = 0 (a rounding of u*'s
+      // sign changes the value only at second order in the rounding)
+      const bool at0 = U == 0.0 || !(rho - kappa * dlo > 0.0);
+      lb2 = at0 ? dlo * dlo + rho * rho : (capped ? hU * hU + rU * rU : free_min);
+    }
+    open |= (g == win || g == other || lb2 > thr) ? 0u : (1u << g);
+  }
+  return open;
+}
+
 DH_HD bool certify(const Tri<double>& A, const Tri<double>& B, const V3<double>& P,
                    const V3<double>& Q) {
   const V3<double> n = sub(Q, P);
@@ -84,6 +372,34 @@ DH_HD bool certify(const Tri<double>& A, const Tri<double>& B, const V3<double>&
     ok = ok && (dot(n, sub(Q, B.v[k])) <= tol);
   }
   return ok;
+}
+
+// The features of `open` besides the two candidates e (edge-edge, value Ve,
+// points Pe, Qe) and v (vertex-face) evaluated with bfSolver's arithmetic and
+// folded into the two family minima with bfSolver's order (least value,
+// lowest code on equal values).  Every feature outside `open` has a value
+// above min(Ve, Vv) (exact_choice), so bfSolver's minima are among the
+// evaluated ones; the caller then applies bfSolver's final rule
+// (edge-edge when Vv > Ve).
+DH_HD void fold_feature(const Tri<double>& A, const Tri<double>& B, int g, int& e, double& Ve,
+                        V3<double>& Pe, V3<double>& Qe, int& v, double& Vv, V3<double>& Pv,
+                        V3<double>& Qv) {
+  V3<double> P, Q;
+  if (g < 9) {
+    const double val = eval_ee_value(A, B, g, P, Q);
+    const bool better = val < Ve || (val == Ve && g < e);
+    e = better ? g : e;
+    Ve = better ? val : Ve;
+    Pe = sel3(better, P, Pe);
+    Qe = sel3(better, Q, Qe);
+  } else {
+    const double val = eval_vf_value(A, B, g, P, Q);
+    const bool better = val < Vv || (val == Vv && g < v);
+    v = better ? g : v;
+    Vv = better ? val : Vv;
+    Pv = sel3(better, P, Pv);
+    Qv = sel3(better, Q, Qv);
+  }
 }
 
 // (s, t) moved into the parameter triangle s, t >= 0, s + t <= 1.

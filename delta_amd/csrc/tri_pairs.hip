@@ -4,6 +4,7 @@
 #include <cmath>
 
 #include "exact_math.hpp"
+#include "gen_pairs.hpp"
 #include "runtime.hpp"
 #include "tri_pairs.hpp"
 #include "zoltan_math.hpp"
@@ -79,64 +80,8 @@ __global__ __launch_bounds__(256) void k_tri_pairs_exact(const double* __restric
 }
 
 // ----------------------------------------------------------------- generator
-// splitmix64 (Steele, Lea, Flood 2014): counter-based uniform stream per pair.
-__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
-  z += 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-struct Stream {
-  uint64_t key;
-  uint64_t ctr;
-  __device__ double uniform() {  // [0, 1)
-    const uint64_t v = splitmix64(key + (ctr++) * 0x9E3779B97F4A7C15ull);
-    return static_cast<double>(v >> 11) * 0x1.0p-53;
-  }
-};
-
-__device__ __forceinline__ V3<double> cross(const V3<double>& a, const V3<double>& b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-__device__ __forceinline__ V3<double> axpy(double s, const V3<double>& a, const V3<double>& b) {
-  return {s * a.x + b.x, s * a.y + b.y, s * a.z + b.z};
-}
-__device__ __forceinline__ V3<double> scale(double s, const V3<double>& a) {
-  return {s * a.x, s * a.y, s * a.z};
-}
-__device__ __forceinline__ V3<double> normalize(const V3<double>& a) {
-  return scale(1.0 / sqrt(dot(a, a)), a);
-}
-
-// does segment pq cross triangle abc (strictly)?
-__device__ bool seg_crosses_tri(const V3<double>& p, const V3<double>& q, const V3<double>& a,
-                                const V3<double>& b, const V3<double>& c) {
-  const V3<double> n = cross(sub(b, a), sub(c, a));
-  const double o1 = dot(n, sub(p, a)), o2 = dot(n, sub(q, a));
-  if ((o1 > 0 && o2 > 0) || (o1 < 0 && o2 < 0)) return false;
-  const V3<double> r = sub(q, p);
-  const double v0 = dot(r, cross(sub(a, p), sub(b, p)));
-  const double v1 = dot(r, cross(sub(b, p), sub(c, p)));
-  const double v2 = dot(r, cross(sub(c, p), sub(a, p)));
-  return (v0 >= 0 && v1 >= 0 && v2 >= 0) || (v0 <= 0 && v1 <= 0 && v2 <= 0);
-}
-
-__device__ bool tris_intersect(const Tri<double>& A, const Tri<double>& B) {
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    if (seg_crosses_tri(A.v[k], A.v[(k + 1) % 3], B.v[0], B.v[1], B.v[2])) return true;
-    if (seg_crosses_tri(B.v[k], B.v[(k + 1) % 3], A.v[0], A.v[1], A.v[2])) return true;
-  }
-  return false;
-}
-
-__device__ double bf_dist(const Tri<double>& A, const Tri<double>& B) {
-  V3<double> p, q;
-  bf_solver(A, B, p, q);
-  return pq_dist(p, q);
-}
-
+// One lane per pair (gen_pairs.hpp); pair i writes only its own column i of
+// the 18 x n SoA block and entry i of target and kind.
 __global__ __launch_bounds__(256) void k_gen_tri_pairs(uint64_t seed, int64_t n,
                                                        const GenParams gp,
                                                        double* __restrict__ soa,
@@ -144,74 +89,17 @@ __global__ __launch_bounds__(256) void k_gen_tri_pairs(uint64_t seed, int64_t n,
                                                        uint8_t* __restrict__ kind) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Stream ra{splitmix64(seed * 2 + 1) ^ splitmix64(static_cast<uint64_t>(i)), 0};
-  Stream rb{splitmix64(seed * 2 + 2) ^ splitmix64(static_cast<uint64_t>(i) + 0x1234567ull), 0};
-  const double L = gp.box;
   Tri<double> A, B;
-  for (int k = 0; k < 3; ++k) A.v[k] = {L * ra.uniform(), L * ra.uniform(), L * ra.uniform()};
-  const double u_kind = rb.uniform();
-  const double lmin = log(gp.sep_min), lmax = log(gp.sep_max);
-  const double s = exp(lmin + rb.uniform() * (lmax - lmin));
+  double s;
   uint8_t kd;
-  if (u_kind < gp.frac_parallel) {
-    kd = 1;
-    // an edge of B nearly parallel (angle < max_angle) to edge a0a1 of A, at
-    // distance s in the edge's outward Voronoi wedge; B lies beyond it.
-    const V3<double> e = sub(A.v[1], A.v[0]);
-    const double len = sqrt(dot(e, e));
-    const V3<double> eh = scale(1.0 / len, e);
-    const V3<double> g = sub(A.v[0], A.v[2]);
-    const V3<double> m = normalize(axpy(-dot(g, eh), eh, g));
-    const V3<double> nA = normalize(cross(e, sub(A.v[2], A.v[0])));
-    const double phi = (2.0 * rb.uniform() - 1.0) * 1.0;
-    const V3<double> w = axpy(cos(phi), m, scale(sin(phi), nA));
-    const double tau0 = 0.1 + 0.3 * rb.uniform();
-    const V3<double> b0 = axpy(s, w, axpy(tau0, e, A.v[0]));
-    const double th = gp.max_angle * rb.uniform();
-    const V3<double> dir = axpy(cos(th), eh, scale(sin(th), cross(w, eh)));
-    const V3<double> b1 = axpy((0.3 + 0.3 * rb.uniform()) * len, dir, b0);
-    const V3<double> mid = scale(0.5, V3<double>{b0.x + b1.x, b0.y + b1.y, b0.z + b1.z});
-    const V3<double> b2 = axpy((0.3 + 0.7 * rb.uniform()) * len, w, mid);
-    B.v[0] = b0;
-    B.v[1] = b1;
-    B.v[2] = b2;
-  } else {
-    for (int k = 0; k < 3; ++k) B.v[k] = {L * rb.uniform(), L * rb.uniform(), L * rb.uniform()};
-    // centre B on A's centroid
-    V3<double> ca = {(A.v[0].x + A.v[1].x + A.v[2].x) / 3, (A.v[0].y + A.v[1].y + A.v[2].y) / 3,
-                     (A.v[0].z + A.v[1].z + A.v[2].z) / 3};
-    V3<double> cb = {(B.v[0].x + B.v[1].x + B.v[2].x) / 3, (B.v[0].y + B.v[1].y + B.v[2].y) / 3,
-                     (B.v[0].z + B.v[1].z + B.v[2].z) / 3};
-    const V3<double> sh = sub(ca, cb);
-    for (int k = 0; k < 3; ++k) B.v[k] = axpy(1.0, sh, B.v[k]);
-    if (u_kind < gp.frac_parallel + gp.frac_intersect) {
-      kd = 2;  // pushed through A: centroids coincide
-    } else {
-      kd = 0;
-      // uniform axis; the true distance along t*axis is convex in t and zero
-      // at t = 0, so bisection on "separated and bf distance >= s" converges
-      const double z = 2.0 * rb.uniform() - 1.0;
-      const double ph = 6.283185307179586 * rb.uniform();
-      const double rr = sqrt(fmax(0.0, 1.0 - z * z));
-      const V3<double> ax = {rr * cos(ph), rr * sin(ph), z};
-      double lo = 0.0, hi = s + 4.0 * L;
-      for (int it = 0; it < gp.bisection_steps; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        Tri<double> Bm;
-        for (int k = 0; k < 3; ++k) Bm.v[k] = axpy(mid, ax, B.v[k]);
-        const bool ok = !tris_intersect(A, Bm) && bf_dist(A, Bm) >= s;
-        if (ok) hi = mid; else lo = mid;
-      }
-      for (int k = 0; k < 3; ++k) B.v[k] = axpy(hi, ax, B.v[k]);
-    }
-  }
+  gen_pair(seed, i, gp, A, B, s, kd);
   for (int k = 0; k < 3; ++k) {
-    soa[(0 + k) * n + i] = A.v[k].x;
-    soa[(3 + k) * n + i] = A.v[k].y;
-    soa[(6 + k) * n + i] = A.v[k].z;
-    soa[(9 + k) * n + i] = B.v[k].x;
-    soa[(12 + k) * n + i] = B.v[k].y;
-    soa[(15 + k) * n + i] = B.v[k].z;
+    soa[soa_index(0 + k, n, i)] = A.v[k].x;
+    soa[soa_index(3 + k, n, i)] = A.v[k].y;
+    soa[soa_index(6 + k, n, i)] = A.v[k].z;
+    soa[soa_index(9 + k, n, i)] = B.v[k].x;
+    soa[soa_index(12 + k, n, i)] = B.v[k].y;
+    soa[soa_index(15 + k, n, i)] = B.v[k].z;
   }
   if (target) target[i] = s;
   if (kind) kind[i] = kd;

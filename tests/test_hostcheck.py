@@ -141,3 +141,79 @@ def test_feature_evaluation_and_certificate(hc):
     assert call[sep].any(axis=1).mean() > 0.999
     # intersecting triangles: no feature ever certifies
     assert not call[cross].any()
+
+
+def _decide(hc, s, ce, cv):
+    n = s.shape[1]
+    P = np.zeros(3 * n); Q = np.zeros(3 * n); st = np.zeros(n, np.int32)
+    hc.hc_learned_decide(ol.dptr(s), C.c_long(n), ce.ctypes.data_as(C.c_void_p),
+                         cv.ctypes.data_as(C.c_void_p), ol.dptr(P), ol.dptr(Q),
+                         st.ctypes.data_as(C.c_void_p))
+    return P, Q, st
+
+
+def host_pairs(hc, n, seed=3):
+    """n pairs of the unique-pair workload from the host build of its
+    generator (gen_pairs.hpp), with their kinds; asserts that every column
+    index stayed inside the 18 x n block."""
+    s = np.zeros(18 * n); t = np.zeros(n); k = np.zeros(n, np.uint8)
+    f = hc.hc_gen_tri_pairs
+    f.restype = C.c_long
+    f.argtypes = [C.c_ulonglong, C.c_long, C.c_long] + [C.c_double] * 6 + [C.c_int] + \
+        [C.c_void_p] * 3
+    bad = f(seed, 0, n, 1e-3, 1e-7, 1e-2, 0.05, 0.01, 1e-4, 48, s.ctypes.data, t.ctypes.data,
+            k.ctypes.data)
+    assert bad == 0
+    return s.reshape(18, n), t, k
+
+
+def test_exact_choice_is_bfsolver_bit_for_bit(hc):
+    """The learned path's decision (pair_features.hpp: two candidate
+    features, bfSolver's rule, exact_choice(), tie folding), host build,
+    against the oracle's bfSolver: whenever it accepts a pair its P and Q
+    are the oracle's bit for bit, for the ideal candidates (bfSolver's own
+    two minima), for the runner-up edge-edge feature and for random
+    candidates (an untrained network).  Pairs: the unique-pair workload's
+    distribution, the compiled reference's golden pairs and hand-built
+    degenerate pairs (ties, parallel edges and faces, touching, intersecting).
+    With the ideal candidates it accepts all separated generated pairs but a
+    few near-parallel ones."""
+    import scenes
+    gen, _, kind = host_pairs(hc, 1 << 14)
+    g = np.load(os.path.join(ol.ROOT, "tests", "golden", "pair_solvers.npz"))
+    s = np.ascontiguousarray(np.concatenate([gen, soa_of(g["A"], g["B"]),
+                                             scenes.degenerate_pairs(motions=24)], axis=1))
+    n = s.shape[1]
+    ngen = gen.shape[1]
+    P0 = np.zeros(3 * n); Q0 = np.zeros(3 * n); d0 = np.zeros(n)
+    ol.oracle().or_tri_pairs(0, ol.dptr(s), C.c_long(n), C.c_double(2e-4), ol.dptr(P0),
+                             ol.dptr(Q0), ol.dptr(d0), None)
+    vals = np.zeros(15 * n)
+    hc.hc_feature_values(ol.dptr(s), C.c_long(n), ol.dptr(vals))
+    vals = vals.reshape(n, 15)
+    # first index of the least value, as bfSolver's strict < loops pick
+    ve = np.where(np.isnan(vals[:, :9]), np.inf, vals[:, :9])
+    vv = np.where(np.isnan(vals[:, 9:]), np.inf, vals[:, 9:])
+    ideal_e = np.argmin(ve, axis=1).astype(np.int32)
+    ideal_v = (9 + np.argmin(vv, axis=1)).astype(np.int32)
+    second_e = np.argsort(ve, axis=1, kind="stable")[:, 1].astype(np.int32)
+    rng = np.random.default_rng(7)
+    rand_e = rng.integers(0, 9, n).astype(np.int32)
+    rand_v = rng.integers(9, 15, n).astype(np.int32)
+    P3 = P0.reshape(3, n); Q3 = Q0.reshape(3, n)
+    for name, ce, cv in (("ideal", ideal_e, ideal_v), ("runner-up", second_e, ideal_v),
+                         ("random", rand_e, rand_v)):
+        P, Q, st = _decide(hc, s, ce, cv)
+        acc = st > 0
+        Pa = P.reshape(3, n)[:, acc]; Qa = Q.reshape(3, n)[:, acc]
+        assert np.array_equal(bits(Pa), bits(P3[:, acc])), name
+        assert np.array_equal(bits(Qa), bits(Q3[:, acc])), name
+        if name == "ideal":
+            sep = kind != 2
+            assert acc[:ngen][sep].mean() > 0.985, acc[:ngen][sep].mean()
+            assert acc[:ngen][kind == 0].mean() > 0.999
+            # ties were resolved by evaluating a few more features
+            assert (st[:ngen] >= 3).any()
+            assert np.mean(np.maximum(st[:ngen] - 2, 0)) < 1.5
+        if name == "random":
+            assert acc.mean() > 0.05  # random guesses are still right sometimes
