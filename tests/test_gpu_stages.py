@@ -2,6 +2,7 @@
 phase candidate set, batched detection on it, contact forces, and the fused
 update given identical forces.  Each stage starts from identical inputs, so a
 mismatch names its stage."""
+import copy
 import ctypes as C
 
 import numpy as np
@@ -164,3 +165,32 @@ def test_exclusive_scan(n):
     _lib.check(L.dh_exclusive_scan(d_counts.ptr, n, offs.ptr, None), "scan")
     want = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])
     assert np.array_equal(offs.download(), want)
+
+
+def test_broadphase_after_eager_steps_equals_step_and_all_pairs():
+    """dh_broadphase on the C1 lattice before each of 24 eager dh_step calls:
+    its candidate set equals the all-pairs box test on the downloaded
+    geometry, and its count equals the candidate count of the step, which
+    runs the same broad phase on the same geometry.  The lattice's initial
+    overlaps fling the particles apart (484 pairs at step 0, none at step 8),
+    so the comparison covers full and empty candidate sets."""
+    hs = engine.c1_scene()
+    ds = engine.DeviceScene(hs)
+    p = engine.step_params(method=_lib.DH_BF, dt=1e-6, spring=1e5, damper=0.1, friction=0.0,
+                           cell=hs.cell_size())
+    cell = hs.cell_size()
+    L = _lib.lib()
+    counts = []
+    for s in range(24):
+        _lib.check(L.dh_synchronize(), "sync")
+        got, _ = gpu_pairs(ds, hs, cell)
+        again, _ = gpu_pairs(ds, hs, cell)
+        cur = copy.copy(hs)
+        cur.x, cur.y, cur.z = ds.download("x"), ds.download("y"), ds.download("z")
+        want = host_candidates(cur)
+        info = engine.step(ds, p)
+        assert len(got) == len(want) == info.candidate_pairs, \
+            (s, len(got), len(again), len(want), info.candidate_pairs)
+        assert got == want and again == want, s
+        counts.append(len(want))
+    assert counts[0] > 400 and 0 in counts, counts
