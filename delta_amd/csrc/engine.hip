@@ -278,9 +278,37 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
   return DH_OK;
 }
 
+}  // extern "C"
+
+namespace dh {
+
+// Every replay shares the library's scratch with every other graph's
+// replays: each launch waits for the previous launch of any graph (an event
+// recorded after it), on whatever stream either ran.
+static hipEvent_t g_replay_done = nullptr;
+static bool g_replay_recorded = false;
+
+}  // namespace dh
+
+extern "C" {
+
 int dh_step_graph_launch(dh_step_graph* g, void* stream) {
   if (!g || !g->exec) return set_error(DH_ERR_ARG, "dh_step_graph_launch: no graph");
-  DH_HIP_CHECK(hipGraphLaunch(g->exec, stream ? as_stream(stream) : g->stream));
+  const hipStream_t st = stream ? as_stream(stream) : g->stream;
+  if (stream) {
+    // a non-blocking stream would not wait for the eager entry points' work
+    // on the null stream, which uses the same scratch
+    unsigned flags = 0;
+    DH_HIP_CHECK(hipStreamGetFlags(st, &flags));
+    if (flags & hipStreamNonBlocking)
+      return set_error(DH_ERR_ARG, "dh_step_graph_launch: non-blocking stream");
+  }
+  if (!g_replay_done)
+    DH_HIP_CHECK(hipEventCreateWithFlags(&g_replay_done, hipEventDisableTiming));
+  if (g_replay_recorded) DH_HIP_CHECK(hipStreamWaitEvent(st, g_replay_done, 0));
+  DH_HIP_CHECK(hipGraphLaunch(g->exec, st));
+  DH_HIP_CHECK(hipEventRecord(g_replay_done, st));
+  g_replay_recorded = true;
   return DH_OK;
 }
 

@@ -557,7 +557,13 @@ int dh_step(dh_scene* scene, const dh_step_params* params, dh_step_info* info, v
  * restores its own copy of the state and recreates the graph with larger
  * capacities.
  * A graph keeps the scene pointers it was created with, and shares the
- * library's scratch with the other entry points: serialise their use. */
+ * library's scratch with the other entry points.  dh_step_graph_launch
+ * replays on `stream`, or on the graph's own blocking stream when NULL; a
+ * non-blocking stream is rejected (DH_ERR_ARG), since it would not wait for
+ * eager work on the null stream.  Each launch waits for the previous launch
+ * of any graph, so replays of different graphs never overlap; eager entry
+ * points called on a stream other than the null stream must be ordered with
+ * replays by the caller. */
 typedef struct dh_step_graph dh_step_graph;
 int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, int64_t pair_cap,
                          int64_t contact_cap, dh_step_graph** out);
