@@ -12,6 +12,7 @@
 // non-orthogonal rotation update (dynamics.cpp:78) can cause -- are kept out
 // of the grid and tested against every particle.  Output pairs are sorted (i ascending, then
 // j ascending) regardless of hash-bucket order.
+#include <mutex>
 #include <vector>
 
 #include "runtime.hpp"
@@ -454,6 +455,9 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_grid_fill(GridArgs g,
 }
 
 int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
+int scan_counts_with(const int* counts, int n, long long* offs, long long* sums,
+                     hipStream_t st);
+int scan_sums_len(int n);
 int scan_reserve(int n);
 
 struct BpScratch {
@@ -484,6 +488,38 @@ static unsigned table_size(int n) {
   unsigned tbl = 1;
   while (tbl < 2u * static_cast<unsigned>(n)) tbl <<= 1;
   return tbl;
+}
+
+// The device buffers of one broad phase over n particles and ntri
+// triangles (entries of each, in order of the layout below).
+struct BpBuffers {
+  Aabb* box;           // n + 1
+  Aabb* cbox;          // ntri / 64 + n + 1
+  int* ccount;         // tbl + 1
+  int* cfill;          // tbl
+  long long* cstart;   // tbl + 1
+  int* sorted;         // n + 1
+  unsigned* cellh;     // n + 1
+  int* count;          // n + 1
+  long long* start;    // n + 1
+  int* nlarge;         // 1
+  int* large;          // n + 1
+  int* lcount;         // (n + 1) * large_chunks(n)
+  int* keep;           // (n + 1) * kBpKeep
+  long long* sums;     // scan_sums_len(max(tbl, n)); null: the scan's own scratch
+};
+
+static BpBuffers library_buffers() {
+  return BpBuffers{g_bp.box.as<Aabb>(),     g_bp.cbox.as<Aabb>(),    g_bp.ccount.as<int>(),
+                   g_bp.cfill.as<int>(),    g_bp.cstart.as<long long>(), g_bp.sorted.as<int>(),
+                   g_bp.cellh.as<unsigned>(), g_bp.count.as<int>(),  g_bp.start.as<long long>(),
+                   g_bp.err.as<int>(),      g_bp.obst.as<int>(),     g_bp.lcount.as<int>(),
+                   g_bp.keep.as<int>(),     nullptr};
+}
+
+static int scan_into(const BpBuffers& b, const int* counts, int n, long long* offs,
+                     hipStream_t st) {
+  return b.sums ? scan_counts_with(counts, n, offs, b.sums, st) : scan_counts(counts, n, offs, st);
 }
 
 // The particle boxes of the last broad phase (lo xyz, hi xyz per particle:
@@ -529,91 +565,168 @@ int broadphase_reserve(int n, int ntri) {
   return scan_reserve(static_cast<int>(tbl) > n ? static_cast<int>(tbl) : n);
 }
 
+// Clears the counters the grid passes add to (fill kernels, no memset, so
+// that captured work may contain it).
+static void grid_clear(const BpBuffers& b, unsigned tbl, hipStream_t st) {
+  zero_async(ZeroRegions{{reinterpret_cast<unsigned*>(b.nlarge),
+                          reinterpret_cast<unsigned*>(b.ccount),
+                          reinterpret_cast<unsigned*>(b.cfill), nullptr},
+                         {1, static_cast<long long>(tbl), static_cast<long long>(tbl), 0}},
+             st);
+}
+
 // Boxes, the sorted cell list and the partner counts, on cleared counters.
-static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double cell, GridArgs& g,
-                      hipStream_t st) {
+static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double cell,
+                      const BpBuffers& b, GridArgs& g, hipStream_t st) {
   const int n = geom->nparticles;
   const unsigned tbl = table_size(n);
   hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
-                     geom->eps, g_bp.box.as<Aabb>(), g_bp.cbox.as<Aabb>());
-  g.box = g_bp.box.as<Aabb>();
+                     geom->eps, b.box, b.cbox);
+  g.box = b.box;
   g.obstacle = obstacle_dev;
   g.n = n;
   g.h = cell;
   g.inv_h = 1.0 / cell;
   g.mask = tbl - 1;
-  g.ccount = g_bp.ccount.as<int>();
-  g.cfill = g_bp.cfill.as<int>();
-  g.cstart = g_bp.cstart.as<long long>();
-  g.sorted = g_bp.sorted.as<int>();
-  g.cellh = g_bp.cellh.as<unsigned>();
-  g.large = g_bp.obst.as<int>();
-  g.nlarge = g_bp.err.as<int>();
+  g.ccount = b.ccount;
+  g.cfill = b.cfill;
+  g.cstart = b.cstart;
+  g.sorted = b.sorted;
+  g.cellh = b.cellh;
+  g.large = b.large;
+  g.nlarge = b.nlarge;
   g.chunk = large_chunk_len(n);
   g.nchunks = large_chunks(n);
-  g.lcount = g_bp.lcount.as<int>();
-  g.keep = g_bp.keep.as<int>();
+  g.lcount = b.lcount;
+  g.keep = b.keep;
   hipLaunchKernelGGL(k_grid_bin, grid_for(n, 256), 256, 0, st, g);
   int rc;
-  if ((rc = scan_counts(g_bp.ccount.as<int>(), static_cast<int>(tbl), g_bp.cstart.as<long long>(),
-                        st)))
-    return rc;
+  if ((rc = scan_into(b, b.ccount, static_cast<int>(tbl), b.cstart, st))) return rc;
   hipLaunchKernelGGL(k_grid_scatter, grid_for(n, 256), 256, 0, st, g);
-  hipLaunchKernelGGL(k_grid_count2, grid_for(n, 2 * kBpWaves), 64 * kBpWaves, 0, st, g,
-                     g_bp.count.as<int>());
+  hipLaunchKernelGGL(k_grid_count2, grid_for(n, 2 * kBpWaves), 64 * kBpWaves, 0, st, g, b.count);
   const long long items = static_cast<long long>(n) * g.nchunks;  // bound on the device count
   hipLaunchKernelGGL(k_large_count, large_grid(items), 64 * kBpWaves, 0, st, g);
-  hipLaunchKernelGGL(k_large_sum, large_grid(n), 64 * kBpWaves, 0, st, g, g_bp.count.as<int>());
+  hipLaunchKernelGGL(k_large_sum, large_grid(n), 64 * kBpWaves, 0, st, g, b.count);
   DH_HIP_CHECK(hipGetLastError());
-  return scan_counts(g_bp.count.as<int>(), n, g_bp.start.as<long long>(), st);
+  return scan_into(b, b.count, n, b.start, st);
 }
 
-int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
-                    long long cap, long long* npairs, hipStream_t st) {
+static void grid_fill(const GridArgs& g, const BpBuffers& b, long long cap, int* pairs,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_grid_fill, grid_for(g.n, kBpWaves), 64 * kBpWaves, 0, st, g, b.start, cap,
+                     pairs);
+  hipLaunchKernelGGL(k_large_fill, large_grid(static_cast<long long>(g.n) * g.nchunks),
+                     64 * kBpWaves, 0, st, g, b.start, cap, pairs);
+}
+
+// The eager broad phase on the buffers b: the total is read back, and the
+// pairs are written only if it fits cap.
+static int broadphase_run(const dh_geometry* geom, const int* obstacle_dev, double cell,
+                          const BpBuffers& b, int* pairs, long long cap, long long* npairs,
+                          hipStream_t st) {
   const int n = geom->nparticles;
-  int rc;
-  if ((rc = broadphase_reserve(n, geom->ntri_total))) return rc;
-  const unsigned tbl = table_size(n);
-  DH_HIP_CHECK(hipMemsetAsync(g_bp.err.ptr, 0, sizeof(int), st));
-  DH_HIP_CHECK(hipMemsetAsync(g_bp.ccount.ptr, 0, sizeof(int) * tbl, st));
-  DH_HIP_CHECK(hipMemsetAsync(g_bp.cfill.ptr, 0, sizeof(int) * tbl, st));
+  grid_clear(b, table_size(n), st);
   GridArgs g;
-  if ((rc = grid_build(geom, obstacle_dev, cell, g, st))) return rc;
+  int rc;
+  if ((rc = grid_build(geom, obstacle_dev, cell, b, g, st))) return rc;
   long long total = 0;
-  DH_HIP_CHECK(hipMemcpyAsync(&total, g_bp.start.as<long long>() + n, sizeof(long long),
-                              hipMemcpyDeviceToHost, st));
+  DH_HIP_CHECK(hipMemcpyAsync(&total, b.start + n, sizeof(long long), hipMemcpyDeviceToHost, st));
   DH_HIP_CHECK(hipStreamSynchronize(st));
   *npairs = total;
   if (*npairs > cap) return DH_ERR_CAPACITY;
-  hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
-                     g_bp.start.as<long long>(), cap, pairs);
-  hipLaunchKernelGGL(k_large_fill, large_grid(static_cast<long long>(n) * g.nchunks),
-                     64 * kBpWaves, 0, st, g, g_bp.start.as<long long>(), cap, pairs);
+  grid_fill(g, b, cap, pairs, st);
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
 }
 
-// The broad phase for a captured step: the same kernels with fills instead of
-// memsets and no host read-back.  *np_dev (the total, which may exceed cap)
-// is left at the returned device address; pairs beyond cap are not written.
+int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
+                    long long cap, long long* npairs, hipStream_t st) {
+  int rc;
+  if ((rc = broadphase_reserve(geom->nparticles, geom->ntri_total))) return rc;
+  return broadphase_run(geom, obstacle_dev, cell, library_buffers(), pairs, cap, npairs, st);
+}
+
+// The broad phase for a captured step: the same kernels with no host
+// read-back.  *np_dev (the total, which may exceed cap) is left at the
+// returned device address; pairs beyond cap are not written.
 int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
                      long long cap, const long long** np_dev, hipStream_t st) {
   const int n = geom->nparticles;
   int rc;
   if ((rc = broadphase_reserve(n, geom->ntri_total))) return rc;
-  const unsigned tbl = table_size(n);
-  zero_async(ZeroRegions{{g_bp.err.as<unsigned>(), g_bp.ccount.as<unsigned>(),
-                          g_bp.cfill.as<unsigned>(), nullptr},
-                         {1, static_cast<long long>(tbl), static_cast<long long>(tbl), 0}},
-             st);
+  const BpBuffers b = library_buffers();
+  grid_clear(b, table_size(n), st);
   GridArgs g;
-  if ((rc = grid_build(geom, obstacle_dev, cell, g, st))) return rc;
-  hipLaunchKernelGGL(k_grid_fill, grid_for(n, kBpWaves), 64 * kBpWaves, 0, st, g,
-                     g_bp.start.as<long long>(), cap, pairs);
-  hipLaunchKernelGGL(k_large_fill, large_grid(static_cast<long long>(n) * g.nchunks),
-                     64 * kBpWaves, 0, st, g, g_bp.start.as<long long>(), cap, pairs);
+  if ((rc = grid_build(geom, obstacle_dev, cell, b, g, st))) return rc;
+  grid_fill(g, b, cap, pairs, st);
   DH_HIP_CHECK(hipGetLastError());
-  *np_dev = g_bp.start.as<long long>() + n;
+  *np_dev = b.start + n;
+  return DH_OK;
+}
+
+// ------------------------------------------------ broad phase, no allocation
+// One statically allocated device array holds every buffer of a broad phase
+// of at most DH_NOALLOC_BP_PARTICLES particles and DH_NOALLOC_BP_TRIANGLES
+// triangles (8,523,264 bytes).  Each device keeps its own copy for the life of
+// the process; every call on a device shares it, so calls that overlap on one
+// device race.
+constexpr size_t kNaAlign = 256;
+
+static size_t na_round(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+
+// Carves the buffers of (n, ntri) out of base; returns the bytes used.
+static size_t noalloc_layout(char* base, int n, int ntri, BpBuffers* b) {
+  const size_t tbl = table_size(n);
+  const size_t np1 = static_cast<size_t>(n) + 1;
+  const int scan_n = static_cast<int>(tbl) > n ? static_cast<int>(tbl) : n;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off += na_round(bytes);
+    return p;
+  };
+  BpBuffers t;
+  t.box = reinterpret_cast<Aabb*>(take(sizeof(Aabb) * np1));
+  t.cbox = reinterpret_cast<Aabb*>(take(sizeof(Aabb) * ((ntri + 63) / 64 + np1)));
+  t.ccount = reinterpret_cast<int*>(take(sizeof(int) * (tbl + 1)));
+  t.cfill = reinterpret_cast<int*>(take(sizeof(int) * tbl));
+  t.cstart = reinterpret_cast<long long*>(take(sizeof(long long) * (tbl + 1)));
+  t.sorted = reinterpret_cast<int*>(take(sizeof(int) * np1));
+  t.cellh = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * np1));
+  t.count = reinterpret_cast<int*>(take(sizeof(int) * np1));
+  t.start = reinterpret_cast<long long*>(take(sizeof(long long) * np1));
+  t.nlarge = reinterpret_cast<int*>(take(sizeof(int)));
+  t.large = reinterpret_cast<int*>(take(sizeof(int) * np1));
+  t.lcount = reinterpret_cast<int*>(take(sizeof(int) * np1 * large_chunks(n)));
+  t.keep = reinterpret_cast<int*>(take(sizeof(int) * np1 * kBpKeep));
+  t.sums = reinterpret_cast<long long*>(take(sizeof(long long) * scan_sums_len(scan_n)));
+  if (b) *b = t;
+  return off;
+}
+
+// Bytes of the largest layout (every buffer grows with n and ntri).
+constexpr size_t kNaScratchBytes = 8523264;
+
+__device__ __align__(256) char scratch[kNaScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_na_ptr[kMaxDevices];
+static std::mutex g_na_mu;
+
+// The device address of this device's `scratch` (looked up on the first
+// call on each device, with that device current).
+static int noalloc_scratch(char** out) {
+  int dev = 0;
+  DH_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(DH_ERR_UNSUPPORTED, "dh_broadphase_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    DH_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_ptr[dev] = p;
+  }
+  *out = static_cast<char*>(g_na_ptr[dev]);
   return DH_OK;
 }
 
@@ -633,4 +746,37 @@ extern "C" int dh_broadphase(const dh_geometry* geom, const int* obstacle_dev, d
   rc = broadphase_impl(geom, obstacle_dev, cell, pairs_dev, cap, &np, as_stream(stream));
   *npairs = static_cast<int>(np);
   return rc;
+}
+
+extern "C" int dh_broadphase_noalloc(const dh_geometry* geom, const int* obstacle_dev,
+                                     double cell, int* pairs_dev, int cap, int* npairs,
+                                     void* stream) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!geom || !npairs || !(cell > 0) || cap < 0 || (cap > 0 && !pairs_dev) ||
+      geom->nparticles < 0 || geom->ntri_total < 0)
+    return set_error(DH_ERR_ARG, "dh_broadphase_noalloc: bad arguments");
+  *npairs = 0;
+  if (geom->nparticles > DH_NOALLOC_BP_PARTICLES || geom->ntri_total > DH_NOALLOC_BP_TRIANGLES)
+    return set_error(DH_ERR_CAPACITY,
+                     "dh_broadphase_noalloc: %d particles / %d triangles exceed the fixed "
+                     "scratch (at most %d / %d)",
+                     geom->nparticles, geom->ntri_total, DH_NOALLOC_BP_PARTICLES,
+                     DH_NOALLOC_BP_TRIANGLES);
+  if (geom->nparticles < 2) return DH_OK;
+  char* base = nullptr;
+  if ((rc = noalloc_scratch(&base))) return rc;
+  BpBuffers b;
+  if (noalloc_layout(base, geom->nparticles, geom->ntri_total, &b) > kNaScratchBytes)
+    return set_error(DH_ERR_CAPACITY, "dh_broadphase_noalloc: scratch layout too large");
+  long long np = 0;
+  rc = broadphase_run(geom, obstacle_dev, cell, b, pairs_dev, cap, &np, as_stream(stream));
+  *npairs = static_cast<int>(np);
+  return rc;
+}
+
+// Bytes the fixed scratch must hold for the largest accepted sizes (a
+// build-time check of kNaScratchBytes, exported for tests).
+extern "C" size_t dh_broadphase_noalloc_scratch_bytes(int nparticles, int ntri) {
+  return noalloc_layout(nullptr, nparticles, ntri, nullptr);
 }

@@ -1195,23 +1195,32 @@ int scan_reserve(int n) {
   return g_scratch.sums.grow(sizeof(long long) * (nb + 1));
 }
 
-// offs[0..n]: exclusive prefix sums of counts, offs[n] = total
-int scan_counts(const int* counts, int n, long long* offs, hipStream_t st) {
+// offs[0..n]: exclusive prefix sums of counts, offs[n] = total; `sums`
+// holds scan_sums_len(n) block sums (unused when n <= 1,024).
+int scan_counts_with(const int* counts, int n, long long* offs, long long* sums,
+                     hipStream_t st) {
   if (n <= 1024) {
     hipLaunchKernelGGL(k_scan_single, 1, 1024, 0, st, counts, n, offs);
     DH_HIP_CHECK(hipGetLastError());
     return DH_OK;
   }
-  const int nb = (n + 1023) / 1024 > 0 ? (n + 1023) / 1024 : 1;
-  int rc = g_scratch.sums.grow(sizeof(long long) * (nb + 1));
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_scan_blocks, nb, 1024, 0, st, counts, n, offs,
-                     g_scratch.sums.as<long long>());
-  hipLaunchKernelGGL(k_scan_sums, 1, 1024, 0, st, g_scratch.sums.as<long long>(), nb);
-  hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, g_scratch.sums.as<long long>(),
-                     counts);
+  const int nb = (n + 1023) / 1024;
+  hipLaunchKernelGGL(k_scan_blocks, nb, 1024, 0, st, counts, n, offs, sums);
+  hipLaunchKernelGGL(k_scan_sums, 1, 1024, 0, st, sums, nb);
+  hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, sums, counts);
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
+}
+
+int scan_sums_len(int n) { return (n + 1023) / 1024 > 0 ? (n + 1023) / 1024 + 1 : 2; }
+
+// The same with the library's own block-sum scratch.
+int scan_counts(const int* counts, int n, long long* offs, hipStream_t st) {
+  if (n > 1024) {
+    int rc = g_scratch.sums.grow(sizeof(long long) * scan_sums_len(n));
+    if (rc) return rc;
+  }
+  return scan_counts_with(counts, n, offs, g_scratch.sums.as<long long>(), st);
 }
 
 // Workgroups of a detection launch over `items`: one per item, or the

@@ -470,6 +470,24 @@ int dh_update_rotation_matrix(double angular[3], const double refAngular[3],
 int dh_broadphase(const dh_geometry* geom, const int* obstacle_dev, double cell,
                   int* pairs_dev, int cap, int* npairs, void* stream);
 
+/* dh_broadphase without any allocation: no counterpart in the reference.
+ * For callers that cannot call hipMalloc (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  Same arguments,
+ * results and errors as dh_broadphase; its working buffers live in one fixed
+ * device array of the library (8,523,264 bytes per device, held until the
+ * process ends) instead of library buffers that grow on demand.  Accepts at
+ * most DH_NOALLOC_BP_PARTICLES particles and DH_NOALLOC_BP_TRIANGLES
+ * triangles (DH_ERR_CAPACITY beyond).  Calls on one device share that array:
+ * calls that overlap on one device race, and must be ordered by the caller
+ * (one stream, or events).  The pair count is still read back to the host
+ * (one stream synchronisation), as in dh_broadphase. */
+#define DH_NOALLOC_BP_PARTICLES 16384
+#define DH_NOALLOC_BP_TRIANGLES (1 << 20)
+int dh_broadphase_noalloc(const dh_geometry* geom, const int* obstacle_dev, double cell,
+                          int* pairs_dev, int cap, int* npairs, void* stream);
+/* Bytes of that array a broad phase of (nparticles, ntri) uses (host only). */
+size_t dh_broadphase_noalloc_scratch_bytes(int nparticles, int ntri);
+
 /* Exclusive prefix sum of n ints into n+1 int64 offsets (offs[n] = total). */
 int dh_exclusive_scan(const int* counts_dev, int n, int64_t* offs_dev, void* stream);
 

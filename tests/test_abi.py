@@ -58,3 +58,18 @@ def test_tuning_table_validates_without_gpu():
     finally:
         tuning.set(**d)
     assert tuning.get() == d
+
+
+def test_noalloc_broadphase_scratch_fits_its_limits():
+    """The fixed array of dh_broadphase_noalloc holds the buffers of the
+    largest accepted broad phase exactly, and every smaller one (host-only
+    layout arithmetic, no device)."""
+    path = os.path.join(ROOT, "delta_amd", "libdelta_hip.so")
+    lib = C.CDLL(path)
+    f = lib.dh_broadphase_noalloc_scratch_bytes
+    f.restype = C.c_size_t
+    f.argtypes = [C.c_int, C.c_int]
+    full = f(16384, 1 << 20)
+    assert full == 8523264
+    for n, t in [(2, 2), (65, 5122), (1000, 80000), (4097, 1 << 20), (16384, 0), (8191, 999999)]:
+        assert 0 < f(n, t) <= full, (n, t)
