@@ -135,13 +135,13 @@ def contact_errors(P0, Q0, d0, P, Q, d, margin=2e-4):
             "max_depth_error_m": float(np.abs(d - d0)[contact].max())}
 
 
-def learned_accuracy(batch):
+def learned_accuracy(batch, method="learned"):
     """Learned results against the FP64 brute force over every pair."""
     import ctypes as C
     from delta_amd import _lib
     batch.run("bf64")
     P0, Q0, d0, _ = batch.results()
-    batch.run("learned")
+    batch.run(method)
     P, Q, d, fl = batch.results()
     cnt = _lib.DhPairCounters()
     _lib.check(_lib.lib().dh_tri_pairs_counters(C.byref(cnt)), "counters")
@@ -158,6 +158,9 @@ def learned_accuracy(batch):
            "pairs_in_band": int(band.sum())}
     out.update(contact_errors(P0, Q0, d0, P, Q, d, margin))
     out["bitwise_equal_fraction"] = float(np.mean(d.view(np.uint64) == d0.view(np.uint64)))
+    out["pq_bitwise_equal_fraction"] = float(np.mean(
+        np.all(P.view(np.uint64) == P0.view(np.uint64), 0) &
+        np.all(Q.view(np.uint64) == Q0.view(np.uint64), 0)))
     return out
 
 

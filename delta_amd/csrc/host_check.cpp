@@ -277,22 +277,26 @@ extern "C" void hc_feature_values(const double* soa, long n, double* vals) {
 
 // The learned path's decision for given candidates (the network's most likely
 // edge-edge feature ce[i] and vertex-face feature cv[i]), as k_learned makes
-// it: both evaluated, bfSolver's rule, exact_choice(), ties folded.  status:
+// it: both evaluated, bfSolver's rule, the acceptance test, ties folded.  status:
 // 0 sent to the brute force (P, Q untouched), 1 accepted as is, 2 + k
 // accepted after folding k tied features; P, Q (component-major) the result.
 // on: on-plane vertex counts at the result (A in bits 0-1, B in bits 2-3).
-extern "C" void hc_learned_decide(const double* soa, long n, const int* ce, const int* cv,
-                                  double* Pout, double* Qout, int* status) {
+static void learned_decide(int kind, const double* soa, long n, const int* ce, const int* cv,
+                           double* Pout, double* Qout, int* status) {
+  const bool exact = kind == 1;
   for (long i = 0; i < n; ++i) {
     Tri<double> A = load(soa, n, i, 0), B = load(soa, n, i, 9);
     int fe = ce[i], fv = cv[i];
     V3<double> Pe, Qe, Pv, Qv;
-    double Ve = dh::eval_ee_value(A, B, fe, Pe, Qe);
-    double Vv = dh::eval_vf_value(A, B, fv, Pv, Qv);
+    unsigned sAe, sBe, sAv, sBv;
+    double Ve = dh::eval_ee_value(A, B, fe, Pe, Qe, sAe, sBe);
+    double Vv = dh::eval_vf_value(A, B, fv, Pv, Qv, sAv, sBv);
     const bool take_e = Vv > Ve;
     const int win0 = take_e ? fe : fv;
     V3<double> P = take_e ? Pe : Pv, Q = take_e ? Qe : Qv;
-    const unsigned open = dh::exact_choice(A, B, P, Q, win0, take_e ? fv : fe, take_e ? Ve : Vv);
+    const unsigned open =
+        exact ? dh::exact_choice(A, B, P, Q, win0, take_e ? fv : fe, take_e ? Ve : Vv)
+              : dh::support_choice(A, B, P, Q, take_e ? sAe : sAv, take_e ? sBe : sBv, win0);
     if (open == dh::kUnresolvedAll) {
       status[i] = 0;
       continue;
@@ -312,3 +316,17 @@ extern "C" void hc_learned_decide(const double* soa, long n, const int* ce, cons
     Qout[i] = Q.x; Qout[n + i] = Q.y; Qout[2 * n + i] = Q.z;
   }
 }
+
+// exact_choice (the proof of bfSolver's choice) and support_choice (the
+// learned path's test).
+extern "C" void hc_learned_decide(const double* soa, long n, const int* ce, const int* cv,
+                                  double* Pout, double* Qout, int* status) {
+  learned_decide(1, soa, n, ce, cv, Pout, Qout, status);
+}
+
+extern "C" void hc_learned_decide_support(const double* soa, long n, const int* ce,
+                                          const int* cv, double* Pout, double* Qout,
+                                          int* status) {
+  learned_decide(0, soa, n, ce, cv, Pout, Qout, status);
+}
+

@@ -61,13 +61,16 @@ __global__ __launch_bounds__(256) void k_labels(const double* __restrict__ soa, 
   bary[4 * i + 1] = clampf(be);
   bary[4 * i + 2] = clampf(ga);
   bary[4 * i + 3] = clampf(de);
-  // exact path: bfSolver's own two minima as the candidates do not pass the
-  // exact-choice test (an intersecting pair, or no closest pair certified)
-  V3<double> Pe, Qe, Pv, Qv;
-  const double Ve = eval_ee_value(A, B, ssid, Pe, Qe);
-  const double Vv = eval_vf_value(A, B, 9 + vid, Pv, Qv);
-  const unsigned open = exact_choice(A, B, P, Q, code, edge ? 9 + vid : ssid, edge ? Ve : Vv);
-  flag[i] = (frame && open != kUnresolvedAll) ? 0.0f : 1.0f;
+  // exact path: bfSolver's own winner does not pass the support check (an
+  // intersecting or touching pair, a near-parallel or near-coplanar closest
+  // pair, a degenerate triangle)
+  V3<double> Pw, Qw;
+  unsigned sA, sB;
+  if (edge)
+    eval_ee_value(A, B, code, Pw, Qw, sA, sB);
+  else
+    eval_vf_value(A, B, code, Pw, Qw, sA, sB);
+  flag[i] = (frame && support_choice(A, B, P, Q, sA, sB, code) != kUnresolvedAll) ? 0.0f : 1.0f;
 }
 
 // ------------------------------------------------------------- forward core
@@ -381,7 +384,7 @@ constexpr int kTiles = 4 + 8 + 2 + 2;  // + W3 output tile 1 (barycentric output
 
 // Inference modes of k_learned.
 enum LearnedMode : int {
-  kModeFeatures = 0,  // feature guesses + certificate (DH_PAIR_LEARNED)
+  kModeFeatures = 0,  // feature guesses + support check, ties folded (DH_PAIR_LEARNED)
   kModeWarm = 1,      // network barycentrics as the penalty start (DH_PAIR_LEARNED_PENALTY)
   kModeCold = 2       // the same penalty from the reference's 0.33 start (DH_PAIR_PENALTY_CERT)
 };
@@ -441,7 +444,7 @@ __device__ inline v8s stage_tile(const float* W, int t, int lane) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(3))) void k_learned(LearnedArgs a) {
+__device__ __forceinline__ void learned_body(const LearnedArgs& a) {
   __shared__ v8s w_s[kTiles * 64];
   __shared__ float b_s[2 * kHid + kOut];
   __shared__ unsigned wg_tally[3];  // certified, features evaluated, queued
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
     if (MODE == kModeFeatures) {
       // the lane's own pair: the best-ranked edge-edge and the best-ranked
       // vertex-face feature, both evaluated (no divergence between the two
-      // kinds), the nearer one certified
+      // kinds), the nearer one checked
       int fe = 0, fv = 9;
       float le = -3.0e38f, lv = -3.0e38f;
 #pragma unroll
@@ -592,26 +595,26 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
       }
       const bool exact_path =
           !frame || (__uint_as_float(io_s[wave][kFlagOut * kCol + lane]) > 0.f);
-      V3<double> Pv, Qv;
-      double Ve = eval_ee_value(A, B, fe, P, Q);
+      V3<double> Pe, Qe, Pv, Qv;
+      unsigned sAe, sBe, sAv, sBv;
+      double Ve = eval_ee_value(A, B, fe, Pe, Qe, sAe, sBe);
       // one feature at a time: an empty asm makes the vertex-face code depend
       // on the edge-edge result, so the two evaluations are not interleaved
       asm volatile("" : "+v"(fv) : "v"(Ve));
-      double Vv = eval_vf_value(A, B, fv, Pv, Qv);
+      double Vv = eval_vf_value(A, B, fv, Pv, Qv, sAv, sBv);
       // bfSolver's final rule between its two minima (bf.cpp:1102)
       const bool take_e = Vv > Ve;
-      P = sel3(take_e, P, Pv);
-      Q = sel3(take_e, Q, Qv);
       const int win0 = take_e ? fe : fv;
-      const unsigned open =
-          (valid && !exact_path)
-              ? exact_choice(A, B, P, Q, win0, take_e ? fv : fe, take_e ? Ve : Vv)
-              : kUnresolvedAll;
+      unsigned open = kUnresolvedAll;
+      if (valid && !exact_path)
+        open = support_choice(A, B, sel3(take_e, Pe, Pv), sel3(take_e, Qe, Qv),
+                              take_e ? sAe : sAv, take_e ? sBe : sBv, win0);
       evals += (valid && !exact_path) ? 2 : 0;
       // Ties of a degenerate closest pair (a vertex against an edge or a
       // vertex): the few features that reach the same points are evaluated
-      // and folded into the two minima in bfSolver's order.  The wave runs
-      // its lanes' edge-edge ones together, then its vertex-face ones.
+      // and folded into the two minima in bfSolver's order, each minimum
+      // keeping its points.  The wave runs its lanes' edge-edge ones
+      // together, then its vertex-face ones.
       unsigned rest = (open == kUnresolvedAll) ? 0u : open & ~((1u << fe) | (1u << fv));
       unsigned rest_e = rest & 0x1FFu, rest_v = rest & 0x7E00u;
       while (__ballot(rest_e != 0u)) {
@@ -623,6 +626,8 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
           const bool better = val < Ve || (val == Ve && g < fe);
           fe = better ? g : fe;
           Ve = better ? val : Ve;
+          Pe = sel3(better, Pg, Pe);
+          Qe = sel3(better, Qg, Qe);
           ++evals;
         }
       }
@@ -635,17 +640,15 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
           const bool better = val < Vv || (val == Vv && g < fv);
           fv = better ? g : fv;
           Vv = better ? val : Vv;
+          Pv = sel3(better, Pg, Pv);
+          Qv = sel3(better, Qg, Qv);
           ++evals;
         }
       }
-      // the final pick; its points again when it is not the first winner
-      const int win = Vv > Ve ? fe : fv;
-      if (__ballot(open != kUnresolvedAll && win != win0)) {
-        if (open != kUnresolvedAll && win != win0) {
-          eval_feature(A, B, win, P, Q);
-          ++evals;
-        }
-      }
+      // the final pick (bfSolver's rule again on the folded minima)
+      const bool fin_e = Vv > Ve;
+      P = sel3(fin_e, Pe, Pv);
+      Q = sel3(fin_e, Qe, Qv);
       done = valid && open != kUnresolvedAll;
     } else {
       // penalty Newton from the network's barycentric guess (or 0.33), the
@@ -731,6 +734,14 @@ __global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_e
   }
 }
 
+// Two waves per SIMD: the tie folding keeps both families' minima and their
+// points live (at three waves the registers spill).
+template <int MODE>
+__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(2))) void
+k_learned(LearnedArgs a) {
+  learned_body<MODE>(a);
+}
+
 struct LearnedState {
   DevBuf queue, qsoa, wg_counts, model_copy;
   const float* model = nullptr;  // points into model_copy once set
@@ -761,11 +772,11 @@ int learned_pairs(int mode, const double* soa, int64_t n, double* P, double* Q, 
                 g_learned.qsoa.as<double>(), qcap, region, g_learned.wg_counts.as<unsigned>()};
   if (n > 0) {
     if (mode == kModeWarm)
-      hipLaunchKernelGGL(k_learned<kModeWarm>, static_cast<unsigned>(grid), 64 * kInferWaves, 0,
-                         st, a);
+      hipLaunchKernelGGL(k_learned<kModeWarm>, static_cast<unsigned>(grid), 64 * kInferWaves,
+                         0, st, a);
     else if (mode == kModeCold)
-      hipLaunchKernelGGL(k_learned<kModeCold>, static_cast<unsigned>(grid), 64 * kInferWaves, 0,
-                         st, a);
+      hipLaunchKernelGGL(k_learned<kModeCold>, static_cast<unsigned>(grid), 64 * kInferWaves,
+                         0, st, a);
     else
       hipLaunchKernelGGL(k_learned<kModeFeatures>, static_cast<unsigned>(grid), 64 * kInferWaves,
                          0, st, a);

@@ -23,6 +23,9 @@ namespace dh {
 
 constexpr double kCertRel = 4e-7;  // distance error bound 8e-7 relative
 
+// No feature set: the pair goes to the brute force.
+constexpr unsigned kUnresolvedAll = 0xFFFFu;
+
 template <typename T>
 DH_HD V3<T> cross3(const V3<T>& a, const V3<T>& b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
@@ -73,27 +76,48 @@ DH_HD void eval_feature(const Tri<double>& A, const Tri<double>& B, int code, V3
 }
 
 // eval_ee, also returning the value bfSolver's edge-edge loop compares for
-// this feature: |w + sc u - tc v| (edge_edge, bf.cpp:586-713).
+// this feature: |w + sc u - tc v| (edge_edge, bf.cpp:586-713), and the
+// supports of P and Q: the vertex bits of the smallest sub-simplex of A (B)
+// that holds the point (a clamped parameter puts it on an edge's end).
 DH_HD double eval_ee_value(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
-                           V3<double>& Q) {
+                           V3<double>& Q, unsigned& suppA, unsigned& suppB) {
   const int k = code / 3, m = code - 3 * (code / 3);
-  const V3<double> ha = pick3(A, k), oa = pick3(A, k == 2 ? 0 : k + 1);
-  const V3<double> hb = pick3(B, m), ob = pick3(B, m == 2 ? 0 : m + 1);
+  const int ko = k == 2 ? 0 : k + 1, mo = m == 2 ? 0 : m + 1;
+  const V3<double> ha = pick3(A, k), oa = pick3(A, ko);
+  const V3<double> hb = pick3(B, m), ob = pick3(B, mo);
   const V3<double> u = sub(ha, oa), v = sub(hb, ob), w = sub(oa, ob);
   const double a = dot(u, u), b = dot(u, v), c = dot(v, v), d = dot(u, w), e = dot(v, w);
   double sc, tc;
   segseg(a, b, c, d, e, sc, tc);
   P = {oa.x + (u.x * sc), oa.y + (u.y * sc), oa.z + (u.z * sc)};
   Q = {ob.x + (v.x * tc), ob.y + (v.y * tc), ob.z + (v.z * tc)};
+  suppA = (sc == 1.0 ? 0u : 1u << ko) | (sc == 0.0 ? 0u : 1u << k);
+  suppB = (tc == 1.0 ? 0u : 1u << mo) | (tc == 0.0 ? 0u : 1u << m);
   const V3<double> dP = {w.x + (sc * u.x) - (tc * v.x), w.y + (sc * u.y) - (tc * v.y),
                          w.z + (sc * u.z) - (tc * v.z)};
   return dsqrt(dP.x * dP.x + dP.y * dP.y + dP.z * dP.z);
 }
 
-// eval_vf, also returning the value bfSolver's vertex-face loop compares:
-// the square root of the clamped quadratic form (vertex_face, bf.cpp:875-1100).
-DH_HD double eval_vf_value(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+DH_HD double eval_ee_value(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
                            V3<double>& Q) {
+  unsigned sa, sb;
+  return eval_ee_value(A, B, code, P, Q, sa, sb);
+}
+
+// Vertex bits (of f0, f1, f2) of the face point pt_tri's parameters (s, t)
+// name: P = f0 + t (f2 - f0) + s (f1 - f0); an edge or a vertex when a
+// parameter sits on its bound.
+DH_HD unsigned face_support(double s, double t) {
+  const bool s0 = s == 0.0, t0 = t == 0.0;
+  const bool s1 = s == 1.0, t1 = t == 1.0;
+  return s1 ? 2u : t1 ? 4u : (s0 && t0) ? 1u : s0 ? 5u : t0 ? 3u : (s + t == 1.0) ? 6u : 7u;
+}
+
+// eval_vf, also returning the value bfSolver's vertex-face loop compares:
+// the square root of the clamped quadratic form (vertex_face, bf.cpp:875-1100),
+// and the supports of P and Q (the vertex alone on its side).
+DH_HD double eval_vf_value(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+                           V3<double>& Q, unsigned& suppA, unsigned& suppB) {
   const int j = code - 9;
   const bool onA = j < 3;
   const V3<double> f0 = sel3(onA, A.v[0], B.v[0]);
@@ -109,8 +133,121 @@ DH_HD double eval_vf_value(const Tri<double>& A, const Tri<double>& B, int code,
                           f0.z + ((f2.z - f0.z) * t) + ((f1.z - f0.z) * s)};
   P = sel3(onA, onf, pt);
   Q = sel3(onA, pt, onf);
+  const unsigned fs = face_support(s, t), vs = 1u << (onA ? j : j - 3);
+  suppA = onA ? fs : vs;
+  suppB = onA ? vs : fs;
   return dsqrt(sqr);
 }
+
+DH_HD double eval_vf_value(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
+                           V3<double>& Q) {
+  unsigned sa, sb;
+  return eval_vf_value(A, B, code, P, Q, sa, sb);
+}
+
+// ------------------------------------------------------- support choice
+//
+// support_choice() is the learned path's cheap acceptance test.  P, Q are
+// the winner's points and suppA, suppB their supports (the smallest
+// sub-simplices holding them).  The pair is accepted when
+//   * (P, Q) is a closest pair up to rounding: with n = Q - P every vertex
+//     of A has m.(P - a) >= -tol and every vertex of B m.(b - Q) >= -tol
+//     (m = n / |n|, tol as in exact_choice below); intersecting or
+//     touching triangles (|n| below 2^-30 of the coordinates' scale) never
+//     pass;
+//   * every vertex outside the supports is lifted off that plane by an angle
+//     of at least kLift as seen from P (Q): m.(P - a) >= kLift |a - P|_1;
+//   * for an edge-edge winner whose points are both inside their edges, the
+//     edges are not parallel: |u x v|^2 >= kLift^2 |u|^2 |v|^2;
+//   * neither triangle is degenerate (the same test on two of its edges:
+//     a thin triangle's angles are all small or it has two near-parallel
+//     edges, so one angle test at vertex 0 rules out coincident and
+//     collinear vertices).
+// Then the closest pair is unique and isolated, and the features that reach
+// it are those whose vertex sets contain both supports (the ties of a
+// degenerate support, a vertex against an edge or a vertex): the returned
+// set names them, to be evaluated and folded in bfSolver's order.  A feature
+// outside the set reaches a value within rounding of the winner's only at
+// points within a few 1e-10 of (P, Q) for the lift and the scales of these
+// workloads, so bfSolver's choice is reproduced bit for bit unless such a
+// near-tie occurs, and its points to that distance always (measured against
+// the brute force on every pair; exact_choice() below proves the choice and
+// is the exact mode).  Returns kUnresolvedAll when the test fails.
+constexpr double kLift = 1e-3;
+
+// Features whose vertex sets contain suppA (of A) and suppB (of B).
+DH_HD unsigned tie_features(unsigned suppA, unsigned suppB) {
+  unsigned out = 0u;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const unsigned ea = (1u << k) | (1u << (k == 2 ? 0 : k + 1));
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const unsigned eb = (1u << m) | (1u << (m == 2 ? 0 : m + 1));
+      out |= ((suppA & ~ea) == 0u && (suppB & ~eb) == 0u) ? (1u << (3 * k + m)) : 0u;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    out |= ((suppB & ~(1u << j)) == 0u) ? (1u << (9 + j)) : 0u;   // vertex j of B, face A
+    out |= ((suppA & ~(1u << j)) == 0u) ? (1u << (12 + j)) : 0u;  // vertex j of A, face B
+  }
+  return out;
+}
+
+DH_HD unsigned support_choice(const Tri<double>& A, const Tri<double>& B, const V3<double>& P,
+                              const V3<double>& Q, unsigned suppA, unsigned suppB, int win) {
+  using std::fabs;
+  using std::fmax;
+  const V3<double> n = sub(Q, P);
+  const double d2 = dot(n, n);
+  if (!(d2 > 0.0)) return kUnresolvedAll;
+  const double inv_d = 1.0 / std::sqrt(d2);
+  const V3<double> m = {n.x * inv_d, n.y * inv_d, n.z * inv_d};
+  double M = fmax(fmax(fmax(fabs(P.x), fabs(P.y)), fmax(fabs(P.z), fabs(Q.x))),
+                  fmax(fabs(Q.y), fabs(Q.z)));
+  // two passes over the six vertices (no per-vertex arrays: registers): the
+  // scales, then the slacks against them
+  double L = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const V3<double> v = k < 3 ? A.v[k] : B.v[k - 3];
+    const V3<double> r = k < 3 ? sub(P, v) : sub(v, Q);
+    L = fmax(L, fabs(r.x) + fabs(r.y) + fabs(r.z));
+    M = fmax(M, fmax(fmax(fabs(v.x), fabs(v.y)), fabs(v.z)));
+  }
+  const double tol = 0x1.0p-47 * (2.0 * L + M + 2.0 * M * L * inv_d);
+  // touching or intersecting triangles (d within rounding of the scale)
+  // never pass: their many zero-valued features tie
+  if (!(d2 > 0x1.0p-60 * (L + M) * (L + M))) return kUnresolvedAll;
+  const unsigned supp = suppA | (suppB << 3);
+  // neither triangle degenerate (an angle below kLift at vertex 0: coincident
+  // or collinear vertices give several features the same points)
+  const V3<double> ua = sub(A.v[1], A.v[0]), va = sub(A.v[2], A.v[0]);
+  const V3<double> ub = sub(B.v[1], B.v[0]), vb = sub(B.v[2], B.v[0]);
+  const V3<double> xa = cross3(ua, va), xb = cross3(ub, vb);
+  bool ok = dot(xa, xa) > kLift * kLift * dot(ua, ua) * dot(va, va) &&
+            dot(xb, xb) > kLift * kLift * dot(ub, ub) * dot(vb, vb);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const V3<double> v = k < 3 ? A.v[k] : B.v[k - 3];
+    const V3<double> r = k < 3 ? sub(P, v) : sub(v, Q);
+    const double sl = dot(m, r);
+    const double r1 = fabs(r.x) + fabs(r.y) + fabs(r.z);
+    ok = ok && (sl >= -tol) && (((supp >> k) & 1u) || sl >= kLift * r1);
+  }
+  // both points inside their edges: the edges must not be parallel
+  const bool interior = win < 9 && __builtin_popcount(suppA) == 2 && __builtin_popcount(suppB) == 2;
+  if (interior) {
+    const int k = win / 3, mm = win - 3 * (win / 3);
+    const V3<double> u = sub(pick3(A, k), pick3(A, k == 2 ? 0 : k + 1));
+    const V3<double> v = sub(pick3(B, mm), pick3(B, mm == 2 ? 0 : mm + 1));
+    const V3<double> x = cross3(u, v);
+    ok = ok && dot(x, x) > kLift * kLift * dot(u, u) * dot(v, v);
+  }
+  return ok ? tie_features(suppA, suppB) : kUnresolvedAll;
+}
+
 
 // ---------------------------------------------------------- exact choice
 //
@@ -172,42 +309,83 @@ DH_HD double eval_vf_value(const Tri<double>& A, const Tri<double>& B, int code,
 //     LB^2 > (Vw (1 + 2^-40) + 2^-45 (L + Vw))^2 + 2^-42 (L + Vw)^2,
 // which dominates both error terms.  NaN inputs fail every comparison and
 // are never accepted.
-constexpr unsigned kUnresolvedAll = 0xFFFFu;
+//
+// The per-feature bounds are evaluated in binary32 on quantities divided by
+// d (so every value is near 1 or a ratio of lengths, far from binary32's
+// range limits).  Each input enters as a bound in the safe direction: the
+// slacks and dlo rounded down by a factor 1 - 2^-22, the in-plane steps
+// rounded up, the threshold rounded up.  The bound is increasing in d, the
+// slacks and rho and decreasing in U, so these inputs give a valid lower
+// bound.  Its binary32 evaluation (hardware reciprocal and square root, a
+// few ulp each, and the in-plane distances' errors below the 2^-16 L
+// widening) is then scaled by 1 - 2^-16.  The closed form's branch is
+// decided on rounded values; a wrong decision occurs only where the
+// unconstrained minimiser u* lies within rounding of 0 or U, and then
+// changes the value only at second order, far inside that factor.  A
+// feature whose true bound clears the threshold by less than about 2^-15
+// stays open and is evaluated, which costs time, never correctness.
 
 struct SideGeom {
-  double slack[3];   // m.(P - a) for A (m.(b - Q) for B), binary64
-  float px[3], py[3];  // in-plane coordinates relative to P (Q), binary32
-  unsigned on;       // bit k: vertex k on the plane
+  float slack[3];      // slack (m.(P - a) for A, m.(b - Q) for B) less tol, over d, >= 0
+  float px[3], py[3];  // in-plane coordinates relative to P (Q), over d
+  unsigned on;         // bit k: vertex k on the plane
 };
 
 // v[i] for a run-time i in 0..2, as selects (no dynamically indexed arrays,
 // which would live in scratch memory on the GPU)
 DH_HD float pickf(const float v[3], int i) { return i == 2 ? v[2] : (i == 1 ? v[1] : v[0]); }
 
+// Reciprocal and square root within a few ulp (the hardware's on the GPU).
+DH_HD float rcp_ulp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+DH_HD float sqrt_ulp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return std::sqrt(x);
+#endif
+}
+
+// The coordinate of the vertex named by the single bit `bit` (1, 2 or 4), as
+// a sum of selects (the compiler keeps it branch-free; exactly one term is
+// nonzero).
+DH_HD float pick_bit(const float v[3], unsigned bit) {
+  return ((bit & 1u) ? v[0] : 0.f) + ((bit & 2u) ? v[1] : 0.f) + ((bit & 4u) ? v[2] : 0.f);
+}
+
 // In-plane distance between the on-plane hulls of vertex sets ma (of A) and
 // mb (of B), each one vertex or an edge; 0 when neither is a single vertex.
+// Branch-free: the hulls' end points are picked by selects.
 DH_HD float inplane_gap(const SideGeom& sa, unsigned ma, const SideGeom& sb, unsigned mb) {
-  const int na = __builtin_popcount(ma), nb = __builtin_popcount(mb);
-  if ((na != 1 && nb != 1) || na > 2 || nb > 2) return 0.f;
+  const unsigned la = ma & (0u - ma), lb = mb & (0u - mb);  // lowest vertex bits
+  const unsigned ha = (ma ^ la) ? (ma ^ la) : la;            // the other end (sets of <= 2)
+  const unsigned hb = (mb ^ lb) ? (mb ^ lb) : lb;
+  // sets are subsets of {1, 2, 4}: single vertices 1, 2, 4; edges 3, 5, 6
+  const bool a_single = la != 0u && la == ma, b_single = lb != 0u && lb == mb;
+  const bool a_pair = ma == 3u || ma == 5u || ma == 6u;
+  const bool b_pair = mb == 3u || mb == 5u || mb == 6u;
+  const bool none = !((a_single && (b_single || b_pair)) || (b_single && a_pair));
+  const float a0x = pick_bit(sa.px, la), a0y = pick_bit(sa.py, la);
+  const float a1x = pick_bit(sa.px, ha), a1y = pick_bit(sa.py, ha);
+  const float b0x = pick_bit(sb.px, lb), b0y = pick_bit(sb.py, lb);
+  const float b1x = pick_bit(sb.px, hb), b1y = pick_bit(sb.py, hb);
   // p: the single vertex; (q0, q1): the other hull (q0 == q1 for a vertex)
-  const bool a_single = na == 1;
-  const unsigned mp = a_single ? ma : mb, mq = a_single ? mb : ma;
-  const int ip = __builtin_ctz(mp);
-  const int iq0 = __builtin_ctz(mq);
-  const int iq1 = 31 - __builtin_clz(mq);
-  const float px = a_single ? pickf(sa.px, ip) : pickf(sb.px, ip);
-  const float py = a_single ? pickf(sa.py, ip) : pickf(sb.py, ip);
-  const float ax = a_single ? pickf(sb.px, iq0) : pickf(sa.px, iq0);
-  const float ay = a_single ? pickf(sb.py, iq0) : pickf(sa.py, iq0);
-  const float bx = a_single ? pickf(sb.px, iq1) : pickf(sa.px, iq1);
-  const float by = a_single ? pickf(sb.py, iq1) : pickf(sa.py, iq1);
+  const float px = a_single ? a0x : b0x, py = a_single ? a0y : b0y;
+  const float ax = a_single ? b0x : a0x, ay = a_single ? b0y : a0y;
+  const float bx = a_single ? b1x : a1x, by = a_single ? b1y : a1y;
   const float ex = bx - ax, ey = by - ay;
   const float wx = px - ax, wy = py - ay;
   const float ee = ex * ex + ey * ey;
-  float t = ee > 0.f ? (wx * ex + wy * ey) / ee : 0.f;
+  float t = ee > 0.f ? (wx * ex + wy * ey) * rcp_ulp(ee) : 0.f;
   t = t < 0.f ? 0.f : (t > 1.f ? 1.f : t);
   const float dx = wx - t * ex, dy = wy - t * ey;
-  return std::sqrt(dx * dx + dy * dy);
+  return none ? 0.f : sqrt_ulp(dx * dx + dy * dy);
 }
 
 // Vertex sets of feature `code`: bit k of *ma is vertex k of A (same for B).
@@ -229,7 +407,6 @@ DH_HD unsigned exact_choice(const Tri<double>& A, const Tri<double>& B, const V3
                             const V3<double>& Q, int win, int other, double Vw) {
   using std::fabs;
   using std::fmax;
-  using std::fmin;
   using std::sqrt;
   const V3<double> n = sub(Q, P);
   const double d2 = dot(n, n);
@@ -251,30 +428,31 @@ DH_HD unsigned exact_choice(const Tri<double>& A, const Tri<double>& B, const V3
   e1y *= k1;
   e1z *= k1;
   const float e2x = my * e1z - mz * e1y, e2y = mz * e1x - mx * e1z, e2z = mx * e1y - my * e1x;
-  // per vertex: slack (binary64), in-plane coordinates (binary32), sizes
+  // per vertex: slack (binary64), in-plane coordinates over d (binary32), sizes
   double La = 0.0, Lb = 0.0;
   double M = fmax(fmax(fmax(fabs(P.x), fabs(P.y)), fmax(fabs(P.z), fabs(Q.x))),
                   fmax(fabs(Q.y), fabs(Q.z)));
+  double sla[3], slb[3];
   SideGeom ga, gb;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const V3<double> r = sub(A.v[k], P);
-    ga.slack[k] = -dot(m, r);
+    sla[k] = -dot(m, r);
     La = fmax(La, fabs(r.x) + fabs(r.y) + fabs(r.z));
     M = fmax(M, fmax(fmax(fabs(A.v[k].x), fabs(A.v[k].y)), fabs(A.v[k].z)));
-    const float x = static_cast<float>(r.x), y = static_cast<float>(r.y),
-                z = static_cast<float>(r.z);
+    const float x = static_cast<float>(r.x * inv_d), y = static_cast<float>(r.y * inv_d),
+                z = static_cast<float>(r.z * inv_d);
     ga.px[k] = x * e1x + y * e1y + z * e1z;
     ga.py[k] = x * e2x + y * e2y + z * e2z;
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const V3<double> r = sub(B.v[k], Q);
-    gb.slack[k] = dot(m, r);
+    slb[k] = dot(m, r);
     Lb = fmax(Lb, fabs(r.x) + fabs(r.y) + fabs(r.z));
     M = fmax(M, fmax(fmax(fabs(B.v[k].x), fabs(B.v[k].y)), fabs(B.v[k].z)));
-    const float x = static_cast<float>(r.x), y = static_cast<float>(r.y),
-                z = static_cast<float>(r.z);
+    const float x = static_cast<float>(r.x * inv_d), y = static_cast<float>(r.y * inv_d),
+                z = static_cast<float>(r.z * inv_d);
     gb.px[k] = x * e1x + y * e1y + z * e1z;
     gb.py[k] = x * e2x + y * e2y + z * e2z;
   }
@@ -282,11 +460,15 @@ DH_HD unsigned exact_choice(const Tri<double>& A, const Tri<double>& B, const V3
   const double tol = 0x1.0p-48 * (L + M + M * L * inv_d);
   ga.on = gb.on = 0u;
   bool cert = true;
+  constexpr float kDown = 1.0f - 0x1.0p-22f, kUp = 1.0f + 0x1.0p-22f;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    cert = cert && (ga.slack[k] >= -tol) && (gb.slack[k] >= -tol);
-    ga.on |= (ga.slack[k] <= 4.0 * tol) ? (1u << k) : 0u;
-    gb.on |= (gb.slack[k] <= 4.0 * tol) ? (1u << k) : 0u;
+    cert = cert && (sla[k] >= -tol) && (slb[k] >= -tol);
+    ga.on |= (sla[k] <= 4.0 * tol) ? (1u << k) : 0u;
+    gb.on |= (slb[k] <= 4.0 * tol) ? (1u << k) : 0u;
+    // less the slacks' rounding, over d, rounded down
+    ga.slack[k] = static_cast<float>(fmax(0.0, sla[k] - tol) * inv_d) * kDown;
+    gb.slack[k] = static_cast<float>(fmax(0.0, slb[k] - tol) * inv_d) * kDown;
   }
   if (!cert) return kUnresolvedAll;
   // in-plane distances between the vertices of each side (01, 02, 12)
@@ -296,65 +478,64 @@ DH_HD unsigned exact_choice(const Tri<double>& A, const Tri<double>& B, const V3
     const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
     const float ux = ga.px[j] - ga.px[i], uy = ga.py[j] - ga.py[i];
     const float vx = gb.px[j] - gb.px[i], vy = gb.py[j] - gb.py[i];
-    da[k] = std::sqrt(ux * ux + uy * uy);
-    db[k] = std::sqrt(vx * vx + vy * vy);
+    da[k] = sqrt_ulp(ux * ux + uy * uy);
+    db[k] = sqrt_ulp(vx * vx + vy * vy);
   }
   // m.(y - x) >= d - 4 tol when x, y use on-plane vertices (their slacks are
-  // >= -tol each, computed within tol), and n's own rounding
-  const double dlo = fmax(d * (1.0 - 0x1.0p-50) - 4.0 * tol, 0.0);
-  const double widen = 0x1.0p-16 * L;
+  // >= -tol each, computed within tol), and n's own rounding; over d
+  const float dl =
+      static_cast<float>(fmax(d * (1.0 - 0x1.0p-50) - 4.0 * tol, 0.0) * inv_d) * kDown;
+  const float widen = static_cast<float>(0x1.0p-16 * L * inv_d) * kUp;
   const double lv = L + Vw;
   const double thr_lin = Vw * (1.0 + 0x1.0p-40) + 0x1.0p-45 * lv;
-  const double thr = thr_lin * thr_lin + 0x1.0p-42 * lv * lv;
+  const float thr =
+      static_cast<float>((thr_lin * thr_lin + 0x1.0p-42 * lv * lv) * inv_d * inv_d) * kUp;
   unsigned open = 0u;
 #pragma unroll 1
   for (int g = 0; g < 15; ++g) {
     unsigned ma, mb;
     feature_vertices(g, ma, mb);
     const unsigned a0 = ma & ga.on, a1 = ma & ~ga.on, b0 = mb & gb.on, b1 = mb & ~gb.on;
-    // least off-plane slack of each side (less the slacks' rounding), and
-    // the largest in-plane step from an on-plane to an off-plane vertex
-    double sa = 1e300, sb = 1e300;
-    float pa = 0.f, pb = 0.f;
+    // least off-plane slack of each side, and the largest in-plane step
+    // from an on-plane to an off-plane vertex
+    float sa = 3.0e38f, sb = 3.0e38f, pa = 0.f, pb = 0.f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const bool ak = (a1 >> k) & 1u, bk = (b1 >> k) & 1u;
-      sa = ak ? fmin(sa, fmax(0.0, ga.slack[k] - tol)) : sa;
-      sb = bk ? fmin(sb, fmax(0.0, gb.slack[k] - tol)) : sb;
+      const unsigned ak = (a1 >> k) & 1u, bk = (b1 >> k) & 1u;
+      sa = ak ? fminf(sa, ga.slack[k]) : sa;
+      sb = bk ? fminf(sb, gb.slack[k]) : sb;
       // pair k joins vertices (i, j): an on-plane one and an off-plane one
       const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
-      const bool pa_k = (((a0 >> i) & 1u) && ((a1 >> j) & 1u)) || (((a0 >> j) & 1u) && ((a1 >> i) & 1u));
-      const bool pb_k = (((b0 >> i) & 1u) && ((b1 >> j) & 1u)) || (((b0 >> j) & 1u) && ((b1 >> i) & 1u));
+      const unsigned pa_k = ((a0 >> i) & (a1 >> j) & 1u) | ((a0 >> j) & (a1 >> i) & 1u);
+      const unsigned pb_k = ((b0 >> i) & (b1 >> j) & 1u) | ((b0 >> j) & (b1 >> i) & 1u);
       pa = pa_k ? fmaxf(pa, da[k]) : pa;
       pb = pb_k ? fmaxf(pb, db[k]) : pb;
     }
-    double lb2;
-    if (!a0 || !b0) {
-      // a side entirely off the plane: the slab bound
-      const double lb = dlo + (a0 ? 0.0 : sa) + (b0 ? 0.0 : sb);
-      lb2 = lb * lb;
-    } else {
-      const double rho = fmax(0.0, static_cast<double>(inplane_gap(ga, a0, gb, b0)) - widen);
-      const double PA = a1 ? static_cast<double>(pa) + widen : 0.0;
-      const double PB = b1 ? static_cast<double>(pb) + widen : 0.0;
-      // slack per unit of in-plane step (the cheaper side)
-      const double ka = PA > 0.0 ? sa / PA : 1e300, kb = PB > 0.0 ? sb / PB : 1e300;
-      const double kappa = fmin(ka, kb);
-      const double U = PA + PB;
-      // min over u >= 0 of (dlo + kappa u)^2 + (rho - u)^2 is
-      // (dlo + kappa rho)^2 / (1 + kappa^2) at u* = (rho - kappa dlo) /
+    // a side entirely off the plane: the slab bound
+    const float lb = dl + (a0 ? 0.f : sa) + (b0 ? 0.f : sb);
+    float lb2 = lb * lb;
+    {
+      const float rho = fmaxf(0.f, inplane_gap(ga, a0, gb, b0) - widen);
+      const float PA = a1 ? (pa + widen) * kUp : 0.f;
+      const float PB = b1 ? (pb + widen) * kUp : 0.f;
+      // slack per unit of in-plane step (the cheaper side), at most 2^60
+      const float ka = PA > 0.f ? sa * rcp_ulp(PA) * kDown : 0x1.0p60f;
+      const float kb = PB > 0.f ? sb * rcp_ulp(PB) * kDown : 0x1.0p60f;
+      const float kappa = fminf(fminf(ka, kb), 0x1.0p60f);
+      const float U = PA + PB;
+      // min over 0 <= u <= min(rho, U) of (dl + kappa u)^2 + (rho - u)^2:
+      // (dl + kappa rho)^2 / (1 + kappa^2) at u* = (rho - kappa dl) /
       // (1 + kappa^2) when 0 < u* <= U; at u = 0 when u* <= 0; at u = U when
-      // u* exceeds the largest step U
-      const bool capped = U < rho && kappa < 1e150 &&
-                          (rho - kappa * dlo) > U * (1.0 + kappa * kappa) * (1.0 + 0x1.0p-20);
-      const double hU = dlo + kappa * U, rU = rho - U;
-      const double free_min = kappa < 1e150 ? (dlo + kappa * rho) * (dlo + kappa * rho) / (1.0 + kappa * kappa)
-                                            : dlo * dlo + rho * rho;
-      // u* <= 0: the minimum over u >= 0 is at u = 0 (a rounding of u*'s
-      // sign changes the value only at second order in the rounding)
-      const bool at0 = U == 0.0 || !(rho - kappa * dlo > 0.0);
-      lb2 = at0 ? dlo * dlo + rho * rho : (capped ? hU * hU + rU * rU : free_min);
+      // u* exceeds U
+      const float lead = rho - kappa * dl;
+      const float k2 = 1.f + kappa * kappa;
+      const bool at0 = U == 0.f || !(lead > 0.f);
+      const bool capped = !at0 && U < rho && lead > U * k2;
+      const float hU = dl + kappa * U, rU = rho - U, fr = dl + kappa * rho;
+      const float lbk = at0 ? dl * dl + rho * rho : (capped ? hU * hU + rU * rU : fr * fr * rcp_ulp(k2));
+      lb2 = (a0 && b0) ? lbk : lb2;
     }
+    lb2 *= 1.0f - 0x1.0p-16f;
     open |= (g == win || g == other || lb2 > thr) ? 0u : (1u << g);
   }
   return open;

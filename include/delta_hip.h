@@ -54,7 +54,8 @@ enum {
   DH_PAIR_PENALTY64 = 1,   /* penaltySolver (4 fixed Newton steps), FP64, bit-exact */
   DH_PAIR_HYBRID64 = 2,    /* penalty, bfSolver where the pair's `failed` is set   */
   DH_PAIR_MIXED = 3,       /* FP32 feature screen + FP64 exact finalisation        */
-  DH_PAIR_LEARNED = 4,     /* MLP warm start / feature guess + FP64 certificate    */
+  DH_PAIR_LEARNED = 4,     /* MLP feature guess + FP64 feature evaluation, support
+                              check and tied features folded; the rest bfSolver */
   DH_PAIR_PENALTY_ADAPT64 = 5, /* penaltySolver(..., int& its), <=16 its, FP64      */
   DH_PAIR_BF64_EXACT = 6,  /* segment-triangle intersection test first (the zoltan
                               segt of zoltan/hybrid.cpp:280-342, in its order of

@@ -143,12 +143,12 @@ def test_feature_evaluation_and_certificate(hc):
     assert not call[cross].any()
 
 
-def _decide(hc, s, ce, cv):
+def _decide(hc, s, ce, cv, fn="hc_learned_decide"):
     n = s.shape[1]
     P = np.zeros(3 * n); Q = np.zeros(3 * n); st = np.zeros(n, np.int32)
-    hc.hc_learned_decide(ol.dptr(s), C.c_long(n), ce.ctypes.data_as(C.c_void_p),
-                         cv.ctypes.data_as(C.c_void_p), ol.dptr(P), ol.dptr(Q),
-                         st.ctypes.data_as(C.c_void_p))
+    getattr(hc, fn)(ol.dptr(s), C.c_long(n), ce.ctypes.data_as(C.c_void_p),
+                    cv.ctypes.data_as(C.c_void_p), ol.dptr(P), ol.dptr(Q),
+                    st.ctypes.data_as(C.c_void_p))
     return P, Q, st
 
 
@@ -167,17 +167,21 @@ def host_pairs(hc, n, seed=3):
     return s.reshape(18, n), t, k
 
 
-def test_exact_choice_is_bfsolver_bit_for_bit(hc):
+@pytest.mark.parametrize("fn", ["hc_learned_decide", "hc_learned_decide_support"])
+def test_exact_choice_is_bfsolver_bit_for_bit(hc, fn):
     """The learned path's decision (pair_features.hpp: two candidate
-    features, bfSolver's rule, exact_choice(), tie folding), host build,
-    against the oracle's bfSolver: whenever it accepts a pair its P and Q
-    are the oracle's bit for bit, for the ideal candidates (bfSolver's own
-    two minima), for the runner-up edge-edge feature and for random
-    candidates (an untrained network).  Pairs: the unique-pair workload's
-    distribution, the compiled reference's golden pairs and hand-built
-    degenerate pairs (ties, parallel edges and faces, touching, intersecting).
-    With the ideal candidates it accepts all separated generated pairs but a
-    few near-parallel ones."""
+    features, bfSolver's rule, the acceptance test -- exact_choice() of the
+    exact mode, support_choice() of the default path -- and tie folding),
+    host build, against the oracle's bfSolver: whenever it accepts a pair
+    its P and Q are the oracle's bit for bit, for the ideal candidates
+    (bfSolver's own two minima), for the runner-up edge-edge feature and for
+    random candidates (an untrained network).  Pairs: the unique-pair
+    workload's distribution, the compiled reference's golden pairs and
+    hand-built degenerate pairs (ties, parallel edges and faces, touching,
+    intersecting).  With the ideal candidates the exact mode accepts all
+    separated generated pairs but a few near-parallel ones; the support test
+    accepts > 99% of the general ones and sends every near-parallel one to
+    the brute force."""
     import scenes
     gen, _, kind = host_pairs(hc, 1 << 14)
     g = np.load(os.path.join(ol.ROOT, "tests", "golden", "pair_solvers.npz"))
@@ -203,12 +207,16 @@ def test_exact_choice_is_bfsolver_bit_for_bit(hc):
     P3 = P0.reshape(3, n); Q3 = Q0.reshape(3, n)
     for name, ce, cv in (("ideal", ideal_e, ideal_v), ("runner-up", second_e, ideal_v),
                          ("random", rand_e, rand_v)):
-        P, Q, st = _decide(hc, s, ce, cv)
+        P, Q, st = _decide(hc, s, ce, cv, fn)
         acc = st > 0
         Pa = P.reshape(3, n)[:, acc]; Qa = Q.reshape(3, n)[:, acc]
         assert np.array_equal(bits(Pa), bits(P3[:, acc])), name
         assert np.array_equal(bits(Qa), bits(Q3[:, acc])), name
-        if name == "ideal":
+        if name == "ideal" and fn == "hc_learned_decide_support":
+            assert acc[:ngen][kind == 0].mean() > 0.99, acc[:ngen][kind == 0].mean()
+            assert not acc[:ngen][kind == 2].any()
+            assert (st[:ngen] >= 3).any()
+        elif name == "ideal":
             sep = kind != 2
             assert acc[:ngen][sep].mean() > 0.985, acc[:ngen][sep].mean()
             assert acc[:ngen][kind == 0].mean() > 0.999
