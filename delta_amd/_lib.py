@@ -11,6 +11,7 @@ DH_ERR_CAPACITY = 3
 # particle-pair methods
 DH_BF, DH_PENALTY, DH_HYBRID_BATCH, DH_HYBRID_TRIANGLE = 0, 1, 2, 3
 DH_PENALTY_STAT, DH_HYBRID_BATCH_STAT, DH_HYBRID_TRIANGLE_STAT, DH_GJK = 4, 5, 6, 7
+DH_BF_LEARNED = 8
 # unique-pair methods
 DH_PAIR_BF64, DH_PAIR_PENALTY64, DH_PAIR_HYBRID64 = 0, 1, 2
 DH_PAIR_MIXED, DH_PAIR_LEARNED, DH_PAIR_PENALTY_ADAPT64 = 3, 4, 5

@@ -1182,6 +1182,12 @@ __global__ __launch_bounds__(256) void k_queue_emit(DetectArgs g, long long live
   atomicAdd(&g.tile_count[item], 1);
 }
 
+int queue_learned(const int2* queue, const int* pairs, int tiles, const int* tri_offset,
+                  const double* x, const double* y, const double* z, const double* eps,
+                  long long n, const unsigned long long* need_dev, int* flags, double* pq,
+                  hipStream_t st);
+int queue_learned_reserve(long long n);
+
 struct DetectScratch {
   DevBuf temp, counter, stats, error, counts, offs, sums, tiles, tstats;
   DevBuf queue, qcounter, qbase, qflags, qpos, qpq, qres, deferred, ndeferred;
@@ -1285,15 +1291,20 @@ static int queue_reserve(long long n) {
 // Solve, scan, reserve and emit over n queue entries (live: all of them, or
 // min(*need_dev, n) in a captured step).
 static int queue_passes(const DetectArgs& a, long long n, const unsigned long long* need_dev,
-                        hipStream_t st) {
+                        bool learned, hipStream_t st) {
   const unsigned grid = grid_for(n, 256);
-  if (a.fp32_screen)
+  int rc;
+  if (learned) {
+    if ((rc = queue_learned(a.queue, a.pairs, a.tiles, a.tri_offset, a.x, a.y, a.z, a.eps, n,
+                            need_dev, g_scratch.qflags.as<int>(), g_scratch.qpq.as<double>(),
+                            st)))
+      return rc;
+  } else if (a.fp32_screen)
     hipLaunchKernelGGL(k_queue_solve<true>, grid, 256, 0, st, a, n, need_dev,
                        g_scratch.qflags.as<int>(), g_scratch.qpq.as<double>());
   else
     hipLaunchKernelGGL(k_queue_solve<false>, grid, 256, 0, st, a, n, need_dev,
                        g_scratch.qflags.as<int>(), g_scratch.qpq.as<double>());
-  int rc;
   if ((rc = scan_counts(g_scratch.qflags.as<int>(), static_cast<int>(n),
                         g_scratch.qpos.as<long long>(), st)))
     return rc;
@@ -1311,7 +1322,7 @@ static int queue_passes(const DetectArgs& a, long long n, const unsigned long lo
 // flags, one reservation of contact slots, the contacts.  Reads the queue
 // length back (one synchronisation); the next call's queue holds this
 // call's need.
-static int solve_queue(const DetectArgs& a, hipStream_t st) {
+static int solve_queue(const DetectArgs& a, bool learned, hipStream_t st) {
   unsigned long long need = 0;
   unsigned ndef = 0;
   DH_HIP_CHECK(hipMemcpyAsync(&need, a.qcounter, sizeof(need), hipMemcpyDeviceToHost, st));
@@ -1332,7 +1343,7 @@ static int solve_queue(const DetectArgs& a, hipStream_t st) {
   if (live == 0) return DH_OK;
   int rc;
   if ((rc = queue_reserve(live))) return rc;
-  return queue_passes(a, live, nullptr, st);
+  return queue_passes(a, live, nullptr, learned, st);
 }
 
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
@@ -1340,6 +1351,10 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
                       long long* offs_out, dh_stats* stats, hipStream_t st,
                       const double* pbox, const double* bbox) {
   int rc;
+  // DH_BF_LEARNED: DH_BF with the survivor queue always on, its pairs
+  // through the learned decision
+  const bool learned = method == DH_BF_LEARNED;
+  if (learned) method = DH_BF;
   const long long temp_cap = cap > 0 ? cap : 1;
   if ((rc = g_scratch.temp.grow(sizeof(dh_contact) * temp_cap))) return rc;
   if ((rc = g_scratch.counter.grow(sizeof(unsigned long long)))) return rc;
@@ -1392,7 +1407,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.nitems_dev = nullptr;
   const long long nitems = static_cast<long long>(npairs) * tiles;
   // the queue costs a read-back and four launches: for many items only
-  if (method == DH_BF && nitems >= tuning().detect_queue_min_items) {
+  if (method == DH_BF && (learned || nitems >= tuning().detect_queue_min_items)) {
     // survivor queue: sized from the last call's need, at least 16 entries
     // per item (an overflowing item solves its own survivors)
     long long qcap = g_scratch.qcap > 16 * nitems + 65536 ? g_scratch.qcap : 16 * nitems + 65536;
@@ -1431,7 +1446,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
       return set_error(DH_ERR_UNSUPPORTED, "dh_detect_pairs: method %d", method);
   }
   DH_HIP_CHECK(hipGetLastError());
-  if (a.queue && (rc = solve_queue(a, st))) return rc;
+  if (a.queue && (rc = solve_queue(a, learned, st))) return rc;
   launch_tile_stats(a.tile_stats, ntiles, a.stats, st);
   unsigned long long total = 0;
   int err = 0;
@@ -1483,7 +1498,8 @@ static int graph_tiles(int pair_cap) { return tiles_for(pair_cap / 2 > 0 ? pair_
 // Survivor-queue entries of a captured DH_BF step (0: no queue).
 static long long graph_qcap(int method, int pair_cap) {
   const long long items = static_cast<long long>(pair_cap) * graph_tiles(pair_cap);
-  if (method != DH_BF || items < tuning().detect_queue_min_items) return 0;
+  if (method == DH_BF_LEARNED) method = DH_BF;
+  else if (method != DH_BF || items < tuning().detect_queue_min_items) return 0;
   const long long q = 16 * items + 65536;
   return q < (1ll << 30) ? q : (1ll << 30);
 }
@@ -1498,10 +1514,12 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
                        const double* bbox) {
   int rc;
   if ((rc = detect_reserve(pair_cap, contact_cap, method))) return rc;
+  const bool learned = method == DH_BF_LEARNED;
+  const long long qcap = graph_qcap(method, pair_cap);
+  if (learned) method = DH_BF;
   zero_async(ZeroRegions{{g_scratch.counter.as<unsigned>(), g_scratch.stats.as<unsigned>(),
                           g_scratch.error.as<unsigned>(),
-                          graph_qcap(method, pair_cap) > 0 ? g_scratch.qcounter.as<unsigned>()
-                                                           : nullptr},
+                          qcap > 0 ? g_scratch.qcounter.as<unsigned>() : nullptr},
                          {2, 2 * ST_COUNT, 1, 2}},
              st);
   DetectArgs a;
@@ -1538,7 +1556,6 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.item_list = nullptr;
   a.nitems_dev = nullptr;
   const unsigned items = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
-  const long long qcap = graph_qcap(method, pair_cap);
   if (qcap > 0) {
     // the survivor queue with its counts kept on the device: every pass is
     // launched for the capacity and reads the live length
@@ -1567,7 +1584,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
     f.nitems_dev = a.ndeferred;
     hipLaunchKernelGGL((k_detect<DH_BF, false>), detect_grid(items), 256, 0, st, f,
                        static_cast<int>(items));
-    if ((rc = queue_passes(a, a.qcap, a.qcounter, st))) return rc;
+    if ((rc = queue_passes(a, a.qcap, a.qcounter, learned, st))) return rc;
   }
   launch_tile_stats(a.tile_stats, ntiles, a.stats, st);
   if (tiles > 1)
@@ -1594,6 +1611,7 @@ int detect_reserve(int pair_cap, long long contact_cap, int method) {
     if ((rc = g_scratch.deferred.grow(sizeof(int) * (items + 1)))) return rc;
     if ((rc = g_scratch.ndeferred.grow(sizeof(unsigned)))) return rc;
     if ((rc = queue_reserve(qcap))) return rc;
+    if (method == DH_BF_LEARNED && (rc = queue_learned_reserve(qcap))) return rc;
   }
   if ((rc = g_scratch.temp.grow(sizeof(dh_contact) * (contact_cap > 0 ? contact_cap : 1)))) return rc;
   if ((rc = g_scratch.counter.grow(sizeof(unsigned long long)))) return rc;

@@ -224,8 +224,9 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
       (params->adaptive && !params->radius) ||
       pair_cap > 0x7fffffffLL || scene->geom.nparticles < 1)
     return set_error(DH_ERR_ARG, "dh_step_graph_create: bad arguments");
-  if (params->method != DH_BF && params->method != DH_PENALTY)
-    return set_error(DH_ERR_UNSUPPORTED, "dh_step_graph_create: method %d (DH_BF or DH_PENALTY)",
+  if (params->method != DH_BF && params->method != DH_PENALTY && params->method != DH_BF_LEARNED)
+    return set_error(DH_ERR_UNSUPPORTED,
+                     "dh_step_graph_create: method %d (DH_BF, DH_PENALTY or DH_BF_LEARNED)",
                      params->method);
   *out = nullptr;
   dh_step_graph* g = new dh_step_graph();

@@ -443,29 +443,240 @@ __device__ inline v8s stage_tile(const float* W, int t, int lane) {
   return cat8(pack4(w[0], w[1], w[2], w[3]), pack4(w[4], w[5], w[6], w[7]));
 }
 
+// Workgroup LDS of the learned kernels: the network's A-operand tiles and
+// biases (staged once per workgroup), and per wave one [16][kCol] dword
+// array in column layout (column = pair): first the inputs as 9 rows of bf16
+// pairs (inputs 2r, 2r+1), then, for the same columns, the 16 outputs as
+// FP32 rows.  A column's inputs are read by the sub-tile that later writes
+// its outputs.  kCol = 64 + 4 spreads per-lane rows and the MFMA
+// operand/result patterns over all 32 banks (two lanes per bank, the minimum
+// for 64 lanes).
+struct LearnedLds {
+  v8s w_s[kTiles * 64];
+  float b_s[2 * kHid + kOut];
+  uint32_t io_s[kInferWaves][16 * kCol];
+};
+
+// Stage the network once per workgroup (no network in the cold mode: W may
+// be null there).  Ends with a workgroup barrier.
 template <int MODE>
-__device__ __forceinline__ void learned_body(const LearnedArgs& a) {
-  __shared__ v8s w_s[kTiles * 64];
-  __shared__ float b_s[2 * kHid + kOut];
-  __shared__ unsigned wg_tally[3];  // certified, features evaluated, queued
-  // Per wave, one [16][kCol] dword array in column layout (column = pair):
-  // first the inputs as 9 rows of bf16 pairs (inputs 2r, 2r+1), then, for
-  // the same columns, the 16 outputs as FP32 rows.  A column's inputs are
-  // read by the sub-tile that later writes its outputs.  kCol = 64 + 4 spreads
-  // per-lane rows and the MFMA operand/result patterns over all 32 banks
-  // (two lanes per bank, the minimum for 64 lanes).
-  __shared__ uint32_t io_s[kInferWaves][16 * kCol];
+__device__ __forceinline__ void stage_network(LearnedLds& L, const float* W) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int t = wave; t < (MODE == kModeCold ? 0 : kTiles); t += kInferWaves)
+    L.w_s[t * 64 + lane] = stage_tile(W, t, lane);
+  for (int i = threadIdx.x; i < (MODE == kModeCold ? 0 : 2 * kHid + kOut); i += 64 * kInferWaves)
+    L.b_s[i] = (i < kHid) ? W[kB1 + i] : (i < 2 * kHid) ? W[kB2 + i - kHid]
+                                                        : W[kB3 + i - 2 * kHid];
+  __syncthreads();
+}
+
+// The learned decision for the wave's 64 pairs (one per lane): canonical
+// inputs, network, the two best-ranked features in binary64, the support
+// test and the tied features (or, in the penalty modes, the Newton solve
+// and its certificate).  done: P, Q are the result; otherwise the pair goes
+// to the brute force.  Wave-uniform control flow: every lane of the wave
+// must call it.
+template <int MODE>
+__device__ __forceinline__ void learned_pair(LearnedLds& L, const Tri<double>& A,
+                                             const Tri<double>& B, bool valid, V3<double>& P,
+                                             V3<double>& Q, bool& done, unsigned& evals) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
-  // stage the network once per workgroup
-  // (no network in the cold mode: a.W may be null there)
-  for (int t = wave; t < (MODE == kModeCold ? 0 : kTiles); t += kInferWaves)
-    w_s[t * 64 + lane] = stage_tile(a.W, t, lane);
-  for (int i = threadIdx.x; i < (MODE == kModeCold ? 0 : 2 * kHid + kOut); i += 64 * kInferWaves)
-    b_s[i] = (i < kHid) ? a.W[kB1 + i] : (i < 2 * kHid) ? a.W[kB2 + i - kHid]
-                                                          : a.W[kB3 + i - 2 * kHid];
+  v8s* w_s = L.w_s;
+  float* b_s = L.b_s;
+  uint32_t (*io_s)[16 * kCol] = L.io_s;
+  bool frame = true;
+  if (MODE != kModeCold) {
+    float in[18];
+    frame = canonical_inputs(A, B, in);
+    uint32_t* col = &io_s[wave][lane];
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+      col[r * kCol] = static_cast<uint32_t>(bf16_bits(in[2 * r])) |
+                      (static_cast<uint32_t>(bf16_bits(in[2 * r + 1])) << 16);
+  }
+  lds_wait();
+  __builtin_amdgcn_wave_barrier();
+  // kSubs sub-tiles of 16 pairs at a time, layer by layer: independent
+  // MFMA chains share each weight tile read from LDS.
+#pragma unroll 1
+  for (int s0 = 0; s0 < (MODE == kModeCold ? 0 : 4); s0 += kSubs) {
+    // opaque per group: keeps the compiler from hoisting the weight and
+    // bias loads out of the loops into ~100 registers
+    int wl = lane, bl = 4 * g;
+    asm volatile("" : "+v"(wl), "+v"(bl));
+    // B operands of layer 1: lane (c, g) holds inputs 8g..8g+7 of its
+    // pair (dword rows 4g..4g+3; only inputs 0..17 are nonzero)
+    v8s x[kSubs];
+#pragma unroll
+    for (int u = 0; u < kSubs; ++u) {
+      const int pl = (s0 + u) * 16 + c;
+      x[u] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (g < 2) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t w = io_s[wave][(4 * g + r) * kCol + pl];
+          x[u][2 * r] = static_cast<short>(w & 0xffffu);
+          x[u][2 * r + 1] = static_cast<short>(w >> 16);
+        }
+      } else if (g == 2) {
+        const uint32_t w = io_s[wave][8 * kCol + pl];
+        x[u][0] = static_cast<short>(w & 0xffffu);
+        x[u][1] = static_cast<short>(w >> 16);
+      }
+    }
+    v4s act1[kSubs][4], act2[kSubs][4];
+#pragma unroll
+    for (int to = 0; to < 4; ++to) {
+      const v8s w = lds_tile(w_s, to, wl);
+      v4f b;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = b_s[to * 16 + bl + j];
+#pragma unroll
+      for (int u = 0; u < kSubs; ++u) {
+        const v4f acc = mfma32(w, x[u], b);
+        act1[u][to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
+      }
+    }
+#pragma unroll
+    for (int to = 0; to < 4; ++to) {
+      const v8s w0 = lds_tile(w_s, 4 + 2 * to, wl), w1 = lds_tile(w_s, 5 + 2 * to, wl);
+      v4f b;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = b_s[kHid + to * 16 + bl + j];
+#pragma unroll
+      for (int u = 0; u < kSubs; ++u) {
+        v4f acc = mfma32(w0, cat8(act1[u][0], act1[u][1]), b);
+        acc = mfma32(w1, cat8(act1[u][2], act1[u][3]), acc);
+        act2[u][to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
+      }
+    }
+    if (MODE == kModeFeatures) {  // output tile 0: feature logits and the exact-path logit
+      const v8s w0 = lds_tile(w_s, 12, wl), w1 = lds_tile(w_s, 13, wl);
+      v4f b;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = b_s[2 * kHid + bl + j];
+#pragma unroll
+      for (int u = 0; u < kSubs; ++u) {
+        v4f acc = mfma32(w0, cat8(act2[u][0], act2[u][1]), b);
+        acc = mfma32(w1, cat8(act2[u][2], act2[u][3]), acc);
+        const int pl = (s0 + u) * 16 + c;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) io_s[wave][(4 * g + j) * kCol + pl] = __float_as_uint(acc[j]);
+      }
+    } else {  // output tile 1: outputs 16..19 (lane group 0) = the penalty start, to rows 0..3
+      const v8s w0 = lds_tile(w_s, 14, wl), w1 = lds_tile(w_s, 15, wl);
+      v4f b;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = b_s[2 * kHid + 16 + bl + j];
+#pragma unroll
+      for (int u = 0; u < kSubs; ++u) {
+        v4f acc = mfma32(w0, cat8(act2[u][0], act2[u][1]), b);
+        acc = mfma32(w1, cat8(act2[u][2], act2[u][3]), acc);
+        const int pl = (s0 + u) * 16 + c;
+        if (g == 0) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) io_s[wave][j * kCol + pl] = __float_as_uint(acc[j]);
+        }
+      }
+    }
+  }
+  lds_wait();
+  __builtin_amdgcn_wave_barrier();
+  done = false;
+  if (MODE == kModeFeatures) {
+    // the lane's own pair: the best-ranked edge-edge and the best-ranked
+    // vertex-face feature, both evaluated (no divergence between the two
+    // kinds), the nearer one checked
+    int fe = 0, fv = 9;
+    float le = -3.0e38f, lv = -3.0e38f;
+#pragma unroll
+    for (int f = 0; f < kNumFeat; ++f) {
+      const float v = __uint_as_float(io_s[wave][f * kCol + lane]);
+      if (f < 9) {
+        fe = v > le ? f : fe;
+        le = v > le ? v : le;
+      } else {
+        fv = v > lv ? f : fv;
+        lv = v > lv ? v : lv;
+      }
+    }
+    const bool exact_path =
+        !frame || (__uint_as_float(io_s[wave][kFlagOut * kCol + lane]) > 0.f);
+    V3<double> Pe, Qe, Pv, Qv;
+    unsigned sAe, sBe, sAv, sBv;
+    double Ve = eval_ee_value(A, B, fe, Pe, Qe, sAe, sBe);
+    // one feature at a time: an empty asm makes the vertex-face code depend
+    // on the edge-edge result, so the two evaluations are not interleaved
+    asm volatile("" : "+v"(fv) : "v"(Ve));
+    double Vv = eval_vf_value(A, B, fv, Pv, Qv, sAv, sBv);
+    // bfSolver's final rule between its two minima (bf.cpp:1102)
+    const bool take_e = Vv > Ve;
+    const int win0 = take_e ? fe : fv;
+    unsigned open = kUnresolvedAll;
+    if (valid && !exact_path)
+      open = support_choice(A, B, sel3(take_e, Pe, Pv), sel3(take_e, Qe, Qv),
+                            take_e ? sAe : sAv, take_e ? sBe : sBv, win0);
+    evals += (valid && !exact_path) ? 2 : 0;
+    // Ties of a degenerate closest pair (a vertex against an edge or a
+    // vertex): the few features that reach the same points are evaluated
+    // and folded into the two minima in bfSolver's order, each minimum
+    // keeping its points.  The wave runs its lanes' edge-edge ones
+    // together, then its vertex-face ones.
+    unsigned rest = (open == kUnresolvedAll) ? 0u : open & ~((1u << fe) | (1u << fv));
+    unsigned rest_e = rest & 0x1FFu, rest_v = rest & 0x7E00u;
+    while (__ballot(rest_e != 0u)) {
+      if (rest_e) {
+        const int g = __builtin_ctz(rest_e);
+        rest_e &= rest_e - 1u;
+        V3<double> Pg, Qg;
+        const double val = eval_ee_value(A, B, g, Pg, Qg);
+        const bool better = val < Ve || (val == Ve && g < fe);
+        fe = better ? g : fe;
+        Ve = better ? val : Ve;
+        Pe = sel3(better, Pg, Pe);
+        Qe = sel3(better, Qg, Qe);
+        ++evals;
+      }
+    }
+    while (__ballot(rest_v != 0u)) {
+      if (rest_v) {
+        const int g = __builtin_ctz(rest_v);
+        rest_v &= rest_v - 1u;
+        V3<double> Pg, Qg;
+        const double val = eval_vf_value(A, B, g, Pg, Qg);
+        const bool better = val < Vv || (val == Vv && g < fv);
+        fv = better ? g : fv;
+        Vv = better ? val : Vv;
+        Pv = sel3(better, Pg, Pv);
+        Qv = sel3(better, Qg, Qv);
+        ++evals;
+      }
+    }
+    // the final pick (bfSolver's rule again on the folded minima)
+    const bool fin_e = Vv > Ve;
+    P = sel3(fin_e, Pe, Pv);
+    Q = sel3(fin_e, Qe, Qv);
+    done = valid && open != kUnresolvedAll;
+  } else {
+    // penalty Newton from the network's barycentric guess (or 0.33), the
+    // result clamped onto the triangles and certified like a feature
+    double x0[4] = {0.33, 0.33, 0.33, 0.33};
+    if (MODE == kModeWarm)
+      for (int k = 0; k < 4; ++k) x0[k] = __uint_as_float(io_s[wave][k * kCol + lane]);
+    const int its = warm_penalty(A, B, x0, P, Q);
+    done = valid && certify(A, B, P, Q);
+    evals += valid ? its : 0;
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void learned_body(const LearnedArgs& a) {
+  __shared__ LearnedLds L;
+  __shared__ unsigned wg_tally[3];  // certified, features evaluated, queued
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x < 3) wg_tally[threadIdx.x] = 0;
-  __syncthreads();
+  stage_network<MODE>(L, a.W);
   unsigned cert_count = 0, evals = 0;  // per lane
   const int64_t qbase = static_cast<int64_t>(blockIdx.x) * a.region;
   const int64_t nchunks = (a.n + 63) / 64;
@@ -474,192 +685,11 @@ __device__ __forceinline__ void learned_body(const LearnedArgs& a) {
     const int64_t i = ch * 64 + lane;
     const bool valid = i < a.n;
     const int64_t ii = valid ? i : 0;
-    bool frame = true;
     const Tri<double> A = load_tri_soa(a.soa, a.n, ii, 0);
     const Tri<double> B = load_tri_soa(a.soa, a.n, ii, 9);
-    if (MODE != kModeCold) {
-      float in[18];
-      frame = canonical_inputs(A, B, in);
-      uint32_t* col = &io_s[wave][lane];
-#pragma unroll
-      for (int r = 0; r < 9; ++r)
-        col[r * kCol] = static_cast<uint32_t>(bf16_bits(in[2 * r])) |
-                        (static_cast<uint32_t>(bf16_bits(in[2 * r + 1])) << 16);
-    }
-    lds_wait();
-    __builtin_amdgcn_wave_barrier();
-    // kSubs sub-tiles of 16 pairs at a time, layer by layer: independent
-    // MFMA chains share each weight tile read from LDS.
-#pragma unroll 1
-    for (int s0 = 0; s0 < (MODE == kModeCold ? 0 : 4); s0 += kSubs) {
-      // opaque per group: keeps the compiler from hoisting the weight and
-      // bias loads out of the loops into ~100 registers
-      int wl = lane, bl = 4 * g;
-      asm volatile("" : "+v"(wl), "+v"(bl));
-      // B operands of layer 1: lane (c, g) holds inputs 8g..8g+7 of its
-      // pair (dword rows 4g..4g+3; only inputs 0..17 are nonzero)
-      v8s x[kSubs];
-#pragma unroll
-      for (int u = 0; u < kSubs; ++u) {
-        const int pl = (s0 + u) * 16 + c;
-        x[u] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (g < 2) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const uint32_t w = io_s[wave][(4 * g + r) * kCol + pl];
-            x[u][2 * r] = static_cast<short>(w & 0xffffu);
-            x[u][2 * r + 1] = static_cast<short>(w >> 16);
-          }
-        } else if (g == 2) {
-          const uint32_t w = io_s[wave][8 * kCol + pl];
-          x[u][0] = static_cast<short>(w & 0xffffu);
-          x[u][1] = static_cast<short>(w >> 16);
-        }
-      }
-      v4s act1[kSubs][4], act2[kSubs][4];
-#pragma unroll
-      for (int to = 0; to < 4; ++to) {
-        const v8s w = lds_tile(w_s, to, wl);
-        v4f b;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = b_s[to * 16 + bl + j];
-#pragma unroll
-        for (int u = 0; u < kSubs; ++u) {
-          const v4f acc = mfma32(w, x[u], b);
-          act1[u][to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
-        }
-      }
-#pragma unroll
-      for (int to = 0; to < 4; ++to) {
-        const v8s w0 = lds_tile(w_s, 4 + 2 * to, wl), w1 = lds_tile(w_s, 5 + 2 * to, wl);
-        v4f b;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = b_s[kHid + to * 16 + bl + j];
-#pragma unroll
-        for (int u = 0; u < kSubs; ++u) {
-          v4f acc = mfma32(w0, cat8(act1[u][0], act1[u][1]), b);
-          acc = mfma32(w1, cat8(act1[u][2], act1[u][3]), acc);
-          act2[u][to] = relu4(pack4(acc[0], acc[1], acc[2], acc[3]));
-        }
-      }
-      if (MODE == kModeFeatures) {  // output tile 0: feature logits and the exact-path logit
-        const v8s w0 = lds_tile(w_s, 12, wl), w1 = lds_tile(w_s, 13, wl);
-        v4f b;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = b_s[2 * kHid + bl + j];
-#pragma unroll
-        for (int u = 0; u < kSubs; ++u) {
-          v4f acc = mfma32(w0, cat8(act2[u][0], act2[u][1]), b);
-          acc = mfma32(w1, cat8(act2[u][2], act2[u][3]), acc);
-          const int pl = (s0 + u) * 16 + c;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) io_s[wave][(4 * g + j) * kCol + pl] = __float_as_uint(acc[j]);
-        }
-      } else {  // output tile 1: outputs 16..19 (lane group 0) = the penalty start, to rows 0..3
-        const v8s w0 = lds_tile(w_s, 14, wl), w1 = lds_tile(w_s, 15, wl);
-        v4f b;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = b_s[2 * kHid + 16 + bl + j];
-#pragma unroll
-        for (int u = 0; u < kSubs; ++u) {
-          v4f acc = mfma32(w0, cat8(act2[u][0], act2[u][1]), b);
-          acc = mfma32(w1, cat8(act2[u][2], act2[u][3]), acc);
-          const int pl = (s0 + u) * 16 + c;
-          if (g == 0) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) io_s[wave][j * kCol + pl] = __float_as_uint(acc[j]);
-          }
-        }
-      }
-    }
-    lds_wait();
-    __builtin_amdgcn_wave_barrier();
     V3<double> P, Q;
-    bool done = false;
-    if (MODE == kModeFeatures) {
-      // the lane's own pair: the best-ranked edge-edge and the best-ranked
-      // vertex-face feature, both evaluated (no divergence between the two
-      // kinds), the nearer one checked
-      int fe = 0, fv = 9;
-      float le = -3.0e38f, lv = -3.0e38f;
-#pragma unroll
-      for (int f = 0; f < kNumFeat; ++f) {
-        const float v = __uint_as_float(io_s[wave][f * kCol + lane]);
-        if (f < 9) {
-          fe = v > le ? f : fe;
-          le = v > le ? v : le;
-        } else {
-          fv = v > lv ? f : fv;
-          lv = v > lv ? v : lv;
-        }
-      }
-      const bool exact_path =
-          !frame || (__uint_as_float(io_s[wave][kFlagOut * kCol + lane]) > 0.f);
-      V3<double> Pe, Qe, Pv, Qv;
-      unsigned sAe, sBe, sAv, sBv;
-      double Ve = eval_ee_value(A, B, fe, Pe, Qe, sAe, sBe);
-      // one feature at a time: an empty asm makes the vertex-face code depend
-      // on the edge-edge result, so the two evaluations are not interleaved
-      asm volatile("" : "+v"(fv) : "v"(Ve));
-      double Vv = eval_vf_value(A, B, fv, Pv, Qv, sAv, sBv);
-      // bfSolver's final rule between its two minima (bf.cpp:1102)
-      const bool take_e = Vv > Ve;
-      const int win0 = take_e ? fe : fv;
-      unsigned open = kUnresolvedAll;
-      if (valid && !exact_path)
-        open = support_choice(A, B, sel3(take_e, Pe, Pv), sel3(take_e, Qe, Qv),
-                              take_e ? sAe : sAv, take_e ? sBe : sBv, win0);
-      evals += (valid && !exact_path) ? 2 : 0;
-      // Ties of a degenerate closest pair (a vertex against an edge or a
-      // vertex): the few features that reach the same points are evaluated
-      // and folded into the two minima in bfSolver's order, each minimum
-      // keeping its points.  The wave runs its lanes' edge-edge ones
-      // together, then its vertex-face ones.
-      unsigned rest = (open == kUnresolvedAll) ? 0u : open & ~((1u << fe) | (1u << fv));
-      unsigned rest_e = rest & 0x1FFu, rest_v = rest & 0x7E00u;
-      while (__ballot(rest_e != 0u)) {
-        if (rest_e) {
-          const int g = __builtin_ctz(rest_e);
-          rest_e &= rest_e - 1u;
-          V3<double> Pg, Qg;
-          const double val = eval_ee_value(A, B, g, Pg, Qg);
-          const bool better = val < Ve || (val == Ve && g < fe);
-          fe = better ? g : fe;
-          Ve = better ? val : Ve;
-          Pe = sel3(better, Pg, Pe);
-          Qe = sel3(better, Qg, Qe);
-          ++evals;
-        }
-      }
-      while (__ballot(rest_v != 0u)) {
-        if (rest_v) {
-          const int g = __builtin_ctz(rest_v);
-          rest_v &= rest_v - 1u;
-          V3<double> Pg, Qg;
-          const double val = eval_vf_value(A, B, g, Pg, Qg);
-          const bool better = val < Vv || (val == Vv && g < fv);
-          fv = better ? g : fv;
-          Vv = better ? val : Vv;
-          Pv = sel3(better, Pg, Pv);
-          Qv = sel3(better, Qg, Qv);
-          ++evals;
-        }
-      }
-      // the final pick (bfSolver's rule again on the folded minima)
-      const bool fin_e = Vv > Ve;
-      P = sel3(fin_e, Pe, Pv);
-      Q = sel3(fin_e, Qe, Qv);
-      done = valid && open != kUnresolvedAll;
-    } else {
-      // penalty Newton from the network's barycentric guess (or 0.33), the
-      // result clamped onto the triangles and certified like a feature
-      double x0[4] = {0.33, 0.33, 0.33, 0.33};
-      if (MODE == kModeWarm)
-        for (int k = 0; k < 4; ++k) x0[k] = __uint_as_float(io_s[wave][k * kCol + lane]);
-      const int its = warm_penalty(A, B, x0, P, Q);
-      done = valid && certify(A, B, P, Q);
-      evals += valid ? its : 0;
-    }
+    bool done;
+    learned_pair<MODE>(L, A, B, valid, P, Q, done, evals);
     // every lane stores (the brute force overwrites the uncertified ones):
     // whole cache lines, no partial-line writes
     if (valid) {
@@ -742,12 +772,161 @@ k_learned(LearnedArgs a) {
   learned_body<MODE>(a);
 }
 
+// ------------------------------------------------ the step's survivor queue
+// The triangle pairs that the DEM step's box mid-phase leaves (the survivor
+// queue of detect.hip, entries {item, iA | iB << 16}) through the learned
+// decision: a wave per 64 entries on a persistent grid.  A finished entry
+// sets flags[e] (its pair emits a contact: d <= epsA + epsB, bf.cpp:125) and,
+// when it emits, its P, Q at pq[6e..6e+6); the rest are listed for the
+// brute force (k_queue_fallback).  Both write exactly what the brute-force
+// queue solve would.
+struct QueueLearnArgs {
+  const int2* queue;
+  const int* pairs;
+  int tiles;  // items per particle pair
+  const int* tri_offset;
+  const double* x;
+  const double* y;
+  const double* z;
+  const double* eps;
+  long long n;                         // entries of the launch
+  const unsigned long long* need_dev;  // live entries (captured step), else null
+  int* flags;
+  double* pq;
+  const float* W;
+  int* fb_list;
+  unsigned* fb_count;
+};
+
+__device__ __forceinline__ Tri<double> load_tri_xyz(const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ z, int t) {
+  Tri<double> r;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) r.v[k] = {x[3 * t + k], y[3 * t + k], z[3 * t + k]};
+  return r;
+}
+
+__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(2))) void
+k_queue_learned(QueueLearnArgs a) {
+  __shared__ LearnedLds L;
+  stage_network<kModeFeatures>(L, a.W);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long live =
+      a.need_dev && static_cast<long long>(*a.need_dev) < a.n ? static_cast<long long>(*a.need_dev)
+                                                              : a.n;
+  const long long nchunks = (a.n + 63) / 64;
+  unsigned evals = 0;
+  for (long long ch = static_cast<long long>(blockIdx.x) * kInferWaves + wave; ch < nchunks;
+       ch += static_cast<long long>(gridDim.x) * kInferWaves) {
+    const long long e = ch * 64 + lane;
+    const int2 en = e < live ? a.queue[e] : make_int2(-1, 0);
+    const bool valid = en.x >= 0;
+    Tri<double> A = {}, B = {};
+    double margin = 0.0;
+    if (valid) {
+      const int k = en.x / a.tiles;
+      const int pa = a.pairs[2 * k], pb = a.pairs[2 * k + 1];
+      const int iA = static_cast<int>(static_cast<unsigned>(en.y) & 0xffffu);
+      const int iB = static_cast<int>(static_cast<unsigned>(en.y) >> 16);
+      A = load_tri_xyz(a.x, a.y, a.z, a.tri_offset[pa] + iA);
+      B = load_tri_xyz(a.x, a.y, a.z, a.tri_offset[pb] + iB);
+      margin = a.eps[pa] + a.eps[pb];
+    }
+    V3<double> P, Q;
+    bool done;
+    learned_pair<kModeFeatures>(L, A, B, valid, P, Q, done, evals);
+    if (e < a.n && !(valid && !done)) {
+      const bool emit = valid && pq_dist(P, Q) <= margin;
+      a.flags[e] = emit ? 1 : 0;
+      if (emit) {
+        double* o = a.pq + 6 * e;
+        o[0] = P.x; o[1] = P.y; o[2] = P.z;
+        o[3] = Q.x; o[4] = Q.y; o[5] = Q.z;
+      }
+    }
+    // the rest to the brute force: one append per wave
+    const bool need = valid && !done;
+    const unsigned long long mask = __ballot(need);
+    if (mask) {
+      unsigned base = 0;
+      if (lane == 0) base = atomicAdd(a.fb_count, static_cast<unsigned>(__popcll(mask)));
+      base = __shfl(base, 0);
+      const int pre = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                                __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0));
+      if (need) a.fb_list[base + pre] = static_cast<int>(e);
+    }
+  }
+}
+
+// The listed entries by the brute force, one lane each (launched for the
+// capacity; the lanes past the list's length return).
+__global__ __launch_bounds__(256) void k_queue_fallback(QueueLearnArgs a) {
+  const long long j = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= static_cast<long long>(*a.fb_count) || j >= a.n) return;
+  const long long e = a.fb_list[j];
+  const int2 en = a.queue[e];
+  const int k = en.x / a.tiles;
+  const int pa = a.pairs[2 * k], pb = a.pairs[2 * k + 1];
+  const int iA = static_cast<int>(static_cast<unsigned>(en.y) & 0xffffu);
+  const int iB = static_cast<int>(static_cast<unsigned>(en.y) >> 16);
+  const Tri<double> A = load_tri_xyz(a.x, a.y, a.z, a.tri_offset[pa] + iA);
+  const Tri<double> B = load_tri_xyz(a.x, a.y, a.z, a.tri_offset[pb] + iB);
+  V3<double> P, Q;
+  bf_solver(A, B, P, Q);
+  const bool emit = pq_dist(P, Q) <= a.eps[pa] + a.eps[pb];
+  a.flags[e] = emit ? 1 : 0;
+  if (emit) {
+    double* o = a.pq + 6 * e;
+    o[0] = P.x; o[1] = P.y; o[2] = P.z;
+    o[3] = Q.x; o[4] = Q.y; o[5] = Q.z;
+  }
+}
+
 struct LearnedState {
   DevBuf queue, qsoa, wg_counts, model_copy;
   const float* model = nullptr;  // points into model_copy once set
   unsigned grid = 0;              // workgroups of the last launch
 };
 static LearnedState g_learned;
+
+struct QueueLearnState {
+  DevBuf fb_list, fb_count;
+};
+static QueueLearnState g_qlearn;
+
+// Buffers of the learned queue solve for n entries (before a capture).
+int queue_learned_reserve(long long n) {
+  int rc;
+  if ((rc = g_qlearn.fb_list.grow(sizeof(int) * (n + 1)))) return rc;
+  return g_qlearn.fb_count.grow(sizeof(unsigned));
+}
+
+// The survivor queue's n entries (live: *need_dev when set) through the
+// learned decision, then the brute force for the rest; fills instead of
+// memsets (captured steps).
+int queue_learned(const int2* queue, const int* pairs, int tiles, const int* tri_offset,
+                  const double* x, const double* y, const double* z, const double* eps,
+                  long long n, const unsigned long long* need_dev, int* flags, double* pq,
+                  hipStream_t st) {
+  if (!g_learned.model)
+    return set_error(DH_ERR_ARG, "DH_BF_LEARNED: no network set (dh_learned_set_model)");
+  int rc;
+  if ((rc = queue_learned_reserve(n))) return rc;
+  QueueLearnArgs a{queue, pairs, tiles, tri_offset, x, y, z, eps, n, need_dev, flags, pq,
+                   g_learned.model, g_qlearn.fb_list.as<int>(), g_qlearn.fb_count.as<unsigned>()};
+  fill_async(g_qlearn.fb_count.as<unsigned>(), 0u, 1, st);
+  if (n > 0) {
+    const long long chunks = (n + 63) / 64;
+    long long grid = (chunks + kInferWaves - 1) / kInferWaves;
+    const long long max_grid = static_cast<long long>(device_cus()) * 4;
+    if (grid > max_grid) grid = max_grid;
+    hipLaunchKernelGGL(k_queue_learned, static_cast<unsigned>(grid), 64 * kInferWaves, 0, st, a);
+    hipLaunchKernelGGL(k_queue_fallback, grid_for(n, 256), 256, 0, st, a);
+  }
+  DH_HIP_CHECK(hipGetLastError());
+  return DH_OK;
+}
 
 int learned_pairs(int mode, const double* soa, int64_t n, double* P, double* Q, double* d,
                   uint8_t* flags, hipStream_t st) {

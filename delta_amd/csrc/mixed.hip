@@ -3,13 +3,33 @@
 //
 // DH_PAIR_MIXED: every pair is solved by bfSolver's arithmetic in binary32 on
 // the caller's binary32 copy (72 bytes per pair), with coordinates taken
-// relative to A's vertex 0.  Pairs whose binary32 distance is below the
-// recheck band (2 * eps_total, or not finite) are appended to the
+// relative to A's vertex 0.  The binary32 solve only picks the closest
+// feature pair; the distance is bounded from both sides in binary64 on the
+// binary32 triangles (which binary64 holds exactly):
+//   * UB = |Q - P| with P, Q that feature's closest points evaluated in
+//     binary64 (points on the triangles, up to binary64 rounding);
+//   * LB = min over B's vertices of m.b - max over A's vertices of m.a,
+//     m = (Q - P)/|Q - P| (a separating direction; its vertices suffice by
+//     convexity), less 2^-49 (RA + RB) for the binary64 rounding of the six
+//     dot products (RA, RB the largest 1-norms of the relative vertices);
+//   * the binary32 copy moved each vertex by at most
+//     2^-24 (|v|_1 + |o|_1 + |v - o|_1) (the copy's rounding of v and of A's
+//     vertex o, and the rounding of the difference), so the true distance
+//     lies within delta (the largest such shift of A plus that of B) of the
+//     binary32 triangles' distance.
+// Then |UB - d| <= err = (UB - LB) + delta for the true distance d, and
+// d >= LB - delta.  A pair is finished in binary32 when LB - delta is at or
+// above the recheck band (2 * eps_total) and err <= 1e-6 (LB - delta): its
+// distance is within 1e-6 relative of the binary64 brute force's by
+// construction, and it is no contact.  Every other pair -- within the band,
+// near-parallel or degenerate (the binary32 feature is then not a closest
+// pair and UB - LB exceeds the allowance), not finite -- is appended to the
 // workgroup's own region of a queue through an LDS counter and, after the
 // workgroup's last chunk, recomputed by the binary64 brute force from the
 // binary64 coordinates.  Every contact (d <= eps_total) is therefore decided
-// in binary64; the pairs finished in binary32 carry binary32 accuracy.
+// in binary64.
 #include "exact_math.hpp"
+#include "pair_features.hpp"
 #include "runtime.hpp"
 
 #include <vector>
@@ -72,18 +92,51 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_mixed(MixedArgs a) {
                 s[(15 + k) * n + ii] - o.z};
     }
     V3<float> p, q;
-    const bool edge = bf_solver(A, B, p, q);
-    const float d32 = pq_dist(p, q);
-    const bool keep = valid && (d32 >= a.band) && (d32 <= 3.0e38f);
+    int ssid = 0, vid = 0;
+    const bool edge = bf_solver_ids(A, B, p, q, ssid, vid);
+    // the feature in binary64 on the binary32 triangles (exact in binary64)
+    Tri<double> Ad, Bd;
+    double RA = 0.0, RB = 0.0, shA = 0.0, shB = 0.0;
+    const double o1 = fabs(static_cast<double>(o.x)) + fabs(static_cast<double>(o.y)) +
+                      fabs(static_cast<double>(o.z));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Ad.v[k] = {A.v[k].x, A.v[k].y, A.v[k].z};
+      Bd.v[k] = {B.v[k].x, B.v[k].y, B.v[k].z};
+      const double ra = fabs(Ad.v[k].x) + fabs(Ad.v[k].y) + fabs(Ad.v[k].z);
+      const double rb = fabs(Bd.v[k].x) + fabs(Bd.v[k].y) + fabs(Bd.v[k].z);
+      RA = fmax(RA, ra);
+      RB = fmax(RB, rb);
+      // |v|_1 of the absolute vertex <= |v - o|_1 + |o|_1
+      shA = fmax(shA, 2.0 * (ra + o1));
+      shB = fmax(shB, 2.0 * (rb + o1));
+    }
+    V3<double> Pd, Qd;
+    eval_feature(Ad, Bd, edge ? ssid : 9 + vid, Pd, Qd);
+    const V3<double> nv = sub(Qd, Pd);
+    const double ub = sqrt(dot(nv, nv)) * (1.0 + 0x1.0p-50);
+    const double inv = 1.0 / ub;
+    const V3<double> m = {nv.x * inv, nv.y * inv, nv.z * inv};
+    double amax = -INFINITY, bmin = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      amax = fmax(amax, dot(m, Ad.v[k]));
+      bmin = fmin(bmin, dot(m, Bd.v[k]));
+    }
+    const double lb = bmin - amax - 0x1.0p-49 * (RA + RB);
+    const double delta = 0x1.0p-24 * (1.0 + 0x1.0p-20) * (shA + shB);
+    const double dlo = lb - delta;
+    const double err = (ub - lb) + delta;
+    const bool keep = valid && dlo >= static_cast<double>(a.band) && err <= 1e-6 * dlo;
     if (keep) {
       const double ox = o.x, oy = o.y, oz = o.z;
-      a.P[i] = ox + static_cast<double>(p.x);
-      a.P[n + i] = oy + static_cast<double>(p.y);
-      a.P[2 * n + i] = oz + static_cast<double>(p.z);
-      a.Q[i] = ox + static_cast<double>(q.x);
-      a.Q[n + i] = oy + static_cast<double>(q.y);
-      a.Q[2 * n + i] = oz + static_cast<double>(q.z);
-      a.d[i] = static_cast<double>(d32);
+      a.P[i] = ox + Pd.x;
+      a.P[n + i] = oy + Pd.y;
+      a.P[2 * n + i] = oz + Pd.z;
+      a.Q[i] = ox + Qd.x;
+      a.Q[n + i] = oy + Qd.y;
+      a.Q[2 * n + i] = oz + Qd.z;
+      a.d[i] = ub;
       if (a.flags) a.flags[i] = edge ? DH_FLAG_EDGE : 0;
       finished++;
     }

@@ -123,3 +123,61 @@ class Trainer:
     def install(self):
         """Make these parameters the ones DH_PAIR_LEARNED uses."""
         _lib.check(self.L.dh_learned_set_model(self.params.ptr), "set_model")
+
+
+def candidate_pairs(hs):
+    """Particle pairs (i < j, not both obstacles) whose boxes, grown by their
+    margins and 0.5e-9 m, overlap: the broad phase's candidate set, on the
+    host."""
+    off = hs.tri_offset
+    lo = np.empty((hs.n, 3))
+    hi = np.empty((hs.n, 3))
+    for p in range(hs.n):
+        a, b = 3 * off[p], 3 * off[p + 1]
+        v = np.stack([hs.x[a:b], hs.y[a:b], hs.z[a:b]], 1)
+        m = hs.eps[p] + 0.5e-9
+        lo[p] = v.min(0) - m
+        hi[p] = v.max(0) + m
+    i, j = np.triu_indices(hs.n, 1)
+    ok = np.all(lo[i] <= hi[j], 1) & np.all(lo[j] <= hi[i], 1)
+    ok &= ~((hs.obstacle[i] != 0) & (hs.obstacle[j] != 0))
+    return i[ok], j[ok]
+
+
+def triangle_pairs(hs, pi, pj):
+    """Every triangle pair of the particle pairs (pi[k], pj[k]) as an 18 x m
+    SoA block (A's vertices x, y, z, then B's; row 3c + k is coordinate c of
+    vertex k), in pair order, A's triangles outer."""
+    off = hs.tri_offset
+    ia, ib = [], []
+    for i, j in zip(pi, pj):
+        a = np.arange(off[i], off[i + 1])
+        b = np.arange(off[j], off[j + 1])
+        ia.append(np.repeat(a, len(b)))
+        ib.append(np.tile(b, len(a)))
+    ia = np.concatenate(ia)
+    ib = np.concatenate(ib)
+    soa = np.empty((18, len(ia)))
+    for side, t in ((0, ia), (9, ib)):
+        for c, arr in enumerate((hs.x, hs.y, hs.z)):
+            for k in range(3):
+                soa[side + 3 * c + k] = arr[3 * t + k]
+    return soa
+
+
+def c1_pairs(n, seed=6):
+    """n triangle pairs of the 64-icosphere lattice's candidate particle pairs
+    (the DEM step's contact distribution): lattices drawn with shape and
+    rotation seeds 1000 seed + 2s, + 2s + 1 (s = 0, 1, ...) until n pairs are
+    collected, then shuffled with `seed`.  18 x n SoA (host)."""
+    from .engine import c1_scene
+    blocks, total, s = [], 0, 0
+    while total < n:
+        hs = c1_scene(seed_shape=1000 * seed + 2 * s, seed_rot=1000 * seed + 2 * s + 1)
+        pi, pj = candidate_pairs(hs)
+        blocks.append(triangle_pairs(hs, pi, pj))
+        total += blocks[-1].shape[1]
+        s += 1
+    soa = np.concatenate(blocks, axis=1)
+    perm = np.random.default_rng(seed).permutation(soa.shape[1])[:n]
+    return np.ascontiguousarray(soa[:, perm])

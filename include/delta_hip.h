@@ -45,7 +45,12 @@ enum {
   DH_PENALTY_STAT = 4,           /* penaltyStat                penalty.cpp:50-111  */
   DH_HYBRID_BATCH_STAT = 5,      /* hybridBatchStat            hybrid.cpp:489-627  */
   DH_HYBRID_TRIANGLE_STAT = 6,   /* hybridTriangleStat         hybrid.cpp:202-336  */
-  DH_GJK = 7                     /* detection::gjk             gjk.cpp:25-100      */
+  DH_GJK = 7,                    /* detection::gjk             gjk.cpp:25-100      */
+  DH_BF_LEARNED = 8              /* DH_BF's contacts bit for bit; the survivor queue's
+                                    triangle pairs (always on) go through the learned
+                                    narrow phase (network of dh_learned_set_model,
+                                    features in FP64, support test), the rest through
+                                    bfSolver.  No counterpart in the reference.  */
 };
 
 /* Unique triangle-pair kernels (the isolated narrow-phase workload). */

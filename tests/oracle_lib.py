@@ -58,6 +58,20 @@ def oracle():
     return _oracle
 
 
+_oracle_omp = None
+
+
+def oracle_omp():
+    """oracle/liboracle_omp.so: the same restatement with the OpenMP CPU
+    baseline drivers (oracle/cpu_baseline.c)."""
+    global _oracle_omp
+    if _oracle_omp is None:
+        if not os.path.exists(ORACLE_OMP_SO):
+            raise RuntimeError("oracle/liboracle_omp.so is not built (make -C oracle oracle)")
+        _oracle_omp = C.CDLL(ORACLE_OMP_SO)
+    return _oracle_omp
+
+
 def ref_available():
     return os.path.exists(REF_SO)
 
@@ -207,9 +221,11 @@ def _ip(a):
     return a.ctypes.data_as(IP)
 
 
-def or_step(hs, method, spring, damper, friction, dt, gravity):
+def or_step(hs, method, spring, damper, friction, dt, gravity, omp_threads=None):
     """One oracle step on a delta_amd.engine.HostScene, in place.  Returns
-    (contacts, checks, candidate pairs)."""
+    (contacts, checks, candidate pairs).  With omp_threads: the
+    reference-shaped OpenMP step of oracle/cpu_baseline.c (brute force only,
+    rows of each particle pair in parallel; a timing driver)."""
     for f in ("x", "y", "z", "rx", "ry", "rz", "eps", "pos", "refpos", "linvel", "angvel",
               "refangvel", "rot", "inertia", "inverse", "mass"):
         setattr(hs, f, np.ascontiguousarray(getattr(hs, f), dtype=np.float64))
@@ -222,6 +238,12 @@ def or_step(hs, method, spring, damper, friction, dt, gravity):
     g = (C.c_double * 3)(*gravity)
     checks = C.c_int64(0)
     npairs = C.c_int(0)
+    if omp_threads:
+        L = oracle_omp()
+        L.or_step_rows_omp.restype = C.c_int64
+        nc = L.or_step_rows_omp(C.byref(sc), C.byref(fp), C.c_double(dt), g, C.byref(checks),
+                                C.byref(npairs), C.c_int(omp_threads))
+        return nc, checks.value, npairs.value
     L = oracle()
     L.or_step.restype = C.c_int64
     nc = L.or_step(C.byref(sc), method, C.byref(fp), C.c_double(dt), g, C.byref(checks),

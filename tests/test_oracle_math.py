@@ -242,3 +242,22 @@ def test_exact_mode_reports_intersections():
     assert abs(P[2, 0]) < 1e-15 and 0 < P[0, 0] < 1 and 0 < P[1, 0] < 1   # on A's plane z = 0
     assert d0[0] > 0                                                        # bfSolver semantics
     assert not f[1] & 16 and d[1] == d0[1] and np.array_equal(P[:, 1], P0[:, 1])
+
+
+def test_openmp_step_driver_matches_the_oracle_step():
+    """The reference-shaped OpenMP step (oracle/cpu_baseline.c, the DEM
+    CPU baseline: rows of each particle pair in parallel, contacts appended
+    in completion order) finds the oracle step's candidate pairs, triangle
+    checks and contacts on the C1 lattice, and its states agree to rounding
+    of the reordered force sums."""
+    import copy
+    from delta_amd import engine
+    a = engine.c1_scene()
+    b = copy.deepcopy(a)
+    for _ in range(2):
+        r0 = ol.or_step(a, 0, 1e5, 0.1, 0.0, 1e-6, (0.0, -9.81, 0.0))
+        r1 = ol.or_step(b, 0, 1e5, 0.1, 0.0, 1e-6, (0.0, -9.81, 0.0), omp_threads=4)
+        assert r0 == r1
+    for f in ("x", "pos", "linvel", "refangvel", "rot"):
+        x0 = np.asarray(getattr(a, f)); x1 = np.asarray(getattr(b, f))
+        assert np.allclose(x0, x1, rtol=1e-9, atol=1e-12 * np.abs(x0).max()), f
