@@ -302,7 +302,7 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_each_way": 1 << 28, "launches": 3}), flush=True)
 
 
-def dem_c1(a, steps=2000):
+def dem_c1(a, steps=2000, learned=False):
     """64 jittered 80-triangle icospheres on a 4x4x4 lattice (1.05 mm) above a
     2-triangle floor, FP64 brute force, spring-dashpot k=1e5 N/m, damping 0.1,
     margin 1e-4 m, dt = 1e-6 s, `steps` (2000) steps from the initial state:
@@ -372,6 +372,37 @@ def dem_c1(a, steps=2000):
            "update_and_binning": hopper_hbm(hs, dg, prm, L)}
     if tuned:
         out["tuning"] = {"table": tuned[0], "trials_value_ms": tuned[1]}
+    if learned:
+        # the same 2000 steps with the survivor queue through the learned
+        # narrow phase (network installed by the C4 section): the final
+        # state must equal the brute-force run's bit for bit
+        pl = engine.step_params(method=_lib.DH_BF_LEARNED, dt=1e-6, spring=1e5, damper=0.1,
+                                friction=0.0, cell=hs.cell_size())
+        dl = engine.DeviceScene(hs)
+        _lib.check(L.dh_synchronize(), "sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            engine.step(dl, pl)
+        _lib.check(L.dh_synchronize(), "sync")
+        ldt = time.perf_counter() - t0
+        dlg = engine.DeviceScene(hs)
+        g = engine.StepGraph(dlg, pl, pair_cap=2 * max_pairs + 64,
+                             contact_cap=2 * max_contacts + 1024)
+        try:
+            _lib.check(L.dh_synchronize(), "sync")
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                g.launch()
+            _, lov = g.status()
+            lgdt = time.perf_counter() - t0
+        finally:
+            g.close()
+        lsame = all(np.array_equal(ds.download(f).view(np.uint64), x.download(f).view(np.uint64))
+                    for f in ("x", "y", "z", "pos", "linvel", "angvel", "rot") for x in (dl, dlg))
+        out["learned_survivor_queue"] = {
+            "ms_per_step": lgdt * 1e3 / steps, "value": n * steps / lgdt,
+            "eager_ms_per_step": ldt * 1e3 / steps, "overflow": lov,
+            "state_equals_brute_force": bool(lsame)}
     if not a.no_cpu:
         # the reference-shaped OpenMP step (rows of each particle pair in
         # parallel, bf.cpp:92-95,137-141) from the same initial state, over
@@ -601,7 +632,7 @@ LARGE = {
 }
 
 
-def dem_large(a, kind, steps=None, warmup=None):
+def dem_large(a, kind, steps=None, warmup=None, learned=False):
     """The uneven funnel fill (10^4 particles) or the rotating box (2*10^5
     particles): eager warm-up steps size the capacities, the detection
     tiling is autotuned on the live candidate pairs, then the whole step is
@@ -677,6 +708,29 @@ def dem_large(a, kind, steps=None, warmup=None):
         overflow |= ov
     finally:
         g.close()
+    learned_out = None
+    if learned:
+        # the same window with the survivor queue through the learned narrow
+        # phase, captured, from the state the brute-force window reached
+        pl = engine.step_params(method=_lib.DH_BF_LEARNED, dt=cfg["dt"], spring=1e5,
+                                damper=0.1, friction=0.0, cell=hs.cell_size())
+        info = engine.step(ds, pl)
+        _lib.check(L.dh_synchronize(), "sync")
+        gl = engine.StepGraph(ds, pl, pair_cap=2 * max_pairs + 1024,
+                              contact_cap=2 * max_contacts + 4096)
+        try:
+            gl.launch()
+            _, lov = gl.status()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                gl.launch()
+            _, ov2 = gl.status()
+            lgdt = time.perf_counter() - t0
+            lov |= ov2
+        finally:
+            gl.close()
+        learned_out = {"ms_per_step": lgdt * 1e3 / steps, "value": hs.n * steps / lgdt,
+                       "overflow": lov}
     free1 = C.c_size_t()
     _lib.check(L.dh_mem_info(C.byref(free1), C.byref(total)), "dh_mem_info")
     hbm = hopper_hbm(hs, ds, prm, L)
@@ -696,6 +750,7 @@ def dem_large(a, kind, steps=None, warmup=None):
            "tuning": {"table": table,
                       "trials_value_ms": [(t["knob"], t["value"], round(t["ms"], 4))
                                           for t in trials]},
+           "learned_survivor_queue": learned_out,
            "hbm_footprint_bytes": int(free0.value - free1.value),
            "scene_build_s": build_s,
            "hbm": hbm}
@@ -832,16 +887,67 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def c4_section(a, n_train=4 << 20, n_test=1 << 20, epochs=10):
+    """The learned warm start on the DEM step's distribution: the network
+    trained on n_train triangle pairs of the 64-icosphere lattice's
+    candidate particle pairs (lattices of seed 6), evaluated on n_test
+    held-out pairs of other lattices (seed 7): top-1/top-2 feature accuracy,
+    the share the learned decision finishes without the brute force, and both
+    paths' pass times on the held-out pairs (their results are bfSolver's bit
+    for bit).  The trained network stays installed for the learned DEM
+    steps."""
+    from delta_amd import _lib, learn, narrow
+    from delta_amd.device import DeviceArray
+    t0 = time.perf_counter()
+    train = DeviceArray.from_host(learn.c1_pairs(n_train, seed=6))
+    test_soa = learn.c1_pairs(n_test, seed=7)
+    build_s = time.perf_counter() - t0
+    ds = learn.Dataset(train, n_train)
+    t0 = time.perf_counter()
+    tr = learn.Trainer(seed=0)
+    losses = tr.fit(ds, epochs=epochs, batch=8192, lr=3e-3)
+    train_s = time.perf_counter() - t0
+    held = learn.Dataset(DeviceArray.from_host(test_soa), n_test)
+    top1, top2 = tr.accuracy(held)
+    tr.install()
+    a._c4_trainer = tr  # keeps the installed parameters alive
+    batch = narrow.PairBatch.from_host(test_soa)
+    res = {}
+    for m in ("bf64", "learned"):
+        ms, _ = time_method(batch, m, 5, 2)
+        res[m] = ms
+    batch.run("learned")
+    cnt = _lib.DhPairCounters()
+    _lib.check(_lib.lib().dh_tri_pairs_counters(C.byref(cnt)), "counters")
+    _, _, d, _ = batch.results()
+    batch.run("bf64")
+    _, _, d0, _ = batch.results()
+    return {"train_pairs": n_train, "held_out_pairs": n_test, "epochs": epochs,
+            "final_loss": losses[-1], "held_out_top1": float(top1),
+            "held_out_top2": float(top2),
+            "held_out_finished_by_network": cnt.certified / n_test,
+            "held_out_distance_bitwise_equal": bool(np.array_equal(d.view(np.uint64),
+                                                                   d0.view(np.uint64))),
+            "held_out_ms_per_pass": res,
+            "dataset_build_s": build_s, "train_s": train_s,
+            "data": "triangle pairs of the candidate particle pairs of 64-icosphere lattices "
+                    "(train: seed 6, held out: seed 7), labelled by the FP64 brute force"}
+
+
 def dem_sections(a, t_start, budget_s=420.0):
-    """The DEM workloads beside the unique-pair headline: C1 (all 2000
-    steps), a window of the uneven funnel fill and of the rotating box (20
-    captured steps each, after eager warm-up steps); each part is skipped,
+    """The DEM workloads beside the unique-pair headline: the network trained
+    on the DEM step's triangle pairs (C4), C1 (all 2000 steps, brute force
+    and learned survivor queue), a window of the uneven funnel fill and of
+    the rotating box (20 captured steps each after eager warm-up steps, the
+    box also with the learned survivor queue); each part is skipped,
     with the reason, when the run has used `budget_s` seconds, and an error
     in one part is recorded without losing the line."""
     parts = {}
-    for name, fn in (("c1", lambda: dem_c1(a)),
+    for name, fn in (("c4_learned", lambda: c4_section(a)),
+                     ("c1", lambda: dem_c1(a, learned=hasattr(a, "_c4_trainer"))),
                      ("funnel", lambda: dem_large(a, "funnel", steps=20, warmup=3)),
-                     ("drum", lambda: dem_large(a, "drum", steps=20, warmup=2))):
+                     ("drum", lambda: dem_large(a, "drum", steps=20, warmup=2,
+                                                learned=hasattr(a, "_c4_trainer")))):
         used = time.perf_counter() - t_start
         if used > budget_s:
             parts[name] = {"skipped": f"bench time budget ({used:.0f} s of {budget_s:.0f} s used)"}
