@@ -303,3 +303,112 @@ def mass_properties_batch(meshes, density):
     J = tr[:, None, None] * np.eye(3)[None] - C
     rho = np.broadcast_to(np.asarray(density, float), vol.shape)
     return rho * vol, com, rho[:, None, None] * J
+
+
+# ---------------------------------------------------------------- hopper
+# The hand-built hopper of delta/geometry/shape/hopper.cpp:28-1075
+# (generateHopper): an open square funnel of four walls, each made of three
+# triangles from the top rim (width w at y = cy + h/2) to the hatch (width
+# `hatch` at y = cy - h/2), an outer copy of the same funnel (width w +
+# thickness, hatch + 0.005), rings of triangles closing the gap between the
+# two along the top and the bottom rim, the inner walls' orientation
+# reversed, and the result refined by edge bisection
+# (operators/triangle.cpp:225-359, meshDenser).  Coordinates are formed with
+# the reference's operations and order, so the triangle soup is the same
+# bit for bit (a test compares it with the routine compiled from the
+# reference tree).
+
+def _funnel_walls(c, w, h, hatch):
+    """The 12 wall triangles (36 vertices) of one funnel: front (+z), back
+    (-z), left (-x), right (+x) walls, each corner points A..E and three
+    triangles (hopper.cpp:28-304)."""
+    cx, cy, cz = c
+    top, bot = cy + h / 2, cy - h / 2
+    ha = hatch / 2
+
+    def faces():
+        # front: A upper left, B/C lower hatch corners, D upper right, E the
+        # top edge's middle (C.x - hatch/2, as the reference forms it)
+        A = (cx - w / 2, top, cz + w / 2); B = (cx - ha, bot, cz + ha); C = (cx + ha, bot, cz + ha)
+        D = (cx + w / 2, top, cz + w / 2); E = (C[0] - ha, top, cz + w / 2)
+        yield (A, B, E), (B, C, E), (D, E, C)
+        A = (cx - w / 2, top, cz - w / 2); B = (cx - ha, bot, cz - ha); C = (cx + ha, bot, cz - ha)
+        D = (cx + w / 2, top, cz - w / 2); E = (C[0] - ha, top, cz - w / 2)
+        yield (B, A, E), (C, B, E), (C, E, D)
+        A = (cx - w / 2, top, cz - w / 2); B = (cx - ha, bot, cz - ha); C = (cx - ha, bot, cz + ha)
+        D = (cx - w / 2, top, cz + w / 2); E = (cx - w / 2, top, C[2] - ha)
+        yield (A, B, E), (B, C, E), (E, C, D)
+        A = (cx + w / 2, top, cz - w / 2); B = (cx + ha, bot, cz - ha); C = (cx + ha, bot, cz + ha)
+        D = (cx + w / 2, top, cz + w / 2); E = (cx + w / 2, top, C[2] - ha)
+        yield (E, B, A), (B, E, C), (E, D, C)
+
+    return [v for tris in faces() for t in tris for v in t]
+
+
+def _rim_ring(inner, outer):
+    """The 12 triangles closing the gap between an inner and an outer rim
+    (corners in the order back-left, back-right, front-right, front-left),
+    with the midpoints of the outer rim's sides (hopper.cpp:306-1038)."""
+    iA, iB, iC, iD = inner
+    eA, eB, eC, eD = outer
+    mid = lambda p, q: tuple((p[k] + q[k]) / 2 for k in range(3))
+    P = {"A": eA, "B": iA, "D": iB, "E": eB, "G": iC, "H": eC, "J": iD, "K": eD}
+    P["C"] = mid(P["A"], P["E"])
+    P["F"] = mid(P["E"], P["H"])
+    P["I"] = mid(P["K"], P["H"])
+    P["L"] = mid(P["A"], P["K"])
+    return P
+
+
+_TOP_RING = ["ABC", "CBD", "CDE", "EDF", "DGF", "GHF", "HGI", "IGJ", "IJK", "KJL", "JBL", "LBA"]
+_BOTTOM_RING = ["CBA", "DBC", "EDC", "FDE", "FGD", "GFH", "IGH", "JGI", "KJI", "LJK", "JLB", "ABL"]
+# inner walls' vertex swaps that reverse their orientation (hopper.cpp:1040-1055)
+_REVERSE = [(0, 2), (3, 5), (6, 8), (9, 11), (12, 14), (15, 17), (18, 20), (23, 21), (24, 26),
+            (28, 29), (30, 32), (33, 35)]
+
+
+def _bisect_refine(verts, levels, grid_h):
+    """meshDenser (operators/triangle.cpp:305-359): `levels` passes; each
+    splits the triangles whose largest box side is >= grid_h into four at
+    the edge midpoints (biSideSectTriangle, triangle.cpp:225-303)."""
+    # The reference's per-axis maximum starts from the smallest positive
+    # double (structure/Triangle.cpp:142-179), so on an axis where all three
+    # coordinates are negative the "maximum" is that tiny number and the
+    # width is |min|; kept, since it decides which triangles split.
+    floor = np.finfo(float).tiny
+    for _ in range(max(levels, 0)):
+        out = []
+        for t in range(0, len(verts), 3):
+            A, B, C = verts[t], verts[t + 1], verts[t + 2]
+            width = max(abs(min(A[k], B[k], C[k]) - max(floor, A[k], B[k], C[k])) for k in range(3))
+            if width >= grid_h:
+                al = tuple((A[k] + B[k]) / 2.0 for k in range(3))
+                be = tuple((A[k] + C[k]) / 2.0 for k in range(3))
+                ga = tuple((B[k] + C[k]) / 2.0 for k in range(3))
+                out += [be, A, al, ga, al, B, ga, be, al, C, be, ga]
+            else:
+                out += [A, B, C]
+        verts = out
+    return verts
+
+
+def generate_hopper(center, width, thickness, height, hatch, mesh_refinement=0, grid_h=0.0):
+    """delta::geometry::hardcoded::generateHopper (hopper.cpp:1057-1075) as a
+    triangle soup (ntri, 3, 3): inner funnel (orientation reversed), outer
+    funnel (width + thickness, hatch + 0.005), the top and bottom rim rings,
+    then `mesh_refinement` bisection passes with threshold `grid_h`."""
+    c = tuple(float(v) for v in center)
+    inner = _funnel_walls(c, width, height, hatch)
+    outer = _funnel_walls(c, width + thickness, height, hatch + 0.005)
+    # rim corners: upper back-left, back-right, front-right, front-left and
+    # the lower ones, at the reference's vertex slots of a funnel
+    up, lowr = (10, 29, 34, 0), (9, 28, 32, 1)
+    top = _rim_ring([inner[k] for k in up], [outer[k] for k in up])
+    bottom = _rim_ring([inner[k] for k in lowr], [outer[k] for k in lowr])
+    verts = list(inner) + list(outer)
+    verts += [top[ch] for tri in _TOP_RING for ch in tri]
+    verts += [bottom[ch] for tri in _BOTTOM_RING for ch in tri]
+    for a, b in _REVERSE:
+        verts[a], verts[b] = verts[b], verts[a]
+    verts = _bisect_refine(verts, mesh_refinement, grid_h)
+    return np.asarray(verts, float).reshape(-1, 3, 3)

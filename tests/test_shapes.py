@@ -1,6 +1,8 @@
 """Host-side geometry generators: the reference's simplex-integration inertia
 (physics.cpp:48-140) bit for bit against its own compiled source, closed
-forms, and the granulate hull particles (granulate.cpp:53-138)."""
+forms, the granulate hull particles (granulate.cpp:53-138), and the
+hand-built hopper (shape/hopper.cpp:1057-1075) bit for bit against its
+compiled source plus its area and enclosed volume in closed form."""
 import ctypes as C
 import os
 
@@ -11,6 +13,7 @@ import oracle_lib as ol
 from delta_amd import shapes
 
 PHYS = os.path.join(ol.ROOT, "oracle", "_ref", "libphysics_ref.so")
+HOP = os.path.join(ol.ROOT, "oracle", "_ref", "libhopper_ref.so")
 DP = C.POINTER(C.c_double)
 
 
@@ -79,3 +82,54 @@ def test_granulate_particle_is_a_closed_outward_hull():
     # hull vertices at radius in [(1 - 0.25) h / 2, h / 2] about the centre
     r = np.linalg.norm(tris.reshape(-1, 3) - centre, axis=1)
     assert r.min() >= 0.75 * h / 2 * (1 - 1e-12) and r.max() <= h / 2 * (1 + 1e-12)
+
+
+# (center, width, thickness, height, hatch, refinement, grid_h): the reference
+# scenario's hopper (world/scenario/scenarios.cpp:45-51), refined ones, and
+# centres with all-negative axes (the refinement's width rule treats those
+# axes specially, structure/Triangle.cpp:142-179)
+HOPPERS = [((0.5, 0.5, 0.5), 0.20, 0.005, 0.20 / 1.5, 0.05, 0, 0.01),
+           ((0, 0, 0), 0.1, 0.01, 0.08, 0.02, 1, 0.05),
+           ((0.3, -0.2, 0.7), 0.25, 0.02, 0.12, 0.05, 2, 0.03),
+           ((1.0, 2.0, 3.0), 0.5, 0.013, 0.3, 0.07, 3, 0.01),
+           ((-1.0, -2.0, -3.0), 0.3, 0.01, 0.2, 0.04, 3, 0.02),
+           ((0, 0, 0), 0.1, 0.01, 0.08, 0.02, 4, 0.005)]
+
+
+@pytest.mark.skipif(not os.path.exists(HOP), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("args", HOPPERS)
+def test_generate_hopper_equals_reference(args):
+    L = C.CDLL(HOP)
+    L.ref_hopper.restype = C.c_int
+    L.ref_hopper.argtypes = [DP, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                             C.c_double, DP, C.c_int]
+    centre, w, t, h, hatch, refine, grid_h = args
+    cap = 48 * 4 ** refine * 3
+    out = np.zeros(3 * cap)
+    n = L.ref_hopper(np.asarray(centre, float).ctypes.data_as(DP), w, t, h, hatch, refine, grid_h,
+                     out.ctypes.data_as(DP), cap)
+    assert n > 0
+    ref = out[:3 * n].reshape(-1, 3, 3)
+    mine = shapes.generate_hopper(centre, w, t, h, hatch, refine, grid_h)
+    assert mine.shape == ref.shape
+    assert np.array_equal(mine.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("args", HOPPERS[:3])
+def test_hopper_area_and_volume(args):
+    """Walls are four trapezoids per funnel (top w, bottom hatch, slant
+    sqrt(h^2 + ((w - hatch) / 2)^2)); the rims are square annuli; the shell
+    encloses the outer square frustum minus the inner one, h/3 (a^2 + ab +
+    b^2) each, with outward normals throughout (signed volume positive)."""
+    centre, w, t, h, hatch, refine, grid_h = args
+    tris = shapes.generate_hopper(centre, w, t, h, hatch, refine, grid_h)
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    trap = lambda a, b: (a + b) / 2 * np.sqrt(h * h + ((a - b) / 2) ** 2)
+    wo, ho = w + t, hatch + 0.005
+    area = 4 * trap(w, hatch) + 4 * trap(wo, ho) + (wo ** 2 - w ** 2) + (ho ** 2 - hatch ** 2)
+    assert 0.5 * np.linalg.norm(n, axis=1).sum() == pytest.approx(area, rel=1e-12)
+    frustum = lambda a, b: h / 3 * (a * a + a * b + b * b)
+    vol = np.einsum("ij,ij->i", tris[:, 0] - np.asarray(centre, float), n).sum() / 6
+    assert vol == pytest.approx(frustum(wo, ho) - frustum(w, hatch), rel=1e-12)
+    # refinement with threshold 0 splits every triangle at every level
+    assert len(shapes.generate_hopper(centre, w, t, h, hatch, 2, 0.0)) == 48 * 16
