@@ -289,15 +289,6 @@ __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __r
     update_vertex(x[j], y[j], z[j], rx[j], ry[j], rz[j], R, pos, refpos);
 }
 
-__global__ void k_update_angular_one(double* W, const double* R, const double* J,
-                                     const double* Ji, const double* t, double h) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) update_angular(W, R, J, Ji, t, h);
-}
-
-__global__ void k_update_rotation_one(double* w, const double* W, double* R, double h) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) update_rotation(w, W, R, h);
-}
-
 
 struct ForceScratch {
   DevBuf pf, incid, start, fill, entries;
@@ -407,45 +398,23 @@ int dh_integrate(dh_bodies* bodies, const double* force_dev, const double* torqu
                         as_stream(stream));
 }
 
+// The single-particle forms run on the host: one particle's update is ~100
+// floating-point operations, far below a kernel launch and two copies, and
+// the host's libm sin/cos are the reference's (the device ones can differ in
+// the last bit where expmap leaves its series, |h W| >= 0.1745).
 int dh_update_angular(double refAngular[3], const double rotation[9], const double inertia[9],
                       const double inverse[9], const double torque[3], double step) {
-  int rc = ensure_init();
-  if (rc) return rc;
-  static DevBuf buf;
-  if ((rc = buf.grow(sizeof(double) * 33))) return rc;
-  double h[33];
-  for (int i = 0; i < 3; ++i) h[i] = refAngular[i];
-  for (int i = 0; i < 9; ++i) {
-    h[3 + i] = rotation[i];
-    h[12 + i] = inertia[i];
-    h[21 + i] = inverse[i];
-  }
-  for (int i = 0; i < 3; ++i) h[30 + i] = torque[i];
-  DH_HIP_CHECK(hipMemcpy(buf.ptr, h, sizeof(h), hipMemcpyHostToDevice));
-  double* d = buf.as<double>();
-  hipLaunchKernelGGL(k_update_angular_one, 1, 1, 0, 0, d, d + 3, d + 12, d + 21, d + 30, step);
-  DH_HIP_CHECK(hipGetLastError());
-  DH_HIP_CHECK(hipMemcpy(refAngular, d, sizeof(double) * 3, hipMemcpyDeviceToHost));
+  if (!refAngular || !rotation || !inertia || !inverse || !torque)
+    return set_error(DH_ERR_ARG, "dh_update_angular: null pointer");
+  update_angular(refAngular, rotation, inertia, inverse, torque, step);
   return DH_OK;
 }
 
 int dh_update_rotation_matrix(double angular[3], const double refAngular[3], double rotation[9],
                               double step) {
-  int rc = ensure_init();
-  if (rc) return rc;
-  static DevBuf buf;
-  if ((rc = buf.grow(sizeof(double) * 15))) return rc;
-  double h[15];
-  for (int i = 0; i < 3; ++i) h[i] = 0.0;
-  for (int i = 0; i < 3; ++i) h[3 + i] = refAngular[i];
-  for (int i = 0; i < 9; ++i) h[6 + i] = rotation[i];
-  DH_HIP_CHECK(hipMemcpy(buf.ptr, h, sizeof(h), hipMemcpyHostToDevice));
-  double* d = buf.as<double>();
-  hipLaunchKernelGGL(k_update_rotation_one, 1, 1, 0, 0, d, d + 3, d + 6, step);
-  DH_HIP_CHECK(hipGetLastError());
-  DH_HIP_CHECK(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
-  for (int i = 0; i < 3; ++i) angular[i] = h[i];
-  for (int i = 0; i < 9; ++i) rotation[i] = h[6 + i];
+  if (!angular || !refAngular || !rotation)
+    return set_error(DH_ERR_ARG, "dh_update_rotation_matrix: null pointer");
+  update_rotation(angular, refAngular, rotation, step);
   return DH_OK;
 }
 
