@@ -376,8 +376,11 @@ __global__ void k_adam(float* __restrict__ W, float* __restrict__ m, float* __re
 // ------------------------------------------------------------- inference
 // Workgroup of kInferWaves waves; the network's A-operand tiles and biases
 // live once per workgroup in LDS, so registers hold only the pair being
-// certified and the layer accumulators.
-constexpr int kInferWaves = 8;
+// certified and the layer accumulators.  Four waves (one per SIMD) at three
+// workgroups per CU: the decision needs up to 168 registers, so three waves
+// per SIMD is the occupancy without spills, which an eight-wave workgroup
+// (two waves per SIMD each) could not use.
+constexpr int kInferWaves = 4;
 // A-operand tiles (16 x 32) of the three layers for v_mfma_f32_16x16x32:
 // W1 4 output tiles x 1 k-block, W2 4 x 2, W3 (first output tile only) 1 x 2.
 constexpr int kTiles = 4 + 8 + 2 + 2;  // + W3 output tile 1 (barycentric outputs)
@@ -623,7 +626,7 @@ __device__ __forceinline__ void learned_pair(LearnedLds& L, const Tri<double>& A
     // and folded into the two minima in bfSolver's order, each minimum
     // keeping its points.  The wave runs its lanes' edge-edge ones
     // together, then its vertex-face ones.
-    unsigned rest = (open == kUnresolvedAll) ? 0u : open & ~((1u << fe) | (1u << fv));
+    const unsigned rest = (open == kUnresolvedAll) ? 0u : open & ~((1u << fe) | (1u << fv));
     unsigned rest_e = rest & 0x1FFu, rest_v = rest & 0x7E00u;
     while (__ballot(rest_e != 0u)) {
       if (rest_e) {
@@ -764,10 +767,10 @@ __device__ __forceinline__ void learned_body(const LearnedArgs& a) {
   }
 }
 
-// Two waves per SIMD: the tie folding keeps both families' minima and their
-// points live (at three waves the registers spill).
+// Three waves per SIMD (168 registers; the tie folding keeps both families'
+// minima and their points live, and at four waves the registers spill).
 template <int MODE>
-__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(2))) void
+__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(3))) void
 k_learned(LearnedArgs a) {
   learned_body<MODE>(a);
 }
@@ -807,7 +810,7 @@ __device__ __forceinline__ Tri<double> load_tri_xyz(const double* __restrict__ x
   return r;
 }
 
-__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(2))) void
+__global__ __launch_bounds__(64 * kInferWaves) __attribute__((amdgpu_waves_per_eu(3))) void
 k_queue_learned(QueueLearnArgs a) {
   __shared__ LearnedLds L;
   stage_network<kModeFeatures>(L, a.W);

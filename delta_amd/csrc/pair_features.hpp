@@ -31,6 +31,20 @@ DH_HD V3<T> cross3(const V3<T>& a, const V3<T>& b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// Binary32 reciprocal square root (hardware instruction on the device) and
+// fused dot product.
+DH_HD float rsqrt_frame(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / std::sqrt(x);
+#endif
+}
+
+DH_HD float fdot(const V3<float>& a, const V3<float>& b) {
+  return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z));
+}
+
 // Closest points of edge-edge feature `code` (0..8).
 DH_HD void eval_ee(const Tri<double>& A, const Tri<double>& B, int code, V3<double>& P,
                    V3<double>& Q) {
@@ -150,13 +164,16 @@ DH_HD double eval_vf_value(const Tri<double>& A, const Tri<double>& B, int code,
 // support_choice() is the learned path's cheap acceptance test.  P, Q are
 // the winner's points and suppA, suppB their supports (the smallest
 // sub-simplices holding them).  The pair is accepted when
-//   * (P, Q) is a closest pair up to rounding: with n = Q - P every vertex
-//     of A has m.(P - a) >= -tol and every vertex of B m.(b - Q) >= -tol
-//     (m = n / |n|, tol as in exact_choice below); intersecting or
-//     touching triangles (|n| below 2^-30 of the coordinates' scale) never
-//     pass;
-//   * every vertex outside the supports is lifted off that plane by an angle
-//     of at least kLift as seen from P (Q): m.(P - a) >= kLift |a - P|_1;
+//   * every vertex outside the supports is lifted off the plane through P
+//     (Q) orthogonal to n = Q - P by an angle of at least kLift as seen from
+//     P (Q): m.(P - a) >= kLift |a - P|_1 for a in A, m.(b - Q) >= kLift
+//     |b - Q|_1 for b in B (m = n / |n|).  The support vertices lie on that
+//     plane by construction (the feature's own closest points satisfy its
+//     optimality conditions), so with the lifted ones (P, Q) is the unique
+//     closest pair of the triangles;
+//   * the pair is not touching or intersecting: |n| > 2^-28 (M + S), M the
+//     largest coordinate magnitude of P and Q and S the largest vertex
+//     distance |a - P|_1, |b - Q|_1 (many zero-valued features tie there);
 //   * for an edge-edge winner whose points are both inside their edges, the
 //     edges are not parallel: |u x v|^2 >= kLift^2 |u|^2 |v|^2;
 //   * neither triangle is degenerate (the same test on two of its edges:
@@ -173,7 +190,22 @@ DH_HD double eval_vf_value(const Tri<double>& A, const Tri<double>& B, int code,
 // near-tie occurs, and its points to that distance always (measured against
 // the brute force on every pair; exact_choice() below proves the choice and
 // is the exact mode).  Returns kUnresolvedAll when the test fails.
-constexpr double kLift = 1e-3;
+//
+// The tests run in binary32 on binary64 differences (vertex - P, Q - P and
+// the edges, each rounded once to binary32).  Every binary32 quantity is then
+// within a relative 2^-20 of its exact value (a slack m.r within 2^-20 |r|),
+// so each threshold is raised by that much: the binary32 test accepts only
+// pairs the exact test accepts.
+//
+// A vertex against a vertex (suppA = {i}, suppB = {j}) ties six features
+// (the four edge pairs through both vertices and the two vertex-face pairs).
+// With the clamps exact, each of them returns P = A_i and Q = B_j as
+// computed by bfSolver: the vertex itself, o + (h - o) for an edge whose far
+// end o it starts from, or A_0 + (A_i - A_0) on the face.  When those sums
+// reproduce the vertices bit for bit (and no coordinate is zero, whose sign
+// the sums could change) every tie gives the same P and Q, and the winner's
+// are the answer without evaluating the others.
+constexpr float kLift = 1e-3f;
 
 // Features whose vertex sets contain suppA (of A) and suppB (of B).
 DH_HD unsigned tie_features(unsigned suppA, unsigned suppB) {
@@ -195,57 +227,75 @@ DH_HD unsigned tie_features(unsigned suppA, unsigned suppB) {
   return out;
 }
 
+DH_HD V3<float> to_f32(const V3<double>& v) {
+  return {static_cast<float>(v.x), static_cast<float>(v.y), static_cast<float>(v.z)};
+}
+
+DH_HD float norm1f(const V3<float>& v) {
+  using std::fabs;
+  return fabs(v.x) + fabs(v.y) + fabs(v.z);
+}
+
+// |u x v|^2 > (kLift^2 + margin) |u|^2 |v|^2 in binary32.
+DH_HD bool angle_ok(const V3<float>& u, const V3<float>& v) {
+  const V3<float> x = cross3(u, v);
+  return fdot(x, x) > (kLift * kLift + 0x1.0p-18f) * (fdot(u, u) * fdot(v, v));
+}
+
+// o + (h - o) == h bit for bit, per coordinate, and h has no zero coordinate.
+DH_HD bool sum_back(const V3<double>& o, const V3<double>& h) {
+  return (o.x + (h.x - o.x)) == h.x && (o.y + (h.y - o.y)) == h.y && (o.z + (h.z - o.z)) == h.z &&
+         h.x != 0.0 && h.y != 0.0 && h.z != 0.0;
+}
+
 DH_HD unsigned support_choice(const Tri<double>& A, const Tri<double>& B, const V3<double>& P,
                               const V3<double>& Q, unsigned suppA, unsigned suppB, int win) {
   using std::fabs;
   using std::fmax;
-  const V3<double> n = sub(Q, P);
-  const double d2 = dot(n, n);
-  if (!(d2 > 0.0)) return kUnresolvedAll;
-  const double inv_d = 1.0 / std::sqrt(d2);
-  const V3<double> m = {n.x * inv_d, n.y * inv_d, n.z * inv_d};
-  double M = fmax(fmax(fmax(fabs(P.x), fabs(P.y)), fmax(fabs(P.z), fabs(Q.x))),
-                  fmax(fabs(Q.y), fabs(Q.z)));
-  // two passes over the six vertices (no per-vertex arrays: registers): the
-  // scales, then the slacks against them
-  double L = 0.0;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const V3<double> v = k < 3 ? A.v[k] : B.v[k - 3];
-    const V3<double> r = k < 3 ? sub(P, v) : sub(v, Q);
-    L = fmax(L, fabs(r.x) + fabs(r.y) + fabs(r.z));
-    M = fmax(M, fmax(fmax(fabs(v.x), fabs(v.y)), fabs(v.z)));
-  }
-  const double tol = 0x1.0p-47 * (2.0 * L + M + 2.0 * M * L * inv_d);
-  // touching or intersecting triangles (d within rounding of the scale)
-  // never pass: their many zero-valued features tie
-  if (!(d2 > 0x1.0p-60 * (L + M) * (L + M))) return kUnresolvedAll;
+  const V3<float> n = to_f32(sub(Q, P));
+  const float d2 = fdot(n, n);
+  if (!(d2 > 0.f)) return kUnresolvedAll;
+  const float inv_d = rsqrt_frame(d2);
+  const V3<float> m = {n.x * inv_d, n.y * inv_d, n.z * inv_d};
   const unsigned supp = suppA | (suppB << 3);
-  // neither triangle degenerate (an angle below kLift at vertex 0: coincident
-  // or collinear vertices give several features the same points)
-  const V3<double> ua = sub(A.v[1], A.v[0]), va = sub(A.v[2], A.v[0]);
-  const V3<double> ub = sub(B.v[1], B.v[0]), vb = sub(B.v[2], B.v[0]);
-  const V3<double> xa = cross3(ua, va), xb = cross3(ub, vb);
-  bool ok = dot(xa, xa) > kLift * kLift * dot(ua, ua) * dot(va, va) &&
-            dot(xb, xb) > kLift * kLift * dot(ub, ub) * dot(vb, vb);
+  bool ok = true;
+  float S = 0.f;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const V3<double> v = k < 3 ? A.v[k] : B.v[k - 3];
-    const V3<double> r = k < 3 ? sub(P, v) : sub(v, Q);
-    const double sl = dot(m, r);
-    const double r1 = fabs(r.x) + fabs(r.y) + fabs(r.z);
-    ok = ok && (sl >= -tol) && (((supp >> k) & 1u) || sl >= kLift * r1);
+    const V3<float> r = to_f32(k < 3 ? sub(P, A.v[k]) : sub(B.v[k - 3], Q));
+    const float r1 = norm1f(r);
+    S = fmaxf(S, r1);
+    ok = ok && (((supp >> k) & 1u) || fdot(m, r) >= (kLift + 0x1.0p-19f) * r1);
   }
+  const float M = fmaxf(fmaxf(fmaxf(fabs(static_cast<float>(P.x)), fabs(static_cast<float>(P.y))),
+                              fmaxf(fabs(static_cast<float>(P.z)), fabs(static_cast<float>(Q.x)))),
+                        fmaxf(fabs(static_cast<float>(Q.y)), fabs(static_cast<float>(Q.z))));
+  // touching or intersecting triangles never pass
+  const float lim = 0x1.0p-28f * (M + S);
+  ok = ok && d2 > lim * lim;
+  // neither triangle degenerate (an angle below kLift at vertex 0)
+  ok = ok && angle_ok(to_f32(sub(A.v[1], A.v[0])), to_f32(sub(A.v[2], A.v[0]))) &&
+       angle_ok(to_f32(sub(B.v[1], B.v[0])), to_f32(sub(B.v[2], B.v[0])));
   // both points inside their edges: the edges must not be parallel
   const bool interior = win < 9 && __builtin_popcount(suppA) == 2 && __builtin_popcount(suppB) == 2;
   if (interior) {
     const int k = win / 3, mm = win - 3 * (win / 3);
-    const V3<double> u = sub(pick3(A, k), pick3(A, k == 2 ? 0 : k + 1));
-    const V3<double> v = sub(pick3(B, mm), pick3(B, mm == 2 ? 0 : mm + 1));
-    const V3<double> x = cross3(u, v);
-    ok = ok && dot(x, x) > kLift * kLift * dot(u, u) * dot(v, v);
+    const V3<float> u = to_f32(sub(pick3(A, k), pick3(A, k == 2 ? 0 : k + 1)));
+    const V3<float> v = to_f32(sub(pick3(B, mm), pick3(B, mm == 2 ? 0 : mm + 1)));
+    ok = ok && angle_ok(u, v);
   }
-  return ok ? tie_features(suppA, suppB) : kUnresolvedAll;
+  if (!ok) return kUnresolvedAll;
+  // a vertex against a vertex whose ties all give the same points
+  if (__builtin_popcount(suppA) == 1 && __builtin_popcount(suppB) == 1) {
+    const int i = __builtin_ctz(suppA), j = __builtin_ctz(suppB);
+    const V3<double> a = pick3(A, i), b = pick3(B, j);
+    const bool same = sum_back(pick3(A, i == 2 ? 0 : i + 1), a) && sum_back(A.v[0], a) &&
+                      sum_back(pick3(B, j == 2 ? 0 : j + 1), b) && sum_back(B.v[0], b) &&
+                      P.x == a.x && P.y == a.y && P.z == a.z && Q.x == b.x && Q.y == b.y &&
+                      Q.z == b.z;
+    if (same) return 1u << win;
+  }
+  return tie_features(suppA, suppB);
 }
 
 
@@ -636,13 +686,6 @@ DH_HD int bf_feature_code(const Tri<double>& A, const Tri<double>& B, V3<double>
 
 // 1/sqrt(x) for the canonical frame, whose coordinates are rounded to bf16:
 // the hardware approximation (v_rsq_f32, about 1 ulp) on the GPU.
-DH_HD float rsqrt_frame(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_rsqf(x);
-#else
-  return 1.0f / std::sqrt(x);
-#endif
-}
 
 // Canonical frame of a pair (origin at A's centroid, x along A's edge 0->1,
 // z along A's normal, lengths in units of A's longest edge), as 18 floats:
@@ -650,10 +693,6 @@ DH_HD float rsqrt_frame(float x) {
 // taken relative to A0 in binary64; the frame is built in binary32 with
 // fused multiply-adds (the network reads bf16).  Returns false for a
 // degenerate A (no frame).
-DH_HD float fdot(const V3<float>& a, const V3<float>& b) {
-  return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z));
-}
-
 DH_HD bool canonical_inputs(const Tri<double>& A, const Tri<double>& B, float out[18]) {
   V3<float> r[6];
   r[0] = {0.f, 0.f, 0.f};

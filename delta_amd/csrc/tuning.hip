@@ -19,7 +19,7 @@
 
 namespace dh {
 
-static const dh_tuning kDefaults = {8, 8, 256, 4096, 16, 0, 32768, 0};
+static const dh_tuning kDefaults = {6, 8, 256, 4096, 16, 0, 32768, 0};
 static dh_tuning g_tuning = kDefaults;
 static int g_cus = 0;
 

@@ -39,7 +39,7 @@ def test_tuning_table_validates_without_gpu():
     out-of-range values."""
     from delta_amd import _lib, tuning
     d = tuning.default()
-    assert d == {"learned_wg_per_cu": 8, "mixed_wg_per_cu": 8, "exact_block": 256,
+    assert d == {"learned_wg_per_cu": 6, "mixed_wg_per_cu": 8, "exact_block": 256,
                  "detect_tile_target": 4096, "detect_max_tiles": 16, "detect_wg_per_cu": 0,
                  "detect_queue_min_items": 32768, "detect_fp32_screen": 0}
     try:
