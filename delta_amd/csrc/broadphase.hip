@@ -454,6 +454,75 @@ __global__ __launch_bounds__(64 * kBpWaves) void k_grid_fill(GridArgs g,
   }
 }
 
+// Scenes of at most kSmallScene particles: every pair (i < j) tested in one
+// workgroup of 16 waves, in place of the grid's dozen launches.  The same
+// candidate set (boxes overlap, not both obstacles) in the same order (i
+// ascending, then j ascending), with the same outputs: start[0..n] (start[n]
+// the total, which may exceed cap) and the pairs that fit cap.  A wave per
+// particle i sweeps j > i in strips of 64: counts, their prefix sums, then
+// the same sweep writes each partner at start[i] + its rank.
+constexpr int kSmallScene = 1024;
+
+__global__ __launch_bounds__(1024) void k_small_pairs(const Aabb* __restrict__ box,
+                                                      const int* __restrict__ obstacle, int n,
+                                                      long long* __restrict__ start,
+                                                      long long cap, int* __restrict__ pairs) {
+  __shared__ long long s_start[kSmallScene + 1];
+  __shared__ long long s_wave[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = w; i < n; i += 16) {
+    const Aabb bi = box[i];
+    const bool oi = obstacle && obstacle[i];
+    int m = 0;
+    for (int j0 = i + 1; j0 < n; j0 += 64) {
+      const int j = j0 + lane;
+      const bool e = j < n && !(oi && obstacle[j]) && overlap(bi, box[j]);
+      m += __popcll(__ballot(e));
+    }
+    if (lane == 0) s_start[i] = m;
+  }
+  __syncthreads();
+  // exclusive prefix sums: one count per thread, waves, then wave totals
+  const int t = threadIdx.x;
+  const long long c = t < n ? s_start[t] : 0;
+  long long incl = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long u = __shfl_up(incl, o);
+    incl += lane >= o ? u : 0;
+  }
+  if (lane == 63) s_wave[w] = incl;
+  __syncthreads();
+  long long before = 0;
+  for (int q = 0; q < w; ++q) before += s_wave[q];
+  const long long excl = before + incl - c;
+  if (t < n) {
+    s_start[t] = excl;
+    start[t] = excl;
+  }
+  if (t == (n > 0 ? n - 1 : 0)) {
+    s_start[n] = n > 0 ? excl + c : 0;
+    start[n] = n > 0 ? excl + c : 0;
+  }
+  __syncthreads();
+  for (int i = w; i < n; i += 16) {
+    const Aabb bi = box[i];
+    const bool oi = obstacle && obstacle[i];
+    const long long s0 = s_start[i];
+    int found = 0;
+    for (int j0 = i + 1; j0 < n; j0 += 64) {
+      const int j = j0 + lane;
+      const bool e = j < n && !(oi && obstacle[j]) && overlap(bi, box[j]);
+      const unsigned long long mask = __ballot(e);
+      const long long q = s0 + found + bp_lane_prefix(mask);
+      if (e && q < cap) {
+        pairs[2 * q] = i;
+        pairs[2 * q + 1] = j;
+      }
+      found += __popcll(mask);
+    }
+  }
+}
+
 int scan_counts(const int* counts, int n, long long* offs, hipStream_t st);
 int scan_counts_with(const int* counts, int n, long long* offs, long long* sums,
                      hipStream_t st);
@@ -625,6 +694,24 @@ static int broadphase_run(const dh_geometry* geom, const int* obstacle_dev, doub
                           const BpBuffers& b, int* pairs, long long cap, long long* npairs,
                           hipStream_t st) {
   const int n = geom->nparticles;
+  if (n <= kSmallScene) {
+    // one workgroup tests every pair; nothing is written past cap, so the
+    // pairs come with the count
+    if (n > 0) {
+      hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
+                         geom->tri_offset, geom->eps, b.box, b.cbox);
+      hipLaunchKernelGGL(k_small_pairs, 1, 1024, 0, st, b.box, obstacle_dev, n, b.start, cap,
+                         pairs);
+      DH_HIP_CHECK(hipGetLastError());
+    }
+    long long total = 0;
+    if (n > 0)
+      DH_HIP_CHECK(hipMemcpyAsync(&total, b.start + n, sizeof(long long), hipMemcpyDeviceToHost,
+                                  st));
+    DH_HIP_CHECK(hipStreamSynchronize(st));
+    *npairs = total;
+    return total > cap ? DH_ERR_CAPACITY : DH_OK;
+  }
   grid_clear(b, table_size(n), st);
   GridArgs g;
   int rc;
@@ -655,12 +742,21 @@ int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double ce
   int rc;
   if ((rc = broadphase_reserve(n, geom->ntri_total))) return rc;
   const BpBuffers b = library_buffers();
+  *np_dev = b.start + n;
+  if (n <= kSmallScene) {
+    if (n > 0)
+      hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
+                         geom->tri_offset, geom->eps, b.box, b.cbox);
+    hipLaunchKernelGGL(k_small_pairs, 1, 1024, 0, st, b.box, obstacle_dev, n, b.start, cap,
+                       pairs);
+    DH_HIP_CHECK(hipGetLastError());
+    return DH_OK;
+  }
   grid_clear(b, table_size(n), st);
   GridArgs g;
   if ((rc = grid_build(geom, obstacle_dev, cell, b, g, st))) return rc;
   grid_fill(g, b, cap, pairs, st);
   DH_HIP_CHECK(hipGetLastError());
-  *np_dev = b.start + n;
   return DH_OK;
 }
 

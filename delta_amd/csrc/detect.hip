@@ -899,6 +899,70 @@ __global__ __launch_bounds__(256) void k_tile_stats(const unsigned long long* __
   }
 }
 
+// The tail of a captured detection over at most kTailPairs pairs, in one
+// workgroup in place of three launches: the tile counters' sums (as
+// k_tile_stats), the per-pair contact counts (as k_tile_sums; with one tile
+// per pair the detection wrote them directly) and their exclusive prefix
+// sums with the total at offs[npairs] (as scan_counts).
+constexpr int kTailPairs = 4096;
+
+__global__ __launch_bounds__(1024) void k_detect_tail_small(
+    const unsigned long long* __restrict__ ts, long long ntiles,
+    unsigned long long* __restrict__ stats, const int* __restrict__ tile_count, int tiles,
+    int npairs, int* __restrict__ counts, long long* __restrict__ offs) {
+  __shared__ unsigned long long red[kTileStats];
+  __shared__ long long s_wave[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t < kTileStats) red[t] = 0;
+  __syncthreads();
+  unsigned long long acc[kTileStats] = {};
+  for (long long k = t; k < ntiles; k += 1024)
+    for (int q = 0; q < kTileStats; ++q) acc[q] += ts[k * kTileStats + q];
+  for (int q = 0; q < kTileStats; ++q) {
+    unsigned long long v = acc[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0 && v) atomicAdd(&red[q], v);
+  }
+  // four consecutive pairs per thread
+  int c[4];
+  long long local = 0;
+  for (int u = 0; u < 4; ++u) {
+    const int k = 4 * t + u;
+    int v = 0;
+    if (k < npairs) {
+      if (tiles > 1) {
+        for (int q = 0; q < tiles; ++q) v += tile_count[static_cast<long long>(k) * tiles + q];
+        counts[k] = v;
+      } else {
+        v = counts[k];
+      }
+    }
+    c[u] = v;
+    local += v;
+  }
+  long long incl = local;
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long u = __shfl_up(incl, o);
+    incl += lane >= o ? u : 0;
+  }
+  if (lane == 63) s_wave[w] = incl;
+  __syncthreads();
+  long long before = 0;
+  for (int q = 0; q < w; ++q) before += s_wave[q];
+  long long run = before + incl - local;
+  for (int u = 0; u < 4; ++u) {
+    const int k = 4 * t + u;
+    if (k < npairs) offs[k] = run;
+    run += c[u];
+  }
+  if (t == 1023) offs[npairs] = before + incl;
+  if (t < kTileStats && red[t]) {
+    const int slot[kTileStats] = {ST_CHECKS, ST_EVALUATED, ST_FALLBACK_ROWS, ST_PENALTY_FAILS,
+                                  ST_BATCH_FAILS, ST_BATCH_SIZE, ST_CULLED};
+    atomicAdd(&stats[slot[t]], red[t]);
+  }
+}
+
 // GJK over the particles' vertex replicas (gjk.cpp:25-100 with the core of
 // gjk.cpp:418-460), one wave per particle pair: the support searches are
 // lane-parallel (first index of the strict extremum after a wave
@@ -1586,11 +1650,16 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
                        static_cast<int>(items));
     if ((rc = queue_passes(a, a.qcap, a.qcounter, learned, st))) return rc;
   }
-  launch_tile_stats(a.tile_stats, ntiles, a.stats, st);
-  if (tiles > 1)
-    hipLaunchKernelGGL(k_tile_sums, grid_for(pair_cap, 256), 256, 0, st, a.tile_count, tiles,
-                       pair_cap, counts);
-  if ((rc = scan_counts(counts, pair_cap, offs, st))) return rc;
+  if (pair_cap <= kTailPairs) {
+    hipLaunchKernelGGL(k_detect_tail_small, 1, 1024, 0, st, a.tile_stats, ntiles, a.stats,
+                       a.tile_count, tiles, pair_cap, counts, offs);
+  } else {
+    launch_tile_stats(a.tile_stats, ntiles, a.stats, st);
+    if (tiles > 1)
+      hipLaunchKernelGGL(k_tile_sums, grid_for(pair_cap, 256), 256, 0, st, a.tile_count, tiles,
+                         pair_cap, counts);
+    if ((rc = scan_counts(counts, pair_cap, offs, st))) return rc;
+  }
   hipLaunchKernelGGL(k_scatter, grid_for(contact_cap, 256), 256, 0, st,
                      g_scratch.temp.as<dh_contact>(), contact_cap, offs, a.tile_count, tiles, out,
                      g_scratch.counter.as<unsigned long long>());

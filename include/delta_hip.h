@@ -467,12 +467,13 @@ int dh_update_rotation_matrix(double angular[3], const double refAngular[3],
 /* Candidate particle pairs (no counterpart function in the reference, whose
  * engine loops over all pairs, Engine.cpp:272-295): every pair (i<j), not both
  * obstacles, whose AABBs inflated by their margins (+0.5e-9 m each) overlap,
- * found with a uniform hash grid of cell size `cell`.  Obstacles (obstacle_dev,
- * n ints, may be NULL) and particles whose inflated box is wider than `cell`
- * are tested against all others; choose `cell` >= the typical particle's
- * inflated extent.  Writes
- * pairs_dev[2k], pairs_dev[2k+1], sorted ascending; *npairs receives the
- * count (DH_ERR_CAPACITY if more than cap). */
+ * found with a uniform hash grid of cell size `cell` (scenes of at most 1024
+ * particles: every pair tested in one workgroup, same result).  Obstacles
+ * (obstacle_dev, n ints, may be NULL) and particles whose inflated box is
+ * wider than `cell` are tested against all others; choose `cell` >= the
+ * typical particle's inflated extent.  Writes pairs_dev[2k], pairs_dev[2k+1],
+ * sorted ascending; *npairs receives the count (DH_ERR_CAPACITY if more than
+ * cap; the first cap pairs may then have been written). */
 int dh_broadphase(const dh_geometry* geom, const int* obstacle_dev, double cell,
                   int* pairs_dev, int cap, int* npairs, void* stream);
 
