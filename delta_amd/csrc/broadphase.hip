@@ -29,7 +29,8 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
                                                       const int* __restrict__ tri_offset,
                                                       const double* __restrict__ eps,
                                                       Aabb* __restrict__ box,
-                                                      Aabb* __restrict__ cbox) {
+                                                      Aabb* __restrict__ cbox,
+                                                      double* __restrict__ sph) {
   const int i = blockIdx.x;
   const int t0 = tri_offset[i], t1 = tri_offset[i + 1];
   // a particle with more triangles than one row block keeps the box of each
@@ -74,6 +75,24 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
       b.hi[q] = phi[q] + m;
     }
     box[i] = b;
+  }
+  if (sph) {
+    // a bounding sphere: the vertex box's centre and the largest vertex
+    // distance from it (NaN coordinates drop out, as from the box)
+    const double c[3] = {0.5 * (plo[0] + phi[0]), 0.5 * (plo[1] + phi[1]),
+                         0.5 * (plo[2] + phi[2])};
+    double r2 = 0.0;
+    for (int v = 3 * t0 + threadIdx.x; v < 3 * t1; v += 64) {
+      const double dx = x[v] - c[0], dy = y[v] - c[1], dz = z[v] - c[2];
+      r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+    for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o));
+    if (threadIdx.x == 0) {
+      sph[4 * i] = c[0];
+      sph[4 * i + 1] = c[1];
+      sph[4 * i + 2] = c[2];
+      sph[4 * i + 3] = sqrt(r2);
+    }
   }
 }
 
@@ -530,7 +549,8 @@ int scan_sums_len(int n);
 int scan_reserve(int n);
 
 struct BpScratch {
-  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst, lcount, keep, cbox;
+  DevBuf box, ccount, cfill, cstart, sorted, cellh, count, start, err, obst, lcount, keep, cbox,
+      sph;
 };
 static BpScratch g_bp;
 
@@ -576,6 +596,7 @@ struct BpBuffers {
   int* lcount;         // (n + 1) * large_chunks(n)
   int* keep;           // (n + 1) * kBpKeep
   long long* sums;     // scan_sums_len(max(tbl, n)); null: the scan's own scratch
+  double* sph;         // 4 (n + 1): bounding spheres (centre, radius); may be null
 };
 
 static BpBuffers library_buffers() {
@@ -583,7 +604,7 @@ static BpBuffers library_buffers() {
                    g_bp.cfill.as<int>(),    g_bp.cstart.as<long long>(), g_bp.sorted.as<int>(),
                    g_bp.cellh.as<unsigned>(), g_bp.count.as<int>(),  g_bp.start.as<long long>(),
                    g_bp.err.as<int>(),      g_bp.obst.as<int>(),     g_bp.lcount.as<int>(),
-                   g_bp.keep.as<int>(),     nullptr};
+                   g_bp.keep.as<int>(),     nullptr,                 g_bp.sph.as<double>()};
 }
 
 static int scan_into(const BpBuffers& b, const int* counts, int n, long long* offs,
@@ -596,6 +617,8 @@ static int scan_into(const BpBuffers& b, const int* counts, int n, long long* of
 // call on more particles.
 const double* broadphase_boxes() { return g_bp.box.as<double>(); }
 const double* broadphase_block_boxes() { return g_bp.cbox.as<double>(); }
+// The particles' bounding spheres (x, y, z, r) of the last broad phase.
+const double* broadphase_spheres() { return g_bp.sph.as<double>(); }
 
 // The same boxes for the current geometry, without the grid (for callers
 // of the detection that did not run the broad phase).
@@ -604,9 +627,11 @@ int particle_boxes(const dh_geometry* geom, const double** out, hipStream_t st) 
   int rc;
   if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
   if ((rc = g_bp.cbox.grow(sizeof(Aabb) * ((geom->ntri_total + 63) / 64 + n + 1)))) return rc;
+  if ((rc = g_bp.sph.grow(sizeof(double) * 4 * (n + 1)))) return rc;
   if (n > 0)
     hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
-                       geom->tri_offset, geom->eps, g_bp.box.as<Aabb>(), g_bp.cbox.as<Aabb>());
+                       geom->tri_offset, geom->eps, g_bp.box.as<Aabb>(), g_bp.cbox.as<Aabb>(),
+                       g_bp.sph.as<double>());
   DH_HIP_CHECK(hipGetLastError());
   *out = g_bp.box.as<double>();
   return DH_OK;
@@ -618,6 +643,7 @@ int broadphase_reserve(int n, int ntri) {
   const unsigned tbl = table_size(n);
   if ((rc = g_bp.box.grow(sizeof(Aabb) * (n + 1)))) return rc;
   if ((rc = g_bp.cbox.grow(sizeof(Aabb) * ((ntri + 63) / 64 + n + 1)))) return rc;
+  if ((rc = g_bp.sph.grow(sizeof(double) * 4 * (n + 1)))) return rc;
   if ((rc = g_bp.ccount.grow(sizeof(int) * (tbl + 1)))) return rc;
   if ((rc = g_bp.cfill.grow(sizeof(int) * tbl))) return rc;
   if ((rc = g_bp.cstart.grow(sizeof(long long) * (tbl + 1)))) return rc;
@@ -650,7 +676,7 @@ static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double c
   const int n = geom->nparticles;
   const unsigned tbl = table_size(n);
   hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
-                     geom->eps, b.box, b.cbox);
+                     geom->eps, b.box, b.cbox, b.sph);
   g.box = b.box;
   g.obstacle = obstacle_dev;
   g.n = n;
@@ -699,7 +725,7 @@ static int broadphase_run(const dh_geometry* geom, const int* obstacle_dev, doub
     // pairs come with the count
     if (n > 0) {
       hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
-                         geom->tri_offset, geom->eps, b.box, b.cbox);
+                         geom->tri_offset, geom->eps, b.box, b.cbox, b.sph);
       hipLaunchKernelGGL(k_small_pairs, 1, 1024, 0, st, b.box, obstacle_dev, n, b.start, cap,
                          pairs);
       DH_HIP_CHECK(hipGetLastError());
@@ -746,7 +772,7 @@ int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double ce
   if (n <= kSmallScene) {
     if (n > 0)
       hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
-                         geom->tri_offset, geom->eps, b.box, b.cbox);
+                         geom->tri_offset, geom->eps, b.box, b.cbox, b.sph);
     hipLaunchKernelGGL(k_small_pairs, 1, 1024, 0, st, b.box, obstacle_dev, n, b.start, cap,
                        pairs);
     DH_HIP_CHECK(hipGetLastError());
@@ -796,6 +822,7 @@ static size_t noalloc_layout(char* base, int n, int ntri, BpBuffers* b) {
   t.lcount = reinterpret_cast<int*>(take(sizeof(int) * np1 * large_chunks(n)));
   t.keep = reinterpret_cast<int*>(take(sizeof(int) * np1 * kBpKeep));
   t.sums = reinterpret_cast<long long*>(take(sizeof(long long) * scan_sums_len(scan_n)));
+  t.sph = nullptr;  // the detection's sphere test is not part of this entry point
   if (b) *b = t;
   return off;
 }

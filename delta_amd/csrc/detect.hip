@@ -80,6 +80,12 @@ struct DetectArgs {
   // apart from every box of B's triangles, so a block of rows that is all
   // apart is skipped before B's triangles are loaded.
   const double* pbox;
+  // With pbox, DH_BF: the particles' bounding spheres (x, y, z, r) of the
+  // same broad phase.  A pair whose spheres are apart by more than the
+  // margin (and a rounding band) has every triangle pair beyond the margin
+  // and is skipped whole (culled), as the reference's engine loop may skip
+  // it by its own bounding-sphere test (Engine.cpp:272-295).
+  const double* psph;
   // With pbox: the broad phase's boxes of the 64-triangle chunks of the
   // particles with more than kRowBlock triangles (chunk c of particle i at
   // bbox[6 * (tri_offset[i] / 64 + i + c)]).  Rows of a chunk apart from
@@ -409,7 +415,19 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
       defer();
       return;
     }
-    for (int rb = 0; rb < nrows; rb += blk) {
+    // the whole pair beyond the margin by its bounding spheres
+    bool far_pair = false;
+    if (g.psph) {
+      const double* sa = g.psph + 4 * static_cast<long long>(p.pa);
+      const double* sb = g.psph + 4 * static_cast<long long>(p.pb);
+      const double dx = sa[0] - sb[0], dy = sa[1] - sb[1], dz = sa[2] - sb[2];
+      const double mag = fmax(fmax(fmax(fabs(sa[0]), fabs(sa[1])), fmax(fabs(sa[2]), fabs(sb[0]))),
+                              fmax(fabs(sb[1]), fabs(sb[2]))) + sa[3] + sb[3];
+      const double reach = sa[3] + sb[3] + p.margin * (1.0 + 0x1p-20) + mag * 0x1p-36;
+      far_pair = dx * dx + dy * dy + dz * dz > reach * reach;
+      if (far_pair) culled += tid == 0 ? static_cast<unsigned>(nrows * p.nB) : 0u;
+    }
+    for (int rb = 0; !far_pair && rb < nrows; rb += blk) {
       const int rs = r0 + rb;  // first row of the block
       const int nrb = nrows - rb < blk ? nrows - rb : blk;
       int nra = nrb, nca = p.nB;
@@ -594,6 +612,34 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
             g.queue[q] = make_int2(-1, 0);
           defer();
           return;
+        }
+        if (!QONLY && g.fp32_screen) {
+          // the survivors far apart in binary32 drop out, the rest are
+          // compacted in place (order kept) for the binary64 solve
+          int n2 = 0;
+          for (int b = 0; b < nl; b += 256) {
+            const int e = b + tid;
+            int v = 0;
+            bool keep2 = false;
+            if (e < nl) {
+              v = surv[e];
+              int iA, iB;
+              pair_of(v, iA, iB);
+              keep2 = !screened_far(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), p.margin);
+            }
+            const unsigned long long m = __ballot(keep2);
+            if (lane == 0) wave_cnt[wave] = __popcll(m);
+            __syncthreads();
+            int off = 0, cnt = 0;
+            for (int w = 0; w < 4; ++w) {
+              off += (w < wave) ? wave_cnt[w] : 0;
+              cnt += wave_cnt[w];
+            }
+            if (keep2) surv[n2 + off + lane_prefix(m)] = v;
+            n2 += cnt;
+            __syncthreads();
+          }
+          nl = n2;
         }
         for (int b = 0; b < (QONLY ? 0 : nl); b += 256) {
           const int e = b + tid;
@@ -1178,26 +1224,9 @@ __global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
   const Tri<double> A = load_tri(g, g.tri_offset[pa] + iA);
   const Tri<double> B = load_tri(g, g.tri_offset[pb] + iB);
   const double margin = g.eps[pa] + g.eps[pb];
-  if (SCREEN) {
-    // binary32 brute force relative to A's first vertex: a pair at twice
-    // the margin or more there is no contact (binary32's error at these
-    // scales is far below the margin); the rest are decided in binary64
-    const V3<double> o = A.v[0];
-    Tri<float> a32, b32;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      a32.v[k] = {static_cast<float>(A.v[k].x - o.x), static_cast<float>(A.v[k].y - o.y),
-                  static_cast<float>(A.v[k].z - o.z)};
-      b32.v[k] = {static_cast<float>(B.v[k].x - o.x), static_cast<float>(B.v[k].y - o.y),
-                  static_cast<float>(B.v[k].z - o.z)};
-    }
-    V3<float> p32, q32;
-    bf_solver(a32, b32, p32, q32);
-    const float d32 = pq_dist(p32, q32);
-    if (d32 >= static_cast<float>(2.0 * margin) && d32 <= 3.0e38f) {
-      flags[e] = 0;
-      return;
-    }
+  if (SCREEN && screened_far(A, B, margin)) {
+    flags[e] = 0;
+    return;
   }
   V3<double> P, Q;
   bf_solver(A, B, P, Q);
@@ -1413,7 +1442,7 @@ static int solve_queue(const DetectArgs& a, bool learned, hipStream_t st) {
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
                       dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
                       long long* offs_out, dh_stats* stats, hipStream_t st,
-                      const double* pbox, const double* bbox) {
+                      const double* pbox, const double* bbox, const double* psph) {
   int rc;
   // DH_BF_LEARNED: DH_BF with the survivor queue always on, its pairs
   // through the learned decision
@@ -1458,6 +1487,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = nullptr;
   a.pbox = pbox;
+  a.psph = pbox ? psph : nullptr;
   a.fp32_screen = tuning().detect_fp32_screen;
   a.wave_items = 0;
   a.bbox = bbox;
@@ -1575,7 +1605,7 @@ static long long graph_qcap(int method, int pair_cap) {
 int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, int pair_cap,
                        const long long* np_dev, dh_contact* out, long long contact_cap,
                        int* counts, long long* offs, hipStream_t st, const double* pbox,
-                       const double* bbox) {
+                       const double* bbox, const double* psph) {
   int rc;
   if ((rc = detect_reserve(pair_cap, contact_cap, method))) return rc;
   const bool learned = method == DH_BF_LEARNED;
@@ -1608,6 +1638,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.error = g_scratch.error.as<int>();
   a.npairs_dev = np_dev;
   a.pbox = pbox;
+  a.psph = pbox ? psph : nullptr;
   a.fp32_screen = tuning().detect_fp32_screen;
   a.wave_items = 0;
   a.bbox = bbox;
@@ -1736,7 +1767,7 @@ int dh_detect_pairs(int method, const dh_geometry* geom, const int* pairs_dev, i
   if (npairs == 0) return DH_OK;
   long long total = 0;
   rc = detect_pairs_impl(method, geom, pairs_dev, npairs, out_dev, cap, &total, pair_count_dev,
-                         nullptr, stats, as_stream(stream), nullptr, nullptr);
+                         nullptr, stats, as_stream(stream), nullptr, nullptr, nullptr);
   *n_out = static_cast<int>(total);
   if (stats && (method == DH_HYBRID_BATCH_STAT || method == DH_HYBRID_TRIANGLE_STAT) &&
       (rc == DH_OK || rc == DH_ERR_CAPACITY)) {
@@ -1823,7 +1854,7 @@ int dh_detect_pair(int method, const double* xA, const double* yA, const double*
   if ((rc = outbuf.grow(sizeof(dh_contact) * dcap))) return rc;
   long long total = 0;
   rc = detect_pairs_impl(method, &g, dm->pairs, 1, outbuf.as<dh_contact>(), cap, &total, nullptr,
-                         nullptr, stats, nullptr, nullptr, nullptr);
+                         nullptr, stats, nullptr, nullptr, nullptr, nullptr);
   *n_out = static_cast<int>(total);
   if (rc != DH_OK) return rc;
   if (stats && (method == DH_HYBRID_BATCH_STAT || method == DH_HYBRID_TRIANGLE_STAT))

@@ -11,9 +11,10 @@ int broadphase_impl(const dh_geometry* geom, const int* obstacle_dev, double cel
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
                       dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
                       long long* offs_out, dh_stats* stats, hipStream_t st, const double* pbox,
-                      const double* bbox);
+                      const double* bbox, const double* psph);
 const double* broadphase_boxes();
 const double* broadphase_block_boxes();
+const double* broadphase_spheres();
 
 int broadphase_graph(const dh_geometry* geom, const int* obstacle_dev, double cell, int* pairs,
                      long long cap, const long long** np_dev, hipStream_t st);
@@ -21,7 +22,7 @@ int broadphase_reserve(int n, int ntri);
 int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, int pair_cap,
                        const long long* np_dev, dh_contact* out, long long contact_cap,
                        int* counts, long long* offs, hipStream_t st, const double* pbox,
-                       const double* bbox);
+                       const double* bbox, const double* psph);
 int detect_reserve(int pair_cap, long long contact_cap, int method);
 const unsigned long long* detect_counter_dev();
 const int* detect_error_dev();
@@ -91,7 +92,7 @@ extern "C" int dh_step(dh_scene* s, const dh_step_params* p, dh_step_info* info,
       rc = detect_pairs_impl(p->method, &s->geom, g_step.pairs.as<int>(), static_cast<int>(np),
                              g_step.contacts.as<dh_contact>(), g_step.contact_cap, &total,
                              g_step.counts.as<int>(), g_step.offs.as<long long>(), &stats, st,
-                             broadphase_boxes(), broadphase_block_boxes());
+                             broadphase_boxes(), broadphase_block_boxes(), broadphase_spheres());
       if (rc != DH_ERR_CAPACITY) break;
       g_step.contact_cap = 2 * total;
     }
@@ -182,7 +183,7 @@ static int record_step(dh_step_graph* g, hipStream_t st) {
                                static_cast<int>(g->pair_cap), np_dev,
                                g->contacts.as<dh_contact>(), g->contact_cap, g->counts.as<int>(),
                                g->offs.as<long long>(), st, broadphase_boxes(),
-                               broadphase_block_boxes())))
+                               broadphase_block_boxes(), broadphase_spheres())))
     return rc;
   if ((rc = contacts_forces_impl(&s->bodies, g->pairs.as<int>(), static_cast<int>(g->pair_cap),
                                  np_dev, g->contacts.as<dh_contact>(), g->contact_cap,

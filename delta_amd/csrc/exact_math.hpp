@@ -306,6 +306,52 @@ DH_HD T pq_dist(const V3<T>& P, const V3<T>& Q) {
                ((Q.z - P.z) * (Q.z - P.z)));
 }
 
+// Screen before bfSolver in binary64: true when A and B are provably farther
+// apart than the contact margin (no contact, bf.cpp:125).  bfSolver in
+// binary32 (coordinates relative to A's first vertex) gives a direction n =
+// Q32 - P32; every point of A and B projects onto n, and the gap between the
+// two projections divided by |n| bounds the distance from below whatever n
+// is.  (The binary32 points themselves may be far from the closest pair: a
+// small triangle against a large one, such as a floor, cancels most of the
+// bits of the point-face quadratic form, so their distance is no bound.)
+// Rounding: the vertices relative to A's first one are rounded once to
+// binary32 (within 2^-24 of their length), the projections add 3 ulp, so
+// each projection is within 2^-21 |n| S of exact, S the largest relative
+// vertex length; the bound is lowered by 2^-19 |n| S and compared, like the
+// box test, against margin (1 + 2^-20) + mag 2^-40 (mag the largest
+// coordinate magnitude), beyond which bfSolver's binary64 distance exceeds
+// the margin as well.
+DH_HD bool screened_far(const Tri<double>& A, const Tri<double>& B, double margin) {
+  using std::fabs;
+  const V3<double> o = A.v[0];
+  Tri<float> a32, b32;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    a32.v[k] = {static_cast<float>(A.v[k].x - o.x), static_cast<float>(A.v[k].y - o.y),
+                static_cast<float>(A.v[k].z - o.z)};
+    b32.v[k] = {static_cast<float>(B.v[k].x - o.x), static_cast<float>(B.v[k].y - o.y),
+                static_cast<float>(B.v[k].z - o.z)};
+  }
+  V3<float> p32, q32;
+  bf_solver(a32, b32, p32, q32);
+  const V3<float> n = {q32.x - p32.x, q32.y - p32.y, q32.z - p32.z};
+  float amax = -3.0e38f, bmin = 3.0e38f, S = 0.f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const V3<float> a = a32.v[k], b = b32.v[k];
+    amax = fmaxf(amax, n.x * a.x + n.y * a.y + n.z * a.z);
+    bmin = fminf(bmin, n.x * b.x + n.y * b.y + n.z * b.z);
+    S = fmaxf(S, fmaxf(fabs(a.x) + fabs(a.y) + fabs(a.z), fabs(b.x) + fabs(b.y) + fabs(b.z)));
+  }
+  const double nn = std::sqrt(static_cast<double>(n.x) * n.x + static_cast<double>(n.y) * n.y +
+                              static_cast<double>(n.z) * n.z);
+  const double lower = (static_cast<double>(bmin) - static_cast<double>(amax) -
+                        0x1p-19 * nn * static_cast<double>(S)) /
+                       (nn * (1.0 + 0x1p-20));
+  const double mag = fmax(fmax(fabs(o.x), fabs(o.y)), fabs(o.z)) + static_cast<double>(S);
+  return nn > 0.0 && lower > margin * (1.0 + 0x1p-20) + mag * 0x1p-40;
+}
+
 // Penalty Newton solver state (penalty.cpp:278-357).
 template <typename T>
 struct Penalty {

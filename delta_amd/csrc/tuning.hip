@@ -131,9 +131,10 @@ static int sweep(int knob, int dh_tuning::*field, const std::vector<int>& values
 int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int npairs,
                       dh_contact* out, long long cap, long long* n_total, int* pair_count_user,
                       long long* offs_out, dh_stats* stats, hipStream_t st, const double* pbox,
-                      const double* bbox);
+                      const double* bbox, const double* psph);
 int particle_boxes(const dh_geometry* geom, const double** out, hipStream_t st);
 const double* broadphase_block_boxes();
+const double* broadphase_spheres();
 
 static bool is_exact(int method) {
   return method == DH_PAIR_BF64 || method == DH_PAIR_PENALTY64 || method == DH_PAIR_HYBRID64 ||
@@ -216,7 +217,8 @@ int dh_autotune_detect(int method, const dh_geometry* geom, const int* pairs_dev
   long long n_out = 0;
   auto run = [&] {
     const int r = detect_pairs_impl(method, geom, pairs_dev, npairs, out_dev, cap, &n_out,
-                                    nullptr, nullptr, nullptr, st, pbox, broadphase_block_boxes());
+                                    nullptr, nullptr, nullptr, st, pbox, broadphase_block_boxes(),
+                                    broadphase_spheres());
     return r == DH_ERR_CAPACITY
                ? set_error(DH_ERR_CAPACITY, "dh_autotune_detect: %lld contacts, cap %d", n_out, cap)
                : r;

@@ -225,3 +225,53 @@ def test_exact_choice_is_bfsolver_bit_for_bit(hc, fn):
             assert np.mean(np.maximum(st[:ngen] - 2, 0)) < 1.5
         if name == "random":
             assert acc.mean() > 0.05  # random guesses are still right sometimes
+
+
+def test_binary32_screen_is_a_lower_bound(hc):
+    """screened_far (exact_math.hpp), the binary32 screen ahead of the
+    detection's binary64 bfSolver, skips a triangle pair only when the
+    oracle's bfSolver distance exceeds the margin: the recorded pair of a C1
+    particle triangle 0.14 mm above the 1 m floor triangle (a contact whose
+    binary32 bfSolver distance comes out 0.40 mm, three times too far),
+    small triangles against large ones at heights from 1e-6 to 1e-2 m, the
+    unique-pair workload and the degenerate pairs.  It still skips nearly
+    every regular pair beyond three margins."""
+    import scenes
+    margin = 2e-4
+    reg = np.array([[[-0.25007118952381935, -0.066117445166873748, -0.13037171044944115],
+                     [-0.25020061591625897, -0.066418704175281645, -0.13048558368355279],
+                     [-0.25010106787159503, -0.066278463038880575, -0.13027561019189846]],
+                    [[0.53306809858186077, -0.65156380079126563, 0.17061959377314012],
+                     [-0.51184027586962633, 0.11783143606624863, -0.22414886892219765],
+                     [0.034872830782655906, -0.23586428204518717, -0.048686634857703082]]])
+    rng = np.random.default_rng(11)
+    m = 20000
+    small = rng.uniform(-5e-4, 5e-4, (m, 3, 3)) + rng.uniform(-0.4, 0.4, (m, 1, 3)) * [1, 0, 1]
+    big = np.array([[-0.5, 0.0, -0.5], [0.5, 0.0, -0.5], [0.0, 0.0, 0.6]])
+    lift = 10.0 ** rng.uniform(-6, -2, m)
+    small[:, :, 1] += (lift - small[:, :, 1].min(axis=1))[:, None]
+    from delta_amd import shapes
+    R = shapes.random_rotation(shapes.Rng(5))
+    pairs = [(reg[0], reg[1])] + [((small[i] @ R.T), (big @ R.T)) for i in range(m)]
+    s_big = np.zeros((18, len(pairs)))
+    for i, (A, B) in enumerate(pairs):
+        s_big[:9, i] = A.T.reshape(-1)
+        s_big[9:, i] = B.T.reshape(-1)
+    gen, _, kind = host_pairs(hc, 1 << 14)
+    s = np.ascontiguousarray(np.concatenate([s_big, gen, scenes.degenerate_pairs(motions=8)], axis=1))
+    n = s.shape[1]
+    far = np.zeros(n, np.uint8)
+    hc.hc_screened_far(ol.dptr(s), C.c_long(n), C.c_double(margin),
+                       far.ctypes.data_as(C.c_void_p))
+    P = np.zeros(3 * n); Q = np.zeros(3 * n); d = np.zeros(n)
+    ol.oracle().or_tri_pairs(0, ol.dptr(s), C.c_long(n), C.c_double(2 * margin), ol.dptr(P),
+                             ol.dptr(Q), ol.dptr(d), None)
+    assert d[0] <= margin and not far[0]
+    assert np.all(d[far.astype(bool)] > margin)
+    # effective on the regular pairs (the generated workload) beyond three margins
+    g = slice(len(pairs), len(pairs) + gen.shape[1])
+    beyond = d[g] > 3 * margin
+    assert far[g][beyond].mean() > 0.95, far[g][beyond].mean()
+    # and on most small-over-large pairs beyond three margins
+    beyond = d[1:len(pairs)] > 3 * margin
+    assert far[1:len(pairs)][beyond].mean() > 0.9, far[1:len(pairs)][beyond].mean()
