@@ -613,9 +613,11 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
           defer();
           return;
         }
-        if (!QONLY && g.fp32_screen) {
-          // the survivors far apart in binary32 drop out, the rest are
-          // compacted in place (order kept) for the binary64 solve
+        if (!QONLY) {
+          // the survivors that a triangle plane separates by more than the
+          // margin (and, with the binary32 screen, those it proves far)
+          // drop out; the rest are compacted in place (order kept) for the
+          // binary64 solve
           int n2 = 0;
           for (int b = 0; b < nl; b += 256) {
             const int e = b + tid;
@@ -625,7 +627,9 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
               v = surv[e];
               int iA, iB;
               pair_of(v, iA, iB);
-              keep2 = !screened_far(load_tri(g, p.oA + iA), load_tri(g, p.oB + iB), p.margin);
+              const Tri<double> A = load_tri(g, p.oA + iA), B = load_tri(g, p.oB + iB);
+              keep2 = !plane_apart(A, B, p.margin) &&
+                      !(g.fp32_screen && screened_far(A, B, p.margin));
             }
             const unsigned long long m = __ballot(keep2);
             if (lane == 0) wave_cnt[wave] = __popcll(m);
@@ -1224,7 +1228,7 @@ __global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
   const Tri<double> A = load_tri(g, g.tri_offset[pa] + iA);
   const Tri<double> B = load_tri(g, g.tri_offset[pb] + iB);
   const double margin = g.eps[pa] + g.eps[pb];
-  if (SCREEN && screened_far(A, B, margin)) {
+  if (plane_apart(A, B, margin) || (SCREEN && screened_far(A, B, margin))) {
     flags[e] = 0;
     return;
   }

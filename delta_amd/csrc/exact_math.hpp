@@ -306,6 +306,51 @@ DH_HD T pq_dist(const V3<T>& P, const V3<T>& Q) {
                ((Q.z - P.z) * (Q.z - P.z)));
 }
 
+// True when one triangle's plane separates the other triangle from it by
+// more than the contact margin (no contact, bf.cpp:125): with n = (A1 - A0) x
+// (A2 - A0), all of B on one side of A's plane at heights |n.(B_k - A0)| /
+// |n| above the margin (or the same with A and B exchanged).  A lies in its
+// plane, so the smallest height bounds the distance from below.  Rounding:
+// the heights' numerators are within 2^-49.3 |u||v||r| of exact (u, v A's
+// edges, r the vertex offsets) and |n| within 2^-49.8 |u||v|; the test
+// lowers the heights by 2^-46 |u|_1 |v|_1 |r|_1 and raises |n| by 2^-48
+// |u|_1 |v|_1, against the box test's threshold margin (1 + 2^-20) + mag
+// 2^-40, beyond which bfSolver's binary64 distance exceeds the margin.
+// Two faces of near-touching convex particles are separated by their planes
+// almost always when they are apart, so this removes most of the pairs the
+// box test leaves at the cost of two cross products.
+DH_HD bool plane_side_apart(const Tri<double>& A, const Tri<double>& B, double thr) {
+  using std::fabs;
+  const V3<double> u = sub(A.v[1], A.v[0]), v = sub(A.v[2], A.v[0]);
+  const V3<double> n = {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+  const double uv = (fabs(u.x) + fabs(u.y) + fabs(u.z)) * (fabs(v.x) + fabs(v.y) + fabs(v.z));
+  double lo = 1e308, hi = -1e308, R = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const V3<double> r = sub(B.v[k], A.v[0]);
+    const double h = dot(n, r);
+    lo = fmin(lo, h);
+    hi = fmax(hi, h);
+    R = fmax(R, fabs(r.x) + fabs(r.y) + fabs(r.z));
+  }
+  const double E = 0x1p-46 * uv * R;
+  const double nn = std::sqrt(dot(n, n)) + 0x1p-48 * uv;
+  const double h = lo > 0.0 ? lo : -hi;  // the least height, when all on one side
+  return h - E > thr * nn;
+}
+
+DH_HD bool plane_apart(const Tri<double>& A, const Tri<double>& B, double margin) {
+  using std::fabs;
+  double mag = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    mag = fmax(mag, fmax(fmax(fabs(A.v[k].x), fabs(A.v[k].y)),
+                         fmax(fmax(fabs(A.v[k].z), fabs(B.v[k].x)),
+                              fmax(fabs(B.v[k].y), fabs(B.v[k].z)))));
+  const double thr = margin * (1.0 + 0x1p-20) + mag * 0x1p-40;
+  return plane_side_apart(A, B, thr) || plane_side_apart(B, A, thr);
+}
+
 // Screen before bfSolver in binary64: true when A and B are provably farther
 // apart than the contact margin (no contact, bf.cpp:125).  bfSolver in
 // binary32 (coordinates relative to A's first vertex) gives a direction n =

@@ -227,15 +227,18 @@ def test_exact_choice_is_bfsolver_bit_for_bit(hc, fn):
             assert acc.mean() > 0.05  # random guesses are still right sometimes
 
 
-def test_binary32_screen_is_a_lower_bound(hc):
-    """screened_far (exact_math.hpp), the binary32 screen ahead of the
-    detection's binary64 bfSolver, skips a triangle pair only when the
-    oracle's bfSolver distance exceeds the margin: the recorded pair of a C1
-    particle triangle 0.14 mm above the 1 m floor triangle (a contact whose
-    binary32 bfSolver distance comes out 0.40 mm, three times too far),
-    small triangles against large ones at heights from 1e-6 to 1e-2 m, the
-    unique-pair workload and the degenerate pairs.  It still skips nearly
-    every regular pair beyond three margins."""
+@pytest.mark.parametrize("fn", ["hc_screened_far", "hc_plane_apart"])
+def test_detection_screens_are_lower_bounds(hc, fn):
+    """The detection's tests ahead of its binary64 bfSolver (exact_math.hpp:
+    screened_far, the binary32 screen, and plane_apart, the triangle-plane
+    test) skip a triangle pair only when the oracle's bfSolver distance
+    exceeds the margin: the recorded pair of a C1 particle triangle 0.14 mm
+    above the 1 m floor triangle (a contact whose binary32 bfSolver distance
+    comes out 0.40 mm, three times too far), small triangles against large
+    ones at heights from 1e-6 to 1e-2 m, the unique-pair workload and the
+    degenerate pairs.  The binary32 screen still skips nearly every regular
+    pair beyond three margins, and the plane test every small triangle held
+    flat above a large one beyond the margin."""
     import scenes
     margin = 2e-4
     reg = np.array([[[-0.25007118952381935, -0.066117445166873748, -0.13037171044944115],
@@ -261,17 +264,21 @@ def test_binary32_screen_is_a_lower_bound(hc):
     s = np.ascontiguousarray(np.concatenate([s_big, gen, scenes.degenerate_pairs(motions=8)], axis=1))
     n = s.shape[1]
     far = np.zeros(n, np.uint8)
-    hc.hc_screened_far(ol.dptr(s), C.c_long(n), C.c_double(margin),
-                       far.ctypes.data_as(C.c_void_p))
+    getattr(hc, fn)(ol.dptr(s), C.c_long(n), C.c_double(margin), far.ctypes.data_as(C.c_void_p))
     P = np.zeros(3 * n); Q = np.zeros(3 * n); d = np.zeros(n)
     ol.oracle().or_tri_pairs(0, ol.dptr(s), C.c_long(n), C.c_double(2 * margin), ol.dptr(P),
                              ol.dptr(Q), ol.dptr(d), None)
     assert d[0] <= margin and not far[0]
     assert np.all(d[far.astype(bool)] > margin)
-    # effective on the regular pairs (the generated workload) beyond three margins
     g = slice(len(pairs), len(pairs) + gen.shape[1])
     beyond = d[g] > 3 * margin
-    assert far[g][beyond].mean() > 0.95, far[g][beyond].mean()
-    # and on most small-over-large pairs beyond three margins
-    beyond = d[1:len(pairs)] > 3 * margin
-    assert far[1:len(pairs)][beyond].mean() > 0.9, far[1:len(pairs)][beyond].mean()
+    sb = slice(1, len(pairs))
+    if fn == "hc_screened_far":
+        # effective on the regular pairs (the generated workload) and on
+        # most small-over-large pairs beyond three margins
+        assert far[g][beyond].mean() > 0.95, far[g][beyond].mean()
+        assert far[sb][d[sb] > 3 * margin].mean() > 0.9
+    else:
+        # the large triangle's plane separates every small one lifted
+        # entirely above it beyond the margin
+        assert far[sb][lift > 2 * margin].all()
