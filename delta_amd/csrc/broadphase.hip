@@ -486,23 +486,31 @@ __global__ __launch_bounds__(1024) void k_small_pairs(const Aabb* __restrict__ b
                                                       const int* __restrict__ obstacle, int n,
                                                       long long* __restrict__ start,
                                                       long long cap, int* __restrict__ pairs) {
+  // the boxes and obstacle flags staged once: both sweeps read LDS only
+  __shared__ Aabb s_box[kSmallScene];
+  __shared__ unsigned char s_obst[kSmallScene];
   __shared__ long long s_start[kSmallScene + 1];
   __shared__ long long s_wave[16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = threadIdx.x;
+  if (t < n) {
+    s_box[t] = box[t];
+    s_obst[t] = obstacle && obstacle[t] ? 1 : 0;
+  }
+  __syncthreads();
   for (int i = w; i < n; i += 16) {
-    const Aabb bi = box[i];
-    const bool oi = obstacle && obstacle[i];
+    const Aabb bi = s_box[i];
+    const bool oi = s_obst[i] != 0;
     int m = 0;
     for (int j0 = i + 1; j0 < n; j0 += 64) {
       const int j = j0 + lane;
-      const bool e = j < n && !(oi && obstacle[j]) && overlap(bi, box[j]);
+      const bool e = j < n && !(oi && s_obst[j]) && overlap(bi, s_box[j]);
       m += __popcll(__ballot(e));
     }
     if (lane == 0) s_start[i] = m;
   }
   __syncthreads();
   // exclusive prefix sums: one count per thread, waves, then wave totals
-  const int t = threadIdx.x;
   const long long c = t < n ? s_start[t] : 0;
   long long incl = c;
   for (int o = 1; o < 64; o <<= 1) {
@@ -524,13 +532,13 @@ __global__ __launch_bounds__(1024) void k_small_pairs(const Aabb* __restrict__ b
   }
   __syncthreads();
   for (int i = w; i < n; i += 16) {
-    const Aabb bi = box[i];
-    const bool oi = obstacle && obstacle[i];
+    const Aabb bi = s_box[i];
+    const bool oi = s_obst[i] != 0;
     const long long s0 = s_start[i];
     int found = 0;
     for (int j0 = i + 1; j0 < n; j0 += 64) {
       const int j = j0 + lane;
-      const bool e = j < n && !(oi && obstacle[j]) && overlap(bi, box[j]);
+      const bool e = j < n && !(oi && s_obst[j]) && overlap(bi, s_box[j]);
       const unsigned long long mask = __ballot(e);
       const long long q = s0 + found + bp_lane_prefix(mask);
       if (e && q < cap) {
