@@ -722,9 +722,13 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
 // Persistent over the items: workgroup b takes items b, b + gridDim.x, ...
 // (with as many workgroups as items, one each).  Pairs of very uneven cost
 // (a 2,048-triangle funnel against 20-triangle particles) share the
-// workgroups of one launch.
+// workgroups of one launch.  The brute force is held to 168 registers
+// (three waves per SIMD; a few spill), which its latency-bound item loop
+// needs more than the registers.
 template <int M, bool QONLY = false>
-__global__ __launch_bounds__(256) void k_detect(DetectArgs g, int items) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(M == DH_BF && !QONLY ? 3 : 1))) void k_detect(DetectArgs g,
+                                                                                 int items) {
   if (g.item_list && g.nitems_dev && static_cast<int>(*g.nitems_dev) < items)
     items = static_cast<int>(*g.nitems_dev);
   for (int t = blockIdx.x; t < items; t += gridDim.x) {
