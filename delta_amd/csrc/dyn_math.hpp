@@ -3,6 +3,7 @@
 // reference's (forces.cpp:208-220,255-396; dynamics.cpp:26-242); compiled
 // without FMA contraction the results are bit-identical to it.
 #pragma once
+#include "cr_sincos.hpp"
 #include "exact_math.hpp"
 
 namespace dh {
@@ -35,11 +36,17 @@ DH_HD void expmap(double o1, double o2, double o3, double L[9]) {
                     angsq) *
                    angsq;
   } else {
-    using std::cos;
-    using std::sin;
+    // the reference's libm sin and cos; on the device the correctly rounded
+    // double-double ones (cr_sincos.hpp), which equal the host's where the
+    // device libm's can differ in the last bit
     const double ang = dsqrt(angsq);
-    sx = sin(ang) / ang;
-    cx = (1.0 - cos(ang)) / angsq;
+#if defined(__HIP_DEVICE_COMPILE__)
+    sx = cr_sincos(ang, false) / ang;
+    cx = (1.0 - cr_sincos(ang, true)) / angsq;
+#else
+    sx = std::sin(ang) / ang;
+    cx = (1.0 - std::cos(ang)) / angsq;
+#endif
   }
   const double v01 = o1 * o2, v02 = o2 * o3, v12 = o2 * o3;
   const double s0 = sx * o1, s1 = sx * o2, s2 = sx * o3;

@@ -340,3 +340,11 @@ extern "C" void hc_screened_far(const double* soa, long n, double margin, unsign
 extern "C" void hc_plane_apart(const double* soa, long n, double margin, unsigned char* far) {
   for (long i = 0; i < n; ++i) far[i] = dh::plane_apart(load(soa, n, i, 0), load(soa, n, i, 9), margin) ? 1 : 0;
 }
+
+// cr_sincos (the device's sin and cos in expmap's large-angle branch), host build.
+extern "C" void hc_cr_sincos(const double* x, long n, double* s, double* c) {
+  for (long i = 0; i < n; ++i) {
+    s[i] = dh::cr_sincos(x[i], false);
+    c[i] = dh::cr_sincos(x[i], true);
+  }
+}

@@ -173,13 +173,25 @@ def test_learned_degenerate_pairs(train_set):
 def test_network_warm_start_of_the_penalty_newton(train_set):
     """DH_PAIR_LEARNED_PENALTY starts the penalty Newton iteration from the
     network's barycentric outputs; DH_PAIR_PENALTY_CERT from the reference's
-    0.33.  Both certify or fall back, so both are correct; the trained start
-    needs fewer Newton steps and certifies more pairs."""
+    0.33.  Both certify or fall back, so both are correct.  The network's
+    start is far closer to the solution's barycentric parameters than 0.33
+    (held-out pairs, against the brute force's labels); the Newton step
+    count barely follows, because the penalty schedule (three steps at the
+    first regularisation, then x1e5 per step until the relative step is
+    below 1e-20) sets it: the trained start needs slightly fewer steps and
+    certifies at least as many pairs."""
     _, ds = train_set
     tr = learn.Trainer(seed=0)
     tr.fit(ds, epochs=6, batch=4096, lr=3e-3)
     tr.install()
     test = narrow.PairBatch(1 << 16, seed=3)
+    held = learn.Dataset(test.soa, test.n)
+    guess = tr.forward(held)[:, 16:20]
+    truth = held.bary.download()
+    err_net = np.abs(guess - truth).mean()
+    err_033 = np.abs(0.33 - truth).mean()
+    # measured: mean absolute error 0.19 from the network against 0.32 from 0.33
+    assert err_net < 0.75 * err_033, (err_net, err_033)
     test.run("bf64")
     _, _, d0, _ = test.results()
     stats = {}
