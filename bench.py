@@ -423,7 +423,7 @@ def dem_c1(a, steps=2000, learned=False):
                                "sample": f"the first {k} of the {steps} steps from the same "
                                          f"initial state (all the contact-carrying ones), "
                                          f"oracle step with OpenMP over the triangles of A in "
-                                         f"each particle pair and a critical append, "
+                                         f"each particle pair and a critical append (then sorted into row order), "
                                          f"{threads} threads, {dt:.1f} s"}
     return out
 
@@ -612,7 +612,7 @@ def dem_hopper(a):
                                "kind": "port", "cpu": host_cpu(threads),
                                "sample": f"oracle bf over {k} lattice-neighbour particle pairs "
                                          f"({nchk} triangle pairs), OpenMP over the triangles "
-                                         f"of A with a critical append, {threads} threads, "
+                                         f"of A with a critical append (then sorted into row order), {threads} threads, "
                                          f"{cdt:.1f} s; compare with triangle_checks_per_s "
                                          "(the eager steps' checks over their own time)"}
     return out
@@ -764,7 +764,7 @@ def dem_large(a, kind, steps=None, warmup=None, learned=False):
                                "kind": "port", "cpu": host_cpu(threads),
                                "sample": f"oracle bf over {k} neighbouring particle pairs "
                                          f"({nchk} triangle pairs), OpenMP over the triangles "
-                                         f"of A with a critical append, {threads} threads, "
+                                         f"of A with a critical append (then sorted into row order), {threads} threads, "
                                          f"{cdt:.1f} s; compare with eager.triangle_checks_per_s"}
     return out
 

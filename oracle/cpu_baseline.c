@@ -11,9 +11,10 @@
  *                      brute-force detection, forces, update) with that
  *                      row-parallel detection: the CPU baseline of the DEM
  *                      workloads.  Contacts are appended in completion order,
- *                      as in the reference, so force sums may differ from
- *                      or_step's in the last bits: a timing driver, not a
- *                      checker.
+ *                      as in the reference, then each pair's are sorted back
+ *                      into row-major order, so the step equals or_step's
+ *                      bit for bit (and can check the GPU's on scenes the
+ *                      serial oracle would take minutes over).
  */
 #include <math.h>
 #include <omp.h>
@@ -74,6 +75,16 @@ int64_t or_detect_rows_omp(const double* xA, const double* yA, const double* zA,
 
 /* Contacts of one particle pair: rows of A in parallel, each row's contacts
  * appended under a critical section (bf.cpp:92-95,125-141). */
+/* Row-major order (triA, then triB) of one pair's contacts: the serial
+ * driver's order, so that the step after the parallel rows is the serial
+ * step bit for bit whatever order the critical section saw. */
+static int contact_order(const void* a, const void* b) {
+  const dh_contact* x = (const dh_contact*)a;
+  const dh_contact* y = (const dh_contact*)b;
+  if (x->triA != y->triA) return x->triA < y->triA ? -1 : 1;
+  return (x->triB > y->triB) - (x->triB < y->triB);
+}
+
 static int rows_detect(const double* xA, const double* yA, const double* zA, int nA, double epsA,
                        int idA, const double* xB, const double* yB, const double* zB, int nB,
                        double epsB, int idB, int fric, dh_contact** buf, int* cap) {
@@ -118,6 +129,7 @@ static int rows_detect(const double* xA, const double* yA, const double* zA, int
       }
     }
   }
+  qsort(*buf, (size_t)n, sizeof(dh_contact), contact_order);
   return n;
 }
 

@@ -247,9 +247,9 @@ def test_exact_mode_reports_intersections():
 def test_openmp_step_driver_matches_the_oracle_step():
     """The reference-shaped OpenMP step (oracle/cpu_baseline.c, the DEM
     CPU baseline: rows of each particle pair in parallel, contacts appended
-    in completion order) finds the oracle step's candidate pairs, triangle
-    checks and contacts on the C1 lattice, and its states agree to rounding
-    of the reordered force sums."""
+    in completion order, then sorted back into row-major order per pair)
+    equals the oracle step bit for bit on the C1 lattice: candidate pairs,
+    triangle checks, contacts and the whole state."""
     import copy
     from delta_amd import engine
     a = engine.c1_scene()
@@ -258,6 +258,6 @@ def test_openmp_step_driver_matches_the_oracle_step():
         r0 = ol.or_step(a, 0, 1e5, 0.1, 0.0, 1e-6, (0.0, -9.81, 0.0))
         r1 = ol.or_step(b, 0, 1e5, 0.1, 0.0, 1e-6, (0.0, -9.81, 0.0), omp_threads=4)
         assert r0 == r1
-    for f in ("x", "pos", "linvel", "refangvel", "rot"):
+    for f in ("x", "y", "z", "pos", "linvel", "angvel", "refangvel", "rot"):
         x0 = np.asarray(getattr(a, f)); x1 = np.asarray(getattr(b, f))
-        assert np.allclose(x0, x1, rtol=1e-9, atol=1e-12 * np.abs(x0).max()), f
+        assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64)), f
