@@ -23,6 +23,10 @@ struct Aabb {
   double lo[3], hi[3];
 };
 
+// Vertex replicas per lane that the box pass loads at once and keeps for the
+// sphere pass (a particle of up to 64 * kAabbHeld replicas is read once).
+constexpr int kAabbHeld = 8;
+
 __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__ x,
                                                       const double* __restrict__ y,
                                                       const double* __restrict__ z,
@@ -37,54 +41,92 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
   // chunk of 64 of its triangles, chunk c at cbox[t0 / 64 + i + c] (an index
   // that grows with i and c, so no two chunks share it)
   const bool chunked = cbox && t1 - t0 > kRowBlock;
-  const int step = chunked ? 64 : (t1 - t0 > 0 ? t1 - t0 : 1);
+  const int v0 = 3 * t0, nv = t1 > t0 ? 3 * (t1 - t0) : 0;
   double plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b0 = t0; b0 < t1; b0 += step) {
-    const int b1 = b0 + step < t1 ? b0 + step : t1;
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int v = 3 * b0 + threadIdx.x; v < 3 * b1; v += 64) {
-      const double c[3] = {x[v], y[v], z[v]};
-      for (int q = 0; q < 3; ++q) {
-        lo[q] = fmin(lo[q], c[q]);
-        hi[q] = fmax(hi[q], c[q]);
+  // the held replicas (NaN past the end: min, max and the sphere skip NaN)
+  double hx[kAabbHeld], hy[kAabbHeld], hz[kAabbHeld];
+  auto load_block = [&](int b) {
+#pragma unroll
+    for (int k = 0; k < kAabbHeld; ++k) {
+      const int v = b + 64 * k + static_cast<int>(threadIdx.x);
+      const bool in = v < nv;
+      hx[k] = in ? x[v0 + v] : NAN;
+      hy[k] = in ? y[v0 + v] : NAN;
+      hz[k] = in ? z[v0 + v] : NAN;
+    }
+  };
+  if (!chunked) {
+    for (int b = 0; b < nv; b += 64 * kAabbHeld) {
+      load_block(b);
+#pragma unroll
+      for (int k = 0; k < kAabbHeld; ++k) {
+        plo[0] = fmin(plo[0], hx[k]);
+        plo[1] = fmin(plo[1], hy[k]);
+        plo[2] = fmin(plo[2], hz[k]);
+        phi[0] = fmax(phi[0], hx[k]);
+        phi[1] = fmax(phi[1], hy[k]);
+        phi[2] = fmax(phi[2], hz[k]);
       }
     }
     for (int o = 32; o > 0; o >>= 1)
       for (int q = 0; q < 3; ++q) {
-        lo[q] = fmin(lo[q], __shfl_xor(lo[q], o));
-        hi[q] = fmax(hi[q], __shfl_xor(hi[q], o));
+        plo[q] = fmin(plo[q], __shfl_xor(plo[q], o));
+        phi[q] = fmax(phi[q], __shfl_xor(phi[q], o));
       }
-    if (chunked && threadIdx.x == 0) {
-      Aabb b;
+  } else {
+    for (int b0 = t0; b0 < t1; b0 += 64) {
+      const int b1 = b0 + 64 < t1 ? b0 + 64 : t1;
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (int v = 3 * b0 + threadIdx.x; v < 3 * b1; v += 64) {
+        const double c[3] = {x[v], y[v], z[v]};
+        for (int q = 0; q < 3; ++q) {
+          lo[q] = fmin(lo[q], c[q]);
+          hi[q] = fmax(hi[q], c[q]);
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1)
+        for (int q = 0; q < 3; ++q) {
+          lo[q] = fmin(lo[q], __shfl_xor(lo[q], o));
+          hi[q] = fmax(hi[q], __shfl_xor(hi[q], o));
+        }
+      if (threadIdx.x == 0) {
+        Aabb bx;
+        for (int q = 0; q < 3; ++q) {
+          bx.lo[q] = lo[q];
+          bx.hi[q] = hi[q];
+        }
+        cbox[t0 / 64 + i + (b0 - t0) / 64] = bx;
+      }
       for (int q = 0; q < 3; ++q) {
-        b.lo[q] = lo[q];
-        b.hi[q] = hi[q];
+        plo[q] = fmin(plo[q], lo[q]);
+        phi[q] = fmax(phi[q], hi[q]);
       }
-      cbox[t0 / 64 + i + (b0 - t0) / 64] = b;
-    }
-    for (int q = 0; q < 3; ++q) {
-      plo[q] = fmin(plo[q], lo[q]);
-      phi[q] = fmax(phi[q], hi[q]);
     }
   }
   if (threadIdx.x == 0) {
     const double m = eps[i] + 0.5e-9;
-    Aabb b;
+    Aabb bx;
     for (int q = 0; q < 3; ++q) {
-      b.lo[q] = plo[q] - m;
-      b.hi[q] = phi[q] + m;
+      bx.lo[q] = plo[q] - m;
+      bx.hi[q] = phi[q] + m;
     }
-    box[i] = b;
+    box[i] = bx;
   }
   if (sph) {
     // a bounding sphere: the vertex box's centre and the largest vertex
-    // distance from it (NaN coordinates drop out, as from the box)
+    // distance from it (NaN coordinates drop out, as from the box); the
+    // replicas come from registers when one block held them all
     const double c[3] = {0.5 * (plo[0] + phi[0]), 0.5 * (plo[1] + phi[1]),
                          0.5 * (plo[2] + phi[2])};
     double r2 = 0.0;
-    for (int v = 3 * t0 + threadIdx.x; v < 3 * t1; v += 64) {
-      const double dx = x[v] - c[0], dy = y[v] - c[1], dz = z[v] - c[2];
-      r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    const bool held = !chunked && nv <= 64 * kAabbHeld;
+    for (int b = 0; b < nv; b += 64 * kAabbHeld) {
+      if (!held) load_block(b);
+#pragma unroll
+      for (int k = 0; k < kAabbHeld; ++k) {
+        const double dx = hx[k] - c[0], dy = hy[k] - c[1], dz = hz[k] - c[2];
+        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+      }
     }
     for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o));
     if (threadIdx.x == 0) {

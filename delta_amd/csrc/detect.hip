@@ -106,6 +106,14 @@ struct DetectArgs {
   // items from item_list.
   int* deferred;
   unsigned* ndeferred;
+  // Items k_detect_wave takes for their rows' test but cannot stage (B has
+  // more than kWaveRows triangles) and does not cull: k_detect_big runs them.
+  int* biglist;
+  unsigned* nbig;
+  // Items k_lane_cull leaves for k_detect_wave (with wlist, k_detect_wave
+  // takes its items from this list, *nwlist long).
+  int* wlist;
+  unsigned* nwlist;
   const int* item_list;
   const unsigned* nitems_dev;  // with item_list: its length (else the launch's)
   int fp32_screen;             // k_queue_solve screens in binary32 first
@@ -117,6 +125,11 @@ constexpr int kWaveRows = 80;       // rows and columns staged per wave
 constexpr int kWaveSurvivors = 256; // survivors listed per wave
 __device__ __forceinline__ bool wave_item(int nrows, int nA, int nB) {
   return nrows <= kWaveRows && nB <= kWaveRows && nA < 65536;
+}
+// Items whose rows k_detect_wave tests against B's particle box (a larger B,
+// such as a wall against a small particle, is often culled there).
+__device__ __forceinline__ bool wave_rows(int nrows, int nA, int nB) {
+  return nrows <= kWaveRows && nA < 65536 && nB < 65536;
 }
 
 __device__ __forceinline__ Tri<double> load_tri(const DetectArgs& g, int t) {
@@ -756,7 +769,7 @@ __global__ __launch_bounds__(256) void k_detect_big(DetectArgs g, int items) {
           const int nB = g.tri_offset[pb + 1] - g.tri_offset[pb];
           const int r0 = static_cast<int>(static_cast<long long>(tile) * nA / g.tiles);
           const int r1 = static_cast<int>(static_cast<long long>(tile + 1) * nA / g.tiles);
-          big = !wave_item(r1 - r0, nA, nB);
+          big = !wave_rows(r1 - r0, nA, nB);
         }
       }
       const unsigned long long m = __ballot(big);
@@ -770,12 +783,77 @@ __global__ __launch_bounds__(256) void k_detect_big(DetectArgs g, int items) {
       __syncthreads();  // the next item reuses the workgroup's LDS
     }
   }
+  // then the items k_detect_wave (launched before) handed over
+  const int nb = static_cast<int>(*g.nbig);
+  for (int j = blockIdx.x; j < nb; j += gridDim.x) {
+    detect_item<DH_BF, true>(g, g.biglist[j]);
+    __syncthreads();
+  }
+}
+
+// Rows a lane tests alone in k_lane_cull.
+constexpr int kLaneRows = 16;
+
+// One lane per item, before k_detect_wave: an item of k_detect_wave's with at
+// most kLaneRows rows whose rows' boxes all miss B's particle box (grown by
+// the margin) is culled here, with the counters k_detect_wave would write (a
+// wall of few triangles against the many particles inside it); the other
+// items of k_detect_wave's are listed for it.
+__global__ __launch_bounds__(256) void k_lane_cull(DetectArgs g, int items) {
+  const int item = blockIdx.x * 256 + static_cast<int>(threadIdx.x);
+  bool listed = false;
+  if (item < items) {
+    const int k = item / g.tiles, tile = item - k * g.tiles;
+    unsigned long long* ts = g.tile_stats + static_cast<long long>(item) * kTileStats;
+    if (g.npairs_dev && k >= *g.npairs_dev) {
+      g.tile_count[item] = 0;
+      for (int q = 0; q < kTileStats; ++q) ts[q] = 0;
+    } else {
+      const Pair p = load_pair(g, k);
+      const int r0 = static_cast<int>(static_cast<long long>(tile) * p.nA / g.tiles);
+      const int r1 = static_cast<int>(static_cast<long long>(tile + 1) * p.nA / g.tiles);
+      const int nrows = r1 - r0;
+      if (wave_rows(nrows, p.nA, p.nB)) {
+        bool near = !(g.pbox && nrows <= kLaneRows);
+        if (!near) {
+          const double* pb = g.pbox + 6 * static_cast<long long>(p.pb);
+          for (int t = 0; t < nrows && !near; ++t) {
+            double lo[3], hi[3];
+            tri_box(load_tri(g, p.oA + r0 + t), lo, hi);
+            near = !boxes_apart(lo, hi, pb, pb + 3, p.margin);
+          }
+        }
+        if (near) {
+          listed = true;
+        } else {
+          const unsigned total = static_cast<unsigned>(nrows * p.nB);
+          g.tile_count[item] = 0;
+          ts[TS_CHECKS] = tile == 0 ? static_cast<unsigned long long>(p.nA) * p.nB : 0ull;
+          ts[TS_EVALUATED] = total;
+          ts[TS_FALLBACK_ROWS] = 0;
+          ts[TS_PENALTY_FAILS] = 0;
+          ts[TS_BATCH_FAILS] = 0;
+          ts[TS_BATCH_SIZE] = 0;
+          ts[TS_CULLED] = total;
+        }
+      }
+    }
+  }
+  const unsigned long long m = __ballot(listed);
+  if (m) {
+    unsigned base = 0;
+    if ((threadIdx.x & 63) == static_cast<unsigned>(__builtin_ctzll(m)))
+      base = atomicAdd(g.nwlist, static_cast<unsigned>(__popcll(m)));
+    base = __shfl(base, __builtin_ctzll(m));
+    if (listed) g.wlist[base + lane_prefix(m)] = item;
+  }
 }
 
 // The mid-phase of DH_BF for small items, one wave per item and no
 // workgroup barrier: A's rows and B's columns (at most kWaveRows each)
 // staged as boxes in the wave's LDS, rows apart from B's particle box
-// dropped, rows and columns prefiltered against the other side's union,
+// dropped (an item with a larger B that this test does not cull goes to
+// k_detect_big), rows and columns prefiltered against the other side's union,
 // the surviving pairs listed in row-major order and appended to the
 // survivor queue as one range.  Items whose survivors overflow the list or
 // the queue are deferred to the full kernel.  Same tests, order and
@@ -787,7 +865,9 @@ __global__ __launch_bounds__(128) void k_detect_wave(DetectArgs g, int items) {
   __shared__ int surv[kW][kWaveSurvivors];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double* bx = boxes[w];
-  for (int item = blockIdx.x * kW + w; item < items; item += gridDim.x * kW) {
+  const int nw = g.wlist ? static_cast<int>(*g.nwlist) : items;
+  for (int j = blockIdx.x * kW + w; j < nw; j += gridDim.x * kW) {
+    const int item = g.wlist ? g.wlist[j] : j;
     const int k = item / g.tiles, tile = item - k * g.tiles;
     unsigned long long* ts = g.tile_stats + static_cast<long long>(item) * kTileStats;
     if (g.npairs_dev && k >= *g.npairs_dev) {
@@ -799,7 +879,7 @@ __global__ __launch_bounds__(128) void k_detect_wave(DetectArgs g, int items) {
     const int r0 = static_cast<int>(static_cast<long long>(tile) * p.nA / g.tiles);
     const int r1 = static_cast<int>(static_cast<long long>(tile + 1) * p.nA / g.tiles);
     const int nrows = r1 - r0;
-    if (!wave_item(nrows, p.nA, p.nB)) continue;  // the workgroup kernel's
+    if (!wave_rows(nrows, p.nA, p.nB)) continue;  // the workgroup kernel's
     const int total = nrows * p.nB;
     unsigned culled = 0;  // lane 0's count is the item's
     const double* pb = g.pbox ? g.pbox + 6 * static_cast<long long>(p.pb) : nullptr;
@@ -814,6 +894,9 @@ __global__ __launch_bounds__(128) void k_detect_wave(DetectArgs g, int items) {
     bool deferred = false;
     if (pb && !__ballot(near)) {
       culled = total;
+    } else if (p.nB > kWaveRows) {
+      if (lane == 0) g.biglist[atomicAdd(g.nbig, 1u)] = item;
+      continue;
     } else {
       for (int t = lane; t < p.nB; t += 64) {
         double* b = bx + 6 * (kWaveRows + t);
@@ -1343,6 +1426,8 @@ static unsigned detect_grid(long long items) {
 template <int M>
 static void launch_detect(unsigned items, size_t shm, hipStream_t st, const DetectArgs& a) {
   if (M == DH_BF && a.queue && a.wave_items) {
+    hipLaunchKernelGGL(k_lane_cull, items > 0 ? grid_for(items, 256) : 1u, 256, 0, st, a,
+                       static_cast<int>(items));
     const long long wg = (static_cast<long long>(items) + 1) / 2;
     const long long cap = static_cast<long long>(device_cus()) * 16;
     hipLaunchKernelGGL(k_detect_wave, static_cast<unsigned>(wg < cap ? (wg > 0 ? wg : 1) : cap),
@@ -1505,6 +1590,10 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.qbase = nullptr;
   a.deferred = nullptr;
   a.ndeferred = nullptr;
+  a.biglist = nullptr;
+  a.nbig = nullptr;
+  a.wlist = nullptr;
+  a.nwlist = nullptr;
   a.item_list = nullptr;
   a.nitems_dev = nullptr;
   const long long nitems = static_cast<long long>(npairs) * tiles;
@@ -1517,12 +1606,16 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
     if ((rc = g_scratch.queue.grow(sizeof(int2) * qcap))) return rc;
     if ((rc = g_scratch.qcounter.grow(sizeof(unsigned long long)))) return rc;
     if ((rc = g_scratch.qbase.grow(sizeof(long long) * (nitems + 1)))) return rc;
-    if ((rc = g_scratch.deferred.grow(sizeof(int) * (nitems + 1)))) return rc;
-    if ((rc = g_scratch.ndeferred.grow(sizeof(unsigned)))) return rc;
+    if ((rc = g_scratch.deferred.grow(sizeof(int) * 3 * (nitems + 1)))) return rc;
+    if ((rc = g_scratch.ndeferred.grow(sizeof(unsigned) * 3))) return rc;
     DH_HIP_CHECK(hipMemsetAsync(g_scratch.qcounter.ptr, 0, sizeof(unsigned long long), st));
-    DH_HIP_CHECK(hipMemsetAsync(g_scratch.ndeferred.ptr, 0, sizeof(unsigned), st));
+    DH_HIP_CHECK(hipMemsetAsync(g_scratch.ndeferred.ptr, 0, sizeof(unsigned) * 3, st));
     a.deferred = g_scratch.deferred.as<int>();
     a.ndeferred = g_scratch.ndeferred.as<unsigned>();
+    a.biglist = a.deferred + (nitems + 1);
+    a.nbig = a.ndeferred + 1;
+    a.wlist = a.deferred + 2 * (nitems + 1);
+    a.nwlist = a.ndeferred + 2;
     a.wave_items = 1;
     a.queue = g_scratch.queue.as<int2>();
     a.qcap = qcap;
@@ -1656,19 +1749,27 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.qbase = nullptr;
   a.deferred = nullptr;
   a.ndeferred = nullptr;
+  a.biglist = nullptr;
+  a.nbig = nullptr;
+  a.wlist = nullptr;
+  a.nwlist = nullptr;
   a.item_list = nullptr;
   a.nitems_dev = nullptr;
   const unsigned items = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
   if (qcap > 0) {
     // the survivor queue with its counts kept on the device: every pass is
     // launched for the capacity and reads the live length
-    fill_async(g_scratch.ndeferred.as<unsigned>(), 0u, 1, st);
+    fill_async(g_scratch.ndeferred.as<unsigned>(), 0u, 3, st);
     a.queue = g_scratch.queue.as<int2>();
     a.qcap = qcap;
     a.qcounter = g_scratch.qcounter.as<unsigned long long>();
     a.qbase = g_scratch.qbase.as<long long>();
     a.deferred = g_scratch.deferred.as<int>();
     a.ndeferred = g_scratch.ndeferred.as<unsigned>();
+    a.biglist = a.deferred + (items + 1);
+    a.nbig = a.ndeferred + 1;
+    a.wlist = a.deferred + 2 * (items + 1);
+    a.nwlist = a.ndeferred + 2;
     a.wave_items = 1;
   }
   switch (method) {
@@ -1716,8 +1817,8 @@ int detect_reserve(int pair_cap, long long contact_cap, int method) {
     if ((rc = g_scratch.queue.grow(sizeof(int2) * qcap))) return rc;
     if ((rc = g_scratch.qcounter.grow(sizeof(unsigned long long)))) return rc;
     if ((rc = g_scratch.qbase.grow(sizeof(long long) * (items + 1)))) return rc;
-    if ((rc = g_scratch.deferred.grow(sizeof(int) * (items + 1)))) return rc;
-    if ((rc = g_scratch.ndeferred.grow(sizeof(unsigned)))) return rc;
+    if ((rc = g_scratch.deferred.grow(sizeof(int) * 3 * (items + 1)))) return rc;
+    if ((rc = g_scratch.ndeferred.grow(sizeof(unsigned) * 3))) return rc;
     if ((rc = queue_reserve(qcap))) return rc;
     if (method == DH_BF_LEARNED && (rc = queue_learned_reserve(qcap))) return rc;
   }
