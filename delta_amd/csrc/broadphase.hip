@@ -36,6 +36,7 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
                                                       Aabb* __restrict__ cbox,
                                                       double* __restrict__ sph) {
   const int i = blockIdx.x;
+  const int lane = static_cast<int>(threadIdx.x);
   const int t0 = tri_offset[i], t1 = tri_offset[i + 1];
   // a particle with more triangles than one row block keeps the box of each
   // chunk of 64 of its triangles, chunk c at cbox[t0 / 64 + i + c] (an index
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
   auto load_block = [&](int b) {
 #pragma unroll
     for (int k = 0; k < kAabbHeld; ++k) {
-      const int v = b + 64 * k + static_cast<int>(threadIdx.x);
+      const int v = b + 64 * k + lane;
       const bool in = v < nv;
       hx[k] = in ? x[v0 + v] : NAN;
       hy[k] = in ? y[v0 + v] : NAN;
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
     for (int b0 = t0; b0 < t1; b0 += 64) {
       const int b1 = b0 + 64 < t1 ? b0 + 64 : t1;
       double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      for (int v = 3 * b0 + threadIdx.x; v < 3 * b1; v += 64) {
+      for (int v = 3 * b0 + lane; v < 3 * b1; v += 64) {
         const double c[3] = {x[v], y[v], z[v]};
         for (int q = 0; q < 3; ++q) {
           lo[q] = fmin(lo[q], c[q]);
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
           lo[q] = fmin(lo[q], __shfl_xor(lo[q], o));
           hi[q] = fmax(hi[q], __shfl_xor(hi[q], o));
         }
-      if (threadIdx.x == 0) {
+      if (lane == 0) {
         Aabb bx;
         for (int q = 0; q < 3; ++q) {
           bx.lo[q] = lo[q];
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
       }
     }
   }
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     const double m = eps[i] + 0.5e-9;
     Aabb bx;
     for (int q = 0; q < 3; ++q) {
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
       }
     }
     for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o));
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
       sph[4 * i] = c[0];
       sph[4 * i + 1] = c[1];
       sph[4 * i + 2] = c[2];
@@ -725,8 +726,8 @@ static int grid_build(const dh_geometry* geom, const int* obstacle_dev, double c
                       const BpBuffers& b, GridArgs& g, hipStream_t st) {
   const int n = geom->nparticles;
   const unsigned tbl = table_size(n);
-  hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z, geom->tri_offset,
-                     geom->eps, b.box, b.cbox, b.sph);
+  hipLaunchKernelGGL(k_particle_aabb, n, 64, 0, st, geom->x, geom->y, geom->z,
+                     geom->tri_offset, geom->eps, b.box, b.cbox, b.sph);
   g.box = b.box;
   g.obstacle = obstacle_dev;
   g.n = n;

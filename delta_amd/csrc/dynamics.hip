@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_fill_incidence(const int* __restrict__ 
 // Particles with more than this many contact-carrying pairs (walls, floors)
 // are summed by a workgroup each (k_particle_sum_wide); the rest by one lane.
 constexpr int kNarrowSum = 64;
-constexpr int kWideSort = 4096;  // partners rank-sorted in LDS
+constexpr int kWideSort = 4096;  // entries sorted in LDS
 
 __device__ __forceinline__ int entry_partner(const int* __restrict__ pairs, int en) {
   const int pr = en >> 1;
@@ -154,11 +154,24 @@ __global__ __launch_bounds__(256) void k_particle_sum(int n, const int* __restri
                                                       const double* __restrict__ pf,
                                                       double* __restrict__ force,
                                                       double* __restrict__ torque,
-                                                      int* __restrict__ touched) {
+                                                      int* __restrict__ touched,
+                                                      int* __restrict__ wide) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int m = count[i];
-  if (m > kNarrowSum) return;  // k_particle_sum_wide
+  const int m = i < n ? count[i] : 0;
+  // listed for k_particle_sum_wide (wide[0] entries from wide[1]), each
+  // wave's in particle order
+  const bool is_wide = m > kNarrowSum;
+  const unsigned long long wm = __ballot(is_wide);
+  if (wm) {
+    const int lane = static_cast<int>(threadIdx.x & 63), first = __builtin_ctzll(wm);
+    int base = 0;
+    if (lane == first) base = atomicAdd(&wide[0], __popcll(wm));
+    base = __shfl(base, first);
+    const int pre = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(wm >> 32),
+                                              __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(wm), 0));
+    if (is_wide) wide[1 + base + pre] = i;
+  }
+  if (i >= n || is_wide) return;
   double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
   sum_by_selection(pairs, entries, pf, start[i], m, f, t);
   for (int q = 0; q < 3; ++q) {
@@ -168,11 +181,14 @@ __global__ __launch_bounds__(256) void k_particle_sum(int n, const int* __restri
   if (touched) touched[i] = m > 0 ? 1 : 0;
 }
 
-// One workgroup per particle with more than kNarrowSum entries: partners
-// rank-sorted in LDS (distinct for unique pairs), the
-// entries' force rows staged 256 at a time, and one lane adding them in
-// ascending partner order, the same order and arithmetic as the narrow sum.
-__global__ __launch_bounds__(256) void k_particle_sum_wide(int n, const int* __restrict__ pairs,
+// The particles with more than kNarrowSum entries (listed by k_particle_sum),
+// one workgroup at a time: entries sorted by partner in LDS (a bitonic sort;
+// a repeated pair's by position), the entries' force rows staged 256 at a
+// time, and six lanes adding
+// them, one component each, in ascending partner order: the same order and
+// arithmetic as the narrow sum.
+__global__ __launch_bounds__(256) void k_particle_sum_wide(const int* __restrict__ wide,
+                                                           const int* __restrict__ pairs,
                                                            const long long* __restrict__ start,
                                                            const int* __restrict__ count,
                                                            const int* __restrict__ entries,
@@ -180,53 +196,69 @@ __global__ __launch_bounds__(256) void k_particle_sum_wide(int n, const int* __r
                                                            double* __restrict__ force,
                                                            double* __restrict__ torque,
                                                            int* __restrict__ touched) {
-  __shared__ int part[kWideSort];
-  __shared__ int ord[kWideSort];
+  // (partner << 32 | position) of each entry, sorted ascending
+  __shared__ long long key[kWideSort];
   __shared__ double stage[256 * 6];
-  const int i = blockIdx.x;
-  if (i >= n) return;
-  const int m = count[i];
-  if (m <= kNarrowSum) return;  // block-uniform
   const int tid = threadIdx.x;
-  const long long s = start[i];
-  double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
-  if (m > kWideSort) {
-    if (tid == 0) sum_by_selection(pairs, entries, pf, s, m, f, t);
-  } else {
-    for (int e = tid; e < m; e += 256) part[e] = entry_partner(pairs, entries[s + e]);
-    __syncthreads();
-    for (int e = tid; e < m; e += 256) {
-      const int v = part[e];
-      int rank = 0;
-      // ties (a repeated pair) by position, so that ord is a permutation
-      for (int q = 0; q < m; ++q) rank += (part[q] < v || (part[q] == v && q < e)) ? 1 : 0;
-      ord[rank] = e;
-    }
-    __syncthreads();
-    for (int c0 = 0; c0 < m; c0 += 256) {
-      const int r = c0 + tid;
-      if (r < m) {
-        const double* o = entry_forces(pf, entries[s + ord[r]]);
-        for (int q = 0; q < 6; ++q) stage[6 * tid + q] = o[q];
-      }
-      __syncthreads();
+  const int nwide = wide[0];
+  for (int j = blockIdx.x; j < nwide; j += gridDim.x) {
+    const int i = wide[1 + j];
+    const int m = count[i];
+    const long long s = start[i];
+    if (m > kWideSort) {
       if (tid == 0) {
-        const int nr = m - c0 < 256 ? m - c0 : 256;
-        for (int k = 0; k < nr; ++k)
-          for (int q = 0; q < 3; ++q) {
-            f[q] += stage[6 * k + q];
-            t[q] += stage[6 * k + 3 + q];
-          }
+        double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+        sum_by_selection(pairs, entries, pf, s, m, f, t);
+        for (int q = 0; q < 3; ++q) {
+          force[3 * i + q] = f[q];
+          torque[3 * i + q] = t[q];
+        }
       }
+    } else {
+      // bitonic sort of the keys, padded to a power of two with the largest
+      // key: partners ascending, a repeated pair's entries by position
+      int P = 1;
+      while (P < m) P <<= 1;
+      for (int e = tid; e < P; e += 256)
+        key[e] = e < m ? (static_cast<long long>(entry_partner(pairs, entries[s + e])) << 32) | e
+                       : 0x7fffffffffffffffll;
       __syncthreads();
+      for (int k = 2; k <= P; k <<= 1)
+        for (int h = k >> 1; h > 0; h >>= 1) {
+          for (int e = tid; e < P; e += 256) {
+            const int o = e ^ h;
+            if (o > e) {
+              const long long a = key[e], b = key[o];
+              if ((a > b) == ((e & k) == 0)) {
+                key[e] = b;
+                key[o] = a;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      // lane c < 6 adds component c (force x, y, z, torque x, y, z) of
+      // every row in ascending partner order
+      double acc = 0.0;
+      for (int c0 = 0; c0 < m; c0 += 256) {
+        const int r = c0 + tid;
+        if (r < m) {
+          const double* o = entry_forces(pf, entries[s + static_cast<int>(key[r] & 0xffffffffll)]);
+          for (int q = 0; q < 6; ++q) stage[6 * tid + q] = o[q];
+        }
+        __syncthreads();
+        if (tid < 6) {
+          const int nr = m - c0 < 256 ? m - c0 : 256;
+#pragma unroll 16
+          for (int k = 0; k < nr; ++k) acc += stage[6 * k + tid];
+        }
+        __syncthreads();
+      }
+      if (tid < 3) force[3 * i + tid] = acc;
+      else if (tid < 6) torque[3 * i + tid - 3] = acc;
     }
-  }
-  if (tid == 0) {
-    for (int q = 0; q < 3; ++q) {
-      force[3 * i + q] = f[q];
-      torque[3 * i + q] = t[q];
-    }
-    if (touched) touched[i] = 1;
+    if (tid == 0 && touched) touched[i] = 1;
+    __syncthreads();  // the next particle reuses the keys
   }
 }
 
@@ -245,6 +277,7 @@ __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __r
                                                   const double* __restrict__ rz,
                                                   const int* __restrict__ tri_offset) {
   const int i = blockIdx.x;
+  const int lane = static_cast<int>(threadIdx.x);
   const double h = h_dev ? *h_dev : h_arg;  // the adaptive controller's step
   double v[3], W[3], R[9], pos[3], refpos[3], w[3];
   load3(b.linvel, i, v);
@@ -275,7 +308,7 @@ __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __r
     if (grav.g[q] != 0.0 && !kinematic) v[q] += h * grav.g[q];
   for (int q = 0; q < 3; ++q) pos[q] += h * v[q];
   update_rotation(w, W, R, h);
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     for (int q = 0; q < 3; ++q) {
       b.linvel[3 * i + q] = v[q];
       b.refangvel[3 * i + q] = W[q];
@@ -285,13 +318,13 @@ __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __r
     for (int q = 0; q < 9; ++q) b.rot[9 * i + q] = R[q];
   }
   const int v0 = 3 * tri_offset[i], v1 = 3 * tri_offset[i + 1];
-  for (int j = v0 + threadIdx.x; j < v1; j += 64)
+  for (int j = v0 + lane; j < v1; j += 64)
     update_vertex(x[j], y[j], z[j], rx[j], ry[j], rz[j], R, pos, refpos);
 }
 
 
 struct ForceScratch {
-  DevBuf pf, incid, start, fill, entries;
+  DevBuf pf, incid, start, fill, entries, wide;
 };
 static ForceScratch g_fs;
 
@@ -305,6 +338,7 @@ int forces_reserve(int n, int npairs) {
   if ((rc = g_fs.fill.grow(sizeof(int) * (n + 1)))) return rc;
   if ((rc = g_fs.start.grow(sizeof(long long) * (n + 1)))) return rc;
   if ((rc = g_fs.entries.grow(sizeof(int) * (2 * static_cast<size_t>(npairs) + 1)))) return rc;
+  if ((rc = g_fs.wide.grow(sizeof(int) * (static_cast<size_t>(n) + 2)))) return rc;
   return DH_OK;
 }
 
@@ -319,8 +353,9 @@ int contacts_forces_impl(const dh_bodies* bodies, const int* pairs_dev, int npai
   const int n = bodies->n;
   int rc;
   if ((rc = forces_reserve(n, npairs))) return rc;
-  zero_async(ZeroRegions{{g_fs.incid.as<unsigned>(), g_fs.fill.as<unsigned>(), nullptr, nullptr},
-                         {n + 1, n + 1, 0, 0}},
+  zero_async(ZeroRegions{{g_fs.incid.as<unsigned>(), g_fs.fill.as<unsigned>(),
+                          g_fs.wide.as<unsigned>(), nullptr},
+                         {n + 1, n + 1, 1, 0}},
              st);
   const BodiesDev b = to_dev(bodies);
   const ForceConsts k{fp->spring, fp->damper, fp->friction};
@@ -340,10 +375,13 @@ int contacts_forces_impl(const dh_bodies* bodies, const int* pairs_dev, int npai
   if (n > 0) {
     hipLaunchKernelGGL(k_particle_sum, grid_for(n, 256), 256, 0, st, n, pairs_dev,
                        g_fs.start.as<long long>(), g_fs.incid.as<int>(), g_fs.entries.as<int>(),
-                       g_fs.pf.as<double>(), force_dev, torque_dev, touched_dev);
-    hipLaunchKernelGGL(k_particle_sum_wide, n, 256, 0, st, n, pairs_dev,
-                       g_fs.start.as<long long>(), g_fs.incid.as<int>(), g_fs.entries.as<int>(),
-                       g_fs.pf.as<double>(), force_dev, torque_dev, touched_dev);
+                       g_fs.pf.as<double>(), force_dev, torque_dev, touched_dev,
+                       g_fs.wide.as<int>());
+    // a workgroup per listed particle, at most 1,024 workgroups
+    hipLaunchKernelGGL(k_particle_sum_wide, static_cast<unsigned>(n < 1024 ? n : 1024), 256, 0,
+                       st, g_fs.wide.as<int>(), pairs_dev, g_fs.start.as<long long>(),
+                       g_fs.incid.as<int>(), g_fs.entries.as<int>(), g_fs.pf.as<double>(),
+                       force_dev, torque_dev, touched_dev);
     DH_HIP_CHECK(hipGetLastError());
   }
   return DH_OK;
@@ -357,9 +395,9 @@ int integrate_impl(const dh_bodies* bodies, const double* force_dev, const doubl
                    const int* tri_offset_dev, hipStream_t st) {
   if (bodies->n == 0) return DH_OK;
   Gravity g{{gravity[0], gravity[1], gravity[2]}};
-  hipLaunchKernelGGL(k_integrate, bodies->n, 64, 0, st, to_dev(bodies), force_dev, torque_dev,
-                     touched_dev, dt, dt_dev, g, x_dev, y_dev, z_dev, refx_dev, refy_dev, refz_dev,
-                     tri_offset_dev);
+  hipLaunchKernelGGL(k_integrate, bodies->n, 64, 0, st, to_dev(bodies), force_dev,
+                     torque_dev, touched_dev, dt, dt_dev, g, x_dev, y_dev, z_dev, refx_dev,
+                     refy_dev, refz_dev, tri_offset_dev);
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
 }
