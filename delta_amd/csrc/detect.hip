@@ -1182,10 +1182,21 @@ __global__ __launch_bounds__(256) void k_gjk(DetectArgs g) {
 }
 
 // exclusive scan of per-pair counts (two levels, 1024 per block)
+// live (may be null): the scan covers the first min(n, *live) entries only.
+__device__ __forceinline__ int scan_len(int n, const unsigned long long* live) {
+  return live && *live < static_cast<unsigned long long>(n) ? static_cast<int>(*live) : n;
+}
+
 __global__ __launch_bounds__(1024) void k_scan_blocks(const int* __restrict__ in, int n,
                                                       long long* __restrict__ out,
-                                                      long long* __restrict__ block_sums) {
+                                                      long long* __restrict__ block_sums,
+                                                      const unsigned long long* __restrict__ live) {
   __shared__ long long buf[1024];
+  n = scan_len(n, live);
+  if (static_cast<long long>(blockIdx.x) * 1024 >= n) {  // past the live entries
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = 0;
+    return;
+  }
   const int i = blockIdx.x * 1024 + threadIdx.x;
   const long long v = (i < n) ? in[i] : 0;
   buf[threadIdx.x] = v;
@@ -1203,8 +1214,10 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const int* __restrict__ in
 // The whole scan in one workgroup when n <= 1,024 (the small scenes' pair
 // and particle counts): offs[0..n) exclusive, offs[n] the total.
 __global__ __launch_bounds__(1024) void k_scan_single(const int* __restrict__ in, int n,
-                                                      long long* __restrict__ out) {
+                                                      long long* __restrict__ out,
+                                                      const unsigned long long* __restrict__ live) {
   __shared__ long long buf[1024];
+  n = scan_len(n, live);
   const int i = threadIdx.x;
   const long long v = i < n ? in[i] : 0;
   buf[i] = v;
@@ -1249,7 +1262,9 @@ __global__ __launch_bounds__(1024) void k_scan_sums(long long* __restrict__ sums
 // out[n] (out[0] = 0 when n = 0).
 __global__ __launch_bounds__(1024) void k_scan_add(long long* __restrict__ out, int n,
                                                    const long long* __restrict__ sums,
-                                                   const int* __restrict__ in) {
+                                                   const int* __restrict__ in,
+                                                   const unsigned long long* __restrict__ live) {
+  n = scan_len(n, live);
   const int i = blockIdx.x * 1024 + threadIdx.x;
   if (i < n) {
     const long long o = out[i] + sums[blockIdx.x];
@@ -1290,20 +1305,11 @@ __global__ __launch_bounds__(256) void k_scatter(const dh_contact* __restrict__ 
   out[dst] = c;
 }
 
-// Survivor queue, pass 2: one lane per queued triangle pair (bfSolver, the
-// contact test of bf.cpp:125); flags[e] = 1 and P, Q kept when it emits.
-// `n` lanes; with need_dev (a captured step) the live entries are
-// min(*need_dev, n) and the lanes past them clear their flags.
 template <bool SCREEN>
-__global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
-                                                     const unsigned long long* __restrict__ need_dev,
-                                                     int* __restrict__ flags,
-                                                     double* __restrict__ pq) {
-  const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (e >= n) return;
-  const long long live =
-      need_dev && static_cast<long long>(*need_dev) < n ? static_cast<long long>(*need_dev) : n;
-  const int2 en = e < live ? g.queue[e] : make_int2(-1, 0);
+__device__ __forceinline__ void queue_solve_entry(const DetectArgs& g, long long e,
+                                                  int* __restrict__ flags,
+                                                  double* __restrict__ pq) {
+  const int2 en = g.queue[e];
   if (en.x < 0) {
     flags[e] = 0;
     return;
@@ -1330,23 +1336,45 @@ __global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
   }
 }
 
+
+// Survivor queue, pass 2: a lane per queued triangle pair at a time
+// (bfSolver, the contact test of bf.cpp:125); flags[e] = 1 and P, Q kept when
+// it emits.  The entries are the first n, or, with need_dev (a captured step
+// launched for the capacity n), the first min(*need_dev, n); the later passes
+// stop at the same count.
+template <bool SCREEN>
+__global__ __launch_bounds__(256) void k_queue_solve(DetectArgs g, long long n,
+                                                     const unsigned long long* __restrict__ need_dev,
+                                                     int* __restrict__ flags,
+                                                     double* __restrict__ pq) {
+  // the live entries (flags past them are not written: the scan stops there)
+  const long long live =
+      need_dev && static_cast<long long>(*need_dev) < n ? static_cast<long long>(*need_dev) : n;
+  for (long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; e < live;
+       e += static_cast<long long>(gridDim.x) * 256)
+    queue_solve_entry<SCREEN>(g, e, flags, pq);
+}
+
 // Pass 3: the queue's contacts take one range of the contact slots.
-__global__ void k_queue_reserve(DetectArgs g, const long long* __restrict__ pos, long long live,
+__device__ __forceinline__ long long queue_live(long long n,
+                                                const unsigned long long* need_dev) {
+  return need_dev && static_cast<long long>(*need_dev) < n ? static_cast<long long>(*need_dev) : n;
+}
+
+__global__ void k_queue_reserve(DetectArgs g, const long long* __restrict__ pos, long long n,
+                                const unsigned long long* __restrict__ need_dev,
                                 unsigned long long* __restrict__ base) {
   if (threadIdx.x == 0 && blockIdx.x == 0)
-    *base = atomicAdd(g.counter, static_cast<unsigned long long>(pos[live]));
+    *base = atomicAdd(g.counter, static_cast<unsigned long long>(pos[queue_live(n, need_dev)]));
 }
 
 // Pass 4: contacts of the emitting entries, tagged (item, rank within the
 // item): the item's entries are contiguous and in row-major order, so the
 // rank is the number of emitting entries before it in the item's range.
-__global__ __launch_bounds__(256) void k_queue_emit(DetectArgs g, long long live,
-                                                    const int* __restrict__ flags,
-                                                    const long long* __restrict__ pos,
-                                                    const double* __restrict__ pq,
-                                                    const unsigned long long* __restrict__ base) {
-  const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (e >= live || !flags[e]) return;
+__device__ __forceinline__ void queue_emit_entry(const DetectArgs& g, long long e,
+                                                 const long long* __restrict__ pos,
+                                                 const double* __restrict__ pq,
+                                                 const unsigned long long* __restrict__ base) {
   const int2 en = g.queue[e];
   const int item = en.x;
   const int k = item / g.tiles;
@@ -1364,6 +1392,18 @@ __global__ __launch_bounds__(256) void k_queue_emit(DetectArgs g, long long live
     g.temp[slot] = c;
   }
   atomicAdd(&g.tile_count[item], 1);
+}
+
+__global__ __launch_bounds__(256) void k_queue_emit(DetectArgs g, long long n,
+                                                    const unsigned long long* __restrict__ need_dev,
+                                                    const int* __restrict__ flags,
+                                                    const long long* __restrict__ pos,
+                                                    const double* __restrict__ pq,
+                                                    const unsigned long long* __restrict__ base) {
+  const long long live = queue_live(n, need_dev);
+  for (long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; e < live;
+       e += static_cast<long long>(gridDim.x) * 256)
+    if (flags[e]) queue_emit_entry(g, e, pos, pq, base);
 }
 
 int queue_learned(const int2* queue, const int* pairs, int tiles, const int* tri_offset,
@@ -1385,21 +1425,28 @@ int scan_reserve(int n) {
   return g_scratch.sums.grow(sizeof(long long) * (nb + 1));
 }
 
-// offs[0..n]: exclusive prefix sums of counts, offs[n] = total; `sums`
-// holds scan_sums_len(n) block sums (unused when n <= 1,024).
-int scan_counts_with(const int* counts, int n, long long* offs, long long* sums,
-                     hipStream_t st) {
+// Exclusive scan of counts[0..m) into offs[0..m), offs[m] the total, where
+// m = n, or min(n, *live) when live is given (read when the kernels run);
+// `sums` holds scan_sums_len(n) block sums (unused when n <= 1,024).
+static int scan_counts_upto(const int* counts, int n, const unsigned long long* live,
+                            long long* offs, long long* sums, hipStream_t st) {
   if (n <= 1024) {
-    hipLaunchKernelGGL(k_scan_single, 1, 1024, 0, st, counts, n, offs);
+    hipLaunchKernelGGL(k_scan_single, 1, 1024, 0, st, counts, n, offs, live);
     DH_HIP_CHECK(hipGetLastError());
     return DH_OK;
   }
   const int nb = (n + 1023) / 1024;
-  hipLaunchKernelGGL(k_scan_blocks, nb, 1024, 0, st, counts, n, offs, sums);
+  hipLaunchKernelGGL(k_scan_blocks, nb, 1024, 0, st, counts, n, offs, sums, live);
   hipLaunchKernelGGL(k_scan_sums, 1, 1024, 0, st, sums, nb);
-  hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, sums, counts);
+  hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, sums, counts, live);
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
+}
+
+// offs[0..n]: exclusive prefix sums of counts, offs[n] = total.
+int scan_counts_with(const int* counts, int n, long long* offs, long long* sums,
+                     hipStream_t st) {
+  return scan_counts_upto(counts, n, nullptr, offs, sums, st);
 }
 
 int scan_sums_len(int n) { return (n + 1023) / 1024 > 0 ? (n + 1023) / 1024 + 1 : 2; }
@@ -1478,7 +1525,12 @@ static int queue_reserve(long long n) {
 // min(*need_dev, n) in a captured step).
 static int queue_passes(const DetectArgs& a, long long n, const unsigned long long* need_dev,
                         bool learned, hipStream_t st) {
-  const unsigned grid = grid_for(n, 256);
+  // a captured step's passes are launched for the capacity: a persistent grid
+  // strides over the live entries
+  long long g = grid_for(n, 256);
+  const long long gcap = static_cast<long long>(device_cus()) * 16;
+  if (need_dev && g > gcap) g = gcap;
+  const unsigned grid = static_cast<unsigned>(g > 0 ? g : 1);
   int rc;
   if (learned) {
     if ((rc = queue_learned(a.queue, a.pairs, a.tiles, a.tri_offset, a.x, a.y, a.z, a.eps, n,
@@ -1491,12 +1543,14 @@ static int queue_passes(const DetectArgs& a, long long n, const unsigned long lo
   else
     hipLaunchKernelGGL(k_queue_solve<false>, grid, 256, 0, st, a, n, need_dev,
                        g_scratch.qflags.as<int>(), g_scratch.qpq.as<double>());
-  if ((rc = scan_counts(g_scratch.qflags.as<int>(), static_cast<int>(n),
-                        g_scratch.qpos.as<long long>(), st)))
+  const int ni = static_cast<int>(n);
+  if (ni > 1024 && (rc = g_scratch.sums.grow(sizeof(long long) * scan_sums_len(ni)))) return rc;
+  if ((rc = scan_counts_upto(g_scratch.qflags.as<int>(), static_cast<int>(n), need_dev,
+                             g_scratch.qpos.as<long long>(), g_scratch.sums.as<long long>(), st)))
     return rc;
-  hipLaunchKernelGGL(k_queue_reserve, 1, 64, 0, st, a, g_scratch.qpos.as<long long>(), n,
+  hipLaunchKernelGGL(k_queue_reserve, 1, 64, 0, st, a, g_scratch.qpos.as<long long>(), n, need_dev,
                      g_scratch.qres.as<unsigned long long>());
-  hipLaunchKernelGGL(k_queue_emit, grid, 256, 0, st, a, n, g_scratch.qflags.as<int>(),
+  hipLaunchKernelGGL(k_queue_emit, grid, 256, 0, st, a, n, need_dev, g_scratch.qflags.as<int>(),
                      g_scratch.qpos.as<long long>(), g_scratch.qpq.as<double>(),
                      g_scratch.qres.as<unsigned long long>());
   DH_HIP_CHECK(hipGetLastError());
@@ -1786,8 +1840,11 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
     f.queue = nullptr;
     f.item_list = a.deferred;
     f.nitems_dev = a.ndeferred;
-    hipLaunchKernelGGL((k_detect<DH_BF, false>), detect_grid(items), 256, 0, st, f,
-                       static_cast<int>(items));
+    // persistent over the deferred items (usually none): a grid for the
+    // capacity would cost a dispatch per item
+    const unsigned dcap = static_cast<unsigned>(device_cus()) * 4;
+    const unsigned dgrid = detect_grid(items) < dcap ? detect_grid(items) : dcap;
+    hipLaunchKernelGGL((k_detect<DH_BF, false>), dgrid, 256, 0, st, f, static_cast<int>(items));
     if ((rc = queue_passes(a, a.qcap, a.qcounter, learned, st))) return rc;
   }
   if (pair_cap <= kTailPairs) {
