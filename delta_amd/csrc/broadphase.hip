@@ -75,32 +75,53 @@ __global__ __launch_bounds__(64) void k_particle_aabb(const double* __restrict__
         phi[q] = fmax(phi[q], __shfl_xor(phi[q], o));
       }
   } else {
-    for (int b0 = t0; b0 < t1; b0 += 64) {
-      const int b1 = b0 + 64 < t1 ? b0 + 64 : t1;
-      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      for (int v = 3 * b0 + lane; v < 3 * b1; v += 64) {
-        const double c[3] = {x[v], y[v], z[v]};
-        for (int q = 0; q < 3; ++q) {
-          lo[q] = fmin(lo[q], c[q]);
-          hi[q] = fmax(hi[q], c[q]);
+    // kChunkGroup chunks at a time, their loads issued together
+    constexpr int kChunkGroup = 4;
+    for (int g0 = t0; g0 < t1; g0 += 64 * kChunkGroup) {
+      double lo[kChunkGroup][3], hi[kChunkGroup][3];
+#pragma unroll
+      for (int c = 0; c < kChunkGroup; ++c) {
+        const int b0 = g0 + 64 * c, b1 = b0 + 64 < t1 ? b0 + 64 : t1;
+        double cx[3], cy[3], cz[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int v = 3 * b0 + 64 * k + lane;
+          const bool in = v < 3 * b1;
+          cx[k] = in ? x[v] : NAN;
+          cy[k] = in ? y[v] : NAN;
+          cz[k] = in ? z[v] : NAN;
         }
+        // from +-inf, as the box of no (or only NaN) coordinates
+        lo[c][0] = fmin(fmin(fmin(INFINITY, cx[0]), cx[1]), cx[2]);
+        lo[c][1] = fmin(fmin(fmin(INFINITY, cy[0]), cy[1]), cy[2]);
+        lo[c][2] = fmin(fmin(fmin(INFINITY, cz[0]), cz[1]), cz[2]);
+        hi[c][0] = fmax(fmax(fmax(-INFINITY, cx[0]), cx[1]), cx[2]);
+        hi[c][1] = fmax(fmax(fmax(-INFINITY, cy[0]), cy[1]), cy[2]);
+        hi[c][2] = fmax(fmax(fmax(-INFINITY, cz[0]), cz[1]), cz[2]);
       }
       for (int o = 32; o > 0; o >>= 1)
-        for (int q = 0; q < 3; ++q) {
-          lo[q] = fmin(lo[q], __shfl_xor(lo[q], o));
-          hi[q] = fmax(hi[q], __shfl_xor(hi[q], o));
+#pragma unroll
+        for (int c = 0; c < kChunkGroup; ++c)
+          for (int q = 0; q < 3; ++q) {
+            lo[c][q] = fmin(lo[c][q], __shfl_xor(lo[c][q], o));
+            hi[c][q] = fmax(hi[c][q], __shfl_xor(hi[c][q], o));
+          }
+#pragma unroll
+      for (int c = 0; c < kChunkGroup; ++c) {
+        const int b0 = g0 + 64 * c;
+        if (b0 >= t1) break;
+        if (lane == 0) {
+          Aabb bx;
+          for (int q = 0; q < 3; ++q) {
+            bx.lo[q] = lo[c][q];
+            bx.hi[q] = hi[c][q];
+          }
+          cbox[t0 / 64 + i + (b0 - t0) / 64] = bx;
         }
-      if (lane == 0) {
-        Aabb bx;
         for (int q = 0; q < 3; ++q) {
-          bx.lo[q] = lo[q];
-          bx.hi[q] = hi[q];
+          plo[q] = fmin(plo[q], lo[c][q]);
+          phi[q] = fmax(phi[q], hi[c][q]);
         }
-        cbox[t0 / 64 + i + (b0 - t0) / 64] = bx;
-      }
-      for (int q = 0; q < 3; ++q) {
-        plo[q] = fmin(plo[q], lo[q]);
-        phi[q] = fmax(phi[q], hi[q]);
       }
     }
   }

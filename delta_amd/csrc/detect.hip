@@ -31,6 +31,7 @@ constexpr int kMaxRowTriangles = 3333;
 // the most triangles per particle whose boxes are staged in LDS.
 constexpr int kSurvivors = 1024;
 constexpr int kBoxTriangles = kRowBlock;
+constexpr int kChunkFlags = 1024;  // chunks of 64 rows an item tests up front
 
 enum DetectError : int { DE_NONE = 0, DE_REF_UB = 1, DE_TOO_LONG = 2 };
 
@@ -413,6 +414,7 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
     __shared__ double red[4][12];
     __shared__ int nact[2];
     __shared__ unsigned char row_far[kBoxTriangles];  // row in a chunk apart from B
+    __shared__ unsigned char chunk_near[kChunkFlags];  // the item's chunks against B's box
     const bool staged = p.nB <= kBoxTriangles;  // block-uniform
     const int blk = staged ? kBoxTriangles : (nrows > 0 ? nrows : 1);
     bool b_staged = false;  // B's boxes are loaded on the first block that needs them
@@ -440,18 +442,39 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
       far_pair = dx * dx + dy * dy + dz * dz > reach * reach;
       if (far_pair) culled += tid == 0 ? static_cast<unsigned>(nrows * p.nB) : 0u;
     }
+    // A in chunks of 64 rows with their boxes (a wall or a funnel): every
+    // chunk of the item tested against B's particle box once, up front
+    const bool chunked = staged && g.bbox && g.pbox && p.nA > kRowBlock;
+    const double* pbB = g.pbox ? g.pbox + 6 * static_cast<long long>(p.pb) : nullptr;
+    // row r of A (particle-relative) lies in a chunk apart from B's box
+    auto chunk_far = [&](int r) {
+      const double* cb = g.bbox + 6 * (static_cast<long long>(p.oA / 64) + p.pa + r / 64);
+      return boxes_apart(cb, cb + 3, pbB, pbB + 3, p.margin);
+    };
+    const int c_first = r0 / 64;
+    const bool flagged = chunked && nrows > 0 && (r0 + nrows - 1) / 64 - c_first < kChunkFlags;
+    if (flagged && !far_pair) {
+      for (int c = tid; c <= (r0 + nrows - 1) / 64 - c_first; c += 256)
+        chunk_near[c] = chunk_far(64 * (c_first + c)) ? 0 : 1;
+      __syncthreads();
+    }
+    auto row_chunk_far = [&](int r) {
+      return flagged ? chunk_near[r / 64 - c_first] == 0 : chunk_far(r);
+    };
     for (int rb = 0; !far_pair && rb < nrows; rb += blk) {
       const int rs = r0 + rb;  // first row of the block
       const int nrb = nrows - rb < blk ? nrows - rb : blk;
       int nra = nrb, nca = p.nB;
-      const bool chunked = staged && g.bbox && g.pbox && p.nA > kRowBlock;
-      const double* pbB = g.pbox ? g.pbox + 6 * static_cast<long long>(p.pb) : nullptr;
-      // row r of A (particle-relative) lies in a chunk apart from B's box
-      auto chunk_far = [&](int r) {
-        const double* cb = g.bbox + 6 * (static_cast<long long>(p.oA / 64) + p.pa + r / 64);
-        return boxes_apart(cb, cb + 3, pbB, pbB + 3, p.margin);
-      };
-      if (chunked) {
+      if (flagged) {
+        // the same flags in every thread: no barrier
+        bool any_near = false;
+        for (int c = rs / 64; c <= (rs + nrb - 1) / 64; ++c)
+          any_near = any_near || chunk_near[c - c_first] != 0;
+        if (!any_near) {  // every chunk of the block is apart
+          culled += tid == 0 ? static_cast<unsigned>(nrb * p.nB) : 0u;
+          continue;
+        }
+      } else if (chunked) {
         bool any_near = false;
         for (int c = rs / 64 + tid; c <= (rs + nrb - 1) / 64; c += 256)
           any_near = any_near || !chunk_far(64 * c);
@@ -463,7 +486,7 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
       if (staged) {
         bool near = false;
         for (int t = tid; t < nrb; t += 256) {
-          const bool far = chunked && chunk_far(rs + t);
+          const bool far = chunked && row_chunk_far(rs + t);
           row_far[t] = far ? 1 : 0;
           if (far) continue;  // its box is never read
           double* b = boxes + 6 * t;
