@@ -266,6 +266,8 @@ struct Gravity {
   double g[3];
 };
 
+constexpr int kIntegrateBatch = 8;  // vertex replicas per lane per pass of k_integrate
+
 __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __restrict__ force,
                                                   const double* __restrict__ torque,
                                                   const int* __restrict__ touched, double h_arg,
@@ -318,8 +320,23 @@ __global__ __launch_bounds__(64) void k_integrate(BodiesDev b, const double* __r
     for (int q = 0; q < 9; ++q) b.rot[9 * i + q] = R[q];
   }
   const int v0 = 3 * tri_offset[i], v1 = 3 * tri_offset[i + 1];
-  for (int j = v0 + lane; j < v1; j += 64)
-    update_vertex(x[j], y[j], z[j], rx[j], ry[j], rz[j], R, pos, refpos);
+  // kIntegrateBatch replicas per lane at a time, their loads issued together
+  for (int j0 = v0 + lane; j0 < v1; j0 += 64 * kIntegrateBatch) {
+    double hx[kIntegrateBatch], hy[kIntegrateBatch], hz[kIntegrateBatch];
+#pragma unroll
+    for (int k = 0; k < kIntegrateBatch; ++k) {
+      const int j = j0 + 64 * k;
+      const bool in = j < v1;
+      hx[k] = in ? rx[j] : 0.0;
+      hy[k] = in ? ry[j] : 0.0;
+      hz[k] = in ? rz[j] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kIntegrateBatch; ++k) {
+      const int j = j0 + 64 * k;
+      if (j < v1) update_vertex(x[j], y[j], z[j], hx[k], hy[k], hz[k], R, pos, refpos);
+    }
+  }
 }
 
 
