@@ -1298,6 +1298,32 @@ __global__ __launch_bounds__(1024) void k_scan_add(long long* __restrict__ out, 
   }
 }
 
+// k_scan_add for at most 1,024 blocks, without the block-sum scan: each
+// block first adds up the raw sums of the blocks before it.
+__global__ __launch_bounds__(1024) void k_scan_add_prefix(long long* __restrict__ out, int n,
+                                                          const long long* __restrict__ sums,
+                                                          const int* __restrict__ in,
+                                                          const unsigned long long* __restrict__ live) {
+  __shared__ long long part[16];
+  n = scan_len(n, live);
+  const int b = blockIdx.x, tid = threadIdx.x;
+  if (static_cast<long long>(b) * 1024 >= n && !(n == 0 && b == 0)) return;
+  long long v = tid < b ? sums[tid] : 0;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((tid & 63) == 0) part[tid >> 6] = v;
+  __syncthreads();
+  long long base = 0;
+  for (int w = 0; w < 16; ++w) base += part[w];
+  const int i = b * 1024 + tid;
+  if (i < n) {
+    const long long o = out[i] + base;
+    out[i] = o;
+    if (i == n - 1) out[n] = o + in[i];
+  } else if (n == 0 && i == 0) {
+    out[0] = 0;
+  }
+}
+
 // Per-pair contact counts from the per-tile counts.
 __global__ __launch_bounds__(256) void k_tile_sums(const int* __restrict__ tile_count, int tiles,
                                                    int npairs, int* __restrict__ counts) {
@@ -1460,8 +1486,12 @@ static int scan_counts_upto(const int* counts, int n, const unsigned long long* 
   }
   const int nb = (n + 1023) / 1024;
   hipLaunchKernelGGL(k_scan_blocks, nb, 1024, 0, st, counts, n, offs, sums, live);
-  hipLaunchKernelGGL(k_scan_sums, 1, 1024, 0, st, sums, nb);
-  hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, sums, counts, live);
+  if (nb <= 1024) {  // two launches: the add pass sums the block sums before it
+    hipLaunchKernelGGL(k_scan_add_prefix, nb, 1024, 0, st, offs, n, sums, counts, live);
+  } else {
+    hipLaunchKernelGGL(k_scan_sums, 1, 1024, 0, st, sums, nb);
+    hipLaunchKernelGGL(k_scan_add, nb, 1024, 0, st, offs, n, sums, counts, live);
+  }
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
 }
