@@ -119,6 +119,9 @@ struct DetectArgs {
   const unsigned* nitems_dev;  // with item_list: its length (else the launch's)
   int fp32_screen;             // k_queue_solve screens in binary32 first
   int wave_items;              // small items go to k_detect_wave (mid-phase only)
+  // item_skip[item] != 0: k_chunk_cull culled the item and wrote its
+  // counters (a launch of DH_BF without the survivor queue skips it)
+  const unsigned char* item_skip;
 };
 
 // Items of a mid-phase-only launch small enough for one wave each.
@@ -279,6 +282,7 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   unsigned long long* ts = g.tile_stats + static_cast<long long>(item) * kTileStats;
+  if (M == DH_BF && !QONLY && g.item_skip && g.item_skip[item]) return;  // k_chunk_cull's
   if (g.npairs_dev && k >= *g.npairs_dev) {
     if (tid == 0) g.tile_count[item] = 0;
     if (tid < kTileStats) ts[tid] = 0;
@@ -811,6 +815,76 @@ __global__ __launch_bounds__(256) void k_detect_big(DetectArgs g, int items) {
   for (int j = blockIdx.x; j < nb; j += gridDim.x) {
     detect_item<DH_BF, true>(g, g.biglist[j]);
     __syncthreads();
+  }
+}
+
+// One wave per item, before a DH_BF launch without the survivor queue, on
+// scenes with particles of more than kRowBlock triangles (walls, funnels):
+// an item whose A is cut in chunks of 64 rows is culled when no row of a
+// chunk near B's particle box is itself near it, with the counters
+// detect_item would write (its blocks would all be skipped).  The test is
+// detect_item's chunk and row tests, without its workgroup barriers.
+// item_skip[item] = 1 marks the culled and dead items.
+__global__ __launch_bounds__(256) void k_chunk_cull(DetectArgs g, int items,
+                                                    unsigned char* __restrict__ skip) {
+  const int lane = threadIdx.x & 63;
+  const int nw = static_cast<int>(gridDim.x) * 4;
+  for (int item = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+       item < items; item += nw) {
+    const int k = item / g.tiles, tile = item - k * g.tiles;
+    unsigned long long* ts = g.tile_stats + static_cast<long long>(item) * kTileStats;
+    if (g.npairs_dev && k >= *g.npairs_dev) {
+      if (lane == 0) {
+        g.tile_count[item] = 0;
+        skip[item] = 1;
+      }
+      if (lane < kTileStats) ts[lane] = 0;
+      continue;
+    }
+    const Pair p = load_pair(g, k);
+    const int r0 = static_cast<int>(static_cast<long long>(tile) * p.nA / g.tiles);
+    const int r1 = static_cast<int>(static_cast<long long>(tile + 1) * p.nA / g.tiles);
+    const bool chunked = p.nB <= kBoxTriangles && g.bbox && g.pbox && p.nA > kRowBlock;
+    bool near = !(chunked && r1 > r0);
+    if (!near) {
+      const double* pbB = g.pbox + 6 * static_cast<long long>(p.pb);
+      const int c_last = (r1 - 1) / 64;
+      for (int c0 = r0 / 64; c0 <= c_last && !near; c0 += 64) {
+        const int c = c0 + lane;
+        bool cn = false;
+        if (c <= c_last) {
+          const double* cb = g.bbox + 6 * (static_cast<long long>(p.oA / 64) + p.pa + c);
+          cn = !boxes_apart(cb, cb + 3, pbB, pbB + 3, p.margin);
+        }
+        unsigned long long m = __ballot(cn);
+        while (m && !near) {
+          const int cc = c0 + __builtin_ctzll(m);
+          m &= m - 1;
+          const int r = 64 * cc + lane;
+          bool rn = false;
+          if (r >= r0 && r < r1) {
+            double lo[3], hi[3];
+            tri_box(load_tri(g, p.oA + r), lo, hi);
+            rn = !boxes_apart(lo, hi, pbB, pbB + 3, p.margin);
+          }
+          near = __ballot(rn) != 0;
+        }
+      }
+    }
+    if (lane == 0) {
+      skip[item] = near ? 0 : 1;
+      if (!near) {
+        const unsigned long long total = static_cast<unsigned long long>(r1 - r0) * p.nB;
+        g.tile_count[item] = 0;
+        ts[TS_CHECKS] = tile == 0 ? static_cast<unsigned long long>(p.nA) * p.nB : 0ull;
+        ts[TS_EVALUATED] = total;
+        ts[TS_FALLBACK_ROWS] = 0;
+        ts[TS_PENALTY_FAILS] = 0;
+        ts[TS_BATCH_FAILS] = 0;
+        ts[TS_BATCH_SIZE] = 0;
+        ts[TS_CULLED] = total;
+      }
+    }
   }
 }
 
@@ -1463,10 +1537,55 @@ int queue_learned_reserve(long long n);
 
 struct DetectScratch {
   DevBuf temp, counter, stats, error, counts, offs, sums, tiles, tstats;
-  DevBuf queue, qcounter, qbase, qflags, qpos, qpq, qres, deferred, ndeferred;
+  DevBuf queue, qcounter, qbase, qflags, qpos, qpq, qres, deferred, ndeferred, iskip, tmax;
   long long qcap = 0;  // queue entries of the next eager DH_BF call
 };
 static DetectScratch g_scratch;
+
+// The largest triangle count of a particle, per geometry (tri_offset, n,
+// ntri): k_chunk_cull runs only on scenes with particles of more than
+// kRowBlock triangles.  A stale entry changes the speed, never the result
+// (the pre-pass is exact, and without it detect_item does the same tests).
+struct TriMax {
+  const int* tri_offset = nullptr;
+  int n = -1, ntri = -1, max_tris = 0;
+};
+static TriMax g_trimax;
+
+__global__ __launch_bounds__(256) void k_max_tris(const int* __restrict__ off, int n,
+                                                  int* __restrict__ out) {
+  int m = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    m = max(m, off[i + 1] - off[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
+
+// With sync, computes and caches a geometry not cached (one read-back);
+// without (inside a capture), answers 0 for it.
+static int particle_max_tris(const dh_geometry* g, hipStream_t st, bool sync) {
+  if (g_trimax.tri_offset == g->tri_offset && g_trimax.n == g->nparticles &&
+      g_trimax.ntri == g->ntri_total)
+    return g_trimax.max_tris;
+  if (!sync || g->nparticles <= 0) return 0;
+  if (g_scratch.tmax.grow(sizeof(int))) return 0;
+  int m = 0;
+  if (hipMemsetAsync(g_scratch.tmax.ptr, 0, sizeof(int), st) != hipSuccess) return 0;
+  const unsigned grid = grid_for(g->nparticles, 256) < 1024 ? grid_for(g->nparticles, 256) : 1024;
+  hipLaunchKernelGGL(k_max_tris, grid, 256, 0, st, g->tri_offset, g->nparticles,
+                     g_scratch.tmax.as<int>());
+  if (hipMemcpyAsync(&m, g_scratch.tmax.ptr, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return 0;
+  g_trimax = {g->tri_offset, g->nparticles, g->ntri_total, m};
+  return m;
+}
+
+// Before a capture of a step on this geometry: fills the cache above.
+int detect_prepare(const dh_geometry* geom, hipStream_t st) {
+  particle_max_tris(geom, st, true);
+  return DH_OK;
+}
 
 // Scratch of scan_counts for up to n entries (before a capture).
 int scan_reserve(int n) {
@@ -1538,12 +1657,20 @@ static void launch_detect(unsigned items, size_t shm, hipStream_t st, const Dete
     const long long cap = static_cast<long long>(device_cus()) * 32;
     hipLaunchKernelGGL(k_detect_big, static_cast<unsigned>(wg < cap ? (wg > 0 ? wg : 1) : cap),
                        256, shm, st, a, static_cast<int>(items));
-  } else if (M == DH_BF && a.queue)
+  } else if (M == DH_BF && a.queue) {
     hipLaunchKernelGGL((k_detect<DH_BF, true>), detect_grid(items), 256, shm, st, a,
                        static_cast<int>(items));
-  else
+  } else {
+    if (M == DH_BF && a.item_skip) {
+      const long long wg = (static_cast<long long>(items) + 3) / 4;
+      const long long cap = static_cast<long long>(device_cus()) * 32;
+      hipLaunchKernelGGL(k_chunk_cull, static_cast<unsigned>(wg < cap ? (wg > 0 ? wg : 1) : cap),
+                         256, 0, st, a, static_cast<int>(items),
+                         const_cast<unsigned char*>(a.item_skip));
+    }
     hipLaunchKernelGGL((k_detect<M, false>), detect_grid(items), 256, shm, st, a,
                        static_cast<int>(items));
+  }
 }
 
 static void launch_tile_stats(const unsigned long long* ts, long long ntiles,
@@ -1690,6 +1817,7 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   a.psph = pbox ? psph : nullptr;
   a.fp32_screen = tuning().detect_fp32_screen;
   a.wave_items = 0;
+  a.item_skip = nullptr;
   a.bbox = bbox;
   a.queue = nullptr;
   a.qcap = 0;
@@ -1731,6 +1859,11 @@ int detect_pairs_impl(int method, const dh_geometry* geom, const int* pairs, int
   }
   const size_t shm_rows = 2 * kMaxRowTriangles * sizeof(int) + 8 + kMaxRowTriangles * 8;
   const unsigned items = static_cast<unsigned>(static_cast<long long>(npairs) * tiles);
+  if (method == DH_BF && !a.queue && pbox && bbox &&
+      particle_max_tris(geom, st, true) > kRowBlock) {
+    if ((rc = g_scratch.iskip.grow(items > 0 ? items : 1))) return rc;
+    a.item_skip = g_scratch.iskip.as<unsigned char>();
+  }
   switch (method) {
     case DH_BF: launch_detect<DH_BF>(items, 0, st, a); break;
     case DH_PENALTY: launch_detect<DH_PENALTY>(items, 0, st, a); break;
@@ -1849,6 +1982,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.psph = pbox ? psph : nullptr;
   a.fp32_screen = tuning().detect_fp32_screen;
   a.wave_items = 0;
+  a.item_skip = nullptr;
   a.bbox = bbox;
   a.queue = nullptr;  // the queue needs the survivor count on the host
   a.qcap = 0;
@@ -1863,6 +1997,10 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
   a.item_list = nullptr;
   a.nitems_dev = nullptr;
   const unsigned items = static_cast<unsigned>(static_cast<long long>(pair_cap) * tiles);
+  // the chunk pre-pass on the geometries detect_prepare found walls in
+  if (method == DH_BF && qcap == 0 && pbox && bbox &&
+      particle_max_tris(geom, st, false) > kRowBlock)
+    a.item_skip = g_scratch.iskip.as<unsigned char>();
   if (qcap > 0) {
     // the survivor queue with its counts kept on the device: every pass is
     // launched for the capacity and reads the live length
@@ -1941,6 +2079,9 @@ int detect_reserve(int pair_cap, long long contact_cap, int method) {
   const int tiles = graph_tiles(pair_cap);
   if (tiles > 1 &&
       (rc = g_scratch.tiles.grow(sizeof(int) * static_cast<size_t>(pair_cap) * tiles)))
+    return rc;
+  if (method == DH_BF &&
+      (rc = g_scratch.iskip.grow(static_cast<size_t>(pair_cap) * tiles + 1)))
     return rc;
   if ((rc = g_scratch.tstats.grow(sizeof(unsigned long long) * kTileStats *
                                   static_cast<size_t>(pair_cap) * tiles)))

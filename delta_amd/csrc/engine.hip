@@ -24,6 +24,7 @@ int detect_pairs_graph(int method, const dh_geometry* geom, const int* pairs, in
                        int* counts, long long* offs, hipStream_t st, const double* pbox,
                        const double* bbox, const double* psph);
 int detect_reserve(int pair_cap, long long contact_cap, int method);
+int detect_prepare(const dh_geometry* geom, hipStream_t st);
 const unsigned long long* detect_counter_dev();
 const int* detect_error_dev();
 const unsigned long long* detect_stats_dev();
@@ -257,6 +258,8 @@ int dh_step_graph_create(const dh_scene* scene, const dh_step_params* params, in
   // library's scratch buffers with the graph
   hipError_t e = hipStreamCreate(&g->stream);
   if (e == hipSuccess) e = hipMemsetAsync(g->status.ptr, 0, sizeof(long long) * 6, g->stream);
+  // what detection learns of the geometry with a read-back, before the capture
+  if (e == hipSuccess) detect_prepare(&scene->geom, g->stream);
   if (e == hipSuccess) e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) {
     destroy_graph(g);

@@ -338,6 +338,47 @@ def test_steps_through_the_survivor_queue(screen):
         tuning.set(**tuning.default())
 
 
+@pytest.mark.parametrize("queue", [0, 1])
+def test_step_graph_matches_dh_step_on_the_funnel(queue):
+    """The funnel (its 2,048-triangle cone is cut in chunks: without the
+    survivor queue its pairs go through the chunk pre-pass, eager and
+    captured) with particles packed against the cone: 3 replays of the
+    captured step against 3 dh_step calls, the counts and every state array
+    bit for bit."""
+    from delta_amd import tuning, workloads
+    hs = workloads.funnel_scene(n=40, pitch=1.3e-3)
+    ds_eager = engine.DeviceScene(hs)
+    ds_graph = engine.DeviceScene(hs)
+    p = engine.step_params(method=_lib.DH_BF, dt=5e-7, spring=1e1, damper=0.1, friction=0.5,
+                           cell=hs.cell_size())
+    tuning.set(detect_queue_min_items=0 if queue else 1 << 30)
+    try:
+        info = engine.step(engine.DeviceScene(hs), p)
+        _lib.check(_lib.lib().dh_synchronize(), "sync")
+        g = engine.StepGraph(ds_graph, p, pair_cap=2 * info.candidate_pairs + 1024,
+                             contact_cap=2 * info.contacts + 4096)
+        total = 0
+        try:
+            for s in range(3):
+                ie = engine.step(ds_eager, p)
+                _lib.check(_lib.lib().dh_synchronize(), "sync")
+                g.launch()
+                ig, overflow = g.status()
+                assert overflow == 0
+                assert (ig.candidate_pairs, ig.contacts, ig.triangle_checks) == \
+                    (ie.candidate_pairs, ie.contacts, ie.triangle_checks), f"step {s}"
+                for f in STATE:
+                    a = ds_eager.download(f).reshape(-1).view(np.uint64)
+                    b = ds_graph.download(f).reshape(-1).view(np.uint64)
+                    assert np.array_equal(a, b), f"step {s}: {f}"
+                total += ie.contacts
+        finally:
+            g.close()
+        assert total > 0
+    finally:
+        tuning.set(**tuning.default())
+
+
 def test_step_graph_matches_dh_step_on_a_hopper():
     """A 12^3 hopper lattice in its wall prisms (walls with over 64
     contact-carrying pairs take the per-particle workgroup sum): 3 replays
