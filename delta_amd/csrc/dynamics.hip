@@ -42,16 +42,13 @@ __device__ __forceinline__ bool past_pairs(int p, int npairs, const long long* n
   return p >= npairs || (np_dev && p >= *np_dev);
 }
 
-__global__ __launch_bounds__(256) void k_pair_forces(BodiesDev b, const int* __restrict__ pairs,
-                                                     int npairs,
-                                                     const dh_contact* __restrict__ cs,
-                                                     const long long* __restrict__ off, ForceConsts k,
-                                                     double* __restrict__ pf,
-                                                     int* __restrict__ incidence,
-                                                     const long long* __restrict__ np_dev,
-                                                     long long ncap) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (past_pairs(p, npairs, np_dev)) return;
+// Pair p's contact forces on both sides into pf, and its incidence counts
+// when it carries contacts.
+__device__ __forceinline__ void pair_forces_one(const BodiesDev& b, const int* __restrict__ pairs,
+                                                int p, const dh_contact* __restrict__ cs,
+                                                const long long* __restrict__ off,
+                                                const ForceConsts& k, double* __restrict__ pf,
+                                                int* __restrict__ incidence, long long ncap) {
   const int a = pairs[2 * p], c = pairs[2 * p + 1];
   double posA[3], angA[3], linA[3], posB[3], angB[3], linB[3];
   load3(b.pos, a, posA);
@@ -86,6 +83,35 @@ __global__ __launch_bounds__(256) void k_pair_forces(BodiesDev b, const int* __r
   }
 }
 
+__global__ __launch_bounds__(256) void k_pair_forces(BodiesDev b, const int* __restrict__ pairs,
+                                                     int npairs,
+                                                     const dh_contact* __restrict__ cs,
+                                                     const long long* __restrict__ off, ForceConsts k,
+                                                     double* __restrict__ pf,
+                                                     int* __restrict__ incidence,
+                                                     const long long* __restrict__ np_dev,
+                                                     long long ncap) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (past_pairs(p, npairs, np_dev)) return;
+  pair_forces_one(b, pairs, p, cs, off, k, pf, incidence, ncap);
+}
+
+// Pair p's two incidence entries, when it carries contacts (the same test as
+// pair_forces_one's count, offsets clamped to ncap).
+__device__ __forceinline__ void fill_one(const int* __restrict__ pairs, int p,
+                                         const long long* __restrict__ off,
+                                         const long long* __restrict__ start,
+                                         int* __restrict__ fill, int* __restrict__ entries,
+                                         long long ncap) {
+  const long long c0 = off[p] < ncap ? off[p] : ncap, c1 = off[p + 1] < ncap ? off[p + 1] : ncap;
+  if (c1 <= c0) return;
+  const int a = pairs[2 * p], c = pairs[2 * p + 1];
+  const int ia = atomicAdd(&fill[a], 1);
+  entries[start[a] + ia] = 2 * p;  // side A
+  const int ic = atomicAdd(&fill[c], 1);
+  entries[start[c] + ic] = 2 * p + 1;  // side B
+}
+
 // CSR of contact-carrying pairs per particle: entries (pair, side) appended in
 // arbitrary order, ordered by partner in k_particle_sum.
 __global__ __launch_bounds__(256) void k_fill_incidence(const int* __restrict__ pairs, int npairs,
@@ -97,14 +123,7 @@ __global__ __launch_bounds__(256) void k_fill_incidence(const int* __restrict__ 
                                                         long long ncap) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (past_pairs(p, npairs, np_dev)) return;
-  // the same test as k_pair_forces' incidence count (offsets clamped to ncap)
-  const long long c0 = off[p] < ncap ? off[p] : ncap, c1 = off[p + 1] < ncap ? off[p + 1] : ncap;
-  if (c1 <= c0) return;
-  const int a = pairs[2 * p], c = pairs[2 * p + 1];
-  const int ia = atomicAdd(&fill[a], 1);
-  entries[start[a] + ia] = 2 * p;  // side A
-  const int ic = atomicAdd(&fill[c], 1);
-  entries[start[c] + ic] = 2 * p + 1;  // side B
+  fill_one(pairs, p, off, start, fill, entries, ncap);
 }
 
 // Particles with more than this many contact-carrying pairs (walls, floors)
@@ -181,12 +200,78 @@ __global__ __launch_bounds__(256) void k_particle_sum(int n, const int* __restri
   if (touched) touched[i] = m > 0 ? 1 : 0;
 }
 
+// Particle i's sum over its m > kNarrowSum entries, by the NT threads of a
+// workgroup: entries sorted by partner in LDS (a bitonic sort of keys
+// partner << 32 | position, so a repeated pair's by position), the entries'
+// force rows staged 256 at a time, and six lanes adding them, one component
+// each, in ascending partner order: the same order and arithmetic as the
+// narrow sum.  Over kWideSort entries, one lane selects them in order.
+template <int NT>
+__device__ void wide_sum_one(int i, const int* __restrict__ pairs,
+                             const long long* __restrict__ start,
+                             const int* __restrict__ count, const int* __restrict__ entries,
+                             const double* __restrict__ pf, double* __restrict__ force,
+                             double* __restrict__ torque, int* __restrict__ touched,
+                             long long* key, double* stage) {
+  const int tid = threadIdx.x;
+  const int m = count[i];
+  const long long s = start[i];
+  if (m > kWideSort) {
+    if (tid == 0) {
+      double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+      sum_by_selection(pairs, entries, pf, s, m, f, t);
+      for (int q = 0; q < 3; ++q) {
+        force[3 * i + q] = f[q];
+        torque[3 * i + q] = t[q];
+      }
+    }
+  } else {
+    int P = 1;
+    while (P < m) P <<= 1;
+    for (int e = tid; e < P; e += NT)
+      key[e] = e < m ? (static_cast<long long>(entry_partner(pairs, entries[s + e])) << 32) | e
+                     : 0x7fffffffffffffffll;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+      for (int h = k >> 1; h > 0; h >>= 1) {
+        for (int e = tid; e < P; e += NT) {
+          const int o = e ^ h;
+          if (o > e) {
+            const long long a = key[e], b = key[o];
+            if ((a > b) == ((e & k) == 0)) {
+              key[e] = b;
+              key[o] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    // lane c < 6 adds component c (force x, y, z, torque x, y, z) of every
+    // row in ascending partner order
+    double acc = 0.0;
+    for (int c0 = 0; c0 < m; c0 += 256) {
+      const int r = c0 + tid;
+      if (tid < 256 && r < m) {
+        const double* o = entry_forces(pf, entries[s + static_cast<int>(key[r] & 0xffffffffll)]);
+        for (int q = 0; q < 6; ++q) stage[6 * tid + q] = o[q];
+      }
+      __syncthreads();
+      if (tid < 6) {
+        const int nr = m - c0 < 256 ? m - c0 : 256;
+#pragma unroll 16
+        for (int k = 0; k < nr; ++k) acc += stage[6 * k + tid];
+      }
+      __syncthreads();
+    }
+    if (tid < 3) force[3 * i + tid] = acc;
+    else if (tid < 6) torque[3 * i + tid - 3] = acc;
+  }
+  if (tid == 0 && touched) touched[i] = 1;
+  __syncthreads();  // the next particle reuses the keys and the staging rows
+}
+
 // The particles with more than kNarrowSum entries (listed by k_particle_sum),
-// one workgroup at a time: entries sorted by partner in LDS (a bitonic sort;
-// a repeated pair's by position), the entries' force rows staged 256 at a
-// time, and six lanes adding
-// them, one component each, in ascending partner order: the same order and
-// arithmetic as the narrow sum.
+// one workgroup at a time.
 __global__ __launch_bounds__(256) void k_particle_sum_wide(const int* __restrict__ wide,
                                                            const int* __restrict__ pairs,
                                                            const long long* __restrict__ start,
@@ -196,70 +281,82 @@ __global__ __launch_bounds__(256) void k_particle_sum_wide(const int* __restrict
                                                            double* __restrict__ force,
                                                            double* __restrict__ torque,
                                                            int* __restrict__ touched) {
-  // (partner << 32 | position) of each entry, sorted ascending
   __shared__ long long key[kWideSort];
   __shared__ double stage[256 * 6];
-  const int tid = threadIdx.x;
   const int nwide = wide[0];
-  for (int j = blockIdx.x; j < nwide; j += gridDim.x) {
-    const int i = wide[1 + j];
-    const int m = count[i];
-    const long long s = start[i];
-    if (m > kWideSort) {
-      if (tid == 0) {
-        double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
-        sum_by_selection(pairs, entries, pf, s, m, f, t);
-        for (int q = 0; q < 3; ++q) {
-          force[3 * i + q] = f[q];
-          torque[3 * i + q] = t[q];
-        }
-      }
-    } else {
-      // bitonic sort of the keys, padded to a power of two with the largest
-      // key: partners ascending, a repeated pair's entries by position
-      int P = 1;
-      while (P < m) P <<= 1;
-      for (int e = tid; e < P; e += 256)
-        key[e] = e < m ? (static_cast<long long>(entry_partner(pairs, entries[s + e])) << 32) | e
-                       : 0x7fffffffffffffffll;
-      __syncthreads();
-      for (int k = 2; k <= P; k <<= 1)
-        for (int h = k >> 1; h > 0; h >>= 1) {
-          for (int e = tid; e < P; e += 256) {
-            const int o = e ^ h;
-            if (o > e) {
-              const long long a = key[e], b = key[o];
-              if ((a > b) == ((e & k) == 0)) {
-                key[e] = b;
-                key[o] = a;
-              }
-            }
-          }
-          __syncthreads();
-        }
-      // lane c < 6 adds component c (force x, y, z, torque x, y, z) of
-      // every row in ascending partner order
-      double acc = 0.0;
-      for (int c0 = 0; c0 < m; c0 += 256) {
-        const int r = c0 + tid;
-        if (r < m) {
-          const double* o = entry_forces(pf, entries[s + static_cast<int>(key[r] & 0xffffffffll)]);
-          for (int q = 0; q < 6; ++q) stage[6 * tid + q] = o[q];
-        }
-        __syncthreads();
-        if (tid < 6) {
-          const int nr = m - c0 < 256 ? m - c0 : 256;
-#pragma unroll 16
-          for (int k = 0; k < nr; ++k) acc += stage[6 * k + tid];
-        }
-        __syncthreads();
-      }
-      if (tid < 3) force[3 * i + tid] = acc;
-      else if (tid < 6) torque[3 * i + tid - 3] = acc;
-    }
-    if (tid == 0 && touched) touched[i] = 1;
-    __syncthreads();  // the next particle reuses the keys
+  for (int j = blockIdx.x; j < nwide; j += gridDim.x)
+    wide_sum_one<256>(wide[1 + j], pairs, start, count, entries, pf, force, torque, touched, key,
+                      stage);
+}
+
+// Particles, pairs and contact capacity for which k_forces_small does the
+// force stage in one workgroup.
+constexpr int kForcesSmallN = 1024;
+
+// The whole force stage of a small scene (at most kForcesSmallN particles) in
+// one workgroup, phase by phase: the pairs' forces and incidence counts, the
+// exclusive scan of the counts, the incidence entries, the narrow sums, then
+// the wide particles one after another.  Each phase is the multi-launch
+// path's own code, so the result is the same bit for bit.  The counts, fill
+// cursors and offsets live in LDS (its atomics are the workgroup's own); the
+// global rows and entries are each written once and first read after a
+// barrier.
+__global__ __launch_bounds__(1024) void k_forces_small(
+    BodiesDev b, const int* __restrict__ pairs, int npairs, const long long* __restrict__ np_dev,
+    const dh_contact* __restrict__ cs, const long long* __restrict__ off, ForceConsts k,
+    long long ncap, double* __restrict__ pf, int* __restrict__ entries,
+    double* __restrict__ force, double* __restrict__ torque, int* __restrict__ touched) {
+  __shared__ long long key[kWideSort];  // the scan's buffer first, then the sort keys
+  __shared__ double stage[256 * 6];
+  __shared__ long long start[kForcesSmallN + 1];
+  __shared__ int cnt[kForcesSmallN];
+  __shared__ int fill[kForcesSmallN];  // fill cursors, then the wide particles
+  __shared__ int nwide;
+  const int n = b.n, tid = threadIdx.x;
+  const int live = np_dev && *np_dev < npairs ? static_cast<int>(*np_dev) : npairs;
+  if (tid < kForcesSmallN) {
+    cnt[tid] = 0;
+    fill[tid] = 0;
   }
+  if (tid == 0) nwide = 0;
+  __syncthreads();
+  for (int p = tid; p < live; p += 1024) pair_forces_one(b, pairs, p, cs, off, k, pf, cnt, ncap);
+  __syncthreads();
+  // start[0..n) exclusive, start[n] the total
+  const long long v = tid < n ? cnt[tid] : 0;
+  key[tid] = v;
+  __syncthreads();
+  for (int sh = 1; sh < 1024; sh <<= 1) {
+    const long long t = tid >= sh ? key[tid - sh] : 0;
+    __syncthreads();
+    key[tid] += t;
+    __syncthreads();
+  }
+  if (tid < n) start[tid] = key[tid] - v;
+  if (tid == 1023) start[n] = key[1023];
+  __syncthreads();
+  for (int p = tid; p < live; p += 1024) fill_one(pairs, p, off, start, fill, entries, ncap);
+  __syncthreads();
+  int* wide = fill;  // the cursors are spent
+  for (int i = tid; i < n; i += 1024) {
+    const int m = cnt[i];
+    if (m > kNarrowSum) {
+      wide[atomicAdd(&nwide, 1)] = i;
+      continue;
+    }
+    double f[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+    sum_by_selection(pairs, entries, pf, start[i], m, f, t);
+    for (int q = 0; q < 3; ++q) {
+      force[3 * i + q] = f[q];
+      torque[3 * i + q] = t[q];
+    }
+    if (touched) touched[i] = m > 0 ? 1 : 0;
+  }
+  __syncthreads();
+  const int nw = nwide;
+  for (int j = 0; j < nw; ++j)
+    wide_sum_one<1024>(wide[j], pairs, start, cnt, entries, pf, force, torque, touched, key,
+                       stage);
 }
 
 struct Gravity {
@@ -370,12 +467,19 @@ int contacts_forces_impl(const dh_bodies* bodies, const int* pairs_dev, int npai
   const int n = bodies->n;
   int rc;
   if ((rc = forces_reserve(n, npairs))) return rc;
+  const BodiesDev b = to_dev(bodies);
+  const ForceConsts k{fp->spring, fp->damper, fp->friction};
+  if (n <= kForcesSmallN) {  // one workgroup, one launch
+    hipLaunchKernelGGL(k_forces_small, 1, 1024, 0, st, b, pairs_dev, npairs, np_dev,
+                       contacts_dev, pair_offset_dev, k, contact_cap, g_fs.pf.as<double>(),
+                       g_fs.entries.as<int>(), force_dev, torque_dev, touched_dev);
+    DH_HIP_CHECK(hipGetLastError());
+    return DH_OK;
+  }
   zero_async(ZeroRegions{{g_fs.incid.as<unsigned>(), g_fs.fill.as<unsigned>(),
                           g_fs.wide.as<unsigned>(), nullptr},
                          {n + 1, n + 1, 1, 0}},
              st);
-  const BodiesDev b = to_dev(bodies);
-  const ForceConsts k{fp->spring, fp->damper, fp->friction};
   if (npairs > 0) {
     hipLaunchKernelGGL(k_pair_forces, grid_for(npairs, 256), 256, 0, st, b, pairs_dev, npairs,
                        contacts_dev, pair_offset_dev, k, g_fs.pf.as<double>(),
