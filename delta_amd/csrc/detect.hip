@@ -1065,6 +1065,30 @@ __global__ __launch_bounds__(128) void k_detect_wave(DetectArgs g, int items) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (S > 0 && S <= kWaveSurvivors) {
+        // survivors that a triangle plane separates by more than the margin
+        // emit nothing (k_queue_solve's first test): only the rest are
+        // queued, compacted in place in row-major order
+        int S2 = 0;
+        for (int e0 = 0; e0 < S; e0 += 64) {
+          const int e = e0 + lane;
+          int fe = 0;
+          bool keep = false;
+          if (e < S) {
+            fe = surv[w][e];
+            const int ra = fe / nc, cb = fe - ra * nc;
+            keep = !plane_apart(load_tri(g, p.oA + r0 + act[w][ra]),
+                                load_tri(g, p.oB + act[w][kWaveRows + cb]), p.margin);
+          }
+          const unsigned long long m = __ballot(keep);
+          if (keep) surv[w][S2 + lane_prefix(m)] = fe;
+          S2 += __popcll(m);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        S = S2;
+      }
       if (S > kWaveSurvivors) {
         deferred = true;
       } else if (S > 0) {
