@@ -527,8 +527,10 @@ def tune_detection(ds, hs, prm, L, contact_cap):
         raise RuntimeError("tune_detection: the scene has no candidate pairs "
                            "(tune on a geometry with contacts)")
     out_c = DeviceArray((contact_cap * 144,), np.uint8)
+    # the median of 7 timings per value: single timings of these short
+    # launches vary by more than the settings differ
     trials = tuning.autotune_detect(ds.scene.geom, d_pairs.ptr, npairs.value, out_c.ptr,
-                                    contact_cap, reps=2)
+                                    contact_cap, reps=7)
     table = tuning.get()
     print(f"[bench] tuning {table}", file=sys.stderr, flush=True)
     return table, [(t["knob"], t["value"], round(t["ms"], 4)) for t in trials]
@@ -691,7 +693,7 @@ def dem_large(a, kind, steps=None, warmup=None, learned=False):
     ccap = 4 * max_contacts + 4096
     out_c = DeviceArray((ccap * 144,), np.uint8)
     trials = [] if a.no_autotune or a.tuning else tuning.autotune_detect(
-        ds.scene.geom, d_pairs.ptr, npairs.value, out_c.ptr, ccap, reps=2)
+        ds.scene.geom, d_pairs.ptr, npairs.value, out_c.ptr, ccap, reps=7)
     del out_c, d_pairs
     table = tuning.get()
     print(f"[bench] tuning {table}", file=sys.stderr, flush=True)

@@ -818,7 +818,8 @@ k_queue_learned(QueueLearnArgs a) {
   const long long live =
       a.need_dev && static_cast<long long>(*a.need_dev) < a.n ? static_cast<long long>(*a.need_dev)
                                                               : a.n;
-  const long long nchunks = (a.n + 63) / 64;
+  // the live entries only (the later passes stop at the same count)
+  const long long nchunks = (live + 63) / 64;
   unsigned evals = 0;
   for (long long ch = static_cast<long long>(blockIdx.x) * kInferWaves + wave; ch < nchunks;
        ch += static_cast<long long>(gridDim.x) * kInferWaves) {
@@ -839,7 +840,7 @@ k_queue_learned(QueueLearnArgs a) {
     V3<double> P, Q;
     bool done;
     learned_pair<kModeFeatures>(L, A, B, valid, P, Q, done, evals);
-    if (e < a.n && !(valid && !done)) {
+    if (e < live && !(valid && !done)) {
       const bool emit = valid && pq_dist(P, Q) <= margin;
       a.flags[e] = emit ? 1 : 0;
       if (emit) {
@@ -862,12 +863,7 @@ k_queue_learned(QueueLearnArgs a) {
   }
 }
 
-// The listed entries by the brute force, one lane each (launched for the
-// capacity; the lanes past the list's length return).
-__global__ __launch_bounds__(256) void k_queue_fallback(QueueLearnArgs a) {
-  const long long j = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (j >= static_cast<long long>(*a.fb_count) || j >= a.n) return;
-  const long long e = a.fb_list[j];
+__device__ __forceinline__ void queue_fallback_entry(const QueueLearnArgs& a, long long e) {
   const int2 en = a.queue[e];
   const int k = en.x / a.tiles;
   const int pa = a.pairs[2 * k], pb = a.pairs[2 * k + 1];
@@ -884,6 +880,16 @@ __global__ __launch_bounds__(256) void k_queue_fallback(QueueLearnArgs a) {
     o[0] = P.x; o[1] = P.y; o[2] = P.z;
     o[3] = Q.x; o[4] = Q.y; o[5] = Q.z;
   }
+}
+
+// The listed entries by the brute force, a lane each at a time (a captured
+// step launches a persistent grid and strides over the list's length).
+__global__ __launch_bounds__(256) void k_queue_fallback(QueueLearnArgs a) {
+  const long long nfb = static_cast<long long>(*a.fb_count) < a.n
+                            ? static_cast<long long>(*a.fb_count) : a.n;
+  for (long long j = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; j < nfb;
+       j += static_cast<long long>(gridDim.x) * 256)
+    queue_fallback_entry(a, a.fb_list[j]);
 }
 
 struct LearnedState {
@@ -925,7 +931,10 @@ int queue_learned(const int2* queue, const int* pairs, int tiles, const int* tri
     const long long max_grid = static_cast<long long>(device_cus()) * 4;
     if (grid > max_grid) grid = max_grid;
     hipLaunchKernelGGL(k_queue_learned, static_cast<unsigned>(grid), 64 * kInferWaves, 0, st, a);
-    hipLaunchKernelGGL(k_queue_fallback, grid_for(n, 256), 256, 0, st, a);
+    long long fgrid = grid_for(n, 256);
+    const long long fcap = static_cast<long long>(device_cus()) * 16;
+    if (need_dev && fgrid > fcap) fgrid = fcap;
+    hipLaunchKernelGGL(k_queue_fallback, static_cast<unsigned>(fgrid), 256, 0, st, a);
   }
   DH_HIP_CHECK(hipGetLastError());
   return DH_OK;
