@@ -208,6 +208,21 @@ __device__ __forceinline__ Pair load_pair(const DetectArgs& g, int k) {
   return p;
 }
 
+// The particles' bounding spheres (the broad phase's centre and largest
+// vertex distance) apart by more than the margin, with an allowance for the
+// rounding of the spheres and of bfSolver's distance: no triangle pair of
+// the two particles emits a contact.
+__device__ __forceinline__ bool spheres_apart(const DetectArgs& g, const Pair& p) {
+  if (!g.psph) return false;
+  const double* sa = g.psph + 4 * static_cast<long long>(p.pa);
+  const double* sb = g.psph + 4 * static_cast<long long>(p.pb);
+  const double dx = sa[0] - sb[0], dy = sa[1] - sb[1], dz = sa[2] - sb[2];
+  const double mag = fmax(fmax(fmax(fabs(sa[0]), fabs(sa[1])), fmax(fabs(sa[2]), fabs(sb[0]))),
+                          fmax(fabs(sb[1]), fabs(sb[2]))) + sa[3] + sb[3];
+  const double reach = sa[3] + sb[3] + p.margin * (1.0 + 0x1p-20) + mag * 0x1p-36;
+  return dx * dx + dy * dy + dz * dz > reach * reach;
+}
+
 __device__ __forceinline__ bool penalty_pair(const Tri<double>& A, const Tri<double>& B,
                                              double max_err, V3<double>& P, V3<double>& Q) {
   const double x0[4] = {0.33, 0.33, 0.33, 0.33};
@@ -435,17 +450,8 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
       return;
     }
     // the whole pair beyond the margin by its bounding spheres
-    bool far_pair = false;
-    if (g.psph) {
-      const double* sa = g.psph + 4 * static_cast<long long>(p.pa);
-      const double* sb = g.psph + 4 * static_cast<long long>(p.pb);
-      const double dx = sa[0] - sb[0], dy = sa[1] - sb[1], dz = sa[2] - sb[2];
-      const double mag = fmax(fmax(fmax(fabs(sa[0]), fabs(sa[1])), fmax(fabs(sa[2]), fabs(sb[0]))),
-                              fmax(fabs(sb[1]), fabs(sb[2]))) + sa[3] + sb[3];
-      const double reach = sa[3] + sb[3] + p.margin * (1.0 + 0x1p-20) + mag * 0x1p-36;
-      far_pair = dx * dx + dy * dy + dz * dz > reach * reach;
-      if (far_pair) culled += tid == 0 ? static_cast<unsigned>(nrows * p.nB) : 0u;
-    }
+    const bool far_pair = spheres_apart(g, p);
+    if (far_pair) culled += tid == 0 ? static_cast<unsigned>(nrows * p.nB) : 0u;
     // A in chunks of 64 rows with their boxes (a wall or a funnel): every
     // chunk of the item tested against B's particle box once, up front
     const bool chunked = staged && g.bbox && g.pbox && p.nA > kRowBlock;
@@ -891,11 +897,13 @@ __global__ __launch_bounds__(256) void k_chunk_cull(DetectArgs g, int items,
 // Rows a lane tests alone in k_lane_cull.
 constexpr int kLaneRows = 16;
 
-// One lane per item, before k_detect_wave: an item of k_detect_wave's with at
-// most kLaneRows rows whose rows' boxes all miss B's particle box (grown by
-// the margin) is culled here, with the counters k_detect_wave would write (a
-// wall of few triangles against the many particles inside it); the other
-// items of k_detect_wave's are listed for it.
+// One lane per item, before k_detect_wave: an item of k_detect_wave's whose
+// particles' bounding spheres are apart by more than the margin (diagonal
+// neighbours in a packing), or with at most kLaneRows rows whose rows' boxes
+// all miss B's particle box (grown by the margin: a wall of few triangles
+// against the many particles inside it), is culled here, with the counters
+// k_detect_wave would write; the other items of k_detect_wave's are listed
+// for it.
 __global__ __launch_bounds__(256) void k_lane_cull(DetectArgs g, int items) {
   const int item = blockIdx.x * 256 + static_cast<int>(threadIdx.x);
   bool listed = false;
@@ -912,7 +920,9 @@ __global__ __launch_bounds__(256) void k_lane_cull(DetectArgs g, int items) {
       const int nrows = r1 - r0;
       if (wave_rows(nrows, p.nA, p.nB)) {
         bool near = !(g.pbox && nrows <= kLaneRows);
-        if (!near) {
+        if (spheres_apart(g, p)) {
+          near = false;  // culled whole
+        } else if (!near) {
           const double* pb = g.pbox + 6 * static_cast<long long>(p.pb);
           for (int t = 0; t < nrows && !near; ++t) {
             double lo[3], hi[3];
