@@ -223,6 +223,22 @@ __device__ __forceinline__ bool spheres_apart(const DetectArgs& g, const Pair& p
   return dx * dx + dy * dy + dz * dz > reach * reach;
 }
 
+// A triangle's box [lo, hi] apart from a particle's bounding sphere s
+// (centre, radius) by more than the margin, with spheres_apart's allowance:
+// no triangle of that particle is within the margin of the box's triangle.
+__device__ __forceinline__ bool box_sphere_apart(const double* lo, const double* hi,
+                                                 const double* s, double margin) {
+  double d2 = 0.0, mag = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double e = fmax(fmax(lo[q] - s[q], s[q] - hi[q]), 0.0);
+    d2 += e * e;
+    mag = fmax(mag, fmax(fmax(fabs(lo[q]), fabs(hi[q])), fabs(s[q])));
+  }
+  const double reach = s[3] + margin * (1.0 + 0x1p-20) + (mag + s[3]) * 0x1p-36;
+  return d2 > reach * reach;
+}
+
 __device__ __forceinline__ bool penalty_pair(const Tri<double>& A, const Tri<double>& B,
                                              double max_err, V3<double>& P, V3<double>& Q) {
   const double x0[4] = {0.33, 0.33, 0.33, 0.33};
@@ -1033,18 +1049,22 @@ __global__ __launch_bounds__(128) void k_detect_wave(DetectArgs g, int items) {
           const double v = __shfl_xor(u[q], o);
           u[q] = ((q % 6) < 3) ? fmin(u[q], v) : fmax(u[q], v);
         }
-      // ordered active rows and columns
+      // ordered active rows and columns: boxes that meet the other side's
+      // union and are not apart from its particle's bounding sphere
       int na = 0, nc = 0;
       for (int side = 0; side < 2; ++side) {
         const int cnt_all = side == 0 ? nrows : p.nB;
         const double* other = u + 6 * (1 - side);
+        const double* osph =
+            g.psph ? g.psph + 4 * static_cast<long long>(side == 0 ? p.pb : p.pa) : nullptr;
         int n_on = 0;
         for (int t0 = 0; t0 < cnt_all; t0 += 64) {
           const int t = t0 + lane;
           bool on = false;
           if (t < cnt_all) {
             const double* b = bx + 6 * (side == 0 ? t : kWaveRows + t);
-            on = !boxes_apart(b, b + 3, other, other + 3, p.margin);
+            on = !boxes_apart(b, b + 3, other, other + 3, p.margin) &&
+                 !(osph && box_sphere_apart(b, b + 3, osph, p.margin));
           }
           const unsigned long long m = __ballot(on);
           if (on) act[w][side * kWaveRows + n_on + lane_prefix(m)] = static_cast<unsigned char>(t);
