@@ -677,9 +677,11 @@ __device__ __forceinline__ void detect_item(const DetectArgs& g, const int item)
         }
         if (!QONLY) {
           // the survivors that a triangle plane separates by more than the
-          // margin (and, with the binary32 screen, those it proves far)
-          // drop out; the rest are compacted in place (order kept) for the
-          // binary64 solve
+          // margin (and, with the binary32 screen, those it proves far) drop
+          // out; the rest are compacted in place (order kept) for the
+          // binary64 solve.  (The edge-pair axes of the queue paths do not
+          // pay here: an item keeps one wave's bfSolver round as long as a
+          // survivor is left, whatever their number.)
           int n2 = 0;
           for (int b = 0; b < nl; b += 256) {
             const int e = b + tid;
@@ -1096,9 +1098,9 @@ __global__ __launch_bounds__(128) void k_detect_wave(DetectArgs g, int items) {
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (S > 0 && S <= kWaveSurvivors) {
-        // survivors that a triangle plane separates by more than the margin
-        // emit nothing (k_queue_solve's first test): only the rest are
-        // queued, compacted in place in row-major order
+        // survivors that a triangle plane or an edge-pair axis separates by
+        // more than the margin emit nothing (k_queue_solve's first test):
+        // only the rest are queued, compacted in place in row-major order
         int S2 = 0;
         for (int e0 = 0; e0 < S; e0 += 64) {
           const int e = e0 + lane;
@@ -1107,8 +1109,8 @@ __global__ __launch_bounds__(128) void k_detect_wave(DetectArgs g, int items) {
           if (e < S) {
             fe = surv[w][e];
             const int ra = fe / nc, cb = fe - ra * nc;
-            keep = !plane_apart(load_tri(g, p.oA + r0 + act[w][ra]),
-                                load_tri(g, p.oB + act[w][kWaveRows + cb]), p.margin);
+            keep = !axes_apart(load_tri(g, p.oA + r0 + act[w][ra]),
+                               load_tri(g, p.oB + act[w][kWaveRows + cb]), p.margin);
           }
           const unsigned long long m = __ballot(keep);
           if (keep) surv[w][S2 + lane_prefix(m)] = fe;
@@ -1498,7 +1500,7 @@ __device__ __forceinline__ void queue_solve_entry(const DetectArgs& g, long long
   const Tri<double> A = load_tri(g, g.tri_offset[pa] + iA);
   const Tri<double> B = load_tri(g, g.tri_offset[pb] + iB);
   const double margin = g.eps[pa] + g.eps[pb];
-  if (plane_apart(A, B, margin) || (SCREEN && screened_far(A, B, margin))) {
+  if (axes_apart(A, B, margin) || (SCREEN && screened_far(A, B, margin))) {
     flags[e] = 0;
     return;
   }

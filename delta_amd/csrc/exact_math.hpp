@@ -339,6 +339,53 @@ DH_HD bool plane_side_apart(const Tri<double>& A, const Tri<double>& B, double t
   return h - E > thr * nn;
 }
 
+// The separating axes of the nine edge pairs: n = ea x eb for an edge ea of
+// A and eb of B.  Every vertex is projected onto n (relative to A's first
+// vertex); when B's projections lie beyond A's by a gap g, every point of B
+// is at least g / |n| from every point of A.  n is used as computed (any
+// axis bounds the distance), so only the projections round: each is within
+// E = 2^-46 |ea|_1 |eb|_1 R of exact (R the largest relative vertex
+// distance, as in plane_side_apart), and |n| is bounded above as there.
+// True when some axis separates A and B by more than thr.
+DH_HD bool edge_axes_apart(const Tri<double>& A, const Tri<double>& B, double thr) {
+  using std::fabs;
+  V3<double> ra[3], rb[3];
+  double R = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ra[k] = sub(A.v[k], A.v[0]);
+    rb[k] = sub(B.v[k], A.v[0]);
+    R = fmax(R, fmax(fabs(ra[k].x) + fabs(ra[k].y) + fabs(ra[k].z),
+                     fabs(rb[k].x) + fabs(rb[k].y) + fabs(rb[k].z)));
+  }
+#pragma unroll 1
+  for (int k = 0; k < 3; ++k) {
+    const V3<double> ea = sub(A.v[k == 2 ? 0 : k + 1], A.v[k]);
+    const double la = fabs(ea.x) + fabs(ea.y) + fabs(ea.z);
+#pragma unroll 1
+    for (int m = 0; m < 3; ++m) {
+      const V3<double> eb = sub(B.v[m == 2 ? 0 : m + 1], B.v[m]);
+      const V3<double> n = {ea.y * eb.z - ea.z * eb.y, ea.z * eb.x - ea.x * eb.z,
+                            ea.x * eb.y - ea.y * eb.x};
+      const double uv = la * (fabs(eb.x) + fabs(eb.y) + fabs(eb.z));
+      double loA = 1e308, hiA = -1e308, loB = 1e308, hiB = -1e308;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double ha = dot(n, ra[i]), hb = dot(n, rb[i]);
+        loA = fmin(loA, ha);
+        hiA = fmax(hiA, ha);
+        loB = fmin(loB, hb);
+        hiB = fmax(hiB, hb);
+      }
+      const double E = 0x1p-46 * uv * R;
+      const double nn = std::sqrt(dot(n, n)) + 0x1p-48 * uv;
+      const double gap = fmax(loB - hiA, loA - hiB);
+      if (gap - 2.0 * E > thr * nn) return true;
+    }
+  }
+  return false;
+}
+
 DH_HD bool plane_apart(const Tri<double>& A, const Tri<double>& B, double margin) {
   using std::fabs;
   double mag = 0.0;
@@ -349,6 +396,19 @@ DH_HD bool plane_apart(const Tri<double>& A, const Tri<double>& B, double margin
                               fmax(fabs(B.v[k].y), fabs(B.v[k].z)))));
   const double thr = margin * (1.0 + 0x1p-20) + mag * 0x1p-40;
   return plane_side_apart(A, B, thr) || plane_side_apart(B, A, thr);
+}
+
+// plane_apart, then the edge-pair axes (for the pairs the planes leave).
+DH_HD bool axes_apart(const Tri<double>& A, const Tri<double>& B, double margin) {
+  using std::fabs;
+  if (plane_apart(A, B, margin)) return true;
+  double mag = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    mag = fmax(mag, fmax(fmax(fabs(A.v[k].x), fabs(A.v[k].y)),
+                         fmax(fmax(fabs(A.v[k].z), fabs(B.v[k].x)),
+                              fmax(fabs(B.v[k].y), fabs(B.v[k].z)))));
+  return edge_axes_apart(A, B, margin * (1.0 + 0x1p-20) + mag * 0x1p-40);
 }
 
 // Screen before bfSolver in binary64: true when A and B are provably farther

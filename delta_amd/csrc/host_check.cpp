@@ -336,6 +336,11 @@ extern "C" void hc_screened_far(const double* soa, long n, double margin, unsign
   for (long i = 0; i < n; ++i) far[i] = dh::screened_far(load(soa, n, i, 0), load(soa, n, i, 9), margin) ? 1 : 0;
 }
 
+// axes_apart (the triangle planes, then the edge-pair axes) per pair.
+extern "C" void hc_axes_apart(const double* soa, long n, double margin, unsigned char* far) {
+  for (long i = 0; i < n; ++i) far[i] = dh::axes_apart(load(soa, n, i, 0), load(soa, n, i, 9), margin) ? 1 : 0;
+}
+
 // plane_apart (the triangle-plane test before the binary64 bfSolver) per pair.
 extern "C" void hc_plane_apart(const double* soa, long n, double margin, unsigned char* far) {
   for (long i = 0; i < n; ++i) far[i] = dh::plane_apart(load(soa, n, i, 0), load(soa, n, i, 9), margin) ? 1 : 0;

@@ -227,11 +227,12 @@ def test_exact_choice_is_bfsolver_bit_for_bit(hc, fn):
             assert acc.mean() > 0.05  # random guesses are still right sometimes
 
 
-@pytest.mark.parametrize("fn", ["hc_screened_far", "hc_plane_apart"])
+@pytest.mark.parametrize("fn", ["hc_screened_far", "hc_plane_apart", "hc_axes_apart"])
 def test_detection_screens_are_lower_bounds(hc, fn):
     """The detection's tests ahead of its binary64 bfSolver (exact_math.hpp:
-    screened_far, the binary32 screen, and plane_apart, the triangle-plane
-    test) skip a triangle pair only when the oracle's bfSolver distance
+    screened_far, the binary32 screen; plane_apart, the triangle-plane test;
+    axes_apart, the planes and the nine edge-pair axes) skip a triangle pair
+    only when the oracle's bfSolver distance
     exceeds the margin: the recorded pair of a C1 particle triangle 0.14 mm
     above the 1 m floor triangle (a contact whose binary32 bfSolver distance
     comes out 0.40 mm, three times too far), small triangles against large
@@ -282,6 +283,15 @@ def test_detection_screens_are_lower_bounds(hc, fn):
         # the large triangle's plane separates every small one lifted
         # entirely above it beyond the margin
         assert far[sb][lift > 2 * margin].all()
+    if fn == "hc_axes_apart":
+        # the edge axes add to the planes: among the regular pairs beyond
+        # three margins, most of those the planes leave
+        pl = np.zeros(n, np.uint8)
+        hc.hc_plane_apart(ol.dptr(s), C.c_long(n), C.c_double(margin),
+                          pl.ctypes.data_as(C.c_void_p))
+        assert np.all(far[pl.astype(bool)])
+        left = beyond & ~pl[g].astype(bool)
+        assert left.sum() > 100 and far[g][left].mean() > 0.5, far[g][left].mean()
 
 
 def test_device_sincos_is_correctly_rounded(hc):
