@@ -358,11 +358,13 @@ DH_HD bool edge_axes_apart(const Tri<double>& A, const Tri<double>& B, double th
     R = fmax(R, fmax(fabs(ra[k].x) + fabs(ra[k].y) + fabs(ra[k].z),
                      fabs(rb[k].x) + fabs(rb[k].y) + fabs(rb[k].z)));
   }
-#pragma unroll 1
+  // unrolled: a variable index into A.v or B.v would put both triangles
+  // in scratch memory
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
     const V3<double> ea = sub(A.v[k == 2 ? 0 : k + 1], A.v[k]);
     const double la = fabs(ea.x) + fabs(ea.y) + fabs(ea.z);
-#pragma unroll 1
+#pragma unroll
     for (int m = 0; m < 3; ++m) {
       const V3<double> eb = sub(B.v[m == 2 ? 0 : m + 1], B.v[m]);
       const V3<double> n = {ea.y * eb.z - ea.z * eb.y, ea.z * eb.x - ea.x * eb.z,
