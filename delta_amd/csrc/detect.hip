@@ -1385,43 +1385,6 @@ __global__ __launch_bounds__(1024) void k_scan_single(const int* __restrict__ in
   if (i == 1023) out[n] = buf[1023];
 }
 
-// The whole scan in one workgroup for n <= 16,384: thread t takes the run
-// of ceil(n / 1024) <= 16 entries from t * run, adds it up, the workgroup
-// scans the 1,024 run sums in LDS, and each thread writes its run's offsets.  One
-// launch in place of two for the medium scenes' particle and pair counts.
-constexpr int kScanOneMax = 16384;
-__global__ __launch_bounds__(1024) void k_scan_one(const int* __restrict__ in, int n,
-                                                   long long* __restrict__ out,
-                                                   const unsigned long long* __restrict__ live) {
-  __shared__ long long buf[1024];
-  n = scan_len(n, live);
-  const int t = threadIdx.x;
-  const int run = (n + 1023) / 1024;  // at most 16
-  const int b = t * run;
-  long long v[16];
-  long long sum = 0;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    v[k] = (k < run && b + k < n) ? in[b + k] : 0;
-    sum += v[k];
-  }
-  buf[t] = sum;
-  __syncthreads();
-  for (int sh = 1; sh < 1024; sh <<= 1) {
-    const long long o = t >= sh ? buf[t - sh] : 0;
-    __syncthreads();
-    buf[t] += o;
-    __syncthreads();
-  }
-  long long acc = buf[t] - sum;  // exclusive offset of the run
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    if (k < run && b + k < n) out[b + k] = acc;
-    acc += v[k];
-  }
-  if (t == 1023) out[n] = buf[1023];
-}
-
 // Exclusive scan of the block sums in place, by one workgroup: 1,024 at a
 // time with a running carry.
 __global__ __launch_bounds__(1024) void k_scan_sums(long long* __restrict__ sums, int nb) {
@@ -1693,11 +1656,6 @@ static int scan_counts_upto(const int* counts, int n, const unsigned long long* 
                             long long* offs, long long* sums, hipStream_t st) {
   if (n <= 1024) {
     hipLaunchKernelGGL(k_scan_single, 1, 1024, 0, st, counts, n, offs, live);
-    DH_HIP_CHECK(hipGetLastError());
-    return DH_OK;
-  }
-  if (n <= kScanOneMax) {
-    hipLaunchKernelGGL(k_scan_one, 1, 1024, 0, st, counts, n, offs, live);
     DH_HIP_CHECK(hipGetLastError());
     return DH_OK;
   }

@@ -151,11 +151,12 @@ def test_forces_and_update_match_oracle():
                 assert np.allclose(got, want, rtol=1e-14, atol=1e-300), (name, p)
 
 
-@pytest.mark.parametrize("n", [0, 1, 1023, 1024, 1025, 5000, 1 << 20, 3_000_001])
+@pytest.mark.parametrize("n", [0, 1, 1023, 1024, 1025, 5000, 16383, 16384, 16385, 1 << 20,
+                               3_000_001])
 def test_exclusive_scan(n):
     """dh_exclusive_scan against numpy: offs[k] = sum of counts[:k] and
-    offs[n] the total, for sizes around the 1,024-entry blocks and past
-    1,024 blocks (the block sums are scanned 1,024 at a time)."""
+    offs[n] the total, for sizes around the 1,024-entry blocks, 16 blocks
+    and past 1,024 blocks (the block sums are scanned 1,024 at a time)."""
     L = _lib.lib()
     L.dh_exclusive_scan.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     rng = np.random.default_rng(n)
