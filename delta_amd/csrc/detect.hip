@@ -1703,8 +1703,10 @@ static void launch_detect(unsigned items, size_t shm, hipStream_t st, const Dete
   if (M == DH_BF && a.queue && a.wave_items) {
     hipLaunchKernelGGL(k_lane_cull, items > 0 ? grid_for(items, 256) : 1u, 256, 0, st, a,
                        static_cast<int>(items));
+    // at most the workgroups one CU holds at once (3 waves per SIMD at the
+    // kernel's 161 VGPRs, 2 waves per workgroup): no partial second round
     const long long wg = (static_cast<long long>(items) + 1) / 2;
-    const long long cap = static_cast<long long>(device_cus()) * 16;
+    const long long cap = static_cast<long long>(device_cus()) * 6;
     hipLaunchKernelGGL(k_detect_wave, static_cast<unsigned>(wg < cap ? (wg > 0 ? wg : 1) : cap),
                        128, 0, st, a, static_cast<int>(items));
   }
